@@ -1,0 +1,257 @@
+/* op2hip -- MI355X (gfx950) kernels for PyOP2's indirect par_loop hot path.
+ *
+ * C ABI: plain pointers and sizes, no torch or Python types.  Every pointer
+ * documented as "device" must address HIP device memory of the current
+ * device; "host" pointers are ordinary host memory.  Streams are hipStream_t
+ * passed as void* (NULL = the legacy default stream).  Every entry point
+ * returns 0 on success or an OP2HIP_ERR_* code; op2hip_last_error() returns
+ * the message of the last failure on the calling thread.  Entry points never
+ * free or reallocate caller buffers and never synchronise the stream unless
+ * documented (functions returning a host-side count do).
+ *
+ * The element-loop entry points replace the ctypes function the reference
+ * JIT-compiles per par_loop, ``wrap_<kernel>(int start, int end, void*...)``
+ * (pyop2/global_kernel.py:322-331, argtypes :410-417, argument order
+ * pyop2/codegen/builder.py:943-962 and pyop2/parloop.py:203-212): the
+ * iteration range, then each par_loop argument's data pointer in call order,
+ * then the unique maps in first-use order.  A Mat argument, a PETSc handle in
+ * the reference (pyop2/types/mat.py:567-569), is an op2hip_mat descriptor
+ * here.  Two trailing arguments are added: the execution plan and the stream.
+ */
+#ifndef OP2HIP_H
+#define OP2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OP2HIP_ABI_VERSION 1
+
+enum op2hip_status {
+  OP2HIP_OK = 0,
+  OP2HIP_ERR_HIP = 1,        /* a HIP runtime call failed */
+  OP2HIP_ERR_ARG = 2,        /* invalid argument (sizes, NULL, unsupported) */
+  OP2HIP_ERR_WORKSPACE = 3,  /* workspace too small */
+  OP2HIP_ERR_OVERFLOW = 4,   /* a count does not fit int32 (PETSc int) */
+  OP2HIP_ERR_JIT = 5,        /* runtime compilation or module load failed */
+  OP2HIP_ERR_PATTERN = 6     /* Mat insertion outside the sparsity pattern */
+};
+
+/* Conflict-resolution strategies for indirect INC (and Mat) arguments. */
+enum op2hip_strategy {
+  OP2HIP_ATOMIC = 0,     /* one thread per element, device-scope atomics */
+  OP2HIP_COLOURED = 1,   /* greedy element colouring, one launch per colour */
+  OP2HIP_STAGED = 2,     /* LDS-staged tiles, one atomic per tile node */
+  OP2HIP_ROWGATHER = 3   /* Mat only: owner-computes block rows, no atomics */
+};
+
+/* Execution plan of one launch.  Host struct; arrays as annotated. */
+typedef struct op2hip_exec {
+  int32_t strategy;            /* enum op2hip_strategy */
+  int32_t block_size;          /* threads per workgroup (0: kernel default) */
+  /* OP2HIP_COLOURED: elements of [start,end) sorted by colour */
+  const int32_t *colour_elems;     /* device, end-start entries */
+  const int32_t *colour_offsets;   /* host, ncolours+1 offsets into colour_elems */
+  int32_t ncolours;
+  /* OP2HIP_STAGED: tiles of consecutive elements of [start,end) */
+  int32_t tile_size;               /* elements per tile */
+  int32_t ntiles;
+  const int32_t *tile_node_ptr;    /* device, ntiles+1: node list of each tile */
+  const int32_t *tile_nodes;       /* device: global node ids, sorted per tile */
+  const uint16_t *elem_local;      /* device, (end-start) x arity local node ids */
+  int32_t max_tile_nodes;          /* max over tiles of the node-list length */
+  /* OP2HIP_ROWGATHER: batches of consecutive block rows */
+  int32_t nbatches;
+  const int32_t *batch_rows;       /* device, nbatches+1 row boundaries */
+  int32_t batch_max_nnz;           /* max blocks of a batch (sizes LDS) */
+  int32_t batch_max_inc;           /* max incidences of a batch */
+  int32_t batch_max_rows;          /* max rows of a batch */
+  /* Subset iteration: element n = subset[i] for i in [start,end) (or NULL) */
+  const int32_t *subset;           /* device */
+} op2hip_exec;
+
+/* Device CSR / BSR matrix on a fixed pattern (the reference's PETSc AIJ or
+ * BAIJ, pyop2/types/mat.py:672-729).  values holds nnz blocks of
+ * rdim x cdim entries, row-major within a block.  rdim = cdim = 1 is CSR. */
+typedef struct op2hip_mat {
+  const int32_t *rowptr;   /* device, nrows+1 */
+  const int32_t *colidx;   /* device, nnz, sorted within each row */
+  void *values;            /* device, nnz*rdim*cdim of the matrix dtype */
+  int32_t nrows, ncols;    /* block rows / block columns */
+  int32_t rdim, cdim;      /* block shape */
+  int32_t *status;         /* device int32[1]; insertions that missed the pattern are counted here */
+  /* Row-incidence plan (transpose of the row map) for OP2HIP_ROWGATHER */
+  const int32_t *inc_ptr;  /* device, nrows+1 */
+  const int32_t *inc_ent;  /* device, element*rarity + local row, sorted per row */
+} op2hip_mat;
+
+/* One (row map, column map) pair of a Sparsity (pyop2/types/mat.py:118-176). */
+typedef struct op2hip_map_pair {
+  const int32_t *rmap;     /* device, (>= iter_size) x rarity */
+  const int32_t *cmap;     /* device, (>= iter_size) x carity */
+  int32_t rarity, carity;
+  int32_t iter_size;       /* owned size of the iteration set (sparsity.pyx:262) */
+} op2hip_map_pair;
+
+#define OP2HIP_MAX_PAIRS 8
+
+/* ---- library ---------------------------------------------------------- */
+int op2hip_abi_version(void);
+const char *op2hip_last_error(void);
+/* Selects `device` and checks it is gfx950-capable and reachable. */
+int op2hip_init(int32_t device);
+
+/* ---- prefix sums ------------------------------------------------------- */
+/* out[0] = 0, out[i+1] = out[i] + in[i] for i < n (int32; OVERFLOW if the
+ * total exceeds INT32_MAX).  Writes the total to *total_host (synchronises). */
+int op2hip_scan_workspace(int64_t n, size_t *bytes);
+int op2hip_exclusive_scan(int64_t n, const int32_t *in, int32_t *out,
+                          int64_t *total_host, void *work, size_t work_bytes,
+                          void *stream);
+
+/* ---- map transpose (row incidences) ------------------------------------ */
+/* For a map m (iter_size x arity, entries < 0 ignored) over [0, iter_size):
+ * inc_ptr[r] .. inc_ptr[r+1] lists e*arity + i for every m[e][i] == r,
+ * ascending.  count: fills inc_ptr and *total_host (synchronises);
+ * fill: writes inc_ent (inc_ptr from count).  Replaces nothing in the
+ * reference: the row-gather plan of the device assembly. */
+int op2hip_transpose_workspace(int32_t nrows, size_t *bytes);
+int op2hip_transpose_count(int32_t nrows, const int32_t *map, int32_t arity,
+                           int32_t iter_size, int32_t *inc_ptr,
+                           int64_t *total_host, void *work, size_t work_bytes,
+                           void *stream);
+int op2hip_transpose_fill(int32_t nrows, const int32_t *map, int32_t arity,
+                          int32_t iter_size, const int32_t *inc_ptr,
+                          int32_t *inc_ent, void *work, size_t work_bytes,
+                          void *stream);
+
+/* ---- sparsity (replaces pyop2/sparsity.pyx:121-284, build_sparsity and
+ *      fill_with_zeros, non-extruded, one DataSet pair) ------------------ */
+/* Block pattern of the union over `pairs` of rmap[e,:] x cmap[e,:] for
+ * e < iter_size, plus (i, i) for i < min(nrows, ncols) when set_diag.
+ * Step 1 (count): rowptr (device, nrows+1) and *nnz_host (synchronises).
+ * Step 2 (fill): colidx (device, nnz), sorted unique per row.
+ * Each pair's row incidences come from op2hip_transpose_* (inc_ptr/inc_ent
+ * arrays, one per pair, device pointers passed in host arrays). */
+int op2hip_sparsity_count(int32_t nrows, int32_t ncols,
+                          const op2hip_map_pair *pairs, int32_t npairs,
+                          const int32_t *const *inc_ptr,
+                          const int32_t *const *inc_ent, int32_t set_diag,
+                          int32_t *rowptr, int64_t *nnz_host, void *work,
+                          size_t work_bytes, void *stream);
+int op2hip_sparsity_fill(int32_t nrows, int32_t ncols,
+                         const op2hip_map_pair *pairs, int32_t npairs,
+                         const int32_t *const *inc_ptr,
+                         const int32_t *const *inc_ent, int32_t set_diag,
+                         const int32_t *rowptr, int32_t *colidx, void *stream);
+/* Dof (scalar) CSR pattern of a block pattern with rdim x cdim blocks
+ * (the AIJ layout of a non-block-sparse vector Mat, sparsity.pyx:168-173). */
+int op2hip_pattern_expand(int32_t nrows, const int32_t *rowptr,
+                          const int32_t *colidx, int32_t rdim, int32_t cdim,
+                          int32_t *rowptr_out, int32_t *colidx_out,
+                          void *stream);
+
+/* ---- greedy colouring (doc/sphinx/source/plan.rst:46-80), host --------- */
+/* Colours elements [0, nelem) so that no two elements of one colour share
+ * an entry of any map (maps into the same target set share masks).  Host
+ * arrays.  Returns the colour count in *ncolours. */
+int op2hip_colour_greedy(int32_t nelem, int32_t nmaps,
+                         const int32_t *const *maps, const int32_t *arity,
+                         const int32_t *target_id, const int32_t *target_size,
+                         int32_t ntargets, int32_t *colour, int32_t *ncolours);
+/* Row batches for OP2HIP_ROWGATHER: consecutive rows whose incidences and
+ * blocks stay within max_inc / max_nnz (a single larger row gets its own
+ * batch).  Host arrays; batch_rows has room for nrows+1 entries. */
+int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr,
+                            const int32_t *rowptr, int32_t max_inc,
+                            int32_t max_nnz, int32_t *batch_rows,
+                            int32_t *nbatches, int32_t *max_batch_inc,
+                            int32_t *max_batch_nnz, int32_t *max_batch_rows);
+
+/* ---- hand-written element loops ---------------------------------------- */
+/* P1 triangles, op2.Kernel names p1_lumped_mass / p1_laplace_residual /
+ * p1_laplace_matrix (pyop2_amd/elements.py).  X is (nverts, 2). */
+int op2hip_wrap_p1_lumped_mass(int32_t start, int32_t end, double *M,
+                               const double *X, const int32_t *c2v,
+                               const op2hip_exec *ex, void *stream);
+int op2hip_wrap_p1_laplace_residual(int32_t start, int32_t end, double *R,
+                                    const double *X, const double *u,
+                                    const int32_t *c2v, const op2hip_exec *ex,
+                                    void *stream);
+int op2hip_wrap_p1_laplace_matrix(int32_t start, int32_t end,
+                                  const op2hip_mat *A, const double *X,
+                                  const int32_t *c2v, const op2hip_exec *ex,
+                                  void *stream);
+/* P2 tetrahedra, isotropic elasticity (lam, mu), X (nverts, 3) through the
+ * cell->vertex map c2v (arity 4), u and R (nnodes, 3) through the P2 map
+ * c2n (arity 10).  Argument order follows the kernels' signatures:
+ * residual (R, X, u; maps c2n, c2v), matrix (A, X; maps c2n, c2v), fused
+ * (A, R, X, u; maps c2n, c2v).  _f32 variants take float data and a float
+ * matrix. */
+int op2hip_wrap_p2_elasticity_residual(int32_t start, int32_t end, double *R,
+                                       const double *X, const double *u,
+                                       const int32_t *c2n, const int32_t *c2v,
+                                       double lam, double mu,
+                                       const op2hip_exec *ex, void *stream);
+int op2hip_wrap_p2_elasticity_residual_f32(int32_t start, int32_t end, float *R,
+                                           const float *X, const float *u,
+                                           const int32_t *c2n, const int32_t *c2v,
+                                           double lam, double mu,
+                                           const op2hip_exec *ex, void *stream);
+int op2hip_wrap_p2_elasticity_matrix(int32_t start, int32_t end,
+                                     const op2hip_mat *A, const double *X,
+                                     const int32_t *c2n, const int32_t *c2v,
+                                     double lam, double mu,
+                                     const op2hip_exec *ex, void *stream);
+int op2hip_wrap_p2_elasticity_matrix_f32(int32_t start, int32_t end,
+                                         const op2hip_mat *A, const float *X,
+                                         const int32_t *c2n, const int32_t *c2v,
+                                         double lam, double mu,
+                                         const op2hip_exec *ex, void *stream);
+int op2hip_wrap_p2_elasticity_fused(int32_t start, int32_t end,
+                                    const op2hip_mat *A, double *R,
+                                    const double *X, const double *u,
+                                    const int32_t *c2n, const int32_t *c2v,
+                                    double lam, double mu,
+                                    const op2hip_exec *ex, void *stream);
+int op2hip_wrap_p2_elasticity_fused_f32(int32_t start, int32_t end,
+                                        const op2hip_mat *A, float *R,
+                                        const float *X, const float *u,
+                                        const int32_t *c2n, const int32_t *c2v,
+                                        double lam, double mu,
+                                        const op2hip_exec *ex, void *stream);
+/* Adjoint of the P2 residual loop: given Rbar (nnodes, 3), adds
+ * ubar += K(X)^T Rbar and Xbar += d(Rbar . R(u, X))/dX (transposed
+ * scatter/gather of the residual loop; no reference counterpart). */
+int op2hip_wrap_p2_elasticity_adjoint(int32_t start, int32_t end,
+                                      double *ubar, double *Xbar,
+                                      const double *Rbar, const double *X,
+                                      const double *u, const int32_t *c2n,
+                                      const int32_t *c2v, double lam, double mu,
+                                      const op2hip_exec *ex, void *stream);
+
+/* ---- generic loops: runtime-compiled gather/compute/scatter template ---- */
+/* Compiles HIP source (a generated wrapper around the user's C kernel) for
+ * gfx950 with hiprtc and loads it.  *module receives an opaque handle. */
+int op2hip_jit_compile(const char *source, const char *const *options,
+                       int32_t noptions, void **module, char *log,
+                       size_t log_bytes);
+int op2hip_jit_code_object(const char *source, const char *const *options,
+                           int32_t noptions, void *code, size_t *code_bytes,
+                           char *log, size_t log_bytes);
+int op2hip_jit_load(const void *code, void **module);
+int op2hip_jit_unload(void *module);
+/* Launches kernel `name` of `module`: args is an array of pointers to the
+ * argument values (the hipModuleLaunchKernel convention). */
+int op2hip_jit_launch(void *module, const char *name, uint32_t grid,
+                      uint32_t block, uint32_t shared_bytes, void **args,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OP2HIP_H */

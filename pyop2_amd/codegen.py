@@ -1,0 +1,454 @@
+"""Generated gather/compute/scatter kernels for arbitrary C-string kernels.
+
+For a local kernel with no hand-written implementation, the executor emits
+one HIP kernel per GlobalKernel signature: each work-item takes one
+iteration element ``n`` and follows the reference wrapper's per-element
+semantics (``pyop2/codegen/builder.py``):
+
+* indirect Dat READ/RW/MIN/MAX arguments are gathered into private buffers
+  ``buf[i*dim + j] = dat[map[n][i]*dim + j]``; INC/WRITE buffers start at
+  zero; direct Dats pass ``&dat[n*dim]``;
+* Globals: READ passes the device array; INC uses a zeroed buffer and
+  MIN/MAX a copy, reduced over the wavefront before one atomic per wave;
+* Mats: a zeroed ``(ar_r*rdim) x (ar_c*cdim)`` buffer, added (INC) or
+  stored (WRITE) block by block at the slot found by binary search of the
+  row's sorted column indices;
+* the user's C function is compiled unchanged as a device function
+  (``#pragma clang force_cuda_host_device``) and called with pointer
+  adaptors that convert to whatever pointer types it declares;
+* unpack: INC adds (device-scope atomics, or plain adds inside a colour),
+  MIN/MAX atomic min/max, WRITE/RW stores.
+
+Indirect WRITE/RW arguments and Mat WRITE need exclusive targets, so such
+loops run coloured (greedy colouring, one launch per colour); INC-only
+loops run with atomics.  Sources are compiled for gfx950 with hiprtc and
+the code objects cached in memory and under ``<cache_dir>/hip``.
+"""
+import ctypes
+import hashlib
+import os
+import re
+import threading
+
+import numpy as np
+
+from pyop2_amd import device as dev
+from pyop2_amd.configuration import configuration
+from pyop2_amd.datatypes import as_cstr
+from pyop2_amd.exceptions import CompilationError
+from pyop2_amd.global_kernel import DatKernelArg, GlobalKernelArg, MatKernelArg
+from pyop2_amd.types.access import INC, MAX, MIN, READ, RW, WRITE
+from pyop2_amd.types.set import Subset
+
+__all__ = ["GeneratedKernel", "generate_source", "compile_options"]
+
+_PRELUDE = r"""
+typedef signed char int8_t;
+typedef unsigned char uint8_t;
+typedef short int16_t;
+typedef unsigned short uint16_t;
+typedef int int32_t;
+typedef unsigned int uint32_t;
+typedef long long int64_t;
+typedef unsigned long long uint64_t;
+#define restrict __restrict__
+
+/* Converts to any pointer type a user kernel declares for an argument. */
+struct op2_ptr_ {
+  void *p;
+  template <typename U> __host__ __device__ operator U *() const { return static_cast<U *>(p); }
+};
+
+template <typename T> __device__ inline void op2_atomic_add(T *p, T v) { atomicAdd(p, v); }
+__device__ inline void op2_atomic_add(int64_t *p, int64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+}
+__device__ inline void op2_atomic_add(uint64_t *p, uint64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+}
+template <typename T> __device__ inline void op2_atomic_min(T *p, T v) { atomicMin(p, v); }
+template <typename T> __device__ inline void op2_atomic_max(T *p, T v) { atomicMax(p, v); }
+__device__ inline void op2_atomic_min(int64_t *p, int64_t v) { atomicMin(reinterpret_cast<long long *>(p), (long long)v); }
+__device__ inline void op2_atomic_max(int64_t *p, int64_t v) { atomicMax(reinterpret_cast<long long *>(p), (long long)v); }
+__device__ inline void op2_atomic_min(uint64_t *p, uint64_t v) { atomicMin(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v); }
+__device__ inline void op2_atomic_max(uint64_t *p, uint64_t v) { atomicMax(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v); }
+__device__ inline void op2_atomic_min(double *p, double v) {
+  unsigned long long *a = reinterpret_cast<unsigned long long *>(p);
+  unsigned long long old = *a, assumed;
+  while (__longlong_as_double(old) > v) {
+    assumed = old;
+    old = atomicCAS(a, assumed, __double_as_longlong(v));
+    if (old == assumed) break;
+  }
+}
+__device__ inline void op2_atomic_max(double *p, double v) {
+  unsigned long long *a = reinterpret_cast<unsigned long long *>(p);
+  unsigned long long old = *a, assumed;
+  while (__longlong_as_double(old) < v) {
+    assumed = old;
+    old = atomicCAS(a, assumed, __double_as_longlong(v));
+    if (old == assumed) break;
+  }
+}
+__device__ inline void op2_atomic_min(float *p, float v) {
+  unsigned int *a = reinterpret_cast<unsigned int *>(p);
+  unsigned int old = *a, assumed;
+  while (__uint_as_float(old) > v) {
+    assumed = old;
+    old = atomicCAS(a, assumed, __float_as_uint(v));
+    if (old == assumed) break;
+  }
+}
+__device__ inline void op2_atomic_max(float *p, float v) {
+  unsigned int *a = reinterpret_cast<unsigned int *>(p);
+  unsigned int old = *a, assumed;
+  while (__uint_as_float(old) < v) {
+    assumed = old;
+    old = atomicCAS(a, assumed, __float_as_uint(v));
+    if (old == assumed) break;
+  }
+}
+template <typename T> __device__ inline T op2_min(T a, T b) { return b < a ? b : a; }
+template <typename T> __device__ inline T op2_max(T a, T b) { return b > a ? b : a; }
+
+template <typename T> __device__ inline T op2_wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <typename T> __device__ inline T op2_wave_min(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = op2_min(v, (T)__shfl_xor(v, o, 64));
+  return v;
+}
+template <typename T> __device__ inline T op2_wave_max(T v) {
+  for (int o = 32; o > 0; o >>= 1) v = op2_max(v, (T)__shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ inline int32_t op2_find_slot(const int32_t *__restrict__ rowptr,
+                                         const int32_t *__restrict__ colidx,
+                                         int32_t row, int32_t col) {
+  int32_t l = rowptr[row], h = rowptr[row + 1] - 1;
+  while (l <= h) {
+    const int32_t m = l + ((h - l) >> 1);
+    const int32_t c = colidx[m];
+    if (c == col) return m;
+    if (c < col) l = m + 1; else h = m - 1;
+  }
+  return -1;
+}
+"""
+
+_STD_HEADER = re.compile(r'^\s*#\s*include\s*<(math|stdlib|stdio|string|stdint|stdbool|float|limits|complex|inttypes|stddef|assert)\.h>\s*$',
+                         re.M)
+
+# Integer types without wavefront shuffles / atomics (Global reductions refused).
+_SHUFFLE_T = {"int8_t": "int32_t", "uint8_t": "uint32_t", "int16_t": "int32_t",
+              "uint16_t": "uint32_t"}
+
+
+def compile_options():
+    return ["--offload-arch=gfx950", "-O3", "-std=c++17", "-munsafe-fp-atomics",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _strip_headers(text):
+    return _STD_HEADER.sub("", text)
+
+
+def generate_source(gk, map_index, block_flags=None):
+    """HIP source of the wrapper kernel ``op2_wrap`` for GlobalKernel ``gk``.
+
+    ``map_index[(arg, j)]`` is the position of an argument's j-th map in the
+    unique-map list; ``block_flags[arg]`` says whether a Mat argument is
+    stored as BSR (else scalar CSR over dofs).
+    """
+    block_flags = block_flags or {}
+    lk = gk.local_kernel
+    headers = "\n".join(_strip_headers(h) for h in lk.headers)
+    code = _strip_headers(lk.code)
+    params = ["int32_t lo", "int32_t hi", "const int32_t *__restrict__ list", "int32_t atomic"]
+    pre, call, post, red = [], [], [], []
+    nmaps = len(set(map_index.values()))
+    for k, (larg, garg) in enumerate(zip(lk.arguments, gk.arguments)):
+        acc = larg.access
+        t = as_cstr(np.dtype(larg.dtype))
+        if isinstance(garg, DatKernelArg):
+            dim = int(np.prod(garg.dim))
+            params.append(f"{t} *__restrict__ arg{k}")
+            if garg.map_ is None:
+                call.append(f"op2_ptr_{{(void *)(arg{k} + (int64_t)n * {dim})}}")
+                continue
+            m = map_index[(k, 0)]
+            ar = garg.map_.arity
+            pre.append(f"    int32_t mi{k}[{ar}];")
+            pre.append(f"    for (int i = 0; i < {ar}; ++i) mi{k}[i] = map{m}[(int64_t)n * {ar} + i];")
+            pre.append(f"    {t} buf{k}[{ar * dim}];")
+            if acc in (INC, WRITE):
+                pre.append(f"    for (int q = 0; q < {ar * dim}; ++q) buf{k}[q] = ({t})0;")
+            else:
+                pre.append(f"    for (int i = 0; i < {ar}; ++i) for (int j = 0; j < {dim}; ++j)"
+                           f" buf{k}[i * {dim} + j] = mi{k}[i] >= 0 ? arg{k}[(int64_t)mi{k}[i] * {dim} + j] : ({t})0;")
+            call.append(f"op2_ptr_{{(void *)buf{k}}}")
+            if acc == READ:
+                continue
+            tgt = f"arg{k}[(int64_t)mi{k}[i] * {dim} + j]"
+            val = f"buf{k}[i * {dim} + j]"
+            if acc == INC:
+                op = f"if (atomic) op2_atomic_add(&{tgt}, {val}); else {tgt} += {val};"
+            elif acc == MIN:
+                op = f"if (atomic) op2_atomic_min(&{tgt}, {val}); else {tgt} = op2_min({tgt}, {val});"
+            elif acc == MAX:
+                op = f"if (atomic) op2_atomic_max(&{tgt}, {val}); else {tgt} = op2_max({tgt}, {val});"
+            else:
+                op = f"{tgt} = {val};"
+            post.append(f"    for (int i = 0; i < {ar}; ++i) {{ if (mi{k}[i] < 0) continue;"
+                        f" for (int j = 0; j < {dim}; ++j) {{ {op} }} }}")
+        elif isinstance(garg, GlobalKernelArg):
+            dim = int(np.prod(garg.dim)) if len(garg.dim) else 1
+            params.append(f"{t} *__restrict__ arg{k}")
+            if acc == READ:
+                call.append(f"op2_ptr_{{(void *)arg{k}}}")
+                continue
+            # declared before the element guard so every lane takes part in the reduction
+            if acc == INC:
+                red.append(("decl", f"  {t} buf{k}[{dim}];\n  for (int q = 0; q < {dim}; ++q) buf{k}[q] = ({t})0;"))
+            else:
+                red.append(("decl", f"  {t} buf{k}[{dim}];\n  for (int q = 0; q < {dim}; ++q) buf{k}[q] = arg{k}[q];"))
+            call.append(f"op2_ptr_{{(void *)buf{k}}}")
+            fn = {INC: "sum", MIN: "min", MAX: "max"}.get(acc)
+            if fn is None:
+                raise NotImplementedError(f"Global access {acc!r} in a par_loop")
+            if t in _SHUFFLE_T:
+                raise NotImplementedError(f"Global reductions of {t} are not supported on the device")
+            aop = {INC: "op2_atomic_add", MIN: "op2_atomic_min", MAX: "op2_atomic_max"}[acc]
+            red.append(("post", f"  for (int q = 0; q < {dim}; ++q) {{ {t} v = op2_wave_{fn}(buf{k}[q]);"
+                                f" if (lane == 0) {aop}(&arg{k}[q], v); }}"))
+        elif isinstance(garg, MatKernelArg):
+            rd, cd = garg.dims[0][0]
+            params += [f"const int32_t *__restrict__ rowptr{k}", f"const int32_t *__restrict__ colidx{k}",
+                       f"{t} *__restrict__ arg{k}"]
+            rm, cm = map_index[(k, 0)], map_index[(k, 1)]
+            rar, car = garg.maps[0].arity, garg.maps[1].arity
+            ncol = car * cd
+            pre.append(f"    {t} buf{k}[{rar * rd * ncol}];")
+            pre.append(f"    for (int q = 0; q < {rar * rd * ncol}; ++q) buf{k}[q] = ({t})0;")
+            call.append(f"op2_ptr_{{(void *)buf{k}}}")
+            block = bool(block_flags.get(k, False))
+            if acc == INC:
+                store = "if (atomic) op2_atomic_add(&{tgt}, {val}); else {tgt} += {val};"
+            else:
+                store = "{tgt} = {val};"
+            if block:
+                tgt = f"arg{k}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj]"
+                val = f"buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]"
+                post.append(
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + i]; if (r_ < 0) continue;\n"
+                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + j]; if (c_ < 0) continue;\n"
+                    f"        const int32_t s_ = op2_find_slot(rowptr{k}, colidx{k}, r_, c_); if (s_ < 0) {{ atomicAdd(status, 1); continue; }}\n"
+                    f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj) {{ {store.format(tgt=tgt, val=val)} }} }} }}")
+            else:
+                tgt = f"arg{k}[s_]"
+                val = f"buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]"
+                post.append(
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + i]; if (r_ < 0) continue;\n"
+                    f"      for (int ri = 0; ri < {rd}; ++ri) for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + j]; if (c_ < 0) continue;\n"
+                    f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}, colidx{k}, r_ * {rd} + ri, c_ * {cd} + cj);"
+                    f" if (s_ < 0) {{ atomicAdd(status, 1); continue; }} {store.format(tgt=tgt, val=val)} }} }} }}")
+        else:
+            raise NotImplementedError("Passthrough arguments are not supported on the device")
+    for m in range(nmaps):
+        params.append(f"const int32_t *__restrict__ map{m}")
+    params.append("int32_t *__restrict__ status")
+    decls = "\n".join(s for kind, s in red if kind == "decl")
+    reds = "\n".join(s for kind, s in red if kind == "post")
+    src = [_PRELUDE,
+           "#pragma clang force_cuda_host_device begin",
+           headers, code,
+           "#pragma clang force_cuda_host_device end",
+           'extern "C" __global__ void __launch_bounds__(256)',
+           "op2_wrap(" + ", ".join(params) + ")",
+           "{",
+           "  const int32_t t_ = lo + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);",
+           "  const bool active = t_ < hi;",
+           "  const int lane = threadIdx.x & 63;",
+           "  (void)lane; (void)status;",
+           decls,
+           "  if (active) {",
+           "    const int32_t n = list ? list[t_] : t_;",
+           *pre,
+           f"    {lk.name}({', '.join(call)});",
+           *post,
+           "  }",
+           reds,
+           "}"]
+    return "\n".join(s for s in src if s is not None)
+
+
+_gen_lock = threading.Lock()
+_modules = {}
+
+
+def _code_object(src, opts):
+    """Code object bytes for ``src`` (memory, then disk cache, then hiprtc)."""
+    key = hashlib.sha256(("\0".join(opts) + "\0" + src).encode()).hexdigest()[:32]
+    cdir = os.path.join(configuration["cache_dir"], "hip")
+    path = os.path.join(cdir, key + ".co")
+    try:
+        with open(path, "rb") as f:
+            return key, f.read()
+    except OSError:
+        pass
+    L = dev.lib()
+    copts = (ctypes.c_char_p * len(opts))(*[o.encode() for o in opts])
+    size = ctypes.c_size_t(0)
+    log = ctypes.create_string_buffer(65536)
+    rc = L.op2hip_jit_code_object(src.encode(), copts, len(opts), None, ctypes.byref(size), log, len(log))
+    if rc != 0:
+        raise CompilationError("device compilation of the generated wrapper failed:\n"
+                               + log.value.decode(errors="replace") + "\n--- source ---\n" + src)
+    buf = ctypes.create_string_buffer(size.value)
+    rc = L.op2hip_jit_code_object(src.encode(), copts, len(opts), buf, ctypes.byref(size), log, len(log))
+    dev.check(rc, "jit_code_object")
+    data = buf.raw[:size.value]
+    try:
+        os.makedirs(cdir, exist_ok=True)
+        tmp = path + ".tmp%d" % os.getpid()
+        with open(tmp, "wb") as f:
+            f.write(data)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+    return key, data
+
+
+def _module(src):
+    opts = compile_options()
+    key, data = _code_object(src, opts)
+    with _gen_lock:
+        mod = _modules.get(key)
+        if mod is None:
+            dev.device()
+            L = dev.lib()
+            h = ctypes.c_void_p()
+            buf = ctypes.create_string_buffer(data, len(data))
+            dev.check(L.op2hip_jit_load(buf, ctypes.byref(h)), "jit_load")
+            mod = _modules[key] = h
+    return mod
+
+
+class GeneratedKernel:
+    """Executor of a generated wrapper (``(start, end, ptrs, parloop)``)."""
+
+    BLOCK = 256
+
+    def __init__(self, gk):
+        self.gk = gk
+        self.strategy_used = None
+        self._src = None
+        self._mod = None
+
+    def _unique_maps(self, parloop):
+        seen, index = [], {}
+        for k, a in enumerate(parloop.arguments):
+            for j, m in enumerate(a.maps):
+                for q, s in enumerate(seen):
+                    if s is m:
+                        index[(k, j)] = q
+                        break
+                else:
+                    index[(k, j)] = len(seen)
+                    seen.append(m)
+        return seen, index
+
+    def source(self, parloop):
+        if self._src is None:
+            _, index = self._unique_maps(parloop)
+            flags = {}
+            for k, (garg, a) in enumerate(zip(self.gk.arguments, parloop.arguments)):
+                if isinstance(garg, MatKernelArg):
+                    flags[k] = a.data._block
+            self._src = generate_source(self.gk, index, flags)
+        return self._src
+
+    def _conflict_maps(self, parloop):
+        """Maps whose targets must be exclusive within a launch."""
+        out = []
+        need = False
+        for larg, garg, a in zip(self.gk.local_kernel.arguments, self.gk.arguments, parloop.arguments):
+            if isinstance(garg, DatKernelArg) and garg.map_ is not None:
+                if larg.access in (RW, WRITE):
+                    need = True
+                if larg.access in (RW, WRITE, INC, MIN, MAX):
+                    out.append(a.maps[0])
+            elif isinstance(garg, MatKernelArg):
+                if larg.access == WRITE:
+                    need = True
+                out.append(a.maps[0])
+        return (out if need else []), need
+
+    def __call__(self, start, end, ptrs, parloop):
+        if parloop is None:
+            raise RuntimeError("generated loops are launched by a Parloop")
+        if self._mod is None:
+            self._mod = _module(self.source(parloop))
+        maps, _ = self._unique_maps(parloop)
+        conflict, need_colour = self._conflict_maps(parloop)
+        if configuration["strategy"] == "coloured" and not need_colour:
+            conflict, _ = [], False
+            for larg, garg, a in zip(self.gk.local_kernel.arguments, self.gk.arguments, parloop.arguments):
+                if (isinstance(garg, DatKernelArg) and garg.map_ is not None and larg.access != READ) \
+                        or isinstance(garg, MatKernelArg):
+                    conflict.append(a.maps[0])
+            need_colour = bool(conflict)
+        vals = []
+        mat_status = []
+        for garg, a in zip(self.gk.arguments, parloop.arguments):
+            if isinstance(garg, MatKernelArg):
+                mat = a.data
+                vals += [ctypes.c_void_p(dev.ptr(mat.rowptr)), ctypes.c_void_p(dev.ptr(mat.colidx)),
+                         ctypes.c_void_p(dev.ptr(mat.device_values))]
+                mat_status.append(mat._status)
+            else:
+                vals.append(ctypes.c_void_p(dev.ptr(a.data._device_array())))
+        vals += [ctypes.c_void_p(dev.ptr(m.device_values)) for m in maps]
+        status = mat_status[0] if mat_status else _dummy_status()
+        vals.append(ctypes.c_void_p(dev.ptr(status)))
+        iterset = parloop.iterset
+        L = dev.lib()
+        s = dev.stream()
+
+        def launch(lo, hi, lst, atomic):
+            allv = [ctypes.c_int32(lo), ctypes.c_int32(hi), ctypes.c_void_p(lst), ctypes.c_int32(atomic)] + vals
+            argv = (ctypes.c_void_p * len(allv))(*[ctypes.addressof(v) for v in allv])
+            grid = (hi - lo + self.BLOCK - 1) // self.BLOCK
+            dev.check(L.op2hip_jit_launch(self._mod, b"op2_wrap", grid, self.BLOCK, 0, argv, s),
+                      f"launch of generated {self.gk.name}")
+
+        if need_colour and conflict:
+            from pyop2_amd import plans
+            if isinstance(iterset, Subset):
+                elems = iterset.indices[start:end]
+            else:
+                elems = np.arange(start, end, dtype=np.int32)
+            key = (id(iterset), start, end, tuple(id(m) for m in conflict))
+            plan = plans.colour_plan(conflict[0], elems, [(m, id(m.toset)) for m in conflict], key)
+            ep = dev.ptr(plan.elems)
+            for c in range(plan.ncolours):
+                lo, hi = int(plan.offsets[c]), int(plan.offsets[c + 1])
+                if hi > lo:
+                    launch(lo, hi, ep, 0)
+            self.strategy_used = "coloured"
+        else:
+            lst = dev.ptr(iterset.device_indices) if isinstance(iterset, Subset) else 0
+            launch(start, end, lst, 1)
+            self.strategy_used = "atomic"
+
+
+_STATUS = None
+
+
+def _dummy_status():
+    global _STATUS
+    if _STATUS is None:
+        _STATUS = dev.zeros((1,), np.int32)
+    return _STATUS

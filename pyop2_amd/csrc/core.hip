@@ -1,0 +1,609 @@
+// Library core: errors, device init, prefix sums, map transpose, device
+// sparsity builder, pattern expansion, greedy colouring and row batching.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace op2hip {
+
+static thread_local char g_error[1024];
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_error, sizeof(g_error), fmt, ap);
+  va_end(ap);
+}
+
+// ---------------------------------------------------------------------------
+// small utility kernels
+
+__global__ void fill_i32(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static int fill(int32_t *p, int64_t n, int32_t v, hipStream_t s) {
+  if (n <= 0) return OP2HIP_OK;
+  const uint32_t grid = std::min<uint32_t>(ceil_div(n, 256), 16384);
+  fill_i32<<<grid, 256, 0, s>>>(p, n, v);
+  OP2HIP_LAUNCHED();
+  return OP2HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan: tiles of kScanTile items, one 256-thread block per tile.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int64_t block_inclusive_scan(int64_t v, int64_t *lds) {
+  const int t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int64_t add = t >= off ? lds[t - off] : 0;
+    __syncthreads();
+    lds[t] += add;
+    __syncthreads();
+  }
+  return lds[t];
+}
+
+// out[i+1] = tile-local inclusive prefix; tile_sums[b] = tile total.
+__global__ void scan_tiles(int64_t n, const int32_t *__restrict__ in,
+                           int32_t *__restrict__ out, int64_t *__restrict__ tile_sums) {
+  __shared__ int64_t lds[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int64_t v[kScanItems];
+  int64_t run = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    run += (i < n) ? in[i] : 0;
+    v[k] = run;
+  }
+  const int64_t incl = block_inclusive_scan(run, lds);
+  const int64_t excl = incl - run;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (i < n) out[i + 1] = static_cast<int32_t>(excl + v[k]);
+  }
+  if (threadIdx.x == kScanThreads - 1) tile_sums[blockIdx.x] = incl;
+}
+
+// Single block: exclusive scan of tile_sums in place, total to *total.
+__global__ void scan_tile_sums(int64_t ntiles, int64_t *__restrict__ tile_sums,
+                               int64_t *__restrict__ total) {
+  __shared__ int64_t lds[kScanThreads];
+  int64_t carry = 0;
+  for (int64_t b = 0; b < ntiles; b += kScanThreads) {
+    const int64_t i = b + threadIdx.x;
+    const int64_t v = i < ntiles ? tile_sums[i] : 0;
+    const int64_t incl = block_inclusive_scan(v, lds);
+    if (i < ntiles) tile_sums[i] = carry + incl - v;
+    __syncthreads();
+    carry += lds[kScanThreads - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void scan_add_offsets(int64_t n, int32_t *__restrict__ out,
+                                 const int64_t *__restrict__ tile_sums) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  const int64_t off = tile_sums[blockIdx.x];
+  for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+    const int64_t i = base + k;
+    if (i < n) out[i + 1] = static_cast<int32_t>(out[i + 1] + off);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+}
+
+static size_t scan_bytes(int64_t n) {
+  const int64_t ntiles = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
+  return (size_t)(ntiles + 2) * sizeof(int64_t);
+}
+
+static int scan(int64_t n, const int32_t *in, int32_t *out, int64_t *total_host,
+                void *work, size_t work_bytes, hipStream_t s) {
+  OP2HIP_REQUIRE(n >= 0, "scan: negative length");
+  if (work_bytes < scan_bytes(n)) {
+    set_error("scan: workspace %zu < %zu bytes", work_bytes, scan_bytes(n));
+    return OP2HIP_ERR_WORKSPACE;
+  }
+  const int64_t ntiles = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
+  int64_t *tile_sums = static_cast<int64_t *>(work);
+  int64_t *total = tile_sums + ntiles;
+  scan_tiles<<<(uint32_t)ntiles, kScanThreads, 0, s>>>(n, in, out, tile_sums);
+  OP2HIP_LAUNCHED();
+  scan_tile_sums<<<1, kScanThreads, 0, s>>>(ntiles, tile_sums, total);
+  OP2HIP_LAUNCHED();
+  scan_add_offsets<<<(uint32_t)ntiles, kScanThreads, 0, s>>>(n, out, tile_sums);
+  OP2HIP_LAUNCHED();
+  int64_t t = 0;
+  OP2HIP_CHECK(hipMemcpyAsync(&t, total, sizeof(t), hipMemcpyDeviceToHost, s));
+  OP2HIP_CHECK(hipStreamSynchronize(s));
+  if (total_host) *total_host = t;
+  if (t > INT32_MAX) {
+    set_error("scan: total %lld exceeds int32", (long long)t);
+    return OP2HIP_ERR_OVERFLOW;
+  }
+  return OP2HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// map transpose
+
+__global__ void transpose_count(int32_t nrows, const int32_t *__restrict__ map,
+                                int64_t nent, int32_t *__restrict__ counts,
+                                int32_t *__restrict__ bad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nent;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = map[i];
+    if (r < 0) continue;
+    if (r >= nrows) { atomicAdd(bad, 1); continue; }
+    atomicAdd(&counts[r], 1);
+  }
+}
+
+__global__ void transpose_fill(int32_t nrows, const int32_t *__restrict__ map,
+                               int64_t nent, int32_t *__restrict__ cursor,
+                               int32_t *__restrict__ inc_ent) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nent;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = map[i];
+    if (r < 0 || r >= nrows) continue;
+    const int32_t pos = atomicAdd(&cursor[r], 1);
+    inc_ent[pos] = static_cast<int32_t>(i);
+  }
+}
+
+// Sort each row's incidence list ascending (lists are short: one thread each).
+__global__ void transpose_sort(int32_t nrows, const int32_t *__restrict__ inc_ptr,
+                               int32_t *__restrict__ inc_ent) {
+  for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += gridDim.x * blockDim.x) {
+    const int32_t lo = inc_ptr[r], hi = inc_ptr[r + 1];
+    for (int32_t a = lo + 1; a < hi; ++a) {
+      const int32_t v = inc_ent[a];
+      int32_t b = a - 1;
+      while (b >= lo && inc_ent[b] > v) {
+        inc_ent[b + 1] = inc_ent[b];
+        --b;
+      }
+      inc_ent[b + 1] = v;
+    }
+  }
+}
+
+static size_t transpose_bytes(int32_t nrows) {
+  // counts/cursor (nrows int32) + flag + scan workspace, 256-byte aligned parts
+  size_t a = ((size_t)std::max(nrows, 1) * 4 + 255) & ~(size_t)255;
+  return a + 256 + scan_bytes(nrows);
+}
+
+// ---------------------------------------------------------------------------
+// sparsity rows: one 64-thread block per block row
+
+struct PairTable {
+  const int32_t *cmap[OP2HIP_MAX_PAIRS];
+  const int32_t *inc_ptr[OP2HIP_MAX_PAIRS];
+  const int32_t *inc_ent[OP2HIP_MAX_PAIRS];
+  int32_t rarity[OP2HIP_MAX_PAIRS];
+  int32_t carity[OP2HIP_MAX_PAIRS];
+  int32_t npairs;
+};
+
+constexpr int kRowCap = 2048;
+
+__device__ __forceinline__ int32_t candidate(const PairTable &pt, int32_t r, int64_t k,
+                                             const int32_t *lo, const int32_t *cnt,
+                                             int32_t diag_col) {
+  for (int p = 0; p < pt.npairs; ++p) {
+    const int64_t n = (int64_t)cnt[p] * pt.carity[p];
+    if (k < n) {
+      const int32_t inc = lo[p] + (int32_t)(k / pt.carity[p]);
+      const int32_t j = (int32_t)(k % pt.carity[p]);
+      const int32_t e = pt.inc_ent[p][inc] / pt.rarity[p];
+      return pt.cmap[p][(int64_t)e * pt.carity[p] + j];
+    }
+    k -= n;
+  }
+  return diag_col;  // the diagonal candidate (or -1)
+}
+
+__device__ __forceinline__ int32_t block64_sum(int32_t v, int32_t *lds) {
+  // 64-thread block reduction through LDS
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  const int32_t s = lds[0];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ int32_t block64_min(int32_t v, int32_t *lds) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] = min(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  const int32_t s = lds[0];
+  __syncthreads();
+  return s;
+}
+
+template <bool kFill>
+__global__ void __launch_bounds__(64)
+sparsity_rows(int32_t nrows, int32_t ncols, PairTable pt, int32_t set_diag,
+              const int32_t *__restrict__ rowptr, int32_t *__restrict__ out) {
+  __shared__ int32_t cand[kRowCap];
+  __shared__ uint8_t first[kRowCap];
+  __shared__ int32_t red[64];
+  __shared__ int32_t lo[OP2HIP_MAX_PAIRS], cnt[OP2HIP_MAX_PAIRS];
+  for (int32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    if (threadIdx.x < (unsigned)pt.npairs) {
+      const int p = threadIdx.x;
+      lo[p] = pt.inc_ptr[p][r];
+      cnt[p] = pt.inc_ptr[p][r + 1] - lo[p];
+    }
+    __syncthreads();
+    int64_t m = 0;
+    for (int p = 0; p < pt.npairs; ++p) m += (int64_t)cnt[p] * pt.carity[p];
+    const bool has_diag = set_diag && r < ncols;
+    const int32_t diag_col = has_diag ? r : -1;
+    m += has_diag ? 1 : 0;
+    const int32_t base = kFill ? rowptr[r] : 0;
+    if (m <= kRowCap) {
+      for (int64_t k = threadIdx.x; k < m; k += 64)
+        cand[k] = candidate(pt, r, k, lo, cnt, diag_col);
+      __syncthreads();
+      int32_t mine = 0;
+      for (int64_t k = threadIdx.x; k < m; k += 64) {
+        const int32_t c = cand[k];
+        bool f = c >= 0;
+        for (int64_t t = 0; f && t < k; ++t) f = cand[t] != c;
+        first[k] = f;
+        mine += f;
+      }
+      __syncthreads();
+      if (!kFill) {
+        const int32_t total = block64_sum(mine, red);
+        if (threadIdx.x == 0) out[r] = total;
+      } else {
+        for (int64_t k = threadIdx.x; k < m; k += 64) {
+          if (!first[k]) continue;
+          const int32_t c = cand[k];
+          int32_t rank = 0;
+          for (int64_t t = 0; t < m; ++t) rank += (first[t] && cand[t] < c);
+          out[base + rank] = c;
+        }
+      }
+      __syncthreads();
+    } else {
+      // long row: repeated minimum extraction straight from the maps
+      int32_t last = -1, count = 0;
+      for (;;) {
+        int32_t best = INT32_MAX;
+        for (int64_t k = threadIdx.x; k < m; k += 64) {
+          const int32_t c = candidate(pt, r, k, lo, cnt, diag_col);
+          if (c > last && c < best) best = c;
+        }
+        best = block64_min(best, red);
+        if (best == INT32_MAX) break;
+        if (kFill && threadIdx.x == 0) out[base + count] = best;
+        ++count;
+        last = best;
+      }
+      if (!kFill && threadIdx.x == 0) out[r] = count;
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void expand_rowptr(int32_t nrows, const int32_t *__restrict__ rowptr,
+                              int32_t rdim, int32_t cdim, int32_t *__restrict__ out) {
+  for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r <= nrows;
+       r += gridDim.x * blockDim.x) {
+    if (r == nrows) {
+      out[(int64_t)nrows * rdim] = rowptr[nrows] * rdim * cdim;
+      continue;
+    }
+    const int32_t len = rowptr[r + 1] - rowptr[r];
+    for (int ri = 0; ri < rdim; ++ri)
+      out[(int64_t)r * rdim + ri] = rowptr[r] * rdim * cdim + ri * len * cdim;
+  }
+}
+
+__global__ void expand_colidx(int32_t nrows, const int32_t *__restrict__ rowptr,
+                              const int32_t *__restrict__ colidx, int32_t rdim,
+                              int32_t cdim, const int32_t *__restrict__ rowptr_out,
+                              int32_t *__restrict__ out) {
+  for (int32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += gridDim.x * blockDim.x) {
+    const int32_t lo = rowptr[r], len = rowptr[r + 1] - lo;
+    for (int ri = 0; ri < rdim; ++ri) {
+      int32_t *o = out + rowptr_out[(int64_t)r * rdim + ri];
+      for (int32_t s = 0; s < len; ++s)
+        for (int cj = 0; cj < cdim; ++cj) o[s * cdim + cj] = colidx[lo + s] * cdim + cj;
+    }
+  }
+}
+
+static int make_pairs(const op2hip_map_pair *pairs, int32_t npairs,
+                      const int32_t *const *inc_ptr, const int32_t *const *inc_ent,
+                      PairTable *pt) {
+  OP2HIP_REQUIRE(npairs >= 0 && npairs <= OP2HIP_MAX_PAIRS,
+                 "sparsity: %d map pairs (max %d)", npairs, OP2HIP_MAX_PAIRS);
+  std::memset(pt, 0, sizeof(*pt));
+  pt->npairs = npairs;
+  for (int p = 0; p < npairs; ++p) {
+    OP2HIP_REQUIRE(pairs[p].rarity > 0 && pairs[p].carity > 0, "sparsity: bad arity");
+    OP2HIP_REQUIRE(inc_ptr && inc_ent && inc_ptr[p] && inc_ent[p],
+                   "sparsity: missing row incidences of pair %d", p);
+    pt->cmap[p] = pairs[p].cmap;
+    pt->inc_ptr[p] = inc_ptr[p];
+    pt->inc_ent[p] = inc_ent[p];
+    pt->rarity[p] = pairs[p].rarity;
+    pt->carity[p] = pairs[p].carity;
+  }
+  return OP2HIP_OK;
+}
+
+}  // namespace op2hip
+
+using namespace op2hip;
+
+extern "C" {
+
+int op2hip_abi_version(void) { return OP2HIP_ABI_VERSION; }
+
+const char *op2hip_last_error(void) { return g_error; }
+
+int op2hip_init(int32_t device) {
+  int n = 0;
+  OP2HIP_CHECK(hipGetDeviceCount(&n));
+  OP2HIP_REQUIRE(device >= 0 && device < n, "device %d not present (%d devices)", device, n);
+  OP2HIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  OP2HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error("device %d is %s; this library is built for gfx950 only", device,
+              prop.gcnArchName);
+    return OP2HIP_ERR_ARG;
+  }
+  return OP2HIP_OK;
+}
+
+int op2hip_scan_workspace(int64_t n, size_t *bytes) {
+  OP2HIP_REQUIRE(bytes, "scan_workspace: NULL");
+  *bytes = scan_bytes(n);
+  return OP2HIP_OK;
+}
+
+int op2hip_exclusive_scan(int64_t n, const int32_t *in, int32_t *out, int64_t *total_host,
+                          void *work, size_t work_bytes, void *stream) {
+  return scan(n, in, out, total_host, work, work_bytes, stream_of(stream));
+}
+
+int op2hip_transpose_workspace(int32_t nrows, size_t *bytes) {
+  OP2HIP_REQUIRE(bytes && nrows >= 0, "transpose_workspace: bad arguments");
+  *bytes = transpose_bytes(nrows);
+  return OP2HIP_OK;
+}
+
+int op2hip_transpose_count(int32_t nrows, const int32_t *map, int32_t arity,
+                           int32_t iter_size, int32_t *inc_ptr, int64_t *total_host,
+                           void *work, size_t work_bytes, void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nrows >= 0 && arity > 0 && iter_size >= 0 && inc_ptr,
+                 "transpose_count: bad arguments");
+  OP2HIP_REQUIRE(iter_size == 0 || map, "transpose_count: NULL map");
+  if (work_bytes < transpose_bytes(nrows)) {
+    set_error("transpose: workspace %zu < %zu", work_bytes, transpose_bytes(nrows));
+    return OP2HIP_ERR_WORKSPACE;
+  }
+  char *w = static_cast<char *>(work);
+  const size_t a = ((size_t)std::max(nrows, 1) * 4 + 255) & ~(size_t)255;
+  int32_t *counts = reinterpret_cast<int32_t *>(w);
+  int32_t *bad = reinterpret_cast<int32_t *>(w + a);
+  void *scan_work = w + a + 256;
+  int rc;
+  if ((rc = fill(counts, nrows, 0, s))) return rc;
+  if ((rc = fill(bad, 1, 0, s))) return rc;
+  const int64_t nent = (int64_t)iter_size * arity;
+  if (nent > 0) {
+    transpose_count<<<std::min<uint32_t>(ceil_div(nent, 256), 16384), 256, 0, s>>>(
+        nrows, map, nent, counts, bad);
+    OP2HIP_LAUNCHED();
+  }
+  if ((rc = scan(nrows, counts, inc_ptr, total_host, scan_work, work_bytes - a - 256, s)))
+    return rc;
+  int32_t nbad = 0;
+  OP2HIP_CHECK(hipMemcpyAsync(&nbad, bad, sizeof(nbad), hipMemcpyDeviceToHost, s));
+  OP2HIP_CHECK(hipStreamSynchronize(s));
+  if (nbad) {
+    set_error("transpose: %d map entries >= %d (target set size)", nbad, nrows);
+    return OP2HIP_ERR_ARG;
+  }
+  return OP2HIP_OK;
+}
+
+int op2hip_transpose_fill(int32_t nrows, const int32_t *map, int32_t arity,
+                          int32_t iter_size, const int32_t *inc_ptr, int32_t *inc_ent,
+                          void *work, size_t work_bytes, void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nrows >= 0 && arity > 0 && iter_size >= 0 && inc_ptr,
+                 "transpose_fill: bad arguments");
+  if (work_bytes < transpose_bytes(nrows)) {
+    set_error("transpose: workspace %zu < %zu", work_bytes, transpose_bytes(nrows));
+    return OP2HIP_ERR_WORKSPACE;
+  }
+  int32_t *cursor = static_cast<int32_t *>(work);
+  if (nrows > 0)
+    OP2HIP_CHECK(hipMemcpyAsync(cursor, inc_ptr, sizeof(int32_t) * nrows,
+                                hipMemcpyDeviceToDevice, s));
+  const int64_t nent = (int64_t)iter_size * arity;
+  if (nent > 0) {
+    transpose_fill<<<std::min<uint32_t>(ceil_div(nent, 256), 16384), 256, 0, s>>>(
+        nrows, map, nent, cursor, inc_ent);
+    OP2HIP_LAUNCHED();
+  }
+  if (nrows > 0) {
+    transpose_sort<<<std::min<uint32_t>(ceil_div(nrows, 256), 16384), 256, 0, s>>>(
+        nrows, inc_ptr, inc_ent);
+    OP2HIP_LAUNCHED();
+  }
+  return OP2HIP_OK;
+}
+
+int op2hip_sparsity_count(int32_t nrows, int32_t ncols, const op2hip_map_pair *pairs,
+                          int32_t npairs, const int32_t *const *inc_ptr,
+                          const int32_t *const *inc_ent, int32_t set_diag,
+                          int32_t *rowptr, int64_t *nnz_host, void *work,
+                          size_t work_bytes, void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nrows >= 0 && ncols >= 0 && rowptr, "sparsity_count: bad arguments");
+  PairTable pt;
+  int rc;
+  if ((rc = make_pairs(pairs, npairs, inc_ptr, inc_ent, &pt))) return rc;
+  const size_t a = ((size_t)std::max(nrows, 1) * 4 + 255) & ~(size_t)255;
+  if (work_bytes < a + scan_bytes(nrows)) {
+    set_error("sparsity: workspace %zu < %zu", work_bytes, a + scan_bytes(nrows));
+    return OP2HIP_ERR_WORKSPACE;
+  }
+  int32_t *counts = static_cast<int32_t *>(work);
+  if (nrows > 0) {
+    sparsity_rows<false><<<std::min<int32_t>(nrows, 65536), 64, 0, s>>>(
+        nrows, ncols, pt, set_diag, nullptr, counts);
+    OP2HIP_LAUNCHED();
+  }
+  return scan(nrows, counts, rowptr, nnz_host, static_cast<char *>(work) + a,
+              work_bytes - a, s);
+}
+
+int op2hip_sparsity_fill(int32_t nrows, int32_t ncols, const op2hip_map_pair *pairs,
+                         int32_t npairs, const int32_t *const *inc_ptr,
+                         const int32_t *const *inc_ent, int32_t set_diag,
+                         const int32_t *rowptr, int32_t *colidx, void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nrows >= 0 && ncols >= 0 && rowptr, "sparsity_fill: bad arguments");
+  PairTable pt;
+  int rc;
+  if ((rc = make_pairs(pairs, npairs, inc_ptr, inc_ent, &pt))) return rc;
+  if (nrows > 0) {
+    sparsity_rows<true><<<std::min<int32_t>(nrows, 65536), 64, 0, s>>>(
+        nrows, ncols, pt, set_diag, rowptr, colidx);
+    OP2HIP_LAUNCHED();
+  }
+  return OP2HIP_OK;
+}
+
+int op2hip_pattern_expand(int32_t nrows, const int32_t *rowptr, const int32_t *colidx,
+                          int32_t rdim, int32_t cdim, int32_t *rowptr_out,
+                          int32_t *colidx_out, void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nrows >= 0 && rdim > 0 && cdim > 0, "pattern_expand: bad arguments");
+  expand_rowptr<<<std::min<uint32_t>(ceil_div((int64_t)nrows + 1, 256), 16384), 256, 0, s>>>(
+      nrows, rowptr, rdim, cdim, rowptr_out);
+  OP2HIP_LAUNCHED();
+  if (nrows > 0) {
+    expand_colidx<<<std::min<uint32_t>(ceil_div(nrows, 256), 16384), 256, 0, s>>>(
+        nrows, rowptr, colidx, rdim, cdim, rowptr_out, colidx_out);
+    OP2HIP_LAUNCHED();
+  }
+  return OP2HIP_OK;
+}
+
+int op2hip_colour_greedy(int32_t nelem, int32_t nmaps, const int32_t *const *maps,
+                         const int32_t *arity, const int32_t *target_id,
+                         const int32_t *target_size, int32_t ntargets,
+                         int32_t *colour, int32_t *ncolours) {
+  OP2HIP_REQUIRE(nelem >= 0 && nmaps >= 0 && ntargets >= 0 && colour && ncolours,
+                 "colour_greedy: bad arguments");
+  for (int m = 0; m < nmaps; ++m)
+    OP2HIP_REQUIRE(target_id[m] >= 0 && target_id[m] < ntargets,
+                   "colour_greedy: map %d has target %d", m, target_id[m]);
+  std::vector<std::vector<uint32_t>> masks(ntargets);
+  for (int t = 0; t < ntargets; ++t) masks[t].assign(std::max(target_size[t], 1), 0u);
+  std::fill(colour, colour + nelem, -1);
+  int32_t left = nelem, base = 0, highest = 0;
+  while (left > 0) {
+    for (auto &m : masks) std::fill(m.begin(), m.end(), 0u);
+    for (int32_t e = 0; e < nelem; ++e) {
+      if (colour[e] >= 0) continue;
+      uint32_t used = 0;
+      for (int m = 0; m < nmaps; ++m) {
+        const int32_t *row = maps[m] + (int64_t)e * arity[m];
+        for (int i = 0; i < arity[m]; ++i)
+          if (row[i] >= 0) {
+            if (row[i] >= target_size[target_id[m]]) {
+              set_error("colour_greedy: map %d entry %d out of range", m, row[i]);
+              return OP2HIP_ERR_ARG;
+            }
+            used |= masks[target_id[m]][row[i]];
+          }
+      }
+      if (used == 0xFFFFFFFFu) continue;  // try again in the next pass
+      const int c = __builtin_ctz(~used);
+      colour[e] = base + c;
+      highest = std::max(highest, base + c + 1);
+      --left;
+      for (int m = 0; m < nmaps; ++m) {
+        const int32_t *row = maps[m] + (int64_t)e * arity[m];
+        for (int i = 0; i < arity[m]; ++i)
+          if (row[i] >= 0) masks[target_id[m]][row[i]] |= (1u << c);
+      }
+    }
+    base += 32;
+  }
+  *ncolours = highest;
+  return OP2HIP_OK;
+}
+
+int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr, const int32_t *rowptr,
+                            int32_t max_inc, int32_t max_nnz, int32_t *batch_rows,
+                            int32_t *nbatches, int32_t *max_batch_inc,
+                            int32_t *max_batch_nnz, int32_t *max_batch_rows) {
+  OP2HIP_REQUIRE(nrows >= 0 && inc_ptr && rowptr && batch_rows && nbatches &&
+                     max_inc > 0 && max_nnz > 0,
+                 "plan_row_batches: bad arguments");
+  int32_t nb = 0, mi = 0, mn = 0, mr = 0;
+  int32_t r = 0;
+  batch_rows[0] = 0;
+  while (r < nrows) {
+    const int32_t r0 = r;
+    int32_t inc = 0, nnz = 0;
+    while (r < nrows) {
+      const int32_t di = inc_ptr[r + 1] - inc_ptr[r];
+      const int32_t dn = rowptr[r + 1] - rowptr[r];
+      if (r > r0 && (inc + di > max_inc || nnz + dn > max_nnz)) break;
+      inc += di;
+      nnz += dn;
+      ++r;
+    }
+    batch_rows[++nb] = r;
+    mi = std::max(mi, inc);
+    mn = std::max(mn, nnz);
+    mr = std::max(mr, r - r0);
+  }
+  *nbatches = nb;
+  if (max_batch_inc) *max_batch_inc = mi;
+  if (max_batch_nnz) *max_batch_nnz = mn;
+  if (max_batch_rows) *max_batch_rows = mr;
+  return OP2HIP_OK;
+}
+
+}  // extern "C"

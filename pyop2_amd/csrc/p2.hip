@@ -1,0 +1,526 @@
+// P2 tetrahedron, isotropic linear elasticity: residual, BSR(3) element-matrix
+// assembly, fused residual + assembly, and the adjoint of the residual loop.
+//
+// Geometry: J[r][c] = X[c+1][r] - X[0][r], K = J^{-1} (K[m][k] = d xi_m/d x_k),
+// barycentric gradients gl[0] = -(K[0]+K[1]+K[2]), gl[m+1] = K[m].
+// Basis gradients at quadrature point q: grad phi_i = sum_m D[q][i][m] gl[m]
+// with D the derivative of the P2 basis in barycentric coordinates
+// (vertex i: (4 L_i - 1) at m = i; edge (a, b): 4 L_b at m = a, 4 L_a at m = b).
+// 4-point Gauss rule, weight 1/24, so wdet = |det J| / 24.
+//
+// Residual: R_i += wdet sum_q sigma(u)_q grad phi_i, evaluated through the
+// barycentric form  grad u_q = sum_m w_qm (x) gl_m,  w_qm = sum_j D[q][j][m] u_j,
+// and  R_i += wdet sum_q sum_m D[q][i][m] sigma_q gl_m.
+// Element matrix block (i, j): with M = sum_q grad phi_i (x) grad phi_j,
+// B = wdet (lam M + mu M^T + mu tr(M) I).
+#include "rowgather.hpp"
+
+namespace op2hip {
+namespace p2 {
+
+constexpr double kQa = 0.5854101966249685;
+constexpr double kQb = 0.1381966011250105;
+constexpr double kQw = 1.0 / 24.0;
+constexpr int kEa[6] = {0, 0, 0, 1, 1, 2};
+constexpr int kEb[6] = {1, 2, 3, 2, 3, 3};
+
+__host__ __device__ constexpr double bary(int q, int m) { return q == m ? kQa : kQb; }
+
+// d phi_i / d lambda_m at quadrature point q.
+__host__ __device__ constexpr double dcoef(int q, int i, int m) {
+  return i < 4 ? (m == i ? 4.0 * bary(q, i) - 1.0 : 0.0)
+               : (m == kEa[i - 4] ? 4.0 * bary(q, kEb[i - 4])
+                                  : (m == kEb[i - 4] ? 4.0 * bary(q, kEa[i - 4]) : 0.0));
+}
+
+struct DTable {
+  double v[4][10][4];
+  constexpr DTable() : v{} {
+    for (int q = 0; q < 4; ++q)
+      for (int i = 0; i < 10; ++i)
+        for (int m = 0; m < 4; ++m) v[q][i][m] = dcoef(q, i, m);
+  }
+};
+__constant__ DTable kD = DTable();
+
+template <typename T>
+struct Geom {
+  T gl[4][3];
+  T K[3][3];
+  T wdet;
+};
+
+template <typename T>
+__device__ __forceinline__ Geom<T> geometry(const T *__restrict__ X,
+                                            const int32_t *__restrict__ c2v, int32_t e) {
+  const int32_t v0 = c2v[4 * e], v1 = c2v[4 * e + 1], v2 = c2v[4 * e + 2], v3 = c2v[4 * e + 3];
+  const T x0[3] = {X[3 * v0], X[3 * v0 + 1], X[3 * v0 + 2]};
+  T J[3][3];
+  const int32_t vv[3] = {v1, v2, v3};
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) J[r][c] = X[3 * vv[c] + r] - x0[r];
+  const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const T det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const T idet = T(1) / det;
+  Geom<T> g;
+  g.K[0][0] = c00 * idet;
+  g.K[1][0] = c01 * idet;
+  g.K[2][0] = c02 * idet;
+  g.K[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * idet;
+  g.K[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * idet;
+  g.K[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * idet;
+  g.K[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * idet;
+  g.K[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * idet;
+  g.K[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * idet;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    g.gl[0][k] = -(g.K[0][k] + g.K[1][k] + g.K[2][k]);
+    g.gl[1][k] = g.K[0][k];
+    g.gl[2][k] = g.K[1][k];
+    g.gl[3][k] = g.K[2][k];
+  }
+  g.wdet = T(kQw) * fabs(det);
+  return g;
+}
+
+// Barycentric-form gradient of a P2 field at quadrature point q:
+// grad[b][k] = sum_m (sum_j D[q][j][m] f[j][b]) gl[m][k].
+template <typename T, int Q>
+__device__ __forceinline__ void field_grad(const T (&f)[10][3], const Geom<T> &g, T (&grad)[3][3]) {
+  T w[4][3];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < 10; ++j)
+        if (dcoef(Q, j, m) != 0.0) acc += T(dcoef(Q, j, m)) * f[j][b];
+      w[m][b] = acc;
+    }
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T acc = T(0);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc += w[m][b] * g.gl[m][k];
+      grad[b][k] = acc;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void stress(const T (&gu)[3][3], T lam, T mu, T (&s)[3][3]) {
+  const T dv = gu[0][0] + gu[1][1] + gu[2][2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[a][k] = mu * (gu[a][k] + gu[k][a]) + (a == k ? lam * dv : T(0));
+}
+
+// out[i][a] += sum_m D[q][i][m] (s gl_m)[a]
+template <typename T, int Q>
+__device__ __forceinline__ void test_gradients(const T (&s)[3][3], const Geom<T> &g, T (&out)[10][3]) {
+  T sm[4][3];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      sm[m][a] = s[a][0] * g.gl[m][0] + s[a][1] * g.gl[m][1] + s[a][2] * g.gl[m][2];
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      T acc = out[i][a];
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (dcoef(Q, i, m) != 0.0) acc += T(dcoef(Q, i, m)) * sm[m][a];
+      out[i][a] = acc;
+    }
+}
+
+template <typename T, int Q>
+__device__ __forceinline__ void residual_q(const T (&uu)[10][3], const Geom<T> &g, T lam, T mu,
+                                           T (&re)[10][3]) {
+  T gu[3][3], s[3][3];
+  field_grad<T, Q>(uu, g, gu);
+  stress(gu, lam, mu, s);
+  test_gradients<T, Q>(s, g, re);
+}
+
+template <typename T>
+__device__ __forceinline__ void element_residual(const T (&uu)[10][3], const Geom<T> &g, T lam,
+                                                 T mu, T (&re)[10][3]) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) re[i][a] = T(0);
+  residual_q<T, 0>(uu, g, lam, mu, re);
+  residual_q<T, 1>(uu, g, lam, mu, re);
+  residual_q<T, 2>(uu, g, lam, mu, re);
+  residual_q<T, 3>(uu, g, lam, mu, re);
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) re[i][a] *= g.wdet;
+}
+
+template <typename T, bool kAtomic>
+__device__ __forceinline__ void add3(T *p, const T (&v)[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (kAtomic) atomic_add(p + a, v[a]); else plain_add(p + a, v[a]);
+  }
+}
+
+template <typename T, bool kAtomic>
+__global__ void __launch_bounds__(256)
+residual_kernel(int32_t lo, int32_t hi, const int32_t *__restrict__ list, T *__restrict__ R,
+                const T *__restrict__ X, const T *__restrict__ u,
+                const int32_t *__restrict__ c2n, const int32_t *__restrict__ c2v, T lam, T mu) {
+  const int32_t t = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hi) return;
+  const int32_t e = element_of(t, list);
+  int32_t nd[10];
+#pragma unroll
+  for (int j = 0; j < 10; ++j) nd[j] = c2n[10 * e + j];
+  const Geom<T> g = geometry(X, c2v, e);
+  T uu[10][3];
+#pragma unroll
+  for (int j = 0; j < 10; ++j)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) uu[j][b] = u[3 * nd[j] + b];
+  T re[10][3];
+  element_residual(uu, g, lam, mu, re);
+#pragma unroll
+  for (int i = 0; i < 10; ++i) add3<T, kAtomic>(R + 3 * nd[i], re[i]);
+}
+
+// Gradient of basis i at every quadrature point (runtime i, table lookup).
+template <typename T>
+__device__ __forceinline__ void basis_grads(const Geom<T> &g, int i, T (&gi)[4][3]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T d0 = T(kD.v[q][i][0]), d1 = T(kD.v[q][i][1]), d2 = T(kD.v[q][i][2]),
+            d3 = T(kD.v[q][i][3]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      gi[q][k] = d0 * g.gl[0][k] + d1 * g.gl[1][k] + d2 * g.gl[2][k] + d3 * g.gl[3][k];
+  }
+}
+
+// Block (i, j) of the element matrix from the quadrature-point gradients of i.
+template <typename T, int J>
+__device__ __forceinline__ void matrix_block(const T (&gi)[4][3], const Geom<T> &g, T lam, T mu,
+                                             T (&B)[9]) {
+  T M[3][3] = {};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    T gj[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T acc = T(0);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (dcoef(q, J, m) != 0.0) acc += T(dcoef(q, J, m)) * g.gl[m][k];
+      gj[k] = acc;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) M[a][b] += gi[q][a] * gj[b];
+  }
+  const T tr = mu * (M[0][0] + M[1][1] + M[2][2]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      B[3 * a + b] = g.wdet * (lam * M[a][b] + mu * M[b][a] + (a == b ? tr : T(0)));
+}
+
+template <typename T, int J, class F>
+__device__ __forceinline__ void for_blocks(const T (&gi)[4][3], const Geom<T> &g, T lam, T mu,
+                                           F &&f) {
+  T B[9];
+  matrix_block<T, J>(gi, g, lam, mu, B);
+  f(J, B);
+  if constexpr (J + 1 < 10) for_blocks<T, J + 1>(gi, g, lam, mu, f);
+}
+
+// Cell-centric BSR(3) assembly: block (i, j) -> slot by binary search of row
+// c2n[i]'s colidx, 9 atomic (or coloured plain) adds.
+template <typename T, bool kAtomic, bool kFused>
+__global__ void __launch_bounds__(256)
+matrix_kernel(int32_t lo, int32_t hi, const int32_t *__restrict__ list, op2hip_mat A,
+              T *__restrict__ R, const T *__restrict__ X, const T *__restrict__ u,
+              const int32_t *__restrict__ c2n, const int32_t *__restrict__ c2v, T lam, T mu) {
+  const int32_t t = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hi) return;
+  const int32_t e = element_of(t, list);
+  int32_t nd[10];
+#pragma unroll
+  for (int j = 0; j < 10; ++j) nd[j] = c2n[10 * e + j];
+  const Geom<T> g = geometry(X, c2v, e);
+  T *vals = static_cast<T *>(A.values);
+  for (int i = 0; i < 10; ++i) {
+    const int32_t r = nd[i];
+    if (r < 0) continue;
+    T gi[4][3];
+    basis_grads(g, i, gi);
+    const int32_t rlo = A.rowptr[r], rhi = A.rowptr[r + 1];
+    T rs[3] = {T(0), T(0), T(0)};
+    for_blocks<T, 0>(gi, g, lam, mu, [&](int j, const T (&B)[9]) {
+      const int32_t c = nd[j];
+      if (c < 0) return;
+      if (kFused) {
+        const T u0 = u[3 * c], u1 = u[3 * c + 1], u2 = u[3 * c + 2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) rs[a] += B[3 * a] * u0 + B[3 * a + 1] * u1 + B[3 * a + 2] * u2;
+      }
+      const int32_t slot = find_slot(A.colidx, rlo, rhi, c);
+      if (slot < 0) {
+        atomicAdd(A.status, 1);
+        return;
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        if (kAtomic) atomic_add(&vals[9 * (size_t)slot + k], B[k]);
+        else plain_add(&vals[9 * (size_t)slot + k], B[k]);
+      }
+    });
+    if (kFused) add3<T, kAtomic>(R + 3 * r, rs);
+  }
+}
+
+// Row-gather element: block row i of the element matrix (and, fused, the
+// row's residual contribution sum_j B_ij u_j).
+template <typename TT, bool kFused>
+struct MatrixRow {
+  using T = TT;
+  static constexpr int kRar = 10, kCar = 10, kRd = 3, kCd = 3;
+  static constexpr bool kHasRowDat = kFused;
+  const T *X;
+  const T *u;
+  const int32_t *c2n;
+  const int32_t *c2v;
+  T lam, mu;
+  template <class Emit>
+  __device__ __forceinline__ void row(int32_t e, int i, Emit &&emit, T *rsum) const {
+    int32_t nd[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) nd[j] = c2n[10 * e + j];
+    const Geom<T> g = geometry(X, c2v, e);
+    T gi[4][3];
+    basis_grads(g, i, gi);
+    for_blocks<T, 0>(gi, g, lam, mu, [&](int j, const T (&B)[9]) {
+      const int32_t c = nd[j];
+      if (kFused && c >= 0) {
+        const T u0 = u[3 * c], u1 = u[3 * c + 1], u2 = u[3 * c + 2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          rsum[a] += B[3 * a] * u0 + B[3 * a + 1] * u1 + B[3 * a + 2] * u2;
+      }
+      emit(j, c, B);
+    });
+  }
+};
+
+// Adjoint of the residual loop for one element (see header of the entry point).
+template <typename T, int Q>
+__device__ __forceinline__ void adjoint_q(const T (&uu)[10][3], const T (&ee)[10][3],
+                                          const Geom<T> &g, T lam, T mu, T (&ub)[10][3],
+                                          T (&P)[3][3]) {
+  T gu[3][3], ge[3][3], su[3][3], se[3][3];
+  field_grad<T, Q>(uu, g, gu);
+  field_grad<T, Q>(ee, g, ge);
+  stress(gu, lam, mu, su);
+  stress(ge, lam, mu, se);
+  test_gradients<T, Q>(se, g, ub);
+  T phi = T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) phi += su[a][k] * ge[a][k];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T acc = (l == k) ? phi : T(0);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc -= gu[b][l] * se[b][k] + ge[b][l] * su[b][k];
+      P[l][k] += acc;
+    }
+}
+
+template <typename T, bool kAtomic>
+__global__ void __launch_bounds__(256)
+adjoint_kernel(int32_t lo, int32_t hi, const int32_t *__restrict__ list, T *__restrict__ ubar,
+               T *__restrict__ Xbar, const T *__restrict__ Rbar, const T *__restrict__ X,
+               const T *__restrict__ u, const int32_t *__restrict__ c2n,
+               const int32_t *__restrict__ c2v, T lam, T mu) {
+  const int32_t t = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= hi) return;
+  const int32_t e = element_of(t, list);
+  int32_t nd[10];
+#pragma unroll
+  for (int j = 0; j < 10; ++j) nd[j] = c2n[10 * e + j];
+  const Geom<T> g = geometry(X, c2v, e);
+  T uu[10][3], ee[10][3];
+#pragma unroll
+  for (int j = 0; j < 10; ++j)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      uu[j][b] = u[3 * nd[j] + b];
+      ee[j][b] = Rbar[3 * nd[j] + b];
+    }
+  T ub[10][3];
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ub[i][a] = T(0);
+  T P[3][3] = {};
+  adjoint_q<T, 0>(uu, ee, g, lam, mu, ub, P);
+  adjoint_q<T, 1>(uu, ee, g, lam, mu, ub, P);
+  adjoint_q<T, 2>(uu, ee, g, lam, mu, ub, P);
+  adjoint_q<T, 3>(uu, ee, g, lam, mu, ub, P);
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ub[i][a] *= g.wdet;
+    add3<T, kAtomic>(ubar + 3 * nd[i], ub[i]);
+  }
+  // dS/dJ = wdet * P K^T ; X[m+1] gets column m, X[0] minus the row sums.
+  T xb[4][3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    T rowsum = T(0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const T v = g.wdet * (P[l][0] * g.K[m][0] + P[l][1] * g.K[m][1] + P[l][2] * g.K[m][2]);
+      xb[m + 1][l] = v;
+      rowsum += v;
+    }
+    xb[0][l] = -rowsum;
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) add3<T, kAtomic>(Xbar + 3 * c2v[4 * e + v], xb[v]);
+}
+
+template <typename T>
+int residual(int32_t start, int32_t end, T *R, const T *X, const T *u, const int32_t *c2n,
+             const int32_t *c2v, double lam, double mu, const op2hip_exec *ex, hipStream_t s) {
+  const int bs = block_size_of(ex, 256);
+  return for_each_launch(ex, start, end, [&](int32_t lo, int32_t hi, const int32_t *list,
+                                             bool atomic) -> int {
+    if (atomic)
+      residual_kernel<T, true><<<ceil_div(hi - lo, bs), bs, 0, s>>>(lo, hi, list, R, X, u, c2n,
+                                                                    c2v, T(lam), T(mu));
+    else
+      residual_kernel<T, false><<<ceil_div(hi - lo, bs), bs, 0, s>>>(lo, hi, list, R, X, u, c2n,
+                                                                     c2v, T(lam), T(mu));
+    OP2HIP_LAUNCHED();
+    return OP2HIP_OK;
+  });
+}
+
+template <typename T, bool kFused>
+int matrix(int32_t start, int32_t end, const op2hip_mat *A, T *R, const T *X, const T *u,
+           const int32_t *c2n, const int32_t *c2v, double lam, double mu, const op2hip_exec *ex,
+           hipStream_t s) {
+  OP2HIP_REQUIRE(A && A->rowptr && A->colidx && A->values && A->status, "p2 matrix: bad Mat");
+  OP2HIP_REQUIRE(A->rdim == 3 && A->cdim == 3, "p2 matrix: Mat block must be 3x3 (BSR)");
+  if (ex && ex->strategy == OP2HIP_ROWGATHER) {
+    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    return launch_rowgather(start, end, elem, A, ex, R, s);
+  }
+  const int bs = block_size_of(ex, 256);
+  const op2hip_mat Av = *A;
+  return for_each_launch(ex, start, end, [&](int32_t lo, int32_t hi, const int32_t *list,
+                                             bool atomic) -> int {
+    if (atomic)
+      matrix_kernel<T, true, kFused><<<ceil_div(hi - lo, bs), bs, 0, s>>>(
+          lo, hi, list, Av, R, X, u, c2n, c2v, T(lam), T(mu));
+    else
+      matrix_kernel<T, false, kFused><<<ceil_div(hi - lo, bs), bs, 0, s>>>(
+          lo, hi, list, Av, R, X, u, c2n, c2v, T(lam), T(mu));
+    OP2HIP_LAUNCHED();
+    return OP2HIP_OK;
+  });
+}
+
+}  // namespace p2
+}  // namespace op2hip
+
+using namespace op2hip;
+
+extern "C" {
+
+int op2hip_wrap_p2_elasticity_residual(int32_t start, int32_t end, double *R, const double *X,
+                                       const double *u, const int32_t *c2n, const int32_t *c2v,
+                                       double lam, double mu, const op2hip_exec *ex,
+                                       void *stream) {
+  return p2::residual<double>(start, end, R, X, u, c2n, c2v, lam, mu, ex, stream_of(stream));
+}
+
+int op2hip_wrap_p2_elasticity_residual_f32(int32_t start, int32_t end, float *R, const float *X,
+                                           const float *u, const int32_t *c2n,
+                                           const int32_t *c2v, double lam, double mu,
+                                           const op2hip_exec *ex, void *stream) {
+  return p2::residual<float>(start, end, R, X, u, c2n, c2v, lam, mu, ex, stream_of(stream));
+}
+
+int op2hip_wrap_p2_elasticity_matrix(int32_t start, int32_t end, const op2hip_mat *A,
+                                     const double *X, const int32_t *c2n, const int32_t *c2v,
+                                     double lam, double mu, const op2hip_exec *ex, void *stream) {
+  return p2::matrix<double, false>(start, end, A, nullptr, X, nullptr, c2n, c2v, lam, mu, ex,
+                                   stream_of(stream));
+}
+
+int op2hip_wrap_p2_elasticity_matrix_f32(int32_t start, int32_t end, const op2hip_mat *A,
+                                         const float *X, const int32_t *c2n, const int32_t *c2v,
+                                         double lam, double mu, const op2hip_exec *ex,
+                                         void *stream) {
+  return p2::matrix<float, false>(start, end, A, nullptr, X, nullptr, c2n, c2v, lam, mu, ex,
+                                  stream_of(stream));
+}
+
+int op2hip_wrap_p2_elasticity_fused(int32_t start, int32_t end, const op2hip_mat *A, double *R,
+                                    const double *X, const double *u, const int32_t *c2n,
+                                    const int32_t *c2v, double lam, double mu,
+                                    const op2hip_exec *ex, void *stream) {
+  return p2::matrix<double, true>(start, end, A, R, X, u, c2n, c2v, lam, mu, ex,
+                                  stream_of(stream));
+}
+
+int op2hip_wrap_p2_elasticity_fused_f32(int32_t start, int32_t end, const op2hip_mat *A,
+                                        float *R, const float *X, const float *u,
+                                        const int32_t *c2n, const int32_t *c2v, double lam,
+                                        double mu, const op2hip_exec *ex, void *stream) {
+  return p2::matrix<float, true>(start, end, A, R, X, u, c2n, c2v, lam, mu, ex,
+                                 stream_of(stream));
+}
+
+int op2hip_wrap_p2_elasticity_adjoint(int32_t start, int32_t end, double *ubar, double *Xbar,
+                                      const double *Rbar, const double *X, const double *u,
+                                      const int32_t *c2n, const int32_t *c2v, double lam,
+                                      double mu, const op2hip_exec *ex, void *stream) {
+  hipStream_t s = stream_of(stream);
+  const int bs = block_size_of(ex, 256);
+  return for_each_launch(ex, start, end, [&](int32_t lo, int32_t hi, const int32_t *list,
+                                             bool atomic) -> int {
+    if (atomic)
+      p2::adjoint_kernel<double, true><<<ceil_div(hi - lo, bs), bs, 0, s>>>(
+          lo, hi, list, ubar, Xbar, Rbar, X, u, c2n, c2v, lam, mu);
+    else
+      p2::adjoint_kernel<double, false><<<ceil_div(hi - lo, bs), bs, 0, s>>>(
+          lo, hi, list, ubar, Xbar, Rbar, X, u, c2n, c2v, lam, mu);
+    OP2HIP_LAUNCHED();
+    return OP2HIP_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+"""Element kernels for the P1 triangle and P2 tetrahedron workloads.
+
+Each function returns an ``op2.Kernel`` whose C string is an ordinary PyOP2
+local kernel (the form ``op2.Kernel(code, name)`` takes, buffer shapes as the
+reference's wrapper packs them, ``pyop2/codegen/builder.py:340-347`` for Dats
+and ``:519-529`` for Mats).  The same C string is what a CPU PyOP2 run would
+compile.  On the device these kernels are not compiled from the string: their
+code hash is registered with a hand-written HIP implementation (see
+``pyop2_amd.registry``), so ``op2.par_loop`` launches the HIP kernel.
+
+Argument conventions (INC buffers arrive zeroed, as in the reference):
+
+* ``p1_lumped_mass``      ``(double M[3], const double X[3][2])``
+* ``p1_laplace_residual`` ``(double R[3], const double X[3][2], const double u[3])``
+* ``p1_laplace_matrix``   ``(double A[3][3], const double X[3][2])``
+* ``p2_elasticity_residual`` ``(double R[10][3], const double X[4][3], const double u[10][3])``
+* ``p2_elasticity_matrix``   ``(double A[30][30], const double X[4][3])``
+* ``p2_elasticity_fused``    ``(double A[30][30], double R[10][3], const double X[4][3], const double u[10][3])``
+
+The P2 kernels bake the Lame parameters in as literals (as form compilers
+do).  The 4-point Gauss rule (a = 0.5854101966249685, b = 0.1381966011250105,
+w = 1/24 on the reference tetrahedron) integrates the degree-2 integrands of
+the P2 stiffness exactly.
+"""
+import hashlib
+
+__all__ = ["p1_lumped_mass", "p1_laplace_residual", "p1_laplace_matrix",
+           "p2_elasticity_residual", "p2_elasticity_matrix", "p2_elasticity_fused",
+           "kernel_source", "code_digest", "QUAD_A", "QUAD_B", "QUAD_W"]
+
+QUAD_A = 0.5854101966249685
+QUAD_B = 0.1381966011250105
+QUAD_W = 1.0 / 24.0
+
+P1_LUMPED_MASS = r"""
+static void p1_lumped_mass(double M[3], const double X[3][2])
+{
+  const double det = (X[1][0] - X[0][0]) * (X[2][1] - X[0][1])
+                   - (X[2][0] - X[0][0]) * (X[1][1] - X[0][1]);
+  const double m = fabs(det) / 6.0;
+  M[0] += m;
+  M[1] += m;
+  M[2] += m;
+}
+"""
+
+_P1_GEOMETRY = r"""
+  double b[3], c[3];
+  b[0] = X[1][1] - X[2][1];
+  b[1] = X[2][1] - X[0][1];
+  b[2] = X[0][1] - X[1][1];
+  c[0] = X[2][0] - X[1][0];
+  c[1] = X[0][0] - X[2][0];
+  c[2] = X[1][0] - X[0][0];
+  const double det = (X[1][0] - X[0][0]) * (X[2][1] - X[0][1])
+                   - (X[2][0] - X[0][0]) * (X[1][1] - X[0][1]);
+  const double s = 1.0 / (2.0 * fabs(det));
+"""
+
+P1_LAPLACE_RESIDUAL = r"""
+static void p1_laplace_residual(double R[3], const double X[3][2], const double u[3])
+{""" + _P1_GEOMETRY + r"""
+  for (int i = 0; i < 3; ++i) {
+    double acc = 0.0;
+    for (int j = 0; j < 3; ++j)
+      acc += (b[i] * b[j] + c[i] * c[j]) * s * u[j];
+    R[i] += acc;
+  }
+}
+"""
+
+P1_LAPLACE_MATRIX = r"""
+static void p1_laplace_matrix(double A[3][3], const double X[3][2])
+{""" + _P1_GEOMETRY + r"""
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      A[i][j] += (b[i] * b[j] + c[i] * c[j]) * s;
+}
+"""
+
+# Affine tetrahedron geometry: J[r][c] = X[c+1][r] - X[0][r], K = J^{-1}
+# (K[m][k] = d xi_m / d x_k), barycentric gradients gl[0..3], wdet = w |det J|.
+_P2_GEOMETRY = r"""
+  const double qa = %(qa)r, qb = %(qb)r, qw = %(qw)r;
+  double J[3][3], K[3][3], gl[4][3];
+  for (int r = 0; r < 3; ++r)
+    for (int cc = 0; cc < 3; ++cc)
+      J[r][cc] = X[cc + 1][r] - X[0][r];
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double idet = 1.0 / det;
+  K[0][0] = c00 * idet;
+  K[1][0] = c01 * idet;
+  K[2][0] = c02 * idet;
+  K[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * idet;
+  K[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * idet;
+  K[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * idet;
+  K[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * idet;
+  K[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * idet;
+  K[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * idet;
+  for (int k = 0; k < 3; ++k) {
+    gl[0][k] = -(K[0][k] + K[1][k] + K[2][k]);
+    gl[1][k] = K[0][k];
+    gl[2][k] = K[1][k];
+    gl[3][k] = K[2][k];
+  }
+  const double wdet = qw * fabs(det);
+  static const int ea[6] = {0, 0, 0, 1, 1, 2};
+  static const int eb[6] = {1, 2, 3, 2, 3, 3};
+"""
+
+# Physical gradients of the 10 P2 basis functions at quadrature point q.
+_P2_BASIS_GRADS = r"""
+    double L[4] = {qb, qb, qb, qb};
+    L[q] = qa;
+    double G[10][3];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 3; ++k)
+        G[i][k] = (4.0 * L[i] - 1.0) * gl[i][k];
+    for (int e = 0; e < 6; ++e)
+      for (int k = 0; k < 3; ++k)
+        G[4 + e][k] = 4.0 * (L[ea[e]] * gl[eb[e]][k] + L[eb[e]] * gl[ea[e]][k]);
+"""
+
+P2_ELASTICITY_RESIDUAL = r"""
+static void p2_elasticity_residual(double R[10][3], const double X[4][3], const double u[10][3])
+{
+  const double lmbda = %(lam)r, mu = %(mu)r;""" + _P2_GEOMETRY + r"""
+  for (int q = 0; q < 4; ++q) {""" + _P2_BASIS_GRADS + r"""
+    double gu[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int j = 0; j < 10; ++j)
+      for (int bb = 0; bb < 3; ++bb)
+        for (int k = 0; k < 3; ++k)
+          gu[bb][k] += u[j][bb] * G[j][k];
+    const double dv = gu[0][0] + gu[1][1] + gu[2][2];
+    double s[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < 3; ++k)
+        s[a][k] = mu * (gu[a][k] + gu[k][a]) + (a == k ? lmbda * dv : 0.0);
+    for (int i = 0; i < 10; ++i)
+      for (int a = 0; a < 3; ++a) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k)
+          acc += s[a][k] * G[i][k];
+        R[i][a] += wdet * acc;
+      }
+  }
+}
+"""
+
+P2_ELASTICITY_MATRIX = r"""
+static void p2_elasticity_matrix(double A[30][30], const double X[4][3])
+{
+  const double lmbda = %(lam)r, mu = %(mu)r;""" + _P2_GEOMETRY + r"""
+  for (int q = 0; q < 4; ++q) {""" + _P2_BASIS_GRADS + r"""
+    for (int i = 0; i < 10; ++i)
+      for (int j = 0; j < 10; ++j) {
+        const double dot = G[i][0] * G[j][0] + G[i][1] * G[j][1] + G[i][2] * G[j][2];
+        for (int a = 0; a < 3; ++a)
+          for (int bb = 0; bb < 3; ++bb)
+            A[3 * i + a][3 * j + bb] += wdet * (lmbda * G[i][a] * G[j][bb]
+                + mu * (G[i][bb] * G[j][a] + (a == bb ? dot : 0.0)));
+      }
+  }
+}
+"""
+
+
+P2_ELASTICITY_FUSED = r"""
+static void p2_elasticity_fused(double A[30][30], double R[10][3], const double X[4][3], const double u[10][3])
+{
+  const double lmbda = %(lam)r, mu = %(mu)r;""" + _P2_GEOMETRY + r"""
+  for (int q = 0; q < 4; ++q) {""" + _P2_BASIS_GRADS + r"""
+    for (int i = 0; i < 10; ++i)
+      for (int j = 0; j < 10; ++j) {
+        const double dot = G[i][0] * G[j][0] + G[i][1] * G[j][1] + G[i][2] * G[j][2];
+        for (int a = 0; a < 3; ++a)
+          for (int bb = 0; bb < 3; ++bb)
+            A[3 * i + a][3 * j + bb] += wdet * (lmbda * G[i][a] * G[j][bb]
+                + mu * (G[i][bb] * G[j][a] + (a == bb ? dot : 0.0)));
+      }
+  }
+  for (int i = 0; i < 10; ++i)
+    for (int a = 0; a < 3; ++a) {
+      double acc = 0.0;
+      for (int j = 0; j < 10; ++j)
+        for (int bb = 0; bb < 3; ++bb)
+          acc += A[3 * i + a][3 * j + bb] * u[j][bb];
+      R[i][a] += acc;
+    }
+}
+"""
+
+
+def kernel_source(family, lam=1.0, mu=0.5):
+    """C source of element kernel ``family`` (P2 families take lam, mu)."""
+    params = {"qa": QUAD_A, "qb": QUAD_B, "qw": QUAD_W,
+              "lam": float(lam), "mu": float(mu)}
+    table = {
+        "p1_lumped_mass": P1_LUMPED_MASS,
+        "p1_laplace_residual": P1_LAPLACE_RESIDUAL,
+        "p1_laplace_matrix": P1_LAPLACE_MATRIX,
+        "p2_elasticity_residual": P2_ELASTICITY_RESIDUAL % params,
+        "p2_elasticity_matrix": P2_ELASTICITY_MATRIX % params,
+        "p2_elasticity_fused": P2_ELASTICITY_FUSED % params,
+    }
+    try:
+        return table[family]
+    except KeyError:
+        raise ValueError(f"unknown element kernel family {family!r}") from None
+
+
+def code_digest(code, name):
+    """Stable digest of a kernel's (code, name), the key of the HIP registry."""
+    return hashlib.sha256((name + "\0" + code).encode()).hexdigest()
+
+
+def _make(family, **params):
+    from pyop2_amd import registry
+    from pyop2_amd.local_kernel import Kernel
+    code = kernel_source(family, **params)
+    registry.register_element_kernel(code, family, family, params)
+    return Kernel(code, family)
+
+
+def p1_lumped_mass():
+    """Lumped P1 mass: |T|/3 to each vertex."""
+    return _make("p1_lumped_mass")
+
+
+def p1_laplace_residual():
+    """P1 Laplace residual K_e u_e, K_e = (b b^T + c c^T) / (4 |T|)."""
+    return _make("p1_laplace_residual")
+
+
+def p1_laplace_matrix():
+    """P1 Laplace element stiffness K_e (3x3)."""
+    return _make("p1_laplace_matrix")
+
+
+def p2_elasticity_residual(lam=1.0, mu=0.5):
+    """P2 isotropic linear elasticity residual sigma(u):grad(phi_i e_a)."""
+    return _make("p2_elasticity_residual", lam=float(lam), mu=float(mu))
+
+
+def p2_elasticity_matrix(lam=1.0, mu=0.5):
+    """P2 isotropic linear elasticity element matrix (30x30, dof 3*node+comp)."""
+    return _make("p2_elasticity_matrix", lam=float(lam), mu=float(mu))
+
+
+def p2_elasticity_fused(lam=1.0, mu=0.5):
+    """Element matrix A_e and residual R_e = A_e u_e in one kernel."""
+    return _make("p2_elasticity_fused", lam=float(lam), mu=float(mu))

@@ -1,0 +1,156 @@
+"""Global kernel: the drop-in boundary of the par_loop hot path.
+
+The reference's ``GlobalKernel`` (``pyop2/global_kernel.py:250-429``) turns a
+local kernel plus argument descriptors into a JIT-compiled C function
+``wrap_<name>(start, end, ptrs...)`` that ``Parloop._compute`` calls once per
+partition (``parloop.py:224-232``).  Here ``compile()`` returns a device
+executor with the same ``(start, end, *ptrs)`` contract:
+
+* a hand-written HIP entry point of ``libop2hip.so`` when the local kernel
+  is a registered element kernel and the argument signature matches
+  (``pyop2_amd.registry``);
+* otherwise a gather/compute/scatter kernel generated around the user's C
+  code and compiled for gfx950 at run time (``pyop2_amd.codegen``).
+
+The descriptor classes keep the reference's fields and cache keys
+(``global_kernel.py:23-247``).
+"""
+import collections
+import itertools
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+from pyop2_amd.configuration import configuration
+from pyop2_amd.types.access import READ
+from pyop2_amd.utils import cached_property
+
+
+@dataclass(eq=False, frozen=True)
+class MapKernelArg:
+    """A map argument: its arity (extruded offsets are not supported)."""
+    arity: int
+    offset: Optional[Tuple[int, ...]] = None
+    offset_quotient: Optional[Tuple[int, ...]] = None
+
+    @property
+    def cache_key(self):
+        return type(self), self.arity, self.offset, self.offset_quotient
+
+
+@dataclass(frozen=True)
+class GlobalKernelArg:
+    dim: Tuple[int, ...]
+
+    @property
+    def cache_key(self):
+        return type(self), self.dim
+
+    @property
+    def maps(self):
+        return ()
+
+
+@dataclass(frozen=True)
+class DatKernelArg:
+    dim: Tuple[int, ...]
+    map_: MapKernelArg = None
+    index: Optional[Tuple[int, ...]] = None
+
+    @property
+    def is_direct(self):
+        return self.map_ is None
+
+    @property
+    def is_indirect(self):
+        return not self.is_direct
+
+    @property
+    def cache_key(self):
+        map_key = self.map_.cache_key if self.map_ is not None else None
+        return type(self), self.dim, map_key, self.index
+
+    @property
+    def maps(self):
+        return (self.map_,) if self.map_ is not None else ()
+
+
+@dataclass(frozen=True)
+class MatKernelArg:
+    dims: Tuple[Tuple[int, ...], Tuple[int, ...]]
+    maps: Tuple[MapKernelArg, MapKernelArg]
+    unroll: bool = False
+
+    @property
+    def cache_key(self):
+        return type(self), self.dims, tuple(m.cache_key for m in self.maps), self.unroll
+
+
+class PassthroughKernelArg:
+    @property
+    def cache_key(self):
+        return type(self)
+
+    @property
+    def maps(self):
+        return ()
+
+
+class GlobalKernel:
+    """Compiled form of a local kernel over a given argument signature."""
+
+    def __init__(self, local_kernel, arguments, *, extruded=False, extruded_periodic=False,
+                 constant_layers=False, subset=False, iteration_region=None, pass_layer_arg=False):
+        if not len(local_kernel.accesses) == len(arguments):
+            raise ValueError("Number of arguments passed to the local and global kernels do not match")
+        if extruded or extruded_periodic or constant_layers or iteration_region is not None or pass_layer_arg:
+            raise NotImplementedError("extruded iteration is not on the device path")
+        from pyop2_amd.types.glob import Constant
+        if any(isinstance(garg, Constant) and larg.access is not READ
+               for larg, garg in zip(local_kernel.arguments, arguments)):
+            raise ValueError("Constants can only ever be read in a parloop, not modified")
+        counter = itertools.count()
+        seen_maps = collections.defaultdict(lambda: next(counter))
+        self.cache_key = (
+            local_kernel.cache_key,
+            *[a.cache_key for a in arguments],
+            *[seen_maps[m] for a in arguments for m in a.maps],
+            extruded, extruded_periodic, constant_layers, subset,
+            iteration_region, pass_layer_arg, configuration["simd_width"],
+        )
+        self.local_kernel = local_kernel
+        self.arguments = arguments
+        self._subset = subset
+        self._executor = None
+
+    def __call__(self, comm, *args, parloop=None):
+        """Execute over ``args = (start, end, *ptrs)`` on the current stream."""
+        fn = self.compile(comm)
+        start, end, ptrs = args[0], args[1], args[2:]
+        fn(start, end, ptrs, parloop)
+
+    @cached_property
+    def name(self):
+        return f"wrap_{self.local_kernel.name}"
+
+    @cached_property
+    def zipped_arguments(self):
+        return tuple(zip(self.local_kernel.arguments, self.arguments))
+
+    def compile(self, comm=None):
+        """The executor: a hand-written entry point or a generated kernel."""
+        if self._executor is None:
+            from pyop2_amd import registry
+            ex = registry.lookup(self)
+            if ex is None:
+                from pyop2_amd.codegen import GeneratedKernel
+                ex = GeneratedKernel(self)
+            self._executor = ex
+        return self._executor
+
+    @property
+    def is_hand_written(self):
+        from pyop2_amd.codegen import GeneratedKernel
+        return not isinstance(self.compile(), GeneratedKernel)
+
+    def num_flops(self, iterset):
+        return self.local_kernel.num_flops

@@ -1,0 +1,141 @@
+"""Execution plans: map transposes, greedy colourings and row batches.
+
+Plans depend on the data (map values), so they are built on first use and
+cached on the Map (``Map._cache``), the way the reference caches objects
+built on top of Sets and Maps.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from pyop2_amd import device as dev
+
+__all__ = ["transpose", "colour_plan", "row_batches", "Transpose", "ColourPlan", "RowBatches"]
+
+
+@dataclass
+class Transpose:
+    """Row incidences of a map: ``inc_ent[inc_ptr[r]:inc_ptr[r+1]]`` lists
+    ``e * arity + i`` for every ``map[e][i] == r`` (e < iter_size), ascending."""
+    inc_ptr: object
+    inc_ent: object
+    total: int
+
+
+def transpose(m, nrows, iter_size):
+    key = ("transpose", int(nrows), int(iter_size))
+    tr = m._cache.get(key)
+    if tr is not None:
+        return tr
+    L = dev.lib()
+    s = dev.stream()
+    need = ctypes.c_size_t()
+    dev.check(L.op2hip_transpose_workspace(nrows, ctypes.byref(need)), "transpose_workspace")
+    work = dev.workspace(need.value)
+    inc_ptr = dev.empty((nrows + 1,), np.int32)
+    total = ctypes.c_int64()
+    mv = m.device_values
+    dev.check(L.op2hip_transpose_count(nrows, dev.ptr(mv), m.arity, iter_size, dev.ptr(inc_ptr),
+                                       ctypes.byref(total), dev.ptr(work), work.numel(), s),
+              "transpose_count")
+    inc_ent = dev.empty((max(total.value, 1),), np.int32)
+    dev.check(L.op2hip_transpose_fill(nrows, dev.ptr(mv), m.arity, iter_size, dev.ptr(inc_ptr),
+                                      dev.ptr(inc_ent), dev.ptr(work), work.numel(), s),
+              "transpose_fill")
+    tr = Transpose(inc_ptr, inc_ent, int(total.value))
+    m._cache[key] = tr
+    return tr
+
+
+@dataclass
+class ColourPlan:
+    colours: np.ndarray        # host colour of each element of the range
+    elems: object              # device: element ids sorted by colour
+    offsets: np.ndarray        # host int32, ncolours+1
+    ncolours: int
+    _offsets_c: object = None
+
+    def exec_fields(self, ex):
+        self._offsets_c = np.ascontiguousarray(self.offsets, dtype=np.int32)
+        ex.colour_elems = dev.ptr(self.elems)
+        ex.colour_offsets = self._offsets_c.ctypes.data
+        ex.ncolours = self.ncolours
+
+
+def greedy_colours(elements, maps_and_targets):
+    """Greedy colouring (``doc/sphinx/source/plan.rst:46-80``) of ``elements``
+    (host int32 ids) against the rows of each ``(map, target_key)``."""
+    L = dev.lib()
+    targets = {}
+    rows = []
+    for m, tkey in maps_and_targets:
+        tid = targets.setdefault(tkey, (len(targets), m.toset.total_size))[0]
+        rows.append((np.ascontiguousarray(m.values_with_halo[elements], dtype=np.int32), tid))
+    nmaps = len(rows)
+    ptrs = (ctypes.c_void_p * max(nmaps, 1))(*[r.ctypes.data for r, _ in rows])
+    ar = np.array([r.shape[1] for r, _ in rows] or [0], dtype=np.int32)
+    tid = np.array([t for _, t in rows] or [0], dtype=np.int32)
+    tsz = np.zeros(max(len(targets), 1), dtype=np.int32)
+    for _, (i, size) in targets.items():
+        tsz[i] = size
+    out = np.empty(max(len(elements), 1), dtype=np.int32)
+    nc = ctypes.c_int32()
+    dev.check(L.op2hip_colour_greedy(len(elements), nmaps, ptrs, ar.ctypes.data, tid.ctypes.data,
+                                     tsz.ctypes.data, len(targets), out.ctypes.data,
+                                     ctypes.byref(nc)), "colour_greedy")
+    return out[:len(elements)], int(nc.value)
+
+
+def colour_plan(owner, elements, maps_and_targets, key):
+    """Colour plan of host element ids ``elements``, cached on ``owner``."""
+    ck = ("colour", key)
+    plan = owner._cache.get(ck)
+    if plan is not None:
+        return plan
+    elements = np.ascontiguousarray(elements, dtype=np.int32)
+    colours, nc = greedy_colours(elements, maps_and_targets)
+    order = np.argsort(colours, kind="stable")
+    counts = np.bincount(colours, minlength=nc)
+    offsets = np.zeros(nc + 1, dtype=np.int32)
+    np.cumsum(counts, out=offsets[1:])
+    plan = ColourPlan(colours, dev.to_device(elements[order]), offsets, nc)
+    owner._cache[ck] = plan
+    return plan
+
+
+@dataclass
+class RowBatches:
+    batch_rows: object   # device int32, nbatches+1
+    nbatches: int
+    max_inc: int
+    max_nnz: int
+    max_rows: int
+
+    def exec_fields(self, ex):
+        ex.batch_rows = dev.ptr(self.batch_rows)
+        ex.nbatches = self.nbatches
+        ex.batch_max_inc = self.max_inc
+        ex.batch_max_nnz = self.max_nnz
+        ex.batch_max_rows = self.max_rows
+
+
+def row_batches(owner, tr, rowptr, max_inc, max_nnz, key):
+    """Batches of consecutive rows for the row-gather assembly."""
+    ck = ("rowbatch", key, int(max_inc), int(max_nnz))
+    plan = owner._cache.get(ck)
+    if plan is not None:
+        return plan
+    L = dev.lib()
+    ip = np.ascontiguousarray(dev.to_host(tr.inc_ptr), dtype=np.int32)
+    rp = np.ascontiguousarray(dev.to_host(rowptr), dtype=np.int32)
+    nrows = len(ip) - 1
+    br = np.empty(nrows + 1, dtype=np.int32)
+    nb, mi, mn, mr = (ctypes.c_int32() for _ in range(4))
+    dev.check(L.op2hip_plan_row_batches(nrows, ip.ctypes.data, rp.ctypes.data, int(max_inc),
+                                        int(max_nnz), br.ctypes.data, ctypes.byref(nb),
+                                        ctypes.byref(mi), ctypes.byref(mn), ctypes.byref(mr)),
+              "plan_row_batches")
+    plan = RowBatches(dev.to_device(br[:nb.value + 1]), nb.value, mi.value, mn.value, mr.value)
+    owner._cache[ck] = plan
+    return plan

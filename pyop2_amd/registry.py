@@ -1,0 +1,229 @@
+"""Registry of hand-written HIP element loops.
+
+``pyop2_amd.elements`` registers the digest of each element kernel's C
+source with a family name.  ``lookup(global_kernel)`` returns an executor
+for a GlobalKernel whose local kernel is registered and whose argument
+signature (access modes, dims, arities, dtypes, map sharing) is the one the
+HIP entry point implements; anything else goes to the generated path.
+
+Conflict resolution (``configuration["strategy"]``):
+* ``atomic``   -- device-scope atomics (FP64 atomics are native on gfx950);
+* ``coloured`` -- greedy colouring, one launch per colour, plain adds;
+* ``rowgather``-- Mat arguments: owner-computes rows, no atomics;
+* ``auto``     -- the autotuner's choice (``pyop2_amd.autotune``).
+"""
+import ctypes
+
+import numpy as np
+
+from pyop2_amd import device as dev
+from pyop2_amd.configuration import configuration
+from pyop2_amd.elements import code_digest
+from pyop2_amd.global_kernel import DatKernelArg, MatKernelArg
+from pyop2_amd.types.access import INC, READ
+from pyop2_amd.types.set import Subset
+
+__all__ = ["register_element_kernel", "lookup", "FAMILIES"]
+
+_REGISTERED = {}
+
+
+def register_element_kernel(code, name, family, params):
+    _REGISTERED[code_digest(code, name)] = (family, dict(params))
+
+
+# family -> (argument specs, map sharing, entry points)
+# spec: ("dat", access, dim, arity) | ("mat", access, (rdim, cdim), (rarity, carity))
+FAMILIES = {
+    "p1_lumped_mass": dict(
+        args=[("dat", INC, (1,), 3), ("dat", READ, (2,), 3)],
+        maps=[[(0, 0), (1, 0)]], entry={"f64": "op2hip_wrap_p1_lumped_mass"}),
+    "p1_laplace_residual": dict(
+        args=[("dat", INC, (1,), 3), ("dat", READ, (2,), 3), ("dat", READ, (1,), 3)],
+        maps=[[(0, 0), (1, 0), (2, 0)]], entry={"f64": "op2hip_wrap_p1_laplace_residual"}),
+    "p1_laplace_matrix": dict(
+        args=[("mat", INC, (1, 1), (3, 3)), ("dat", READ, (2,), 3)],
+        maps=[[(0, 0), (0, 1), (1, 0)]], entry={"f64": "op2hip_wrap_p1_laplace_matrix"}),
+    "p2_elasticity_residual": dict(
+        args=[("dat", INC, (3,), 10), ("dat", READ, (3,), 4), ("dat", READ, (3,), 10)],
+        maps=[[(0, 0), (2, 0)]],
+        entry={"f64": "op2hip_wrap_p2_elasticity_residual",
+               "f32": "op2hip_wrap_p2_elasticity_residual_f32"}),
+    "p2_elasticity_matrix": dict(
+        args=[("mat", INC, (3, 3), (10, 10)), ("dat", READ, (3,), 4)],
+        maps=[[(0, 0), (0, 1)]],
+        entry={"f64": "op2hip_wrap_p2_elasticity_matrix",
+               "f32": "op2hip_wrap_p2_elasticity_matrix_f32"}),
+    "p2_elasticity_fused": dict(
+        args=[("mat", INC, (3, 3), (10, 10)), ("dat", INC, (3,), 10), ("dat", READ, (3,), 4),
+              ("dat", READ, (3,), 10)],
+        maps=[[(0, 0), (0, 1), (1, 0), (3, 0)]],
+        entry={"f64": "op2hip_wrap_p2_elasticity_fused",
+               "f32": "op2hip_wrap_p2_elasticity_fused_f32"}),
+    "p2_elasticity_adjoint": dict(
+        args=[("dat", INC, (3,), 10), ("dat", INC, (3,), 4), ("dat", READ, (3,), 10),
+              ("dat", READ, (3,), 4), ("dat", READ, (3,), 10)],
+        maps=[[(0, 0), (2, 0), (4, 0)], [(1, 0), (3, 0)]],
+        entry={"f64": "op2hip_wrap_p2_elasticity_adjoint"}),
+}
+
+# Row-batch budgets of the row-gather assembly (LDS <= 64 KiB per workgroup).
+_BATCH = {("p1_laplace_matrix", "f64"): (1024, 2048),
+          ("p2_elasticity_matrix", "f64"): (512, 640),
+          ("p2_elasticity_matrix", "f32"): (512, 1200),
+          ("p2_elasticity_fused", "f64"): (512, 640),
+          ("p2_elasticity_fused", "f32"): (512, 1200)}
+
+
+def _variant(dtypes):
+    dts = {np.dtype(d) for d in dtypes}
+    if dts == {np.dtype(np.float64)}:
+        return "f64"
+    if dts == {np.dtype(np.float32)}:
+        return "f32"
+    return None
+
+
+def lookup(gk):
+    """Executor for ``gk`` if a hand-written entry point implements it."""
+    lk = gk.local_kernel
+    hit = _REGISTERED.get(code_digest(lk.code, lk.name))
+    if hit is None:
+        return None
+    family, params = hit
+    spec = FAMILIES[family]
+    if len(gk.arguments) != len(spec["args"]):
+        return None
+    for (kind, access, dims, arity), larg, garg in zip(spec["args"], lk.arguments, gk.arguments):
+        if larg.access != access:
+            return None
+        if kind == "dat":
+            if not isinstance(garg, DatKernelArg) or garg.map_ is None:
+                return None
+            if tuple(garg.dim) != dims or garg.map_.arity != arity:
+                return None
+        else:
+            if not isinstance(garg, MatKernelArg) or garg.unroll:
+                return None
+            if tuple(garg.dims[0][0]) != dims or tuple(m.arity for m in garg.maps) != arity:
+                return None
+    variant = _variant(lk.dtypes)
+    if variant not in spec["entry"]:
+        return None
+    return HandWrittenKernel(gk, family, params, variant)
+
+
+class HandWrittenKernel:
+    """Launches a registered element loop through its C entry point."""
+
+    def __init__(self, gk, family, params, variant):
+        self.gk = gk
+        self.family = family
+        self.params = params
+        self.variant = variant
+        self.spec = FAMILIES[family]
+        self.fn = getattr(dev.lib(), self.spec["entry"][variant])
+        self.strategy_used = None
+
+    def _check_maps(self, parloop):
+        """Arguments that the entry point reads through one map must share it."""
+        maps = [a.maps if hasattr(a, "maps") else () for a in parloop.arguments]
+        if any(m.has_negative for ms in maps for m in ms):
+            return False   # the hand-written loops assume every entry has a target
+        for group in self.spec["maps"]:
+            objs = {id(maps[i][j]) for i, j in group}
+            if len(objs) != 1:
+                return False
+        return True
+
+    def _choose(self, parloop):
+        has_mat = any(s[0] == "mat" for s in self.spec["args"])
+        subset = isinstance(parloop.iterset, Subset)
+        want = configuration["strategy"]
+        if want == "auto":
+            from pyop2_amd import autotune
+            want = autotune.choice(self, parloop)
+        if want == "rowgather" and (not has_mat or subset):
+            want = "atomic"
+        if want == "staged":
+            want = "atomic"
+        if has_mat and want == "rowgather":
+            mat = parloop.arguments[0].data
+            if mat.block_shape != self.spec["args"][0][2]:
+                want = "atomic"
+        return want
+
+    def _inc_maps(self, parloop):
+        out = []
+        for (kind, access, _, _), arg in zip(self.spec["args"], parloop.arguments):
+            if access == INC:
+                out.append(arg.maps[0])
+        return out
+
+    def __call__(self, start, end, ptrs, parloop):
+        if parloop is None:
+            raise RuntimeError("hand-written loops are launched by a Parloop")
+        if not self._check_maps(parloop):
+            # same values through different Map objects: take the generated path
+            from pyop2_amd.codegen import GeneratedKernel
+            return GeneratedKernel(self.gk)(start, end, ptrs, parloop)
+        self.launch(start, end, parloop, self._choose(parloop))
+
+    def launch(self, start, end, parloop, strategy, outputs=None):
+        """Launch with ``strategy``; ``outputs`` optionally replaces the
+        written arguments' device pointers (autotuning on scratch copies)."""
+        ex = dev.ExecPlan()
+        ex.strategy = dev.STRATEGIES[strategy]
+        keep = []
+        iterset = parloop.iterset
+        if isinstance(iterset, Subset):
+            ex.subset = dev.ptr(iterset.device_indices)
+        args = parloop.arguments
+        desc = None
+        if strategy == "coloured":
+            from pyop2_amd import plans
+            if isinstance(iterset, Subset):
+                elems = iterset.indices[start:end]
+            else:
+                elems = np.arange(start, end, dtype=np.int32)
+            owner = self._inc_maps(parloop)[0]
+            key = (id(iterset), start, end, tuple(id(m) for m in self._inc_maps(parloop)))
+            plan = plans.colour_plan(owner, elems,
+                                     [(m, id(m.toset)) for m in self._inc_maps(parloop)], key)
+            plan.exec_fields(ex)
+            keep.append(plan)
+        if self.spec["args"][0][0] == "mat":
+            mat = args[0].data
+            rmap = args[0].maps[0]
+            if strategy == "rowgather":
+                from pyop2_amd import plans
+                tr = plans.transpose(rmap, mat.sparsity.nrows, rmap.iterset.size)
+                budget = _BATCH[(self.family, self.variant)]
+                rb = plans.row_batches(rmap, tr, mat.rowptr, budget[0], budget[1],
+                                       (id(mat.sparsity), mat.block_shape, self.variant))
+                rb.exec_fields(ex)
+                keep.append(rb)
+                desc = mat.descriptor(tr.inc_ptr, tr.inc_ent)
+            else:
+                desc = mat.descriptor()
+            if outputs and "mat" in outputs:
+                desc.values = dev.ptr(outputs["mat"])
+        cargs = []
+        for k, ((kind, access, _, _), arg) in enumerate(zip(self.spec["args"], args)):
+            if kind == "mat":
+                cargs.append(ctypes.addressof(desc))
+            else:
+                t = outputs.get(k) if outputs else None
+                cargs.append(dev.ptr(t if t is not None else arg.data._device_array()))
+        seen = []
+        for a in args:
+            for m in a.maps:
+                if not any(m is s for s in seen):
+                    seen.append(m)
+        cargs += [dev.ptr(m.device_values) for m in seen]
+        if self.family.startswith("p2_"):
+            cargs += [ctypes.c_double(self.params["lam"]), ctypes.c_double(self.params["mu"])]
+        rc = self.fn(start, end, *cargs, ctypes.byref(ex), dev.stream())
+        dev.check(rc, self.spec["entry"][self.variant])
+        self.strategy_used = strategy
+        del keep

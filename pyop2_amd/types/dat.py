@@ -1,0 +1,339 @@
+"""Dat: values on every element of a DataSet (``pyop2/types/dat.py``).
+
+Storage is ``(total_size,)`` when ``cdim == 1`` else ``(total_size, *dim)``,
+C-contiguous (``dat.py:76``), held on the device between par_loops (see
+:mod:`pyop2_amd.types.data_carrier`).  ``data``/``data_ro``/``data_wo`` and
+the ``*_with_halos`` variants keep the reference's semantics and return
+numpy views of the host copy; ``device_data`` exposes the torch tensor.
+"""
+import contextlib
+
+import numpy as np
+
+from pyop2_amd import configuration as conf, exceptions as ex, utils
+from pyop2_amd.types.access import Access
+from pyop2_amd.types.data_carrier import DataCarrier, DeviceStorage
+from pyop2_amd.types.dataset import DataSet
+from pyop2_amd.types.set import Set
+
+
+class AbstractDat(DataCarrier, DeviceStorage):
+
+    _modes = [Access.READ, Access.WRITE, Access.RW, Access.INC, Access.MIN, Access.MAX]
+
+    @utils.validate_type(('dataset', (DataCarrier, DataSet, Set), ex.DataSetTypeError),
+                         ('name', str, ex.NameTypeError))
+    @utils.validate_dtype(('dtype', None, ex.DataTypeError))
+    def __init__(self, dataset, data=None, dtype=None, name=None):
+        if isinstance(dataset, Dat):
+            self.__init__(dataset.dataset, None, dtype=dataset.dtype, name="copy_of_%s" % dataset.name)
+            dataset.copy(self)
+            return
+        if type(dataset) is Set:
+            dataset = dataset ** 1
+        shape = (dataset.total_size,) + (() if dataset.cdim == 1 else dataset.dim)
+        self._storage_init(data, dtype, shape)
+        self._dataset = dataset
+        self.comm = dataset.comm
+        self.halo_valid = True
+        self._name = name or "dat_#x%x" % id(self)
+        self._halo_frozen = False
+        self._frozen_access_mode = None
+
+    @property
+    def _kernel_args_(self):
+        from pyop2_amd import device as dev
+        return (dev.ptr(self._device_array()),)
+
+    @utils.validate_in(('access', _modes, ex.ModeValueError))
+    def __call__(self, access, path=None):
+        from pyop2_amd.parloop import DatLegacyArg
+        if conf.configuration["type_check"] and path and path.toset != self.dataset.set:
+            raise ex.MapValueError("To Set of Map does not match Set of Dat.")
+        return DatLegacyArg(self, path, access)
+
+    def __getitem__(self, idx):
+        if idx != 0:
+            raise ex.IndexValueError("Can only extract component 0 from %r" % self)
+        return self
+
+    @utils.cached_property
+    def split(self):
+        return (self,)
+
+    @property
+    def dataset(self):
+        return self._dataset
+
+    @property
+    def dim(self):
+        return self.dataset.dim
+
+    @property
+    def cdim(self):
+        return self.dataset.cdim
+
+    @property
+    def shape(self):
+        return self._shape
+
+    @property
+    def data(self):
+        """Writable numpy view of the owned values (marks the Dat modified)."""
+        self.increment_dat_version()
+        if self.dataset.total_size > 0 and self._dtype.itemsize == 0:
+            raise RuntimeError("Illegal access: no data associated with this Dat!")
+        self.halo_valid = False
+        a = self._host_array()
+        self._host_modified()
+        v = a[:self.dataset.size].view()
+        v.setflags(write=True)
+        return v
+
+    @property
+    def data_with_halos(self):
+        self.increment_dat_version()
+        self.global_to_local_begin(Access.RW)
+        self.global_to_local_end(Access.RW)
+        self.halo_valid = False
+        a = self._host_array()
+        self._host_modified()
+        v = a.view()
+        v.setflags(write=True)
+        return v
+
+    @property
+    def data_ro(self):
+        """Read-only numpy view of the owned values."""
+        v = self._host_array()[:self.dataset.size].view()
+        v.setflags(write=False)
+        return v
+
+    @property
+    def data_ro_with_halos(self):
+        self.global_to_local_begin(Access.READ)
+        self.global_to_local_end(Access.READ)
+        v = self._host_array().view()
+        v.setflags(write=False)
+        return v
+
+    @property
+    def data_wo(self):
+        return self.data
+
+    @property
+    def data_wo_with_halos(self):
+        self.increment_dat_version()
+        self.halo_valid = False
+        a = self._host_array()
+        self._host_modified()
+        v = a.view()
+        v.setflags(write=True)
+        return v
+
+    def save(self, filename):
+        """Write the data array to file ``filename`` in NumPy format."""
+        np.save(filename, self.data_ro)
+
+    def load(self, filename):
+        """Read values written by :meth:`save`."""
+        if filename[-4:] != ".npy":
+            filename = filename + ".npy"
+        self.data[:] = np.load(filename)
+
+    @property
+    def nbytes(self):
+        return self.dtype.itemsize * self.dataset.total_size * self.dataset.cdim
+
+    def zero(self, subset=None):
+        """Zero the data (on the device when that copy is current)."""
+        self.increment_dat_version()
+        if subset is None:
+            if self._where in (2, 3):   # device copy current: zero it there
+                self._device_array().zero_()
+                self._device_modified()
+            else:
+                self._host_array()[...] = 0
+                self._host_modified()
+            self.halo_valid = True
+        elif subset.superset != self.dataset.set:
+            raise ex.MapValueError("The subset and dataset are incompatible")
+        else:
+            self.data[subset.owned_indices] = 0
+
+    def copy(self, other, subset=None):
+        """Copy the data in this Dat into another."""
+        if other is self:
+            return
+        if subset is None:
+            if self._where in (2, 3) and self.halo_valid:
+                src = self._device_array()
+                dst = other._device_array()
+                dst.copy_(src)
+                other._device_modified()
+                other.halo_valid = True
+            elif self.halo_valid:
+                other._host_array()[...] = self._host_array()
+                other._host_modified()
+                other.halo_valid = True
+            else:
+                other.data[:] = self.data_ro
+        elif subset.superset != self.dataset.set:
+            raise ex.MapValueError("The subset and dataset are incompatible")
+        else:
+            other.data[subset.owned_indices] = self.data_ro[subset.owned_indices]
+
+    def __iter__(self):
+        yield self
+
+    def __len__(self):
+        return 1
+
+    def __str__(self):
+        return "OP2 Dat: %s on (%s) with datatype %s" % (self._name, self._dataset, self.dtype.name)
+
+    def __repr__(self):
+        return "Dat(%r, None, %r, %r)" % (self._dataset, self.dtype, self._name)
+
+    # -- pointwise algebra on the device (torch) ------------------------------
+    def _check_shape(self, other):
+        if other.dataset.dim != self.dataset.dim:
+            raise ValueError('Mismatched shapes in operands %s and %s' % (self.dataset.dim, other.dataset.dim))
+
+    def _operand(self, other):
+        from pyop2_amd.types.glob import Global
+        if isinstance(other, AbstractDat):
+            self._check_shape(other)
+            return other._device_array()[:self.dataset.size]
+        if isinstance(other, Global):
+            return other._device_array()
+        return other
+
+    def _iop(self, other, op):
+        a = self._device_array()[:self.dataset.size]
+        getattr(a, op)(self._operand(other))
+        self._device_modified()
+        self.increment_dat_version()
+        return self
+
+    def _op(self, other, op):
+        ret = type(self)(self.dataset, None, self.dtype)
+        ret._device_array()[:self.dataset.size] = getattr(self._device_array()[:self.dataset.size], op)(self._operand(other))
+        ret._device_modified()
+        return ret
+
+    def __iadd__(self, other):
+        return self._iop(other, "add_")
+
+    def __isub__(self, other):
+        return self._iop(other, "sub_")
+
+    def __imul__(self, other):
+        return self._iop(other, "mul_")
+
+    def __itruediv__(self, other):
+        return self._iop(other, "div_")
+
+    def __add__(self, other):
+        return self._op(other, "add")
+
+    def __radd__(self, other):
+        return self + other
+
+    def __sub__(self, other):
+        return self._op(other, "sub")
+
+    def __mul__(self, other):
+        return self._op(other, "mul")
+
+    def __rmul__(self, other):
+        return self * other
+
+    def __truediv__(self, other):
+        return self._op(other, "div")
+
+    def __neg__(self):
+        return self._op(-1, "mul")
+
+    def inner(self, other):
+        """Inner product of the owned values (a float)."""
+        self._check_shape(other)
+        a = self._device_array()[:self.dataset.size]
+        b = other._device_array()[:self.dataset.size]
+        return float((a * b).sum().item())
+
+    @property
+    def norm(self):
+        """Euclidean norm of the owned values."""
+        return self.inner(self) ** 0.5
+
+    # -- halo exchange hooks (single rank: only the INC halo reset) ---------------
+    def global_to_local_begin(self, access_mode):
+        halo = self.dataset.halo
+        if halo is None or self._halo_frozen:
+            return
+        if not self.halo_valid and access_mode in {Access.READ, Access.RW}:
+            halo.global_to_local_begin(self, Access.WRITE)
+        elif access_mode in {Access.INC, Access.MIN, Access.MAX}:
+            from pyop2_amd.datatypes import dtype_limits
+            min_, max_ = dtype_limits(self.dtype)
+            val = {Access.MAX: min_, Access.MIN: max_, Access.INC: 0}[access_mode]
+            t = self._device_array()
+            t[self.dataset.size:] = val
+            self._device_modified()
+
+    def global_to_local_end(self, access_mode):
+        halo = self.dataset.halo
+        if halo is None or self._halo_frozen:
+            return
+        if not self.halo_valid and access_mode in {Access.READ, Access.RW}:
+            halo.global_to_local_end(self, Access.WRITE)
+            self.halo_valid = True
+        elif access_mode in {Access.INC, Access.MIN, Access.MAX}:
+            self.halo_valid = False
+
+    def local_to_global_begin(self, insert_mode):
+        halo = self.dataset.halo
+        if halo is None or self._halo_frozen:
+            return
+        halo.local_to_global_begin(self, insert_mode)
+
+    def local_to_global_end(self, insert_mode):
+        halo = self.dataset.halo
+        if halo is None or self._halo_frozen:
+            return
+        halo.local_to_global_end(self, insert_mode)
+        self.halo_valid = False
+
+    def frozen_halo(self, access_mode):
+        return frozen_halo(self, access_mode)
+
+    def freeze_halo(self, access_mode):
+        if self._halo_frozen:
+            raise RuntimeError("Expected an unfrozen halo")
+        self._halo_frozen = True
+        self._frozen_access_mode = access_mode
+
+    def unfreeze_halo(self):
+        if not self._halo_frozen:
+            raise RuntimeError("Expected a frozen halo")
+        self._halo_frozen = False
+        self._frozen_access_mode = None
+
+
+class Dat(AbstractDat):
+    """OP2 vector data (see :class:`AbstractDat`)."""
+
+    @contextlib.contextmanager
+    def vec_context(self, access):
+        raise NotImplementedError("PETSc Vec views are not part of the device path")
+
+
+@contextlib.contextmanager
+def frozen_halo(dat, access_mode):
+    """Temporarily disable halo exchanges of ``dat``."""
+    dat.freeze_halo(access_mode)
+    try:
+        yield
+    finally:
+        dat.unfreeze_halo()

@@ -1,0 +1,560 @@
+"""Sparsity and Mat on the device (``pyop2/types/mat.py``).
+
+``Sparsity`` keeps the reference's argument processing, identity cache and
+``nnz``/``onnz`` semantics (``mat.py:26-291``), but the pattern is built on
+the device by the op2hip sparsity builder (the reference fills a PETSc
+preallocator, ``sparsity.pyx:121-284``): the sorted unique union of every
+``rmap[e, :] x cmap[e, :]`` (owned e) plus the diagonal.
+
+``Mat`` is a device CSR matrix, or BSR with ``dim x dim`` blocks when the
+reference would create a PETSc BAIJ (``rdim == cdim > 1`` and
+``block_sparsity``, ``mat.py:685-699``), pre-filled with zeros on the exact
+pattern.  Insertions outside the pattern are counted on the device and
+raised as :class:`PatternError` by ``assemble()`` (PETSc's
+``NEW_NONZERO_LOCATION_ERR``).
+"""
+import ctypes
+import itertools
+from collections.abc import Sequence
+
+import numpy as np
+
+from pyop2_amd import caching, configuration as conf, exceptions as ex, utils
+from pyop2_amd.types.access import Access
+from pyop2_amd.types.data_carrier import DataCarrier
+from pyop2_amd.types.dataset import DataSet
+from pyop2_amd.types.map import Map
+from pyop2_amd.types.set import MixedSet, Subset
+
+
+class Sparsity(caching.ObjectCached):
+    """Non-zero structure of a matrix from pairs of Maps.
+
+    ``Sparsity((row_dset, col_dset), [(rmap, cmap, None), ...])`` or the
+    dict form ``{(0, 0): [(rmap, cmap, None)]}``.
+    """
+
+    def __init__(self, dsets, maps_and_regions, name=None, nest=None, block_sparse=None,
+                 diagonal_block=True):
+        if self._initialized:
+            return
+        self._dsets = dsets
+        self._maps_and_regions = maps_and_regions
+        self._block_sparse = block_sparse
+        self._diagonal_block = diagonal_block
+        rset, cset = self.dsets
+        self.comm = rset.comm
+        self._has_diagonal = (rset == cset) and diagonal_block
+        self._dims = (((rset.cdim, cset.cdim),),)
+        self._name = name or "sparsity_#x%x" % id(self)
+        self._nested = False
+        self._blocks = [[self]]
+        self._pattern = None
+        self._dof_pattern = None
+        self._initialized = True
+
+    _cache = {}
+
+    @classmethod
+    @utils.validate_type(('name', str, ex.NameTypeError))
+    def _process_args(cls, dsets, maps_and_regions, name=None, nest=None, block_sparse=None,
+                      diagonal_block=True):
+        if len(dsets) != 2:
+            raise RuntimeError(f"dsets must be a tuple of two DataSets: got {dsets}")
+        for dset in dsets:
+            if not isinstance(dset, DataSet) and dset is not None:
+                raise ex.DataSetTypeError("All data sets must be of type DataSet, not type %r" % type(dset))
+            if isinstance(getattr(dset, "set", None), MixedSet):
+                raise NotImplementedError("mixed sparsities are not on the device path")
+        if isinstance(maps_and_regions, Sequence):
+            maps_and_regions = {(0, 0): maps_and_regions}
+        elif not isinstance(maps_and_regions, dict):
+            raise TypeError(f"maps_and_regions must be dict or Sequence: got {type(maps_and_regions)}")
+        processed = {(0, 0): frozenset()}
+        for key, val in maps_and_regions.items():
+            i, j = key
+            if i >= 1 or j >= 1:
+                raise RuntimeError(f"(i, j) must be < (1, 1): got {(i, j)}")
+            pv = set()
+            for rmap, cmap, iteration_regions in set(val):
+                for m in (rmap, cmap):
+                    if not isinstance(m, Map):
+                        raise ex.MapTypeError("All maps must be of type map, not type %r" % type(m))
+                    if len(m.values_with_halo) == 0 and m.iterset.total_size > 0:
+                        raise ex.MapValueError("Unpopulated map values when trying to build sparsity.")
+                if rmap.toset is not dsets[0].set or cmap.toset is not dsets[1].set:
+                    raise RuntimeError("Map toset must be the same as DataSet set")
+                if rmap.iterset is not cmap.iterset:
+                    raise RuntimeError("Iterset of both maps in a pair must be the same")
+                if iteration_regions is not None:
+                    raise NotImplementedError("iteration regions (extruded) are not on the device path")
+                pv.add((rmap, cmap, None))
+            if pv:
+                processed[key] = frozenset(pv)
+        cache = dsets[0].set
+        if nest is None:
+            nest = conf.configuration["matnest"]
+        if block_sparse is None:
+            block_sparse = conf.configuration["block_sparsity"]
+        kwargs = {"name": name, "nest": nest, "block_sparse": block_sparse,
+                  "diagonal_block": diagonal_block}
+        return (cache,) + (tuple(dsets), processed), kwargs
+
+    @classmethod
+    def _cache_key(cls, dsets, maps_and_regions, name, nest, block_sparse, diagonal_block, *args, **kwargs):
+        return (dsets, tuple(maps_and_regions.items()), nest, block_sparse)
+
+    def __getitem__(self, idx):
+        try:
+            i, j = idx
+            return self._blocks[i][j]
+        except TypeError:
+            return self._blocks[idx]
+
+    @property
+    def dsets(self):
+        return self._dsets
+
+    @utils.cached_property
+    def rcmaps(self):
+        return {key: [(r, c) for r, c, _ in sorted(val, key=lambda t: (id(t[0]), id(t[1])))]
+                for key, val in self._maps_and_regions.items()}
+
+    @utils.cached_property
+    def iteration_regions(self):
+        return {key: [reg for _, _, reg in val] for key, val in self._maps_and_regions.items()}
+
+    @property
+    def dims(self):
+        return self._dims
+
+    @utils.cached_property
+    def shape(self):
+        return (1, 1)
+
+    @property
+    def nested(self):
+        return self._nested
+
+    @property
+    def name(self):
+        return self._name
+
+    def __iter__(self):
+        for row in self._blocks:
+            for s in row:
+                yield s
+
+    def __str__(self):
+        return "OP2 Sparsity: dsets %s, maps_and_regions %s, name %s" % (self._dsets, self._maps_and_regions, self._name)
+
+    def __repr__(self):
+        return "Sparsity(%r, %r, name=%r)" % (self.dsets, self._maps_and_regions, self.name)
+
+    def __contains__(self, other):
+        rm, cm = other[0], other[1]
+        return any(r is rm and c is cm for r, c in self.rcmaps[(0, 0)])
+
+    # -- the pattern -----------------------------------------------------------
+    @property
+    def nrows(self):
+        """Block rows (the row Set's elements)."""
+        return self._dsets[0].set.total_size
+
+    @property
+    def ncols(self):
+        return self._dsets[1].set.total_size
+
+    def _build(self):
+        if self._pattern is not None:
+            return self._pattern
+        from pyop2_amd import device as dev, plans
+        L = dev.lib()
+        s = dev.stream()
+        pairs = self.rcmaps[(0, 0)]
+        nrows, ncols = self.nrows, self.ncols
+        arr = (dev.MapPair * max(len(pairs), 1))()
+        incp, ince = [], []
+        for k, (rmap, cmap) in enumerate(pairs):
+            size = rmap.iterset.size
+            tr = plans.transpose(rmap, nrows, size)
+            incp.append(tr.inc_ptr)
+            ince.append(tr.inc_ent)
+            arr[k] = dev.MapPair(dev.ptr(rmap.device_values), dev.ptr(cmap.device_values),
+                                 rmap.arity, cmap.arity, size)
+        ip = (ctypes.c_void_p * max(len(pairs), 1))(*[dev.ptr(t) for t in incp])
+        ie = (ctypes.c_void_p * max(len(pairs), 1))(*[dev.ptr(t) for t in ince])
+        rowptr = dev.empty((nrows + 1,), np.int32)
+        need = ctypes.c_size_t()
+        dev.check(L.op2hip_scan_workspace(nrows, ctypes.byref(need)), "scan_workspace")
+        work = dev.workspace(need.value + 256 + 4 * (nrows + 64))
+        nnz = ctypes.c_int64()
+        dev.check(L.op2hip_sparsity_count(nrows, ncols, arr, len(pairs), ip, ie,
+                                          int(self._has_diagonal), dev.ptr(rowptr),
+                                          ctypes.byref(nnz), dev.ptr(work), work.numel(), s),
+                  "sparsity_count")
+        colidx = dev.empty((max(nnz.value, 1),), np.int32)
+        dev.check(L.op2hip_sparsity_fill(nrows, ncols, arr, len(pairs), ip, ie,
+                                         int(self._has_diagonal), dev.ptr(rowptr),
+                                         dev.ptr(colidx), s), "sparsity_fill")
+        self._pattern = (rowptr, colidx[:nnz.value], int(nnz.value))
+        return self._pattern
+
+    @property
+    def block_pattern(self):
+        """Device ``(rowptr, colidx)`` of the block pattern (int32)."""
+        rowptr, colidx, _ = self._build()
+        return rowptr, colidx
+
+    @property
+    def block_nnz(self):
+        return self._build()[2]
+
+    def dof_pattern(self):
+        """Device ``(rowptr, colidx)`` of the dof (scalar) pattern."""
+        rdim, cdim = self._dims[0][0]
+        if rdim == 1 and cdim == 1:
+            return self.block_pattern
+        if self._dof_pattern is None:
+            from pyop2_amd import device as dev
+            rowptr, colidx, nnz = self._build()
+            rp = dev.empty((self.nrows * rdim + 1,), np.int32)
+            ci = dev.empty((max(nnz * rdim * cdim, 1),), np.int32)
+            dev.check(dev.lib().op2hip_pattern_expand(self.nrows, dev.ptr(rowptr), dev.ptr(colidx),
+                                                      rdim, cdim, dev.ptr(rp), dev.ptr(ci),
+                                                      dev.stream()), "pattern_expand")
+            self._dof_pattern = (rp, ci[:nnz * rdim * cdim])
+        return self._dof_pattern
+
+    @utils.cached_property
+    def nnz(self):
+        """Non-zeros per row as the reference reports them (block rows when
+        block-sparse with square blocks, else dof rows)."""
+        from pyop2_amd import device as dev
+        rowptr, _, _ = self._build()
+        nnz = np.diff(dev.to_host(rowptr)).astype(np.int32)
+        rdim, cdim = self._dims[0][0]
+        if not (self._block_sparse and rdim == cdim):
+            nnz = np.repeat(nnz * cdim, rdim)
+        return nnz
+
+    @utils.cached_property
+    def onnz(self):
+        return np.zeros_like(self.nnz)
+
+    @utils.cached_property
+    def nz(self):
+        return int(self.nnz.sum())
+
+    @utils.cached_property
+    def onz(self):
+        return 0
+
+
+class Mat(DataCarrier):
+    """Device matrix on a Sparsity (CSR, or BSR for square vector blocks)."""
+
+    ASSEMBLED = "ASSEMBLED"
+    INSERT_VALUES = "INSERT_VALUES"
+    ADD_VALUES = "ADD_VALUES"
+
+    _modes = [Access.WRITE, Access.INC]
+
+    @utils.validate_type(('sparsity', Sparsity, ex.SparsityTypeError),
+                         ('name', str, ex.NameTypeError))
+    def __init__(self, sparsity, dtype=None, name=None, mat_type=None):
+        self._sparsity = sparsity
+        self.comm = sparsity.comm
+        self._datatype = np.dtype(dtype or np.float64)
+        if self._datatype not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise RuntimeError("Can only create a matrix of type float64 (or float32), %s is not supported"
+                               % self._datatype)
+        self._name = name or "mat_#x%x" % id(self)
+        self.mat_type = mat_type
+        rdim, cdim = self.dims[0][0]
+        self._block = bool(rdim == cdim and rdim > 1 and sparsity._block_sparse)
+        self._values = None
+        self._status = None
+        self._desc = None
+        self.assembly_state = Mat.ASSEMBLED
+
+    def _init(self):
+        if self._values is not None:
+            return
+        from pyop2_amd import device as dev
+        rdim, cdim = self.dims[0][0]
+        if self._block:
+            rowptr, colidx = self._sparsity.block_pattern
+            bs = rdim * cdim
+            nrows, ncols, brd, bcd = self._sparsity.nrows, self._sparsity.ncols, rdim, cdim
+        else:
+            rowptr, colidx = self._sparsity.dof_pattern()
+            bs = 1
+            nrows, ncols, brd, bcd = self._sparsity.nrows * rdim, self._sparsity.ncols * cdim, 1, 1
+        nnz = colidx.numel()
+        self._rowptr, self._colidx = rowptr, colidx
+        self._values = dev.zeros((max(nnz * bs, 1),), self._datatype)
+        self._status = dev.zeros((1,), np.int32)
+        self._nnz = nnz
+        self._bs = bs
+        d = dev.MatDesc()
+        d.rowptr, d.colidx, d.values = dev.ptr(rowptr), dev.ptr(colidx), dev.ptr(self._values)
+        d.nrows, d.ncols, d.rdim, d.cdim = nrows, ncols, brd, bcd
+        d.status = dev.ptr(self._status)
+        d.inc_ptr = d.inc_ent = None
+        self._desc = d
+
+    # -- device views -------------------------------------------------------------
+    @property
+    def rowptr(self):
+        self._init()
+        return self._rowptr
+
+    @property
+    def colidx(self):
+        self._init()
+        return self._colidx
+
+    @property
+    def device_values(self):
+        """Device values tensor: nnz blocks of ``bs`` entries (row-major)."""
+        self._init()
+        return self._values
+
+    @property
+    def block_shape(self):
+        rdim, cdim = self.dims[0][0]
+        return (rdim, cdim) if self._block else (1, 1)
+
+    def descriptor(self, inc_ptr=None, inc_ent=None):
+        """``op2hip_mat`` for the C ABI (optionally with row incidences)."""
+        from pyop2_amd import device as dev
+        self._init()
+        d = dev.MatDesc.from_buffer_copy(self._desc)
+        d.inc_ptr = dev.ptr(inc_ptr) if inc_ptr is not None else None
+        d.inc_ent = dev.ptr(inc_ent) if inc_ent is not None else None
+        return d
+
+    @property
+    def _kernel_args_(self):
+        self._init()
+        return (ctypes.addressof(self._desc),)
+
+    # -- API ------------------------------------------------------------------------
+    @utils.validate_in(('access', _modes, ex.ModeValueError))
+    def __call__(self, access, path, lgmaps=None, unroll_map=False):
+        from pyop2_amd.parloop import MatLegacyArg
+        path_maps = utils.as_tuple(path, Map, 2)
+        if conf.configuration["type_check"] and tuple(path_maps) not in self.sparsity:
+            raise ex.MapValueError("Path maps not in sparsity maps")
+        if lgmaps is not None:
+            raise NotImplementedError("lgmaps are not supported on the device path")
+        return MatLegacyArg(self, path_maps, access, lgmaps, unroll_map)
+
+    @property
+    def sparsity(self):
+        return self._sparsity
+
+    @property
+    def dims(self):
+        return self._sparsity._dims
+
+    @property
+    def dtype(self):
+        return self._datatype
+
+    @property
+    def nrows(self):
+        return self._sparsity.nrows * self.dims[0][0][0]
+
+    @property
+    def ncols(self):
+        return self._sparsity.ncols * self.dims[0][0][1]
+
+    @property
+    def nblock_rows(self):
+        return self._sparsity.nrows
+
+    @property
+    def nblock_cols(self):
+        return self._sparsity.ncols
+
+    @property
+    def nblocks(self):
+        return 1
+
+    @property
+    def is_mixed(self):
+        return False
+
+    @property
+    def nbytes(self):
+        rdim, cdim = self.dims[0][0]
+        mult = rdim * cdim if self._sparsity._block_sparse else 1
+        return (self._sparsity.nz + self._sparsity.onz) * self.dtype.itemsize * mult
+
+    def __iter__(self):
+        yield self
+
+    def __getitem__(self, idx):
+        return self
+
+    @property
+    def blocks(self):
+        return [[self]]
+
+    def change_assembly_state(self, new_state):
+        if new_state == Mat.ASSEMBLED or self.assembly_state == Mat.ASSEMBLED:
+            self.assembly_state = new_state
+        elif new_state != self.assembly_state:
+            self._flush_assembly()
+            self.assembly_state = new_state
+
+    def _flush_assembly(self):
+        """Device writes are ordered on the stream: nothing to flush."""
+
+    def _check_status(self):
+        from pyop2_amd import device as dev
+        if self._status is None:
+            return
+        n = int(self._status.item())
+        if n:
+            self._status.zero_()
+            raise ex.PatternError(f"{n} Mat insertions fell outside the sparsity pattern of {self.name}")
+
+    def assemble(self):
+        """Finalise the matrix (checks that every insertion hit the pattern)."""
+        self._init()
+        self._check_status()
+        self.change_assembly_state(Mat.ASSEMBLED)
+
+    def zero(self):
+        """Zero every stored value (the pattern is kept)."""
+        self._init()
+        self._values.zero_()
+
+    def zero_rows(self, rows, diag_val=1.0):
+        """Zero rows (dof rows) except the diagonal, set to ``diag_val``."""
+        self.assemble()
+        rows = rows.indices if isinstance(rows, Subset) else rows
+        rows = np.asarray(rows, dtype=np.int64).ravel()
+        rp, ci, vals = self._host_csr()
+        rd, cd = self.block_shape
+        for r in rows:
+            br, ri = divmod(int(r), rd)
+            for s in range(rp[br], rp[br + 1]):
+                blk = vals[s]
+                blk[ri, :] = 0.0
+                if ci[s] * cd <= r < (ci[s] + 1) * cd:
+                    blk[ri, r - ci[s] * cd] = diag_val
+        self._values.copy_(self._to_dev_values(vals))
+
+    def set_local_diagonal_entries(self, rows, diag_val=1.0, idx=None):
+        rows = np.asarray(rows, dtype=np.int64)
+        rbs, _ = self.dims[0][0]
+        if rbs > 1:
+            rows = rbs * rows + idx if idx is not None else np.dstack([rbs * rows + i for i in range(rbs)]).flatten()
+        if len(rows):
+            self._set_entries(rows, rows, np.full(len(rows), diag_val), add=False)
+
+    def addto_values(self, rows, cols, values):
+        """Add a dense block of values at (block) ``rows`` x ``cols``."""
+        self._blocked_entries(rows, cols, values, add=True)
+
+    def set_values(self, rows, cols, values):
+        """Set a dense block of values at (block) ``rows`` x ``cols``."""
+        self._blocked_entries(rows, cols, values, add=False)
+
+    def _blocked_entries(self, rows, cols, values, add):
+        rd, cd = self.dims[0][0]
+        rows = np.atleast_1d(np.asarray(rows, dtype=np.int64))
+        cols = np.atleast_1d(np.asarray(cols, dtype=np.int64))
+        vals = np.asarray(values, dtype=self.dtype).reshape(len(rows) * rd, len(cols) * cd)
+        rr, cc, vv = [], [], []
+        for i, r in enumerate(rows):
+            for ri in range(rd):
+                for j, c in enumerate(cols):
+                    for cj in range(cd):
+                        rr.append(r * rd + ri)
+                        cc.append(c * cd + cj)
+                        vv.append(vals[i * rd + ri, j * cd + cj])
+        if vv:
+            self._set_entries(np.array(rr), np.array(cc), np.array(vv), add)
+
+    def _set_entries(self, rows, cols, vals, add):
+        rp, ci, v = self._host_csr()
+        rd, cd = self.block_shape
+        for r, c, x in zip(rows, cols, vals):
+            br, ri = divmod(int(r), rd)
+            bc, cj = divmod(int(c), cd)
+            lo, hi = rp[br], rp[br + 1]
+            k = lo + np.searchsorted(ci[lo:hi], bc)
+            if k >= hi or ci[k] != bc:
+                raise ex.PatternError(f"entry ({r}, {c}) is not in the sparsity pattern")
+            if add:
+                v[k][ri, cj] += x
+            else:
+                v[k][ri, cj] = x
+        self._values.copy_(self._to_dev_values(v))
+
+    def _to_dev_values(self, v):
+        from pyop2_amd import device as dev
+        return dev.to_device(np.ascontiguousarray(v).reshape(-1))[:self._values.numel()]
+
+    def _host_csr(self):
+        """Host ``(rowptr, colidx, values[nnz, rd, cd])`` copies."""
+        from pyop2_amd import device as dev
+        self._init()
+        rd, cd = self.block_shape
+        rp = dev.to_host(self._rowptr)
+        ci = dev.to_host(self._colidx)
+        v = dev.to_host(self._values)[:self._nnz * rd * cd].reshape(self._nnz, rd, cd).copy()
+        return rp, ci, v
+
+    def csr(self):
+        """Host scalar CSR ``(rowptr, colidx, values)`` (BSR expanded)."""
+        rp, ci, v = self._host_csr()
+        rd, cd = self.block_shape
+        if rd == 1 and cd == 1:
+            return rp, ci, v.reshape(-1)
+        nb = len(rp) - 1
+        rows, cols, vals = [], [], []
+        for r in range(nb):
+            for s in range(rp[r], rp[r + 1]):
+                for a in range(rd):
+                    for b in range(cd):
+                        rows.append(r * rd + a)
+                        cols.append(ci[s] * cd + b)
+                        vals.append(v[s, a, b])
+        order = np.lexsort((np.array(cols), np.array(rows)))
+        rows, cols, vals = np.array(rows)[order], np.array(cols)[order], np.array(vals)[order]
+        srp = np.zeros(nb * rd + 1, dtype=np.int64)
+        np.cumsum(np.bincount(rows, minlength=nb * rd), out=srp[1:])
+        return srp, cols, vals
+
+    @property
+    def values(self):
+        """Dense numpy copy (refused above 1e6 entries, as in the reference)."""
+        self.assemble()
+        if self.nrows * self.ncols > 1000000:
+            raise ValueError("Printing dense matrix with more than 1 million entries not allowed.\n"
+                             "Are you sure you wanted to do this?")
+        rp, ci, v = self._host_csr()
+        rd, cd = self.block_shape
+        out = np.zeros((self.nrows, self.ncols), dtype=self.dtype)
+        for r in range(len(rp) - 1):
+            for s in range(rp[r], rp[r + 1]):
+                out[r * rd:(r + 1) * rd, ci[s] * cd:(ci[s] + 1) * cd] += v[s]
+        return out
+
+    @property
+    def dat_version(self):
+        if self.assembly_state != Mat.ASSEMBLED:
+            raise RuntimeError("Should not ask for state counter if the matrix is not assembled.")
+        return 0
+
+    def __str__(self):
+        return "OP2 Mat: %s, sparsity (%s), datatype %s" % (self._name, self._sparsity, self._datatype.name)
+
+    def __repr__(self):
+        return "Mat(%r, %r, %r)" % (self._sparsity, self._datatype, self._name)

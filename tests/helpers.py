@@ -1,0 +1,94 @@
+"""Shared helpers of the tests: op2 objects for the synthetic meshes and the
+oracle's answer for the same inputs."""
+import numpy as np
+
+from oracle import seqwrap, sparsity as osp
+from pyop2_amd import elements, op2
+
+
+def relerr(a, b):
+    """Normwise relative error ||a - b|| / ||b||."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+class TriProblem:
+    """op2 objects of a P1 triangle mesh (optionally with a core/owned split)."""
+
+    def __init__(self, mesh, core=None):
+        self.m = mesh
+        n = mesh.ncells
+        sizes = n if core is None else (core, n, n)
+        self.cells = op2.Set(sizes, "cells")
+        self.verts = op2.Set(mesh.nverts, "verts")
+        self.c2v = op2.Map(self.cells, self.verts, 3, mesh.cell_vertex, "c2v")
+        self.X = op2.Dat(self.verts ** 2, mesh.coords, name="X")
+        self.u = op2.Dat(self.verts, mesh.u, name="u")
+
+    def sparsity(self):
+        return op2.Sparsity((self.verts ** 1, self.verts ** 1), [(self.c2v, self.c2v, None)])
+
+
+class TetProblem:
+    """op2 objects of a P2 tetrahedral mesh."""
+
+    def __init__(self, mesh, dtype=np.float64, core=None):
+        self.m = mesh
+        n = mesh.ncells
+        sizes = n if core is None else (core, n, n)
+        self.cells = op2.Set(sizes, "cells")
+        self.verts = op2.Set(mesh.nverts, "verts")
+        self.nodes = op2.Set(mesh.nnodes, "nodes")
+        self.c2v = op2.Map(self.cells, self.verts, 4, mesh.cell_vertex, "c2v")
+        self.c2n = op2.Map(self.cells, self.nodes, 10, mesh.cell_node, "c2n")
+        self.X = op2.Dat(self.verts ** 3, mesh.coords.astype(dtype), dtype, name="X")
+        self.u = op2.Dat(self.nodes ** 3, mesh.u.astype(dtype), dtype, name="u")
+
+    def sparsity(self):
+        return op2.Sparsity((self.nodes ** 3, self.nodes ** 3), [(self.c2n, self.c2n, None)])
+
+
+def oracle_p1(family, mesh):
+    """Oracle result of a P1 family on ``mesh`` (numpy)."""
+    code = elements.kernel_source(family)
+    cv = mesh.cell_vertex
+    if family == "p1_lumped_mass":
+        out = np.zeros(mesh.nverts)
+        seqwrap.run(code, family, [seqwrap.DatArg(out, "INC", cv), seqwrap.DatArg(mesh.coords, "READ", cv)],
+                    [(0, mesh.ncells)])
+        return out
+    if family == "p1_laplace_residual":
+        out = np.zeros(mesh.nverts)
+        seqwrap.run(code, family, [seqwrap.DatArg(out, "INC", cv), seqwrap.DatArg(mesh.coords, "READ", cv),
+                                   seqwrap.DatArg(mesh.u, "READ", cv)], [(0, mesh.ncells)])
+        return out
+    if family == "p1_laplace_matrix":
+        rp, ci = osp.pattern(mesh.nverts, mesh.nverts, [(cv, cv, mesh.ncells)])
+        A = seqwrap.CSRMat(rp, ci, np.zeros(len(ci)))
+        err = seqwrap.run(code, family, [seqwrap.MatArg(A, "INC", cv, cv), seqwrap.DatArg(mesh.coords, "READ", cv)],
+                          [(0, mesh.ncells)])
+        assert err == 0
+        return A
+    raise ValueError(family)
+
+
+def oracle_p2(family, mesh, lam=1.0, mu=0.5, u=None, coords=None):
+    code = elements.kernel_source(family, lam=lam, mu=mu)
+    cv, cn = mesh.cell_vertex, mesh.cell_node
+    X = mesh.coords if coords is None else coords
+    uu = mesh.u if u is None else u
+    if family == "p2_elasticity_residual":
+        out = np.zeros((mesh.nnodes, 3))
+        seqwrap.run(code, family, [seqwrap.DatArg(out, "INC", cn), seqwrap.DatArg(X, "READ", cv),
+                                   seqwrap.DatArg(uu, "READ", cn)], [(0, mesh.ncells)])
+        return out
+    if family == "p2_elasticity_matrix":
+        rp, ci = osp.pattern(mesh.nnodes, mesh.nnodes, [(cn, cn, mesh.ncells)])
+        A = seqwrap.CSRMat(rp, ci, np.zeros((len(ci), 3, 3)), 3, 3, block=True)
+        err = seqwrap.run(code, family, [seqwrap.MatArg(A, "INC", cn, cn, 3, 3), seqwrap.DatArg(X, "READ", cv)],
+                          [(0, mesh.ncells)])
+        assert err == 0
+        return A
+    raise ValueError(family)

@@ -1,0 +1,77 @@
+"""P2 tetrahedral elasticity loops on the device against the oracle.
+
+FP64: 1e-12 normwise; FP32: 1e-5 normwise against the FP64 oracle (the
+reference Mat is FP64-only, so FP32 has no reference counterpart).
+"""
+import numpy as np
+import pytest
+
+from helpers import TetProblem, oracle_p2, relerr
+from pyop2_amd import elements, op2
+from pyop2_amd.configuration import configuration
+from pyop2_amd.meshes import unit_cube_tet_p2, workload_rng
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tet8():
+    return unit_cube_tet_p2(8, rng=workload_rng(3))
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+@pytest.mark.parametrize("split", ["all", "half"])
+def test_p2_residual_f64(device, tet8, strategy, split):
+    configuration["strategy"] = strategy
+    core = None if split == "all" else tet8.ncells // 2
+    p = TetProblem(tet8, core=core)
+    R = op2.Dat(p.nodes ** 3, name="R")
+    op2.par_loop(elements.p2_elasticity_residual(), p.cells, R(op2.INC, p.c2n), p.X(op2.READ, p.c2v),
+                 p.u(op2.READ, p.c2n))
+    assert relerr(R.data_ro, oracle_p2("p2_elasticity_residual", tet8)) < 1e-12
+
+
+def test_p2_residual_f32(device, tet8):
+    p = TetProblem(tet8, dtype=np.float32)
+    R = op2.Dat(p.nodes ** 3, dtype=np.float32, name="R")
+    op2.par_loop(elements.p2_elasticity_residual(), p.cells, R(op2.INC, p.c2n), p.X(op2.READ, p.c2v),
+                 p.u(op2.READ, p.c2n))
+    ref = oracle_p2("p2_elasticity_residual", tet8, coords=tet8.coords.astype(np.float32).astype(np.float64),
+                    u=tet8.u.astype(np.float32).astype(np.float64))
+    assert relerr(R.data_ro, ref) < 1e-5
+
+
+@pytest.mark.parametrize("strategy", ["rowgather", "atomic", "coloured"])
+def test_p2_bsr_assembly_f64(device, tet8, strategy):
+    configuration["strategy"] = strategy
+    p = TetProblem(tet8)
+    A = op2.Mat(p.sparsity(), np.float64)
+    op2.par_loop(elements.p2_elasticity_matrix(), p.cells, A(op2.INC, (p.c2n, p.c2n)), p.X(op2.READ, p.c2v))
+    A.assemble()
+    ref = oracle_p2("p2_elasticity_matrix", tet8)
+    rp, ci, v = A._host_csr()
+    assert np.array_equal(rp, ref.rowptr) and np.array_equal(ci, ref.colidx)
+    assert relerr(v, ref.values) < 1e-12
+
+
+def test_p2_fused_f64(device, tet8):
+    p = TetProblem(tet8)
+    A = op2.Mat(p.sparsity(), np.float64)
+    R = op2.Dat(p.nodes ** 3, name="R")
+    op2.par_loop(elements.p2_elasticity_fused(), p.cells, A(op2.INC, (p.c2n, p.c2n)), R(op2.INC, p.c2n),
+                 p.X(op2.READ, p.c2v), p.u(op2.READ, p.c2n))
+    A.assemble()
+    ref = oracle_p2("p2_elasticity_matrix", tet8)
+    _, _, v = A._host_csr()
+    assert relerr(v, ref.values) < 1e-12
+    assert relerr(R.data_ro, oracle_p2("p2_elasticity_residual", tet8)) < 1e-12
+
+
+def test_p2_bsr_assembly_f32(device, tet8):
+    p = TetProblem(tet8, dtype=np.float32)
+    A = op2.Mat(p.sparsity(), np.float32)
+    op2.par_loop(elements.p2_elasticity_matrix(), p.cells, A(op2.INC, (p.c2n, p.c2n)), p.X(op2.READ, p.c2v))
+    A.assemble()
+    ref = oracle_p2("p2_elasticity_matrix", tet8, coords=tet8.coords.astype(np.float32).astype(np.float64))
+    _, _, v = A._host_csr()
+    assert relerr(v, ref.values) < 1e-5
