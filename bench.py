@@ -1,0 +1,307 @@
+"""Benchmark of the indirect INC par_loop and CSR assembly on one MI355X.
+
+One step = one ``op2.par_loop`` INC of the 3x3 P1 Laplace element matrices
+into the CSR Mat of the 1024x1024 unit square (2,097,152 cells, 1,050,625
+vertices, randomly permuted vertex numbering), inputs resident in HBM.
+``value`` is cells/s over the timed steps.  The same run also measures, with
+HIP events around each launch: the P1 lumped mass / Laplace residual on the
+256^2 square, the P2 tetrahedral elasticity residual, BSR(3) assembly and
+fused residual + assembly on the 64^3 cube (float64 and float32), and the
+adjoint/forward time ratio of the P2 residual loop.  ``cpu_baseline`` times
+the oracle's restatement of the reference's sequential wrapper, built with
+the reference's flags, on one host core.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cells/s + eff. HBM GB/s: indirect INC par_loop & CSR assembly; adjoint/fwd ratio"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_ms(timer, name):
+    t = timer.times(name)
+    return float(np.mean(t)) if t else float("nan")
+
+
+def time_loop(pl, reps, name):
+    """Average kernel duration (HIP events) of ``reps`` executions of ``pl``."""
+    from pyop2_amd import device as dev
+    pl()
+    dev.torch().cuda.synchronize()
+    with dev.launch_timer() as t:
+        for _ in range(reps):
+            pl()
+    dev.torch().cuda.synchronize()
+    return kernel_ms(t, name)
+
+
+def cpu_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count()
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def c2_bytes(m, nnz):
+    return (m.ncells * 3 * 4 + m.nverts * 2 * 8 + nnz * 8 * 2 + nnz * 4 + (m.nverts + 1) * 4)
+
+
+def run(args):
+    from pyop2_amd import device as dev, elements, op2
+    from pyop2_amd.configuration import configuration
+    from pyop2_amd.meshes import unit_cube_tet_p2, unit_square_tri, workload_rng
+    torch = dev.torch()
+    d = dev.device()
+    gpu_name = torch.cuda.get_device_name(d)
+    log(f"device {d} {gpu_name}")
+    configuration["strategy"] = args.strategy
+    out = {}
+
+    # ---- primary: P1 Laplacian CSR assembly (1024^2, permuted) ----------------------
+    n2 = 256 if args.quick else 1024
+    m2 = unit_square_tri(n2, rng=workload_rng(2), permute=True)
+    cells = op2.Set(m2.ncells, "cells")
+    verts = op2.Set(m2.nverts, "verts")
+    c2v = op2.Map(cells, verts, 3, m2.cell_vertex, "c2v")
+    X = op2.Dat(verts ** 2, m2.coords, name="X")
+    t0 = time.time()
+    sp = op2.Sparsity((verts ** 1, verts ** 1), [(c2v, c2v, None)])
+    nnz = sp.block_nnz
+    torch.cuda.synchronize()
+    t_sp = time.time() - t0
+    A = op2.Mat(sp, np.float64)
+    kern = elements.p1_laplace_matrix()
+    step = op2.ParLoop(kern, cells, A(op2.INC, (c2v, c2v)), X(op2.READ, c2v))
+    log(f"C2 mesh {n2}^2: {m2.ncells} cells, nnz {nnz}, sparsity build {t_sp * 1e3:.1f} ms (host clock)")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    with dev.launch_timer() as timer:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    kms = kernel_ms(timer, "p1_laplace_matrix")
+    strategy_used = step.global_kernel.compile().strategy_used
+    ms_per_step = wall * 1e3 / args.steps
+    cells_per_s = m2.ncells / (wall / args.steps)
+    alg = c2_bytes(m2, nnz)
+    achieved = alg / (kms * 1e-3) / 1e9
+    log(f"C2 assembly: {ms_per_step:.3f} ms/step, kernel {kms * 1e3:.1f} us ({strategy_used}), "
+        f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
+
+    # parity of the timed Mat against the oracle (values scale with the step count)
+    from oracle import seqwrap, sparsity as osp
+    A.assemble()
+    rp, ci = osp.pattern(m2.nverts, m2.nverts, [(m2.cell_vertex, m2.cell_vertex, m2.ncells)])
+    ref = seqwrap.CSRMat(rp, ci, np.zeros(len(ci)))
+    code = elements.kernel_source("p1_laplace_matrix")
+    cargs = [seqwrap.MatArg(ref, "INC", m2.cell_vertex, m2.cell_vertex),
+             seqwrap.DatArg(m2.coords, "READ", m2.cell_vertex)]
+    seqwrap.run(code, "p1_laplace_matrix", cargs, [(0, m2.ncells)])
+    grp, gci, gv = A.csr()
+    nrun = args.steps + args.warmup
+    pattern_exact = bool(np.array_equal(grp, rp) and np.array_equal(gci, ci))
+    c2_err = float(np.linalg.norm(gv / nrun - ref.values) / np.linalg.norm(ref.values))
+    log(f"C2 parity: pattern bit-exact {pattern_exact}, values normwise rel err {c2_err:.2e}")
+
+    # CPU baseline: the oracle's sequential wrapper with the reference's flags
+    cpu = None
+    if not args.no_cpu:
+        ref.values[:] = 0
+        seqwrap.run(code, "p1_laplace_matrix", cargs, [(0, m2.ncells)], flavour="fast")
+        ts = []
+        budget = time.time() + 20.0
+        while len(ts) < 5 and (time.time() < budget or len(ts) < 2):
+            t0 = time.perf_counter()
+            seqwrap.run(code, "p1_laplace_matrix", cargs, [(0, m2.ncells)], flavour="fast")
+            ts.append(time.perf_counter() - t0)
+        tcpu = float(np.median(ts))
+        cpu = {"value": m2.ncells / tcpu, "unit": "cells/s", "cores": 1, "kind": "port",
+               "sample": f"full {n2}^2 workload ({m2.ncells} cells), median of {len(ts)} runs of the "
+                         f"oracle's sequential wrapper (gcc -march=native -O3 -ffast-math) on one core of "
+                         f"{cpu_model()} ({cpu_cores()} cores visible)"}
+        log(f"C2 CPU baseline: {tcpu * 1e3:.1f} ms per pass = {cpu['value']:.3e} cells/s "
+            f"(GPU step {m2.ncells / (kms * 1e-3) / cpu['value']:.0f}x faster by kernel time)")
+    out["c2"] = dict(cells=m2.ncells, nnz=nnz, kernel_us=kms * 1e3, ms_per_step=ms_per_step,
+                     strategy=strategy_used, alg_bytes=alg, gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+                     pattern_bit_exact=pattern_exact, relerr_vs_cpu=c2_err,
+                     sparsity_build_ms_host=t_sp * 1e3)
+    del A, sp
+
+    if not args.primary_only:
+        out.update(secondary(args))
+
+    result = {
+        "metric": METRIC,
+        "value": cells_per_s,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"P1 Laplacian CSR assembly: par_loop INC of 3x3 element matrices into a "
+                               f"float64 CSR Mat, {n2}x{n2} unit square ({m2.ncells} cells, {m2.nverts} "
+                               f"vertices, {nnz} nnz), randomly permuted vertex numbering",
+                   "strategy": strategy_used, "device": gpu_name},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "rowgather_kernel<P1MatrixRow>" if strategy_used == "rowgather"
+                     else f"p1_matrix_kernel ({strategy_used})",
+                     "kernel_us": kms * 1e3, "alg_bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+        "secondary": out,
+    }
+    return result
+
+
+def secondary(args):
+    """Configs 1, 3 and 4 (kernel-time measurements)."""
+    from pyop2_amd import device as dev, elements, op2
+    from pyop2_amd.meshes import unit_cube_tet_p2, unit_square_tri, workload_rng
+    torch = dev.torch()
+    reps = args.reps
+    res = {}
+
+    # C1: P1 lumped mass and Laplace residual, 256^2
+    m1 = unit_square_tri(256, rng=workload_rng(1))
+    cells = op2.Set(m1.ncells, "cells")
+    verts = op2.Set(m1.nverts, "verts")
+    c2v = op2.Map(cells, verts, 3, m1.cell_vertex, "c2v")
+    X = op2.Dat(verts ** 2, m1.coords, name="X")
+    u = op2.Dat(verts, m1.u, name="u")
+    M = op2.Dat(verts, name="M")
+    R = op2.Dat(verts, name="R")
+    pm = op2.ParLoop(elements.p1_lumped_mass(), cells, M(op2.INC, c2v), X(op2.READ, c2v))
+    pr = op2.ParLoop(elements.p1_laplace_residual(), cells, R(op2.INC, c2v), X(op2.READ, c2v), u(op2.READ, c2v))
+    base = m1.ncells * 12 + m1.nverts * 16
+    for key, pl, name, alg in (("c1a_mass", pm, "p1_lumped_mass", base + m1.nverts * 16),
+                               ("c1b_residual", pr, "p1_laplace_residual", base + m1.nverts * 24)):
+        ms = time_loop(pl, reps, name)
+        res[key] = dict(cells=m1.ncells, kernel_us=ms * 1e3, cells_per_s=m1.ncells / (ms * 1e-3),
+                        gbs=alg / (ms * 1e-3) / 1e9, alg_bytes=alg,
+                        strategy=pl.global_kernel.compile().strategy_used)
+        log(f"C1 {key}: {ms * 1e3:.2f} us, {res[key]['gbs']:.0f} GB/s ({res[key]['strategy']})")
+
+    # C3: P2 elasticity, 64^3 (16^3 with --quick)
+    n3 = 16 if args.quick else 64
+    m3 = unit_cube_tet_p2(n3, rng=workload_rng(3))
+    for dt, tag in ((np.float64, "f64"), (np.float32, "f32")):
+        isz = np.dtype(dt).itemsize
+        cells = op2.Set(m3.ncells, "cells")
+        vset = op2.Set(m3.nverts, "verts")
+        nodes = op2.Set(m3.nnodes, "nodes")
+        c2v = op2.Map(cells, vset, 4, m3.cell_vertex, "c2v")
+        c2n = op2.Map(cells, nodes, 10, m3.cell_node, "c2n")
+        X = op2.Dat(vset ** 3, m3.coords.astype(dt), dt, name="X")
+        u = op2.Dat(nodes ** 3, m3.u.astype(dt), dt, name="u")
+        R = op2.Dat(nodes ** 3, dtype=dt, name="R")
+        sp = op2.Sparsity((nodes ** 3, nodes ** 3), [(c2n, c2n, None)])
+        nnzb = sp.block_nnz
+        A = op2.Mat(sp, dt)
+        maps_b = m3.ncells * (40 + 16)
+        x_b = m3.nverts * 3 * isz
+        u_b = m3.nnodes * 3 * isz
+        r_b = 2 * u_b
+        asm_b = nnzb * 9 * isz * 2 + nnzb * 4 + (m3.nnodes + 1) * 4 + maps_b + x_b
+        loops = [
+            ("residual", op2.ParLoop(elements.p2_elasticity_residual(), cells, R(op2.INC, c2n),
+                                     X(op2.READ, c2v), u(op2.READ, c2n)),
+             "p2_elasticity_residual", maps_b + x_b + u_b + r_b),
+            ("assembly", op2.ParLoop(elements.p2_elasticity_matrix(), cells, A(op2.INC, (c2n, c2n)),
+                                     X(op2.READ, c2v)), "p2_elasticity_matrix", asm_b),
+            ("fused", op2.ParLoop(elements.p2_elasticity_fused(), cells, A(op2.INC, (c2n, c2n)),
+                                  R(op2.INC, c2n), X(op2.READ, c2v), u(op2.READ, c2n)),
+             "p2_elasticity_fused", asm_b + u_b + r_b),
+        ]
+        for key, pl, name, alg in loops:
+            ms = time_loop(pl, reps, name)
+            k = f"c3_{key}_{tag}"
+            res[k] = dict(cells=m3.ncells, block_nnz=nnzb, kernel_us=ms * 1e3,
+                          cells_per_s=m3.ncells / (ms * 1e-3), gbs=alg / (ms * 1e-3) / 1e9,
+                          frac=alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
+                          strategy=pl.global_kernel.compile().strategy_used)
+            log(f"C3 {k}: {ms * 1e3:.1f} us, {res[k]['gbs']:.0f} GB/s = {res[k]['frac']:.1%} "
+                f"({res[k]['strategy']})")
+        if tag == "f64":
+            res["c4_adjoint"] = adjoint_ratio(m3, cells, c2v, c2n, X, u, reps)
+        del A, sp
+        torch.cuda.empty_cache()
+    return res
+
+
+def adjoint_ratio(m3, cells, c2v, c2n, X, u, reps):
+    """t(adjoint loop) / t(forward residual loop), P2 64^3, float64."""
+    from pyop2_amd import adjoint, device as dev
+    torch = dev.torch()
+    prob = adjoint.P2Elasticity(cells, c2v, c2n, X, u)
+    uu = u.device_data_ro.clone().requires_grad_(True)
+    xx = X.device_data_ro.clone().requires_grad_(True)
+    for _ in range(2):
+        J = prob.objective(uu, xx)
+        J.backward()
+    torch.cuda.synchronize()
+    with dev.launch_timer() as t:
+        for _ in range(reps):
+            J = prob.objective(uu, xx)
+            J.backward()
+    torch.cuda.synchronize()
+    fwd = kernel_ms(t, "p2_elasticity_residual")
+    adj = kernel_ms(t, "p2_elasticity_adjoint")
+    log(f"C4 adjoint: forward {fwd * 1e3:.1f} us, adjoint {adj * 1e3:.1f} us, ratio {adj / fwd:.2f}")
+    return dict(forward_us=fwd * 1e3, adjoint_us=adj * 1e3, ratio=adj / fwd)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=10, help="repetitions of each secondary loop")
+    ap.add_argument("--strategy", default="auto")
+    ap.add_argument("--quick", action="store_true", help="small meshes (smoke of the harness)")
+    ap.add_argument("--primary-only", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    result = run(args)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -344,8 +344,8 @@ class GeneratedKernel:
     def __init__(self, gk):
         self.gk = gk
         self.strategy_used = None
-        self._src = None
-        self._mod = None
+        self._src = {}
+        self._mods = {}
 
     def _unique_maps(self, parloop):
         seen, index = [], {}
@@ -360,15 +360,17 @@ class GeneratedKernel:
                     seen.append(m)
         return seen, index
 
+    def _flags(self, parloop):
+        return tuple((k, a.data._block) for k, (garg, a) in enumerate(zip(self.gk.arguments, parloop.arguments))
+                     if isinstance(garg, MatKernelArg))
+
     def source(self, parloop):
-        if self._src is None:
+        flags = self._flags(parloop)
+        src = self._src.get(flags)
+        if src is None:
             _, index = self._unique_maps(parloop)
-            flags = {}
-            for k, (garg, a) in enumerate(zip(self.gk.arguments, parloop.arguments)):
-                if isinstance(garg, MatKernelArg):
-                    flags[k] = a.data._block
-            self._src = generate_source(self.gk, index, flags)
-        return self._src
+            src = self._src[flags] = generate_source(self.gk, index, dict(flags))
+        return src
 
     def _conflict_maps(self, parloop):
         """Maps whose targets must be exclusive within a launch."""
@@ -389,8 +391,10 @@ class GeneratedKernel:
     def __call__(self, start, end, ptrs, parloop):
         if parloop is None:
             raise RuntimeError("generated loops are launched by a Parloop")
-        if self._mod is None:
-            self._mod = _module(self.source(parloop))
+        flags = self._flags(parloop)
+        mod = self._mods.get(flags)
+        if mod is None:
+            mod = self._mods[flags] = _module(self.source(parloop))
         maps, _ = self._unique_maps(parloop)
         conflict, need_colour = self._conflict_maps(parloop)
         if configuration["strategy"] == "coloured" and not need_colour:
@@ -421,8 +425,9 @@ class GeneratedKernel:
             allv = [ctypes.c_int32(lo), ctypes.c_int32(hi), ctypes.c_void_p(lst), ctypes.c_int32(atomic)] + vals
             argv = (ctypes.c_void_p * len(allv))(*[ctypes.addressof(v) for v in allv])
             grid = (hi - lo + self.BLOCK - 1) // self.BLOCK
-            dev.check(L.op2hip_jit_launch(self._mod, b"op2_wrap", grid, self.BLOCK, 0, argv, s),
-                      f"launch of generated {self.gk.name}")
+            with dev.timed_launch("op2_wrap:" + self.gk.name):
+                rc = L.op2hip_jit_launch(mod, b"op2_wrap", grid, self.BLOCK, 0, argv, s)
+            dev.check(rc, f"launch of generated {self.gk.name}")
 
         if need_colour and conflict:
             from pyop2_amd import plans

@@ -265,3 +265,63 @@ def to_host(tensor, out=None):
 def workspace(nbytes):
     """A device byte buffer of at least ``nbytes`` (torch allocation)."""
     return torch().empty(max(int(nbytes), 8), dtype=torch().uint8, device=device())
+
+
+class _LaunchTimer:
+    """Records a HIP event pair on the launch stream around every library
+    launch while active (``with device.launch_timer() as t: ...``);
+    ``t.times(name)`` then gives each launch's duration in ms."""
+
+    def __init__(self):
+        self.events = []
+
+    def times(self, name=None):
+        out = []
+        for n, e0, e1 in self.events:
+            if name is None or n == name:
+                e1.synchronize()
+                out.append(e0.elapsed_time(e1))
+        return out
+
+
+class _TimerState:
+    active = None   # process-wide: autograd runs backward on its own thread
+
+
+_timer = _TimerState()
+
+
+class launch_timer:
+    def __enter__(self):
+        self.t = _LaunchTimer()
+        _timer.active = self.t
+        return self.t
+
+    def __exit__(self, *exc):
+        _timer.active = None
+        return False
+
+
+class timed_launch:
+    """Context around one launch: event pair when a launch_timer is active."""
+
+    __slots__ = ("name", "e1", "t")
+
+    def __init__(self, name):
+        self.name = name
+        self.t = _timer.active
+
+    def __enter__(self):
+        if self.t is not None:
+            tc = torch().cuda
+            s = tc.current_stream(device())
+            e0 = tc.Event(enable_timing=True)
+            self.e1 = tc.Event(enable_timing=True)
+            e0.record(s)
+            self.t.events.append((self.name, e0, self.e1))
+        return self
+
+    def __exit__(self, *exc):
+        if self.t is not None:
+            self.e1.record(torch().cuda.current_stream(device()))
+        return False

@@ -16,6 +16,8 @@ Argument conventions (INC buffers arrive zeroed, as in the reference):
 * ``p2_elasticity_residual`` ``(double R[10][3], const double X[4][3], const double u[10][3])``
 * ``p2_elasticity_matrix``   ``(double A[30][30], const double X[4][3])``
 * ``p2_elasticity_fused``    ``(double A[30][30], double R[10][3], const double X[4][3], const double u[10][3])``
+* ``p2_elasticity_adjoint``  ``(double ub[10][3], double Xb[4][3], const double Rbar[10][3], const double X[4][3], const double u[10][3])``
+* ``p2_elasticity_tlm``      ``(double dR[10][3], const double X[4][3], const double u[10][3], const double dX[4][3], const double du[10][3])``
 
 The P2 kernels bake the Lame parameters in as literals (as form compilers
 do).  The 4-point Gauss rule (a = 0.5854101966249685, b = 0.1381966011250105,
@@ -26,6 +28,7 @@ import hashlib
 
 __all__ = ["p1_lumped_mass", "p1_laplace_residual", "p1_laplace_matrix",
            "p2_elasticity_residual", "p2_elasticity_matrix", "p2_elasticity_fused",
+           "p2_elasticity_adjoint", "p2_elasticity_tlm",
            "kernel_source", "code_digest", "QUAD_A", "QUAD_B", "QUAD_W"]
 
 QUAD_A = 0.5854101966249685
@@ -194,6 +197,132 @@ static void p2_elasticity_fused(double A[30][30], double R[10][3], const double 
 """
 
 
+# Adjoint of the residual for J-type functionals: with E = Rbar_e,
+#   ub += K_e^T E                    (= the residual operator applied to E),
+#   Xb += d(E . R_e(u, X))/dX,  d/dJ = wdet sum_q [phi I - gu^T sE - gE^T su] K^T,
+# phi = su : gE, J[r][c] = X[c+1][r] - X[0][r] (X[m+1] gets column m, X[0] minus the sum).
+P2_ELASTICITY_ADJOINT = r"""
+static void p2_elasticity_adjoint(double ub[10][3], double Xb[4][3], const double E[10][3],
+                                  const double X[4][3], const double u[10][3])
+{
+  const double lmbda = %(lam)r, mu = %(mu)r;""" + _P2_GEOMETRY + r"""
+  double P[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int q = 0; q < 4; ++q) {""" + _P2_BASIS_GRADS + r"""
+    double gu[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double ge[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int j = 0; j < 10; ++j)
+      for (int bb = 0; bb < 3; ++bb)
+        for (int k = 0; k < 3; ++k) {
+          gu[bb][k] += u[j][bb] * G[j][k];
+          ge[bb][k] += E[j][bb] * G[j][k];
+        }
+    const double dvu = gu[0][0] + gu[1][1] + gu[2][2];
+    const double dve = ge[0][0] + ge[1][1] + ge[2][2];
+    double su[3][3], se[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < 3; ++k) {
+        su[a][k] = mu * (gu[a][k] + gu[k][a]) + (a == k ? lmbda * dvu : 0.0);
+        se[a][k] = mu * (ge[a][k] + ge[k][a]) + (a == k ? lmbda * dve : 0.0);
+      }
+    for (int i = 0; i < 10; ++i)
+      for (int a = 0; a < 3; ++a) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k)
+          acc += se[a][k] * G[i][k];
+        ub[i][a] += wdet * acc;
+      }
+    double phi = 0.0;
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < 3; ++k)
+        phi += su[a][k] * ge[a][k];
+    for (int l = 0; l < 3; ++l)
+      for (int k = 0; k < 3; ++k) {
+        double acc = (l == k) ? phi : 0.0;
+        for (int bb = 0; bb < 3; ++bb)
+          acc -= gu[bb][l] * se[bb][k] + ge[bb][l] * su[bb][k];
+        P[l][k] += acc;
+      }
+  }
+  for (int l = 0; l < 3; ++l) {
+    double rowsum = 0.0;
+    for (int m = 0; m < 3; ++m) {
+      const double v = wdet * (P[l][0] * K[m][0] + P[l][1] * K[m][1] + P[l][2] * K[m][2]);
+      Xb[m + 1][l] += v;
+      rowsum += v;
+    }
+    Xb[0][l] -= rowsum;
+  }
+}
+"""
+
+# Tangent-linear residual: dR = dR/du du + dR/dX dX, with dK = -K dJ K,
+# d|det J| = |det J| tr(K dJ) and grad phi linear in the barycentric gradients.
+P2_ELASTICITY_TLM = r"""
+static void p2_elasticity_tlm(double dR[10][3], const double X[4][3], const double u[10][3],
+                              const double dX[4][3], const double du[10][3])
+{
+  const double lmbda = %(lam)r, mu = %(mu)r;""" + _P2_GEOMETRY + r"""
+  double dJ[3][3], dK[3][3], dgl[4][3];
+  for (int r = 0; r < 3; ++r)
+    for (int cc = 0; cc < 3; ++cc)
+      dJ[r][cc] = dX[cc + 1][r] - dX[0][r];
+  double trKdJ = 0.0;
+  for (int m = 0; m < 3; ++m)
+    for (int r = 0; r < 3; ++r)
+      trKdJ += K[m][r] * dJ[r][m];
+  for (int m = 0; m < 3; ++m)
+    for (int k = 0; k < 3; ++k) {
+      double acc = 0.0;
+      for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc)
+          acc += K[m][r] * dJ[r][cc] * K[cc][k];
+      dK[m][k] = -acc;
+    }
+  for (int k = 0; k < 3; ++k) {
+    dgl[0][k] = -(dK[0][k] + dK[1][k] + dK[2][k]);
+    dgl[1][k] = dK[0][k];
+    dgl[2][k] = dK[1][k];
+    dgl[3][k] = dK[2][k];
+  }
+  const double dwdet = wdet * trKdJ;
+  for (int q = 0; q < 4; ++q) {""" + _P2_BASIS_GRADS + r"""
+    double dG[10][3];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 3; ++k)
+        dG[i][k] = (4.0 * L[i] - 1.0) * dgl[i][k];
+    for (int e = 0; e < 6; ++e)
+      for (int k = 0; k < 3; ++k)
+        dG[4 + e][k] = 4.0 * (L[ea[e]] * dgl[eb[e]][k] + L[eb[e]] * dgl[ea[e]][k]);
+    double gu[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double dgu[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int j = 0; j < 10; ++j)
+      for (int bb = 0; bb < 3; ++bb)
+        for (int k = 0; k < 3; ++k) {
+          gu[bb][k] += u[j][bb] * G[j][k];
+          dgu[bb][k] += du[j][bb] * G[j][k] + u[j][bb] * dG[j][k];
+        }
+    const double dv = gu[0][0] + gu[1][1] + gu[2][2];
+    const double ddv = dgu[0][0] + dgu[1][1] + dgu[2][2];
+    double s[3][3], ds[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int k = 0; k < 3; ++k) {
+        s[a][k] = mu * (gu[a][k] + gu[k][a]) + (a == k ? lmbda * dv : 0.0);
+        ds[a][k] = mu * (dgu[a][k] + dgu[k][a]) + (a == k ? lmbda * ddv : 0.0);
+      }
+    for (int i = 0; i < 10; ++i)
+      for (int a = 0; a < 3; ++a) {
+        double acc0 = 0.0, acc1 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          acc0 += s[a][k] * G[i][k];
+          acc1 += ds[a][k] * G[i][k] + s[a][k] * dG[i][k];
+        }
+        dR[i][a] += dwdet * acc0 + wdet * acc1;
+      }
+  }
+}
+"""
+
+
 def kernel_source(family, lam=1.0, mu=0.5):
     """C source of element kernel ``family`` (P2 families take lam, mu)."""
     params = {"qa": QUAD_A, "qb": QUAD_B, "qw": QUAD_W,
@@ -205,6 +334,8 @@ def kernel_source(family, lam=1.0, mu=0.5):
         "p2_elasticity_residual": P2_ELASTICITY_RESIDUAL % params,
         "p2_elasticity_matrix": P2_ELASTICITY_MATRIX % params,
         "p2_elasticity_fused": P2_ELASTICITY_FUSED % params,
+        "p2_elasticity_adjoint": P2_ELASTICITY_ADJOINT % params,
+        "p2_elasticity_tlm": P2_ELASTICITY_TLM % params,
     }
     try:
         return table[family]
@@ -253,3 +384,14 @@ def p2_elasticity_matrix(lam=1.0, mu=0.5):
 def p2_elasticity_fused(lam=1.0, mu=0.5):
     """Element matrix A_e and residual R_e = A_e u_e in one kernel."""
     return _make("p2_elasticity_fused", lam=float(lam), mu=float(mu))
+
+
+def p2_elasticity_adjoint(lam=1.0, mu=0.5):
+    """Adjoint of the residual loop: (ub INC, Xb INC, Rbar, X, u)."""
+    return _make("p2_elasticity_adjoint", lam=float(lam), mu=float(mu))
+
+
+def p2_elasticity_tlm(lam=1.0, mu=0.5):
+    """Tangent-linear residual: (dR INC, X, u, dX, du) (generated device path)."""
+    from pyop2_amd.local_kernel import Kernel
+    return Kernel(kernel_source("p2_elasticity_tlm", lam=lam, mu=mu), "p2_elasticity_tlm")

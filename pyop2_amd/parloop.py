@@ -392,6 +392,20 @@ class PassthroughArg(LegacyArg):
         return PassthroughParloopArg(self.data)
 
 
+_GLOBAL_KERNELS = {}
+
+
+def _cached_global_kernel(local_knl, global_knl_args, subset, kwargs):
+    """GlobalKernels (and their compiled executors) are reused for equal
+    cache keys, as the reference reuses compiled wrappers
+    (``pyop2/compilation.py:427-489``, memory cache keyed by ``cache_key``)."""
+    gk = GlobalKernel(local_knl, global_knl_args, subset=subset, **kwargs)
+    try:
+        return _GLOBAL_KERNELS.setdefault(gk.cache_key, gk)
+    except TypeError:   # unhashable key component: no caching
+        return gk
+
+
 def ParLoop(*args, **kwargs):
     return LegacyParloop(*args, **kwargs)
 
@@ -407,7 +421,7 @@ def LegacyParloop(local_knl, iterset, *args, **kwargs):
         local_knl.dtypes = tuple(a.dtype for a in args)
     global_knl_args = tuple(a.global_kernel_arg for a in args)
     subset = isinstance(iterset, Subset)
-    global_knl = GlobalKernel(local_knl, global_knl_args, subset=subset, **kwargs)
+    global_knl = _cached_global_kernel(local_knl, global_knl_args, subset, kwargs)
     parloop_args = tuple(a.parloop_arg for a in args)
     return Parloop(global_knl, iterset, parloop_args)
 

@@ -223,7 +223,8 @@ class HandWrittenKernel:
         cargs += [dev.ptr(m.device_values) for m in seen]
         if self.family.startswith("p2_"):
             cargs += [ctypes.c_double(self.params["lam"]), ctypes.c_double(self.params["mu"])]
-        rc = self.fn(start, end, *cargs, ctypes.byref(ex), dev.stream())
+        with dev.timed_launch(self.family):
+            rc = self.fn(start, end, *cargs, ctypes.byref(ex), dev.stream())
         dev.check(rc, self.spec["entry"][self.variant])
         self.strategy_used = strategy
         del keep

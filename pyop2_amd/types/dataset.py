@@ -35,7 +35,10 @@ class DataSet(caching.ObjectCached):
 
     @classmethod
     def _cache_key(cls, iter_set, dim=1, name=None):
-        return (iter_set, utils.as_tuple(dim, numbers.Integral))
+        try:
+            return (iter_set, utils.as_tuple(dim, numbers.Integral))
+        except TypeError:
+            raise ex.DimTypeError("DataSet dim must be an integer or a tuple of integers, not %r" % (dim,))
 
     def __getattr__(self, name):
         if name.startswith("__") or name == "_set":

@@ -24,26 +24,25 @@ class cached_property:
 
 
 def as_tuple(item, type=None, length=None, allow_none=False):
-    # Empty list if we get passed None
+    """``item`` as a tuple: None -> (), iterables -> tuple(item), anything
+    else -> ``length`` (default 1) copies.  Checks the length and the item
+    types when asked (``TypeError`` / ``ValueError``)."""
     if item is None:
-        t = ()
+        out = ()
+    elif isinstance(item, str) or not hasattr(item, "__iter__"):
+        out = (item,) * (length or 1) if not isinstance(item, str) else tuple(item)
     else:
-        # Convert iterable to tuple...
         try:
-            t = tuple(item)
-        # ... or create a list of a single item
+            out = tuple(item)
         except (TypeError, NotImplementedError):
-            t = (item,) * (length or 1)
-    if length and not len(t) == length:
+            out = (item,) * (length or 1)
+    if length and len(out) != length:
         raise ValueError("Tuple needs to be of length %d" % length)
     if type is not None:
-        if allow_none:
-            valid = all((isinstance(i, type) or i is None) for i in t)
-        else:
-            valid = all(isinstance(i, type) for i in t)
-        if not valid:
+        ok = (lambda v: isinstance(v, type) or v is None) if allow_none else (lambda v: isinstance(v, type))
+        if not all(ok(v) for v in out):
             raise TypeError("Items need to be of type %s" % type)
-    return t
+    return out
 
 
 def tuplify(xs):
@@ -60,14 +59,14 @@ class validate_base:
         self._checks = checks
 
     def __call__(self, f):
-        @functools.wraps(f)
         def wrapper(f, *args, **kwargs):
             if self._configuration_type_check():
-                self.nargs = f.__code__.co_argcount
-                self.defaults = f.__defaults__ or ()
-                self.varnames = f.__code__.co_varnames
-                self.file = f.__code__.co_filename
-                self.line = f.__code__.co_firstlineno + 1
+                code_of = inspect.unwrap(f)   # stacked validators: the real signature
+                self.nargs = code_of.__code__.co_argcount
+                self.defaults = code_of.__defaults__ or ()
+                self.varnames = code_of.__code__.co_varnames
+                self.file = code_of.__code__.co_filename
+                self.line = code_of.__code__.co_firstlineno + 1
                 self.check_args(args, kwargs)
             return f(*args, **kwargs)
         return _decorate(f, wrapper)
@@ -148,26 +147,29 @@ class validate_dtype(validate_base):
 
 
 def verify_reshape(data, dtype, shape, allow_none=False):
-    """Verify data is of type dtype and try to reshape it to shape."""
+    """``data`` as a numpy array of ``dtype`` with exactly ``shape``.
+
+    Raises ``DataTypeError`` for an unusable dtype and ``DataValueError``
+    for None (unless ``allow_none``: an empty array) or a shape mismatch.
+    The array is not copied when it already has the dtype; its shape is set
+    in place, which requires contiguous data.
+    """
     try:
-        t = np.dtype(dtype) if dtype is not None else None
+        target = None if dtype is None else np.dtype(dtype)
     except TypeError:
         raise DataTypeError("Invalid data type: %s" % dtype)
-    if data is None and allow_none:
-        return np.asarray([], dtype=t)
-    elif data is None:
+    if data is None:
+        if allow_none:
+            return np.asarray([], dtype=target)
         raise DataValueError("Invalid data: None is not allowed!")
-    else:
-        try:
-            a = np.asarray(data, dtype=t)
-        except ValueError:
-            raise DataValueError("Invalid data: cannot convert to %s!" % dtype)
-        except TypeError:
-            raise DataTypeError("Invalid data type: %s" % dtype)
-        try:
-            # Destructively modify shape.  Fails if data are not
-            # contiguous, but that's what we want anyway.
-            a.shape = shape
-            return a
-        except ValueError:
-            raise DataValueError("Expected data of shape %s, got %s" % (shape, a.shape))
+    try:
+        arr = np.asarray(data, dtype=target)
+    except ValueError:
+        raise DataValueError("Invalid data: cannot convert to %s!" % dtype)
+    except TypeError:
+        raise DataTypeError("Invalid data type: %s" % dtype)
+    try:
+        arr.shape = shape
+    except ValueError:
+        raise DataValueError("Expected data of shape %s, got %s" % (shape, arr.shape))
+    return arr

@@ -1,0 +1,163 @@
+"""Host-side API of ``pyop2_amd.op2`` (CPU): construction, validation errors
+and caching behave as in the reference (``test/unit/test_api.py``,
+``test_caching.py:491-548``, ``test_indirect_loop.py:109-129``,
+``test_matrices.py:575-580``), and a par_loop without a device fails loudly."""
+import numpy as np
+import pytest
+
+from pyop2_amd import exceptions as ex, op2
+from pyop2_amd.configuration import configuration
+
+
+def test_set_sizes_and_parts():
+    s = op2.Set((3, 7, 9), "s")
+    assert (s.core_size, s.size, s.total_size) == (3, 7, 9)
+    assert (s.core_part.offset, s.core_part.size) == (0, 3)
+    assert (s.owned_part.offset, s.owned_part.size) == (3, 4)
+    assert op2.Set(5).sizes == (5, 5, 5)
+    with pytest.raises(ex.SizeTypeError):
+        op2.Set("five")
+    with pytest.raises(ex.NameTypeError):
+        op2.Set(5, 12)
+
+
+def test_subset():
+    s = op2.Set(10)
+    ss = s([7, 2, 2, 5])
+    assert list(ss.indices) == [2, 5, 7]
+    assert ss.superset is s and ss.total_size == 3
+    assert list(ss([0, 2]).indices) == [2, 7]
+    with pytest.raises(ex.SubsetIndexOutOfBounds):
+        op2.Subset(s, [11])
+
+
+def test_dataset_identity_cache():
+    s = op2.Set(4)
+    assert s ** 2 is s ** 2
+    assert s ** 2 is not s ** 3
+    assert (s ** (2, 3)).cdim == 6
+    with pytest.raises(ex.DimTypeError):
+        op2.DataSet(s, "x")
+
+
+def test_map_validation():
+    a, b = op2.Set(3), op2.Set(4)
+    m = op2.Map(a, b, 2, [0, 1, 2, 3, 1, 0])
+    assert m.values.shape == (3, 2) and m.values.dtype == np.int32
+    with pytest.raises(ex.DataValueError):
+        op2.Map(a, b, 2, [0, 1])
+    with pytest.raises(ex.SetTypeError):
+        op2.Map("a", b, 2)
+    with pytest.raises(ex.ArityTypeError):
+        op2.Map(a, b, 2.5)
+
+
+def test_dat_host_semantics(tmp_path):
+    s = op2.Set(5)
+    d = op2.Dat(s ** 2, np.arange(10.0).reshape(5, 2))
+    assert d.data.shape == (5, 2) and d.dtype == np.float64
+    ro = d.data_ro
+    with pytest.raises(ValueError):
+        ro[0, 0] = 1.0
+    d.data[0, 0] = 42.0
+    assert d.data_ro[0, 0] == 42.0
+    z = op2.Dat(s)
+    assert (z.data_ro == 0).all() and z.dtype == np.float64
+    d.save(str(tmp_path / "d"))
+    e = op2.Dat(s ** 2)
+    e.load(str(tmp_path / "d"))
+    assert np.array_equal(e.data_ro, d.data_ro)
+    d.zero()
+    assert (d.data_ro == 0).all()
+    c = op2.Dat(e)
+    assert np.array_equal(c.data_ro, e.data_ro)
+    with pytest.raises(ex.DataSetTypeError):
+        op2.Dat("s")
+
+
+def test_dat_call_checks_map_target():
+    a, b, c = op2.Set(3), op2.Set(4), op2.Set(4)
+    m = op2.Map(a, b, 1, [0, 1, 2])
+    d = op2.Dat(c)
+    with pytest.raises(ex.MapValueError):
+        d(op2.READ, m)
+    with pytest.raises(ex.ModeValueError):
+        d("bogus", m)
+
+
+def test_global():
+    g = op2.Global(2, [1.0, 2.0], name="g")
+    assert g.shape == (2,) and list(g.data_ro) == [1.0, 2.0]
+    g.data = [3.0, 4.0]
+    assert list(g.data_ro) == [3.0, 4.0]
+    with pytest.raises(ex.ModeValueError):
+        g(op2.WRITE)
+    with pytest.raises(ex.ModeValueError):
+        op2.Constant(1, 1.0)(op2.INC)
+
+
+def test_parloop_validation_errors():
+    it, ind = op2.Set(4), op2.Set(4)
+    x = op2.Dat(ind, np.zeros(4))
+    k = op2.Kernel("static void w(double *x) { *x = 1.0; }", "w")
+    with pytest.raises(ex.MapValueError):     # iterset of the map differs
+        op2.par_loop(k, it, x(op2.WRITE, op2.Map(op2.Set(4), ind, 1, [0, 1, 2, 3])))
+    with pytest.raises(ex.MapValueError):     # toset differs from the Dat's set
+        op2.par_loop(k, it, x(op2.WRITE, op2.Map(it, op2.Set(4), 1, [0, 1, 2, 3])))
+    with pytest.raises(ex.MapValueError):     # uninitialised map
+        op2.par_loop(k, it, x(op2.WRITE, op2.Map(it, ind, 1)))
+    with pytest.raises(ex.SetTypeError):
+        op2.par_loop(k, "it", x(op2.WRITE, op2.Map(it, ind, 1, [0, 1, 2, 3])))
+    with pytest.raises(ex.KernelTypeError):
+        op2.par_loop("not a kernel", it)
+    with pytest.raises(ex.MapValueError):     # direct arg on a different set
+        op2.par_loop(k, it, op2.Dat(op2.Set(4))(op2.WRITE))
+
+
+def test_mat_modes_and_sparsity_cache():
+    nodes, els = op2.Set(4), op2.Set(2)
+    en = op2.Map(els, nodes, 3, [0, 1, 3, 2, 3, 1])
+    sp = op2.Sparsity((nodes ** 1, nodes ** 1), [(en, en, None)])
+    assert sp is op2.Sparsity((nodes ** 1, nodes ** 1), [(en, en, None)])
+    assert sp is not op2.Sparsity((nodes ** 2, nodes ** 2), [(en, en, None)])
+    A = op2.Mat(sp)
+    for mode in (op2.READ, op2.RW, op2.MAX, op2.MIN):
+        with pytest.raises(ex.ModeValueError):
+            A(mode, (en, en))
+    other = op2.Map(els, nodes, 3, [0, 1, 3, 2, 3, 1])
+    with pytest.raises(ex.MapValueError):
+        A(op2.INC, (other, other))
+    with pytest.raises(ex.MapValueError):
+        op2.Sparsity((nodes ** 1, nodes ** 1), [(op2.Map(els, nodes, 1), op2.Map(els, nodes, 1), None)])
+    with pytest.raises(RuntimeError):
+        op2.Mat(sp, np.int32)
+
+
+def test_kernel_cache_keys():
+    k1 = op2.Kernel("void a(double *x) {}", "a")
+    k2 = op2.Kernel("void a(double *x) {}", "a")
+    k3 = op2.Kernel("void a(double *x) { }", "a")
+    assert k1._immutable_cache_key == k2._immutable_cache_key != k3._immutable_cache_key
+    with pytest.raises(ex.NameTypeError):
+        op2.Kernel("", 3)
+
+
+def test_configuration():
+    assert configuration["type_check"] is True
+    with pytest.raises(ex.ConfigurationError):
+        configuration["type_check"] = "yes"
+    with pytest.raises(ex.ConfigurationError):
+        configuration["strategy"] = "magic"
+    op2.init(strategy="coloured")
+    assert configuration["strategy"] == "coloured"
+
+
+def test_parloop_without_device_fails_loudly():
+    from pyop2_amd import device as dev
+    if dev.torch().cuda.is_available():
+        pytest.skip("a device is present")
+    it, ind = op2.Set(4), op2.Set(4)
+    x = op2.Dat(ind, np.zeros(4))
+    m = op2.Map(it, ind, 1, [0, 1, 2, 3])
+    with pytest.raises(ex.DeviceError):
+        op2.par_loop(op2.Kernel("static void w(double *x) { *x = 1.0; }", "w"), it, x(op2.WRITE, m))
