@@ -45,7 +45,11 @@ enum op2hip_strategy {
   OP2HIP_ATOMIC = 0,     /* one thread per element, device-scope atomics */
   OP2HIP_COLOURED = 1,   /* greedy element colouring, one launch per colour */
   OP2HIP_STAGED = 2,     /* LDS-staged tiles, one atomic per tile node */
-  OP2HIP_ROWGATHER = 3   /* Mat only: owner-computes block rows, no atomics */
+  OP2HIP_ROWGATHER = 3,  /* Mat only: owner-computes block rows, no atomics;
+                            one lane per (row incidence, column block) */
+  OP2HIP_ROWGATHER_THREAD = 4, /* as ROWGATHER, one thread per row incidence */
+  OP2HIP_ROWGATHER_SLOTS = 5   /* as ROWGATHER_THREAD, with a precomputed slot
+                                  table (no pattern search in the kernel) */
 };
 
 /* Execution plan of one launch.  Host struct; arrays as annotated. */
@@ -69,6 +73,8 @@ typedef struct op2hip_exec {
   int32_t batch_max_nnz;           /* max blocks of a batch (sizes LDS) */
   int32_t batch_max_inc;           /* max incidences of a batch */
   int32_t batch_max_rows;          /* max rows of a batch */
+  const void *batch_desc;          /* device, per batch {r0, nr, z0, nz, i0, ni, 0, 0} (int32) */
+  const void *slot_table;          /* device, per row incidence: op2hip_slot_table layout */
   /* Subset iteration: element n = subset[i] for i in [start,end) (or NULL) */
   const int32_t *subset;           /* device */
 } op2hip_exec;
@@ -170,6 +176,21 @@ int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr,
                             int32_t max_nnz, int32_t *batch_rows,
                             int32_t *nbatches, int32_t *max_batch_inc,
                             int32_t *max_batch_nnz, int32_t *max_batch_rows);
+
+/* Slot table of OP2HIP_ROWGATHER_SLOTS for the batches of a row-batch plan:
+ * per batch a descriptor {r0, nr, z0, nz, i0, ni, 0, 0} (8 int32) and, per
+ * row incidence t (inc_ent order), an entry of entry_bytes(carity) bytes:
+ * uint16 block offset of the row in its batch, uint16 row index in the
+ * batch, then carity uint8 positions of column block j in the row (255: no
+ * column or not in the pattern, counted in the device int32 *bad, which the
+ * call zeroes first). */
+int op2hip_slot_table_bytes(int32_t carity, size_t *entry_bytes);
+int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows,
+                            const int32_t *rowptr, const int32_t *colidx,
+                            const int32_t *inc_ptr, const int32_t *inc_ent,
+                            const int32_t *cmap, int32_t rarity, int32_t carity,
+                            void *table, void *batch_desc, int32_t *bad,
+                            void *stream);
 
 /* ---- hand-written element loops ---------------------------------------- */
 /* P1 triangles, op2.Kernel names p1_lumped_mass / p1_laplace_residual /

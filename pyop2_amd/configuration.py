@@ -65,7 +65,8 @@ class Configuration(dict):
             if not isinstance(value, valid_type):
                 raise ConfigurationError("Values for configuration key %s must be of type %r, not %r"
                                          % (key, valid_type, type(value)))
-        if key == "strategy" and value not in ("auto", "atomic", "coloured", "staged", "rowgather"):
+        if key == "strategy" and value not in ("auto", "atomic", "coloured", "staged", "rowgather",
+                                               "rowgather_thread", "rowgather_slots"):
             raise ConfigurationError(f"unknown strategy {value!r}")
         self._set.add(key)
         super().__setitem__(key, value)

@@ -607,3 +607,58 @@ int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr, const int32_t
 }
 
 }  // extern "C"
+
+#include "rowgather.hpp"
+
+extern "C" {
+
+int op2hip_slot_table_bytes(int32_t carity, size_t *entry_bytes) {
+  OP2HIP_REQUIRE(entry_bytes, "slot_table_bytes: NULL");
+  switch (carity) {
+    case 3: *entry_bytes = sizeof(SlotEntry<3>); return OP2HIP_OK;
+    case 4: *entry_bytes = sizeof(SlotEntry<4>); return OP2HIP_OK;
+    case 10: *entry_bytes = sizeof(SlotEntry<10>); return OP2HIP_OK;
+    default:
+      set_error("slot tables exist for column arities 3, 4 and 10 (got %d)", carity);
+      return OP2HIP_ERR_ARG;
+  }
+}
+
+int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows, const int32_t *rowptr,
+                            const int32_t *colidx, const int32_t *inc_ptr, const int32_t *inc_ent,
+                            const int32_t *cmap, int32_t rarity, int32_t carity, void *table,
+                            void *batch_desc, int32_t *bad, void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nbatches >= 0 && batch_rows && rowptr && colidx && inc_ptr && inc_ent && cmap &&
+                     table && batch_desc && rarity > 0 && bad,
+                 "build_slot_table: bad arguments");
+  int rc = fill(bad, 1, 0, s);
+  if (rc) return rc;
+  if (nbatches > 0) {
+    auto *d = static_cast<BatchDesc *>(batch_desc);
+    switch (carity) {
+      case 3:
+        build_slot_table<3><<<nbatches, 64, 0, s>>>(nbatches, batch_rows, rowptr, colidx, inc_ptr,
+                                                     inc_ent, cmap, rarity,
+                                                     static_cast<SlotEntry<3> *>(table), d, bad);
+        break;
+      case 4:
+        build_slot_table<4><<<nbatches, 64, 0, s>>>(nbatches, batch_rows, rowptr, colidx, inc_ptr,
+                                                     inc_ent, cmap, rarity,
+                                                     static_cast<SlotEntry<4> *>(table), d, bad);
+        break;
+      case 10:
+        build_slot_table<10><<<nbatches, 64, 0, s>>>(nbatches, batch_rows, rowptr, colidx, inc_ptr,
+                                                      inc_ent, cmap, rarity,
+                                                      static_cast<SlotEntry<10> *>(table), d, bad);
+        break;
+      default:
+        set_error("slot tables exist for column arities 3, 4 and 10 (got %d)", carity);
+        return OP2HIP_ERR_ARG;
+    }
+    OP2HIP_LAUNCHED();
+  }
+  return OP2HIP_OK;
+}
+
+}  // extern "C"

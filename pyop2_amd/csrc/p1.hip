@@ -109,14 +109,41 @@ struct P1MatrixRow {
   const int32_t *c2v;
   template <class Emit>
   __device__ __forceinline__ void row(int32_t e, int i, Emit &&emit, double *) const {
-    const int32_t v[3] = {c2v[3 * e], c2v[3 * e + 1], c2v[3 * e + 2]};
-    const P1Geom g = p1_geom(X, v[0], v[1], v[2]);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double k = (g.b[i] * g.b[j] + g.c[i] * g.c[j]) * g.s;
-      emit(j, v[j], &k);
-    }
+    const int32_t v0 = c2v[3 * e], v1 = c2v[3 * e + 1], v2 = c2v[3 * e + 2];
+    const P1Geom g = p1_geom(X, v0, v1, v2);
+    // select row i with constant indices (a runtime index into the
+    // coefficient arrays would place them in scratch memory)
+    const double bi = i == 0 ? g.b[0] : (i == 1 ? g.b[1] : g.b[2]);
+    const double ci = i == 0 ? g.c[0] : (i == 1 ? g.c[1] : g.c[2]);
+    const double k0 = (bi * g.b[0] + ci * g.c[0]) * g.s;
+    const double k1 = (bi * g.b[1] + ci * g.c[1]) * g.s;
+    const double k2 = (bi * g.b[2] + ci * g.c[2]) * g.s;
+    emit(0, v0, &k0);
+    emit(1, v1, &k1);
+    emit(2, v2, &k2);
   }
+};
+
+// Lane-per-block row-gather element: lane j of a group computes K_e[i][j].
+struct P1MatrixLanes {
+  using T = double;
+  using Geom = P1Geom;
+  static constexpr int kRar = 3, kCar = 3, kRd = 1, kCd = 1;
+  static constexpr bool kHasRowDat = false;
+  const double *X;
+  const int32_t *c2v;
+  __device__ __forceinline__ Geom geometry(int32_t e) const {
+    return p1_geom(X, c2v[3 * e], c2v[3 * e + 1], c2v[3 * e + 2]);
+  }
+  __device__ __forceinline__ int32_t col(int32_t e, int j) const { return c2v[3 * e + j]; }
+  __device__ __forceinline__ void block(const Geom &g, int i, int j, double *blk) const {
+    const double bi = i == 0 ? g.b[0] : (i == 1 ? g.b[1] : g.b[2]);
+    const double ci = i == 0 ? g.c[0] : (i == 1 ? g.c[1] : g.c[2]);
+    const double bj = j == 0 ? g.b[0] : (j == 1 ? g.b[1] : g.b[2]);
+    const double cj = j == 0 ? g.c[0] : (j == 1 ? g.c[1] : g.c[2]);
+    blk[0] = (bi * bj + ci * cj) * g.s;
+  }
+  __device__ __forceinline__ void rowdat(int32_t, const double *, double *) const {}
 };
 
 }  // namespace op2hip
@@ -161,6 +188,14 @@ int op2hip_wrap_p1_laplace_matrix(int32_t start, int32_t end, const op2hip_mat *
   OP2HIP_REQUIRE(A && A->rowptr && A->colidx && A->values && A->status, "p1 matrix: bad Mat");
   OP2HIP_REQUIRE(A->rdim == 1 && A->cdim == 1, "p1 matrix: Mat block must be 1x1");
   if (ex && ex->strategy == OP2HIP_ROWGATHER) {
+    P1MatrixLanes elem{X, c2v};
+    return launch_rowgather_lanes(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_SLOTS) {
+    P1MatrixRow elem{X, c2v};
+    return launch_rowgather_slots(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_THREAD) {
     P1MatrixRow elem{X, c2v};
     return launch_rowgather(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
   }

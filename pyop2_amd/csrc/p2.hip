@@ -329,6 +329,45 @@ struct MatrixRow {
   }
 };
 
+// Lane-per-block row-gather element: lane j of a group computes block (i, j).
+template <typename TT, bool kFused>
+struct MatrixLanes {
+  using T = TT;
+  using Geom = p2::Geom<TT>;
+  static constexpr int kRar = 10, kCar = 10, kRd = 3, kCd = 3;
+  static constexpr bool kHasRowDat = kFused;
+  const T *X;
+  const T *u;
+  const int32_t *c2n;
+  const int32_t *c2v;
+  T lam, mu;
+  __device__ __forceinline__ Geom geometry(int32_t e) const { return p2::geometry(X, c2v, e); }
+  __device__ __forceinline__ int32_t col(int32_t e, int j) const { return c2n[10 * e + j]; }
+  __device__ __forceinline__ void block(const Geom &g, int i, int j, T *B) const {
+    T gi[4][3], gj[4][3];
+    basis_grads(g, i, gi);
+    basis_grads(g, j, gj);
+    T M[3][3] = {};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) M[a][b] += gi[q][a] * gj[q][b];
+    const T tr = mu * (M[0][0] + M[1][1] + M[2][2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        B[3 * a + b] = g.wdet * (lam * M[a][b] + mu * M[b][a] + (a == b ? tr : T(0)));
+  }
+  __device__ __forceinline__ void rowdat(int32_t c, const T *B, T *rs) const {
+    const T u0 = u[3 * c], u1 = u[3 * c + 1], u2 = u[3 * c + 2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) rs[a] += B[3 * a] * u0 + B[3 * a + 1] * u1 + B[3 * a + 2] * u2;
+  }
+};
+
 // Adjoint of the residual loop for one element (see header of the entry point).
 template <typename T, int Q>
 __device__ __forceinline__ void adjoint_q(const T (&uu)[10][3], const T (&ee)[10][3],
@@ -434,6 +473,14 @@ int matrix(int32_t start, int32_t end, const op2hip_mat *A, T *R, const T *X, co
   OP2HIP_REQUIRE(A && A->rowptr && A->colidx && A->values && A->status, "p2 matrix: bad Mat");
   OP2HIP_REQUIRE(A->rdim == 3 && A->cdim == 3, "p2 matrix: Mat block must be 3x3 (BSR)");
   if (ex && ex->strategy == OP2HIP_ROWGATHER) {
+    MatrixLanes<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    return launch_rowgather_lanes(start, end, elem, A, ex, R, s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_SLOTS) {
+    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    return launch_rowgather_slots(start, end, elem, A, ex, R, s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_THREAD) {
     MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
     return launch_rowgather(start, end, elem, A, ex, R, s);
   }

@@ -111,6 +111,8 @@ class RowBatches:
     max_inc: int
     max_nnz: int
     max_rows: int
+    slots: dict = None   # (cmap id) -> (table, descriptors)
+    max_row_nnz: int = None
 
     def exec_fields(self, ex):
         ex.batch_rows = dev.ptr(self.batch_rows)
@@ -118,6 +120,39 @@ class RowBatches:
         ex.batch_max_inc = self.max_inc
         ex.batch_max_nnz = self.max_nnz
         ex.batch_max_rows = self.max_rows
+
+    def slot_table(self, tr, rowptr, colidx, cmap, rarity):
+        """Slot table of these batches for column map ``cmap`` (built once),
+        or None when a row is too long for 8-bit slots or the arity has no
+        table layout (the caller then uses another strategy)."""
+        if self.slots is None:
+            self.slots = {}
+        if self.max_row_nnz is None:
+            self.max_row_nnz = int(np.diff(dev.to_host(rowptr)).max(initial=0))
+        if self.max_row_nnz > 255 or cmap.arity not in (3, 4, 10):
+            return None
+        hit = self.slots.get(id(cmap))
+        if hit is None:
+            L = dev.lib()
+            eb = ctypes.c_size_t()
+            dev.check(L.op2hip_slot_table_bytes(cmap.arity, ctypes.byref(eb)), "slot_table_bytes")
+            table = dev.workspace(max(tr.total, 1) * eb.value)
+            desc = dev.empty((max(self.nbatches, 1), 8), np.int32)
+            bad = dev.zeros((1,), np.int32)
+            dev.check(L.op2hip_build_slot_table(self.nbatches, dev.ptr(self.batch_rows), dev.ptr(rowptr),
+                                                dev.ptr(colidx), dev.ptr(tr.inc_ptr), dev.ptr(tr.inc_ent),
+                                                dev.ptr(cmap.device_values), rarity, cmap.arity,
+                                                dev.ptr(table), dev.ptr(desc), dev.ptr(bad), dev.stream()),
+                      "build_slot_table")
+            if int(bad.item()):
+                from pyop2_amd.exceptions import PatternError
+                raise PatternError(f"{int(bad.item())} map entries are not in the sparsity pattern")
+            hit = self.slots[id(cmap)] = (table, desc, cmap)
+        return hit
+
+    def slot_fields(self, ex, hit):
+        ex.slot_table = dev.ptr(hit[0])
+        ex.batch_desc = dev.ptr(hit[1])
 
 
 def row_batches(owner, tr, rowptr, max_inc, max_nnz, key):

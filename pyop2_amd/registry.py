@@ -70,9 +70,9 @@ FAMILIES = {
 # Row-batch budgets of the row-gather assembly (LDS <= 64 KiB per workgroup).
 _BATCH = {("p1_laplace_matrix", "f64"): (1024, 2048),
           ("p2_elasticity_matrix", "f64"): (512, 640),
-          ("p2_elasticity_matrix", "f32"): (512, 1200),
+          ("p2_elasticity_matrix", "f32"): (512, 640),
           ("p2_elasticity_fused", "f64"): (512, 640),
-          ("p2_elasticity_fused", "f32"): (512, 1200)}
+          ("p2_elasticity_fused", "f32"): (512, 640)}
 
 
 def _variant(dtypes):
@@ -143,11 +143,11 @@ class HandWrittenKernel:
         if want == "auto":
             from pyop2_amd import autotune
             want = autotune.choice(self, parloop)
-        if want == "rowgather" and (not has_mat or subset):
+        if want.startswith("rowgather") and (not has_mat or subset):
             want = "atomic"
         if want == "staged":
             want = "atomic"
-        if has_mat and want == "rowgather":
+        if has_mat and want.startswith("rowgather"):
             mat = parloop.arguments[0].data
             if mat.block_shape != self.spec["args"][0][2]:
                 want = "atomic"
@@ -195,7 +195,7 @@ class HandWrittenKernel:
         if self.spec["args"][0][0] == "mat":
             mat = args[0].data
             rmap = args[0].maps[0]
-            if strategy == "rowgather":
+            if strategy.startswith("rowgather"):
                 from pyop2_amd import plans
                 tr = plans.transpose(rmap, mat.sparsity.nrows, rmap.iterset.size)
                 budget = _BATCH[(self.family, self.variant)]
@@ -203,6 +203,13 @@ class HandWrittenKernel:
                                        (id(mat.sparsity), mat.block_shape, self.variant))
                 rb.exec_fields(ex)
                 keep.append(rb)
+                if strategy == "rowgather_slots":
+                    hit = rb.slot_table(tr, mat.rowptr, mat.colidx, args[0].maps[1], rmap.arity)
+                    if hit is None:
+                        strategy = "rowgather_thread"
+                        ex.strategy = dev.STRATEGIES[strategy]
+                    else:
+                        rb.slot_fields(ex, hit)
                 desc = mat.descriptor(tr.inc_ptr, tr.inc_ent)
             else:
                 desc = mat.descriptor()

@@ -41,7 +41,7 @@ def test_p2_residual_f32(device, tet8):
     assert relerr(R.data_ro, ref) < 1e-5
 
 
-@pytest.mark.parametrize("strategy", ["rowgather", "atomic", "coloured"])
+@pytest.mark.parametrize("strategy", ["rowgather_slots", "rowgather_thread", "rowgather", "atomic", "coloured"])
 def test_p2_bsr_assembly_f64(device, tet8, strategy):
     configuration["strategy"] = strategy
     p = TetProblem(tet8)
@@ -54,7 +54,9 @@ def test_p2_bsr_assembly_f64(device, tet8, strategy):
     assert relerr(v, ref.values) < 1e-12
 
 
-def test_p2_fused_f64(device, tet8):
+@pytest.mark.parametrize("strategy", ["rowgather_slots", "rowgather_thread", "atomic"])
+def test_p2_fused_f64(device, tet8, strategy):
+    configuration["strategy"] = strategy
     p = TetProblem(tet8)
     A = op2.Mat(p.sparsity(), np.float64)
     R = op2.Dat(p.nodes ** 3, name="R")
@@ -67,7 +69,9 @@ def test_p2_fused_f64(device, tet8):
     assert relerr(R.data_ro, oracle_p2("p2_elasticity_residual", tet8)) < 1e-12
 
 
-def test_p2_bsr_assembly_f32(device, tet8):
+@pytest.mark.parametrize("strategy", ["rowgather_slots", "rowgather_thread", "atomic"])
+def test_p2_bsr_assembly_f32(device, tet8, strategy):
+    configuration["strategy"] = strategy
     p = TetProblem(tet8, dtype=np.float32)
     A = op2.Mat(p.sparsity(), np.float32)
     op2.par_loop(elements.p2_elasticity_matrix(), p.cells, A(op2.INC, (p.c2n, p.c2n)), p.X(op2.READ, p.c2v))
