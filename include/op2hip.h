@@ -48,8 +48,11 @@ enum op2hip_strategy {
   OP2HIP_ROWGATHER = 3,  /* Mat only: owner-computes block rows, no atomics;
                             one lane per (row incidence, column block) */
   OP2HIP_ROWGATHER_THREAD = 4, /* as ROWGATHER, one thread per row incidence */
-  OP2HIP_ROWGATHER_SLOTS = 5   /* as ROWGATHER_THREAD, with a precomputed slot
+  OP2HIP_ROWGATHER_SLOTS = 5,  /* as ROWGATHER_THREAD, with a precomputed slot
                                   table (no pattern search in the kernel) */
+  OP2HIP_ROWGATHER_STAGED = 6  /* slot-table plan; element block rows staged in
+                                  LDS, each output block summed by one thread
+                                  (no atomics, deterministic) */
 };
 
 /* Execution plan of one launch.  Host struct; arrays as annotated. */
@@ -75,6 +78,9 @@ typedef struct op2hip_exec {
   int32_t batch_max_rows;          /* max rows of a batch */
   const void *batch_desc;          /* device, per batch {r0, nr, z0, nz, i0, ni, 0, 0} (int32) */
   const void *slot_table;          /* device, per row incidence: op2hip_slot_table layout */
+  /* OP2HIP_ROWGATHER_STAGED: gather table (op2hip_gather_table_fill) */
+  const uint8_t *gather_count;     /* device, per Mat block: element blocks added to it */
+  const uint16_t *gather_list;     /* device, per batch: element block positions */
   /* Subset iteration: element n = subset[i] for i in [start,end) (or NULL) */
   const int32_t *subset;           /* device */
 } op2hip_exec;
@@ -182,8 +188,10 @@ int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr,
  * row incidence t (inc_ent order), an entry of entry_bytes(carity) bytes:
  * uint16 block offset of the row in its batch, uint16 row index in the
  * batch, then carity uint8 positions of column block j in the row (255: no
- * column or not in the pattern, counted in the device int32 *bad, which the
- * call zeroes first). */
+ * column or not in the pattern).  bad points to two device int32 counters,
+ * zeroed first: [0] map entries not in the pattern, [1] row incidences whose
+ * element names one column twice (OP2HIP_ROWGATHER_STAGED, which adds one
+ * block per incidence and output, does not take such a table). */
 int op2hip_slot_table_bytes(int32_t carity, size_t *entry_bytes);
 int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows,
                             const int32_t *rowptr, const int32_t *colidx,
@@ -191,6 +199,24 @@ int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows,
                             const int32_t *cmap, int32_t rarity, int32_t carity,
                             void *table, void *batch_desc, int32_t *bad,
                             void *stream);
+
+/* Gather table of OP2HIP_ROWGATHER_STAGED, from a slot table and its batch
+ * descriptors (batches of at most 1024 blocks).  Output block z of a batch
+ * receives the element blocks (t, j) -- row incidence t of the batch, column
+ * block j -- whose slot is z.  Count: gcount[z0 + z] (uint8, one per Mat
+ * block) = their number, batch_total[b] (device int32) = the batch's sum,
+ * *over (device int32, zeroed first) = blocks with more than 255.  Fill,
+ * given batch_off = exclusive prefix sum of batch_total: the batch's list at
+ * glist + batch_off[b] holds t * carity + j (uint16) block by block, in
+ * incidence order; the descriptors' last two fields become (batch_off[b],
+ * batch_total[b]). */
+int op2hip_gather_table_count(int32_t nbatches, int32_t batch_max_nnz, const void *batch_desc,
+                              const void *table, int32_t carity, uint8_t *gcount,
+                              int32_t *batch_total, int32_t *over, void *stream);
+int op2hip_gather_table_fill(int32_t nbatches, int32_t batch_max_nnz, void *batch_desc,
+                             const void *table, int32_t carity, const int32_t *batch_off,
+                             const int32_t *batch_total, const uint8_t *gcount, uint16_t *glist,
+                             void *stream);
 
 /* ---- hand-written element loops ---------------------------------------- */
 /* P1 triangles, op2.Kernel names p1_lumped_mass / p1_laplace_residual /

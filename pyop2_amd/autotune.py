@@ -54,7 +54,7 @@ def candidates(executor, parloop):
     if has_mat:
         if isinstance(parloop.iterset, Subset):
             return ["atomic", "coloured"]
-        return ["rowgather_slots", "rowgather_thread", "rowgather", "atomic"]
+        return ["rowgather_staged", "rowgather_slots", "rowgather_thread", "rowgather", "atomic"]
     return ["atomic", "coloured"]
 
 

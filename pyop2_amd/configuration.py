@@ -66,7 +66,8 @@ class Configuration(dict):
                 raise ConfigurationError("Values for configuration key %s must be of type %r, not %r"
                                          % (key, valid_type, type(value)))
         if key == "strategy" and value not in ("auto", "atomic", "coloured", "staged", "rowgather",
-                                               "rowgather_thread", "rowgather_slots"):
+                                               "rowgather_thread", "rowgather_slots",
+                                               "rowgather_staged"):
             raise ConfigurationError(f"unknown strategy {value!r}")
         self._set.add(key)
         super().__setitem__(key, value)

@@ -632,7 +632,7 @@ int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows, const i
   OP2HIP_REQUIRE(nbatches >= 0 && batch_rows && rowptr && colidx && inc_ptr && inc_ent && cmap &&
                      table && batch_desc && rarity > 0 && bad,
                  "build_slot_table: bad arguments");
-  int rc = fill(bad, 1, 0, s);
+  int rc = fill(bad, 2, 0, s);
   if (rc) return rc;
   if (nbatches > 0) {
     auto *d = static_cast<BatchDesc *>(batch_desc);
@@ -658,6 +658,76 @@ int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows, const i
     }
     OP2HIP_LAUNCHED();
   }
+  return OP2HIP_OK;
+}
+
+int op2hip_gather_table_count(int32_t nbatches, int32_t batch_max_nnz, const void *batch_desc,
+                              const void *table, int32_t carity, uint8_t *gcount,
+                              int32_t *batch_total, int32_t *over, void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nbatches >= 0 && batch_desc && table && gcount && batch_total && over,
+                 "gather_table_count: bad arguments");
+  OP2HIP_REQUIRE(batch_max_nnz <= kGatherMaxBlocks,
+                 "gather tables take batches of at most %d blocks (got %d)", kGatherMaxBlocks,
+                 batch_max_nnz);
+  int rc = fill(over, 1, 0, s);
+  if (rc || nbatches == 0) return rc;
+  const auto *d = static_cast<const BatchDesc *>(batch_desc);
+  switch (carity) {
+    case 3:
+      gather_count_kernel<3><<<nbatches, 256, 0, s>>>(
+          nbatches, d, static_cast<const SlotEntry<3> *>(table), gcount, batch_total, over);
+      break;
+    case 4:
+      gather_count_kernel<4><<<nbatches, 256, 0, s>>>(
+          nbatches, d, static_cast<const SlotEntry<4> *>(table), gcount, batch_total, over);
+      break;
+    case 10:
+      gather_count_kernel<10><<<nbatches, 256, 0, s>>>(
+          nbatches, d, static_cast<const SlotEntry<10> *>(table), gcount, batch_total, over);
+      break;
+    default:
+      set_error("gather tables exist for column arities 3, 4 and 10 (got %d)", carity);
+      return OP2HIP_ERR_ARG;
+  }
+  OP2HIP_LAUNCHED();
+  return OP2HIP_OK;
+}
+
+int op2hip_gather_table_fill(int32_t nbatches, int32_t batch_max_nnz, void *batch_desc,
+                             const void *table, int32_t carity, const int32_t *batch_off,
+                             const int32_t *batch_total, const uint8_t *gcount, uint16_t *glist,
+                             void *stream) {
+  hipStream_t s = stream_of(stream);
+  OP2HIP_REQUIRE(nbatches >= 0 && batch_desc && table && batch_off && batch_total && gcount &&
+                     glist,
+                 "gather_table_fill: bad arguments");
+  OP2HIP_REQUIRE(batch_max_nnz <= kGatherMaxBlocks,
+                 "gather tables take batches of at most %d blocks (got %d)", kGatherMaxBlocks,
+                 batch_max_nnz);
+  if (nbatches == 0) return OP2HIP_OK;
+  auto *d = static_cast<BatchDesc *>(batch_desc);
+  switch (carity) {
+    case 3:
+      gather_fill_kernel<3><<<nbatches, 256, 0, s>>>(nbatches, d,
+                                                      static_cast<const SlotEntry<3> *>(table),
+                                                      batch_off, batch_total, gcount, glist);
+      break;
+    case 4:
+      gather_fill_kernel<4><<<nbatches, 256, 0, s>>>(nbatches, d,
+                                                      static_cast<const SlotEntry<4> *>(table),
+                                                      batch_off, batch_total, gcount, glist);
+      break;
+    case 10:
+      gather_fill_kernel<10><<<nbatches, 256, 0, s>>>(nbatches, d,
+                                                       static_cast<const SlotEntry<10> *>(table),
+                                                       batch_off, batch_total, gcount, glist);
+      break;
+    default:
+      set_error("gather tables exist for column arities 3, 4 and 10 (got %d)", carity);
+      return OP2HIP_ERR_ARG;
+  }
+  OP2HIP_LAUNCHED();
   return OP2HIP_OK;
 }
 

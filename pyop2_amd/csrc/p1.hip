@@ -105,6 +105,7 @@ struct P1MatrixRow {
   using T = double;
   static constexpr int kRar = 3, kCar = 3, kRd = 1, kCd = 1;
   static constexpr bool kHasRowDat = false;
+  static constexpr int kGroups = 1;
   const double *X;
   const int32_t *c2v;
   template <class Emit>
@@ -121,6 +122,10 @@ struct P1MatrixRow {
     emit(0, v0, &k0);
     emit(1, v1, &k1);
     emit(2, v2, &k2);
+  }
+  template <int G, class Emit>
+  __device__ __forceinline__ void row_group(int32_t e, int i, Emit &&emit, double *rs) const {
+    row(e, i, emit, rs);
   }
 };
 
@@ -194,6 +199,10 @@ int op2hip_wrap_p1_laplace_matrix(int32_t start, int32_t end, const op2hip_mat *
   if (ex && ex->strategy == OP2HIP_ROWGATHER_SLOTS) {
     P1MatrixRow elem{X, c2v};
     return launch_rowgather_slots(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_STAGED) {
+    P1MatrixRow elem{X, c2v};
+    return launch_rowgather_staged(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
   }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_THREAD) {
     P1MatrixRow elem{X, c2v};

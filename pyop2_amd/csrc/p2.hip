@@ -213,43 +213,111 @@ __device__ __forceinline__ void basis_grads(const Geom<T> &g, int i, T (&gi)[4][
   }
 }
 
-// Block (i, j) of the element matrix from the quadrature-point gradients of i.
+// Block row i of the element matrix in closed form.  With gi_q the gradient
+// of basis i at point q and L_qn the barycentric coordinates of the points,
+// let G_n = sum_q L_qn gi_q and S = sum_q gi_q.  The rule's moments
+// sum_q L_qn = 1, sum_q L_qn^2 = 0.4, sum_q L_qn L_qm = 0.2 (n != m) give
+//   vertex row i:       4 G_n = (3.2 [n == i] - 0.8) gl_i,   S = 0,
+//   edge row i = (a,b): 4 G_n = 3.2 (gl_a + gl_b) + 3.2 ([n == b] gl_a + [n == a] gl_b),
+//                       S = 4 (gl_a + gl_b),
+// and M_ij = sum_q gi_q (x) gj_q = sum_n (sum_q D[q][j][n] gi_q) (x) gl_n is
+//   vertex column j:    M_ij = (4 G_j - S) (x) gl_j,
+//   edge column (c,d):  M_ij = 4 G_d (x) gl_c + 4 G_c (x) gl_d.
+template <typename T>
+struct RowCoef {
+  T G4[4][3];  // 4 G_n
+  T S[3];
+  T lw, mw;    // wdet * lam, wdet * mu
+};
+
+template <typename T>
+__device__ __forceinline__ T gl_of(const Geom<T> &g, int v, int k) {
+  // select among values (a runtime index, or a select among the array's
+  // elements as lvalues, would place the geometry in scratch memory)
+  const T g0 = g.gl[0][k], g1 = g.gl[1][k], g2 = g.gl[2][k], g3 = g.gl[3][k];
+  const T lo = v == 0 ? g0 : g1;
+  const T hi = v == 2 ? g2 : g3;
+  return v < 2 ? lo : hi;
+}
+
+template <typename T>
+__device__ __forceinline__ RowCoef<T> row_coefficients(const Geom<T> &g, int i, T lam, T mu) {
+  const bool edge = i >= 4;
+  const int ei = i - 4;
+  const int a = edge ? (ei < 3 ? 0 : (ei < 5 ? 1 : 2)) : i;
+  const int b = edge ? (ei < 3 ? ei + 1 : (ei < 5 ? ei - 1 : 3)) : i;
+  const T c0 = edge ? T(3.2) : T(-0.8);
+  const T c1 = edge ? T(3.2) : T(1.6);
+  const T cs = edge ? T(4) : T(0);
+  RowCoef<T> r;
+  T ga[3], gb[3], w[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ga[k] = gl_of(g, a, k);
+    gb[k] = gl_of(g, b, k);
+    w[k] = edge ? ga[k] + gb[k] : ga[k];
+    r.S[k] = cs * w[k];
+  }
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      r.G4[n][k] = c0 * w[k] + c1 * ((n == b ? ga[k] : T(0)) + (n == a ? gb[k] : T(0)));
+  r.lw = g.wdet * lam;
+  r.mw = g.wdet * mu;
+  return r;
+}
+
+// Block (i, j) = wdet (lam M + mu M^T + mu tr(M) I) of row i's coefficients.
 template <typename T, int J>
-__device__ __forceinline__ void matrix_block(const T (&gi)[4][3], const Geom<T> &g, T lam, T mu,
-                                             T (&B)[9]) {
-  T M[3][3] = {};
+__device__ __forceinline__ void closed_block(const RowCoef<T> &r, const Geom<T> &g, T (&B)[9]) {
+  T M[3][3];
+  if constexpr (J < 4) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    T gj[3];
+    for (int a = 0; a < 3; ++a) {
+      const T h = r.G4[J][a] - r.S[a];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      T acc = T(0);
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-        if (dcoef(q, J, m) != 0.0) acc += T(dcoef(q, J, m)) * g.gl[m][k];
-      gj[k] = acc;
+      for (int b = 0; b < 3; ++b) M[a][b] = h * g.gl[J][b];
     }
+  } else {
+    constexpr int c = kEa[J - 4], d = kEb[J - 4];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) M[a][b] += gi[q][a] * gj[b];
+      for (int b = 0; b < 3; ++b) M[a][b] = r.G4[d][a] * g.gl[c][b] + r.G4[c][a] * g.gl[d][b];
   }
-  const T tr = mu * (M[0][0] + M[1][1] + M[2][2]);
+  const T tr = r.mw * (M[0][0] + M[1][1] + M[2][2]);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b)
-      B[3 * a + b] = g.wdet * (lam * M[a][b] + mu * M[b][a] + (a == b ? tr : T(0)));
+      B[3 * a + b] = r.lw * M[a][b] + r.mw * M[b][a] + (a == b ? tr : T(0));
 }
 
 template <typename T, int J, class F>
-__device__ __forceinline__ void for_blocks(const T (&gi)[4][3], const Geom<T> &g, T lam, T mu,
-                                           F &&f) {
+__device__ __forceinline__ void for_blocks(const RowCoef<T> &r, const Geom<T> &g, F &&f) {
   T B[9];
-  matrix_block<T, J>(gi, g, lam, mu, B);
+  closed_block<T, J>(r, g, B);
   f(J, B);
-  if constexpr (J + 1 < 10) for_blocks<T, J + 1>(gi, g, lam, mu, f);
+  if constexpr (J + 1 < 10) for_blocks<T, J + 1>(r, g, f);
 }
+
+// The blocks j with bit j of kMask set.
+template <typename T, unsigned kMask, int J = 0, class F>
+__device__ __forceinline__ void for_blocks_in(const RowCoef<T> &r, const Geom<T> &g, F &&f) {
+  if constexpr (J < 10) {
+    if constexpr (((kMask >> J) & 1u) != 0) {
+      T B[9];
+      closed_block<T, J>(r, g, B);
+      f(J, B);
+    }
+    for_blocks_in<T, kMask, J + 1>(r, g, f);
+  }
+}
+
+// Column groups of the staged assembly, balanced by cost (an edge column
+// block costs about 4/3 of a vertex one): {0,1,4}, {2,3,5}, {6,7}, {8,9}.
+constexpr unsigned kColumnGroups[4] = {0x013u, 0x02Cu, 0x0C0u, 0x300u};
 
 // Cell-centric BSR(3) assembly: block (i, j) -> slot by binary search of row
 // c2n[i]'s colidx, 9 atomic (or coloured plain) adds.
@@ -269,11 +337,10 @@ matrix_kernel(int32_t lo, int32_t hi, const int32_t *__restrict__ list, op2hip_m
   for (int i = 0; i < 10; ++i) {
     const int32_t r = nd[i];
     if (r < 0) continue;
-    T gi[4][3];
-    basis_grads(g, i, gi);
+    const RowCoef<T> rc = row_coefficients(g, i, lam, mu);
     const int32_t rlo = A.rowptr[r], rhi = A.rowptr[r + 1];
     T rs[3] = {T(0), T(0), T(0)};
-    for_blocks<T, 0>(gi, g, lam, mu, [&](int j, const T (&B)[9]) {
+    for_blocks<T, 0>(rc, g, [&](int j, const T (&B)[9]) {
       const int32_t c = nd[j];
       if (c < 0) return;
       if (kFused) {
@@ -303,6 +370,8 @@ struct MatrixRow {
   using T = TT;
   static constexpr int kRar = 10, kCar = 10, kRd = 3, kCd = 3;
   static constexpr bool kHasRowDat = kFused;
+  // 24 prefetched doubles would take the fused kernel past 168 registers
+  static constexpr int kMaxPrefetch = (kFused && sizeof(TT) == 8) ? 16 : 24;
   const T *X;
   const T *u;
   const int32_t *c2n;
@@ -314,9 +383,28 @@ struct MatrixRow {
 #pragma unroll
     for (int j = 0; j < 10; ++j) nd[j] = c2n[10 * e + j];
     const Geom<T> g = geometry(X, c2v, e);
-    T gi[4][3];
-    basis_grads(g, i, gi);
-    for_blocks<T, 0>(gi, g, lam, mu, [&](int j, const T (&B)[9]) {
+    const RowCoef<T> rc = row_coefficients(g, i, lam, mu);
+    for_blocks<T, 0>(rc, g, [&](int j, const T (&B)[9]) {
+      const int32_t c = nd[j];
+      if (kFused && c >= 0) {
+        const T u0 = u[3 * c], u1 = u[3 * c + 1], u2 = u[3 * c + 2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          rsum[a] += B[3 * a] * u0 + B[3 * a + 1] * u1 + B[3 * a + 2] * u2;
+      }
+      emit(j, c, B);
+    });
+  }
+  static constexpr int kGroups = 4;
+  template <int G, class Emit>
+  __device__ __forceinline__ void row_group(int32_t e, int i, Emit &&emit, T *rsum) const {
+    int32_t nd[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+      if ((kColumnGroups[G] >> j) & 1u) nd[j] = c2n[10 * e + j];
+    const Geom<T> g = geometry(X, c2v, e);
+    const RowCoef<T> rc = row_coefficients(g, i, lam, mu);
+    for_blocks_in<T, kColumnGroups[G]>(rc, g, [&](int j, const T (&B)[9]) {
       const int32_t c = nd[j];
       if (kFused && c >= 0) {
         const T u0 = u[3 * c], u1 = u[3 * c + 1], u2 = u[3 * c + 2];
@@ -479,6 +567,10 @@ int matrix(int32_t start, int32_t end, const op2hip_mat *A, T *R, const T *X, co
   if (ex && ex->strategy == OP2HIP_ROWGATHER_SLOTS) {
     MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
     return launch_rowgather_slots(start, end, elem, A, ex, R, s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_STAGED) {
+    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    return launch_rowgather_staged(start, end, elem, A, ex, R, s);
   }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_THREAD) {
     MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};

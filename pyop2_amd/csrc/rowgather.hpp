@@ -364,14 +364,105 @@ __global__ void build_slot_table(int32_t nbatches, const int32_t *__restrict__ b
       SlotEntry<kCar> s;
       s.row_off = static_cast<uint16_t>(zlo - rowptr[r0]);
       s.lr = static_cast<uint16_t>(r - r0);
+      bool repeated = false;
 #pragma unroll
       for (int j = 0; j < kCar; ++j) {
         const int32_t c = cmap[(int64_t)e * kCar + j];
         const int32_t p = c < 0 ? -1 : find_slot(colidx, zlo, zhi, c);
         if (c >= 0 && p < 0) atomicAdd(bad, 1);
         s.slot[j] = (p < 0 || p - zlo > 254) ? uint8_t(255) : static_cast<uint8_t>(p - zlo);
+#pragma unroll
+        for (int jj = 0; jj < j; ++jj) repeated |= s.slot[j] != 255 && s.slot[jj] == s.slot[j];
       }
+      if (repeated) atomicAdd(bad + 1, 1);
       table[t] = s;
+    }
+  }
+}
+
+// Gather table of the staged row gather, built from a slot table.  Output
+// block z of a batch (its z-th Mat block) receives the element blocks
+// (t, j) -- row incidence t of the batch, column block j -- whose slot is z;
+// gather_count[z0 + z] is their number and the batch's gather list holds
+// t * kCar + j for z = 0, 1, ... in turn, each run in incidence order.
+constexpr int kGatherMaxBlocks = 1024;   // blocks of a batch the plan kernels take
+
+template <int kCar>
+__global__ void __launch_bounds__(256)
+gather_count_kernel(int32_t nbatches, const BatchDesc *__restrict__ desc,
+                    const SlotEntry<kCar> *__restrict__ table, uint8_t *__restrict__ gcount,
+                    int32_t *__restrict__ batch_total, int32_t *__restrict__ over) {
+  __shared__ int32_t cnt[kGatherMaxBlocks];
+  __shared__ int32_t total;
+  const int32_t b = blockIdx.x;
+  if (b >= nbatches) return;
+  const BatchDesc d = desc[b];
+  for (int32_t z = threadIdx.x; z < d.nz; z += blockDim.x) cnt[z] = 0;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  for (int32_t q = threadIdx.x; q < d.ni * kCar; q += blockDim.x) {
+    const int32_t t = q / kCar, j = q - t * kCar;
+    const SlotEntry<kCar> se = table[d.i0 + t];
+    if (se.slot[j] != 255) atomicAdd(&cnt[se.row_off + se.slot[j]], 1);
+  }
+  __syncthreads();
+  int32_t mine = 0;
+  for (int32_t z = threadIdx.x; z < d.nz; z += blockDim.x) {
+    const int32_t c = cnt[z];
+    if (c > 255) atomicAdd(over, 1);
+    gcount[(size_t)d.z0 + z] = static_cast<uint8_t>(c > 255 ? 255 : c);
+    mine += c;
+  }
+  atomicAdd(&total, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) batch_total[b] = total;
+}
+
+template <int kCar>
+__global__ void __launch_bounds__(256)
+gather_fill_kernel(int32_t nbatches, BatchDesc *__restrict__ desc,
+                   const SlotEntry<kCar> *__restrict__ table, const int32_t *__restrict__ batch_off,
+                   const int32_t *__restrict__ batch_total, const uint8_t *__restrict__ gcount,
+                   uint16_t *__restrict__ glist) {
+  __shared__ int32_t first[kGatherMaxBlocks];
+  __shared__ int32_t cursor[kGatherMaxBlocks];
+  const int32_t b = blockIdx.x;
+  if (b >= nbatches) return;
+  const BatchDesc d = desc[b];
+  const int32_t off = batch_off[b];
+  if (threadIdx.x == 0) {
+    desc[b].pad0 = off;
+    desc[b].pad1 = batch_total[b];
+    int32_t acc = 0;
+    for (int32_t z = 0; z < d.nz; ++z) {
+      first[z] = acc;
+      cursor[z] = acc;
+      acc += gcount[(size_t)d.z0 + z];
+    }
+  }
+  __syncthreads();
+  uint16_t *list = glist + off;
+  for (int32_t q = threadIdx.x; q < d.ni * kCar; q += blockDim.x) {
+    const int32_t t = q / kCar, j = q - t * kCar;
+    const SlotEntry<kCar> se = table[d.i0 + t];
+    if (se.slot[j] != 255) {
+      const int32_t p = atomicAdd(&cursor[se.row_off + se.slot[j]], 1);
+      list[p] = static_cast<uint16_t>(q);
+    }
+  }
+  __syncthreads();
+  // incidence order within each block's run (runs are short: insertion sort)
+  for (int32_t z = threadIdx.x; z < d.nz; z += blockDim.x) {
+    uint16_t *run = list + first[z];
+    const int32_t n = gcount[(size_t)d.z0 + z];
+    for (int32_t x = 1; x < n; ++x) {
+      const uint16_t v = run[x];
+      int32_t y = x - 1;
+      while (y >= 0 && run[y] > v) {
+        run[y + 1] = run[y];
+        --y;
+      }
+      run[y + 1] = v;
     }
   }
 }
@@ -454,6 +545,17 @@ rowgather_slots_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
   }
 }
 
+// Deepest old-value prefetch of an element (Elem::kMaxPrefetch when given):
+// the register count grows by two (double) per prefetched value.
+template <class Elem, class = void>
+struct MaxPrefetch {
+  static constexpr int64_t value = 24;
+};
+template <class Elem>
+struct MaxPrefetch<Elem, std::void_t<decltype(Elem::kMaxPrefetch)>> {
+  static constexpr int64_t value = Elem::kMaxPrefetch;
+};
+
 template <class Elem>
 int launch_rowgather_slots(int32_t start, int32_t end, const Elem &elem, const op2hip_mat *A,
                            const op2hip_exec *ex, typename Elem::T *rowdat, hipStream_t s) {
@@ -470,9 +572,232 @@ int launch_rowgather_slots(int32_t start, int32_t end, const Elem &elem, const o
   OP2HIP_REQUIRE(lds <= 64 * 1024, "row batch needs %zu bytes of LDS (max 65536)", lds);
   const int bs = block_size_of(ex, 256);
   OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % 64 == 0, "row-gather block size %d", bs);
-  rowgather_slots_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(
-      start, end, elem, *A, static_cast<const BatchDesc *>(ex->batch_desc), ex->nbatches,
-      static_cast<const SlotEntry<Elem::kCar> *>(ex->slot_table), rowdat);
+  // old values of the whole batch are loaded into registers before the
+  // element work starts (up to 24 per thread, which keeps the register count
+  // below the LDS-bound occupancy of these kernels)
+  const auto *descs = static_cast<const BatchDesc *>(ex->batch_desc);
+  const auto *table = static_cast<const SlotEntry<Elem::kCar> *>(ex->slot_table);
+  const int64_t per_thread = std::min<int64_t>(
+      ((int64_t)ex->batch_max_nnz * Elem::kRd * Elem::kCd + bs - 1) / bs, MaxPrefetch<Elem>::value);
+  if (per_thread <= 8)
+    rowgather_slots_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
+                                                                   ex->nbatches, table, rowdat);
+  else if (per_thread <= 16)
+    rowgather_slots_kernel<Elem, 16><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
+                                                                    ex->nbatches, table, rowdat);
+  else
+    rowgather_slots_kernel<Elem, 24><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
+                                                                    ex->nbatches, table, rowdat);
+  OP2HIP_LAUNCHED();
+  return OP2HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Staged row gather: no atomics.  A batch holds at most blockDim / kGroups row
+// incidences.  Phase 1 computes the element block rows of the batch's
+// incidences into LDS; the element's column blocks are split into kGroups
+// groups and thread group g (whole waves) computes group g's blocks for every
+// incidence, so each wave runs one group's code.  Phase 2 gives every output
+// block of the batch to one thread, which adds the element blocks its gather
+// list names (in incidence = element order) to the old value and writes the
+// block once.  No value depends on scheduling.
+//
+// Elem must additionally provide
+//   static constexpr int kGroups;
+//   template <int G, class Emit> void row_group(int32_t e, int i, Emit &&, T *rsum) const;
+// where row_group<G> emits the column blocks of group G of row i of e.
+
+template <class Elem, int G = 0, class Emit>
+__device__ __forceinline__ void row_group_of(const Elem &elem, int g, int32_t e, int i,
+                                             Emit &&emit, typename Elem::T *rsum) {
+  if constexpr (G < Elem::kGroups) {
+    if (g == G) elem.template row_group<G>(e, i, emit, rsum);
+    else row_group_of<Elem, G + 1>(elem, g, e, i, emit, rsum);
+  }
+}
+
+// LDS of the staged kernel: element block rows | row sums (fused) | gather
+// list of the batch | wave totals of the block scan | row incidence offsets.
+template <class Elem>
+struct StagedLayout {
+  using T = typename Elem::T;
+  static constexpr int kBs = Elem::kRd * Elem::kCd;
+  __host__ __device__ static size_t rowbuf(int per_pass) {
+    return (size_t)per_pass * Elem::kCar * kBs * sizeof(T);
+  }
+  __host__ __device__ static size_t rsbuf(int per_pass) {
+    return Elem::kHasRowDat ? (size_t)Elem::kGroups * per_pass * Elem::kRd * sizeof(T) : 0;
+  }
+  __host__ __device__ static size_t list_at(int per_pass) {
+    return (rowbuf(per_pass) + rsbuf(per_pass) + 15) & ~(size_t)15;
+  }
+  __host__ __device__ static size_t wtot_at(int per_pass) {
+    return (list_at(per_pass) + (size_t)per_pass * Elem::kCar * sizeof(uint16_t) + 15) & ~(size_t)15;
+  }
+  __host__ __device__ static size_t inc_at(int per_pass, int kOut) {
+    return wtot_at(per_pass) + (size_t)kOut * 4 * sizeof(int32_t);
+  }
+  static size_t bytes(int per_pass, int kOut, int32_t max_rows) {
+    return inc_at(per_pass, kOut) + (Elem::kHasRowDat ? (size_t)(max_rows + 1) * sizeof(int32_t) : 0);
+  }
+};
+
+template <class Elem, int kOut>
+__global__ void __launch_bounds__(256)
+rowgather_staged_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
+                        const BatchDesc *__restrict__ descs, int32_t nbatches,
+                        const uint8_t *__restrict__ gcount, const uint16_t *__restrict__ glist,
+                        typename Elem::T *__restrict__ rowdat) {
+  using T = typename Elem::T;
+  using Acc = typename AccumOf<T>::type;
+  using L = StagedLayout<Elem>;
+  constexpr int kBs = Elem::kRd * Elem::kCd;
+  constexpr int kCar = Elem::kCar;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int per_pass = blockDim.x / Elem::kGroups;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const BatchDesc d = descs[xcd_contiguous(blockIdx.x, nbatches)];
+  T *__restrict__ vals = static_cast<T *>(A.values) + (size_t)d.z0 * kBs;
+
+  T *rowbuf = reinterpret_cast<T *>(smem);
+  T *rsbuf = rowbuf + (size_t)per_pass * kCar * kBs;
+  uint16_t *slist = reinterpret_cast<uint16_t *>(smem + L::list_at(per_pass));
+  int32_t *wtot = reinterpret_cast<int32_t *>(smem + L::wtot_at(per_pass));
+  int32_t *sinc = reinterpret_cast<int32_t *>(smem + L::inc_at(per_pass, kOut));
+
+  // loads in flight during phase 1: old values and gather counts of this
+  // thread's output blocks, the batch's gather list (pad1 <= ni * kCar)
+  T old[kOut][kBs];
+  int32_t cnt[kOut];
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    const int32_t z = threadIdx.x + o * (int32_t)blockDim.x;
+    cnt[o] = z < d.nz ? int32_t(gcount[(size_t)d.z0 + z]) : 0;
+#pragma unroll
+    for (int k = 0; k < kBs; ++k) old[o][k] = z < d.nz ? vals[(size_t)z * kBs + k] : T(0);
+  }
+  for (int32_t k = threadIdx.x; k < d.pad1; k += blockDim.x) slist[k] = glist[(size_t)d.pad0 + k];
+  if (Elem::kHasRowDat)
+    for (int32_t k = threadIdx.x; k <= d.nr; k += blockDim.x) sinc[k] = A.inc_ptr[d.r0 + k] - d.i0;
+
+  // phase 1: element block rows into LDS
+  const int g = threadIdx.x / per_pass;
+  const int t = threadIdx.x - g * per_pass;
+  if (t < d.ni) {
+    const int32_t ent = A.inc_ent[d.i0 + t];
+    const int32_t e = ent / Elem::kRar;
+    const int i = ent - e * Elem::kRar;
+    T *rb = rowbuf + (size_t)t * kCar * kBs;
+    T rsum[Elem::kRd];
+#pragma unroll
+    for (int a = 0; a < Elem::kRd; ++a) rsum[a] = T(0);
+    auto emit = [&](int j, int32_t, const T *blk) {
+#pragma unroll
+      for (int k = 0; k < kBs; ++k) rb[j * kBs + k] = blk[k];
+    };
+    if (e >= start && e < end) {
+      row_group_of(elem, g, e, i, emit, rsum);
+    } else if (g == 0) {
+      // outside the iteration range: the incidence adds zero blocks
+      for (int k = 0; k < kCar * kBs; ++k) rb[k] = T(0);
+    }
+    if (Elem::kHasRowDat) {
+#pragma unroll
+      for (int a = 0; a < Elem::kRd; ++a) rsbuf[((size_t)g * per_pass + t) * Elem::kRd + a] = rsum[a];
+    }
+  }
+
+  // exclusive scan of the gather counts in block order z = tid + o * blockDim
+  int32_t incl[kOut];
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    int32_t x = cnt[o];
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+      const int32_t y = __shfl_up(x, dd, 64);
+      if (lane >= dd) x += y;
+    }
+    incl[o] = x;
+    if (lane == 63) wtot[o * 4 + wave] = x;
+  }
+  __syncthreads();
+
+  // phase 2: one thread per output block
+  int32_t base = 0;
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    int32_t before = 0, all = 0;
+    for (int w = 0; w < nwaves; ++w) {
+      const int32_t v = wtot[o * 4 + w];
+      before += w < wave ? v : 0;
+      all += v;
+    }
+    const int32_t z = threadIdx.x + o * (int32_t)blockDim.x;
+    const int32_t first = base + before + incl[o] - cnt[o];
+    base += all;
+    if (z >= d.nz) continue;
+    Acc acc[kBs];
+#pragma unroll
+    for (int k = 0; k < kBs; ++k) acc[k] = Acc(0);
+    for (int32_t c = 0; c < cnt[o]; ++c) {
+      const T *blk = rowbuf + (size_t)slist[first + c] * kBs;
+#pragma unroll
+      for (int k = 0; k < kBs; ++k) acc[k] += Acc(blk[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kBs; ++k) vals[(size_t)z * kBs + k] = old[o][k] + T(acc[k]);
+  }
+  if (Elem::kHasRowDat) {
+    for (int32_t lr = threadIdx.x; lr < d.nr; lr += blockDim.x) {
+      Acc r[Elem::kRd];
+#pragma unroll
+      for (int a = 0; a < Elem::kRd; ++a) r[a] = Acc(0);
+      for (int32_t u = sinc[lr]; u < sinc[lr + 1]; ++u)
+#pragma unroll
+        for (int gg = 0; gg < Elem::kGroups; ++gg)
+#pragma unroll
+          for (int a = 0; a < Elem::kRd; ++a) r[a] += Acc(rsbuf[((size_t)gg * per_pass + u) * Elem::kRd + a]);
+      T *rd = rowdat + (size_t)(d.r0 + lr) * Elem::kRd;
+#pragma unroll
+      for (int a = 0; a < Elem::kRd; ++a) rd[a] += T(r[a]);
+    }
+  }
+}
+
+template <class Elem>
+int launch_rowgather_staged(int32_t start, int32_t end, const Elem &elem, const op2hip_mat *A,
+                            const op2hip_exec *ex, typename Elem::T *rowdat, hipStream_t s) {
+  OP2HIP_REQUIRE(ex && ex->batch_desc && ex->gather_count && ex->gather_list && ex->nbatches >= 0,
+                 "staged row gather without its plan (batch descriptors and gather table)");
+  OP2HIP_REQUIRE(A->inc_ent && A->inc_ptr, "row-gather launch without row incidences");
+  OP2HIP_REQUIRE(!ex->subset, "row-gather assembly does not iterate Subsets");
+  OP2HIP_REQUIRE(A->rdim == Elem::kRd && A->cdim == Elem::kCd,
+                 "matrix block %dx%d, kernel assembles %dx%d", A->rdim, A->cdim,
+                 Elem::kRd, Elem::kCd);
+  if (end <= start || ex->nbatches == 0) return OP2HIP_OK;
+  const int bs = block_size_of(ex, 256);
+  OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % (64 * Elem::kGroups) == 0,
+                 "staged row gather: block size %d must be a multiple of %d", bs, 64 * Elem::kGroups);
+  const int per_pass = bs / Elem::kGroups;
+  OP2HIP_REQUIRE(ex->batch_max_inc <= per_pass,
+                 "staged row gather: a batch has %d row incidences, at most %d fit", ex->batch_max_inc,
+                 per_pass);
+  const int32_t nout = (ex->batch_max_nnz + bs - 1) / bs;
+  OP2HIP_REQUIRE(nout <= 4, "staged row gather: %d blocks in a batch, at most %d",
+                 ex->batch_max_nnz, 4 * bs);
+  const int kout = nout <= 1 ? 1 : (nout <= 2 ? 2 : 4);
+  const size_t lds = StagedLayout<Elem>::bytes(per_pass, kout, ex->batch_max_rows);
+  OP2HIP_REQUIRE(lds <= 64 * 1024, "row batch needs %zu bytes of LDS (max 65536)", lds);
+  const auto *descs = static_cast<const BatchDesc *>(ex->batch_desc);
+  if (kout == 1)
+    rowgather_staged_kernel<Elem, 1><<<ex->nbatches, bs, lds, s>>>(
+        start, end, elem, *A, descs, ex->nbatches, ex->gather_count, ex->gather_list, rowdat);
+  else if (kout == 2)
+    rowgather_staged_kernel<Elem, 2><<<ex->nbatches, bs, lds, s>>>(
+        start, end, elem, *A, descs, ex->nbatches, ex->gather_count, ex->gather_list, rowdat);
+  else
+    rowgather_staged_kernel<Elem, 4><<<ex->nbatches, bs, lds, s>>>(
+        start, end, elem, *A, descs, ex->nbatches, ex->gather_count, ex->gather_list, rowdat);
   OP2HIP_LAUNCHED();
   return OP2HIP_OK;
 }

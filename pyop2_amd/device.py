@@ -19,9 +19,10 @@ __all__ = ["lib", "torch", "device", "stream", "check", "ExecPlan", "MatDesc",
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libop2hip.so")
 
 OK, ERR_HIP, ERR_ARG, ERR_WORKSPACE, ERR_OVERFLOW, ERR_JIT, ERR_PATTERN = range(7)
-ATOMIC, COLOURED, STAGED, ROWGATHER, ROWGATHER_THREAD, ROWGATHER_SLOTS = range(6)
+ATOMIC, COLOURED, STAGED, ROWGATHER, ROWGATHER_THREAD, ROWGATHER_SLOTS, ROWGATHER_STAGED = range(7)
 STRATEGIES = {"atomic": ATOMIC, "coloured": COLOURED, "staged": STAGED, "rowgather": ROWGATHER,
-              "rowgather_thread": ROWGATHER_THREAD, "rowgather_slots": ROWGATHER_SLOTS}
+              "rowgather_thread": ROWGATHER_THREAD, "rowgather_slots": ROWGATHER_SLOTS,
+              "rowgather_staged": ROWGATHER_STAGED}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
@@ -47,6 +48,8 @@ class ExecPlan(ctypes.Structure):
         ("batch_max_rows", ctypes.c_int32),
         ("batch_desc", ctypes.c_void_p),
         ("slot_table", ctypes.c_void_p),
+        ("gather_count", ctypes.c_void_p),
+        ("gather_list", ctypes.c_void_p),
         ("subset", ctypes.c_void_p),
     ]
 
@@ -99,6 +102,8 @@ _SIGNATURES = {
     "op2hip_plan_row_batches": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _i32p, _i32p, _i32p, _i32p]),
     "op2hip_slot_table_bytes": (ctypes.c_int, [_I, ctypes.POINTER(ctypes.c_size_t)]),
     "op2hip_build_slot_table": (ctypes.c_int, [_I, _V, _V, _V, _V, _V, _V, _I, _I, _V, _V, _V, _V]),
+    "op2hip_gather_table_count": (ctypes.c_int, [_I, _I, _V, _V, _I, _V, _V, _V, _V]),
+    "op2hip_gather_table_fill": (ctypes.c_int, [_I, _I, _V, _V, _I, _V, _V, _V, _V, _V]),
     "op2hip_wrap_p1_lumped_mass": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V]),
     "op2hip_wrap_p1_laplace_residual": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V, _V]),
     "op2hip_wrap_p1_laplace_matrix": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V]),

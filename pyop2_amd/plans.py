@@ -138,21 +138,59 @@ class RowBatches:
             dev.check(L.op2hip_slot_table_bytes(cmap.arity, ctypes.byref(eb)), "slot_table_bytes")
             table = dev.workspace(max(tr.total, 1) * eb.value)
             desc = dev.empty((max(self.nbatches, 1), 8), np.int32)
-            bad = dev.zeros((1,), np.int32)
+            bad = dev.zeros((2,), np.int32)
             dev.check(L.op2hip_build_slot_table(self.nbatches, dev.ptr(self.batch_rows), dev.ptr(rowptr),
                                                 dev.ptr(colidx), dev.ptr(tr.inc_ptr), dev.ptr(tr.inc_ent),
                                                 dev.ptr(cmap.device_values), rarity, cmap.arity,
                                                 dev.ptr(table), dev.ptr(desc), dev.ptr(bad), dev.stream()),
                       "build_slot_table")
-            if int(bad.item()):
+            nbad, nrep = (int(v) for v in dev.to_host(bad))
+            if nbad:
                 from pyop2_amd.exceptions import PatternError
-                raise PatternError(f"{int(bad.item())} map entries are not in the sparsity pattern")
-            hit = self.slots[id(cmap)] = (table, desc, cmap)
+                raise PatternError(f"{nbad} map entries are not in the sparsity pattern")
+            hit = self.slots[id(cmap)] = (table, desc, cmap, nrep)
         return hit
 
     def slot_fields(self, ex, hit):
         ex.slot_table = dev.ptr(hit[0])
         ex.batch_desc = dev.ptr(hit[1])
+
+    def gather_table(self, hit, nblocks):
+        """Gather table (per-block counts, per-batch lists) of slot table
+        ``hit`` for a Mat of ``nblocks`` blocks, or None when a block takes
+        more than 255 element blocks (built once)."""
+        if len(hit) > 4:
+            return hit[4]
+        L = dev.lib()
+        table, desc, cmap = hit[0], hit[1], hit[2]
+        gcount = dev.empty((max(nblocks, 1),), np.uint8)
+        totals = dev.empty((max(self.nbatches, 1),), np.int32)
+        over = dev.zeros((1,), np.int32)
+        dev.check(L.op2hip_gather_table_count(self.nbatches, self.max_nnz, dev.ptr(desc), dev.ptr(table),
+                                              cmap.arity, dev.ptr(gcount), dev.ptr(totals), dev.ptr(over),
+                                              dev.stream()), "gather_table_count")
+        gt = None
+        if not int(over.item()):
+            tot = dev.to_host(totals)[:self.nbatches].astype(np.int64)
+            off = np.zeros(max(self.nbatches, 1), dtype=np.int64)
+            np.cumsum(tot[:-1], out=off[1:self.nbatches])
+            total = int(tot.sum())
+            if total < 2 ** 31:
+                glist = dev.empty((max(total, 1),), np.uint16)
+                doff = dev.to_device(off.astype(np.int32))
+                dev.check(L.op2hip_gather_table_fill(self.nbatches, self.max_nnz, dev.ptr(desc), dev.ptr(table),
+                                                     cmap.arity, dev.ptr(doff), dev.ptr(totals),
+                                                     dev.ptr(gcount), dev.ptr(glist), dev.stream()),
+                          "gather_table_fill")
+                gt = (gcount, glist)
+        key = id(cmap)
+        self.slots[key] = hit = hit + (gt,)
+        return gt
+
+    @staticmethod
+    def gather_fields(ex, gt):
+        ex.gather_count = dev.ptr(gt[0])
+        ex.gather_list = dev.ptr(gt[1])
 
 
 def row_batches(owner, tr, rowptr, max_inc, max_nnz, key):

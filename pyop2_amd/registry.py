@@ -67,12 +67,20 @@ FAMILIES = {
         entry={"f64": "op2hip_wrap_p2_elasticity_adjoint"}),
 }
 
-# Row-batch budgets of the row-gather assembly (LDS <= 64 KiB per workgroup).
+# Row-batch budgets (max row incidences, max blocks) of the row-gather
+# assembly (LDS <= 64 KiB per workgroup).
 _BATCH = {("p1_laplace_matrix", "f64"): (1024, 2048),
           ("p2_elasticity_matrix", "f64"): (512, 640),
           ("p2_elasticity_matrix", "f32"): (512, 640),
           ("p2_elasticity_fused", "f64"): (512, 640),
           ("p2_elasticity_fused", "f32"): (512, 640)}
+# The staged variant holds one batch's element block rows in LDS: at most
+# 256 / (column groups of the element) incidences per batch.
+_BATCH_STAGED = {("p1_laplace_matrix", "f64"): (256, 512),
+                 ("p2_elasticity_matrix", "f64"): (64, 256),
+                 ("p2_elasticity_matrix", "f32"): (64, 256),
+                 ("p2_elasticity_fused", "f64"): (64, 256),
+                 ("p2_elasticity_fused", "f32"): (64, 256)}
 
 
 def _variant(dtypes):
@@ -198,18 +206,36 @@ class HandWrittenKernel:
             if strategy.startswith("rowgather"):
                 from pyop2_amd import plans
                 tr = plans.transpose(rmap, mat.sparsity.nrows, rmap.iterset.size)
-                budget = _BATCH[(self.family, self.variant)]
-                rb = plans.row_batches(rmap, tr, mat.rowptr, budget[0], budget[1],
-                                       (id(mat.sparsity), mat.block_shape, self.variant))
+                key = (id(mat.sparsity), mat.block_shape, self.variant)
+                rb = None
+                if strategy == "rowgather_staged":
+                    budget = _BATCH_STAGED[(self.family, self.variant)]
+                    rb = plans.row_batches(rmap, tr, mat.rowptr, budget[0], budget[1], key)
+                    if rb.max_inc > budget[0]:
+                        # a row with more incidences than one batch can stage
+                        rb, strategy = None, "rowgather_slots"
+                if rb is None:
+                    budget = _BATCH[(self.family, self.variant)]
+                    rb = plans.row_batches(rmap, tr, mat.rowptr, budget[0], budget[1], key)
                 rb.exec_fields(ex)
                 keep.append(rb)
-                if strategy == "rowgather_slots":
+                if strategy in ("rowgather_slots", "rowgather_staged"):
                     hit = rb.slot_table(tr, mat.rowptr, mat.colidx, args[0].maps[1], rmap.arity)
                     if hit is None:
                         strategy = "rowgather_thread"
-                        ex.strategy = dev.STRATEGIES[strategy]
+                    elif strategy == "rowgather_staged" and hit[3]:
+                        # elements repeating a column need one add per occurrence
+                        strategy = "rowgather_thread"
                     else:
                         rb.slot_fields(ex, hit)
+                        if strategy == "rowgather_staged":
+                            gt = rb.gather_table(hit, mat.sparsity.block_nnz)
+                            if gt is None:
+                                strategy = "rowgather_thread"
+                            else:
+                                rb.gather_fields(ex, gt)
+                                keep.append(gt)
+                ex.strategy = dev.STRATEGIES[strategy]
                 desc = mat.descriptor(tr.inc_ptr, tr.inc_ent)
             else:
                 desc = mat.descriptor()
