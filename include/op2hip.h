@@ -63,13 +63,16 @@ typedef struct op2hip_exec {
   const int32_t *colour_elems;     /* device, end-start entries */
   const int32_t *colour_offsets;   /* host, ncolours+1 offsets into colour_elems */
   int32_t ncolours;
-  /* OP2HIP_STAGED: tiles of consecutive elements of [start,end) */
-  int32_t tile_size;               /* elements per tile */
+  /* OP2HIP_STAGED: tiles of the elements of [start,end) (op2hip_plan_tiles);
+     per staged map m (in the order the entry point documents): */
   int32_t ntiles;
-  const int32_t *tile_node_ptr;    /* device, ntiles+1: node list of each tile */
-  const int32_t *tile_nodes;       /* device: global node ids, sorted per tile */
-  const uint16_t *elem_local;      /* device, (end-start) x arity local node ids */
-  int32_t max_tile_nodes;          /* max over tiles of the node-list length */
+  const int32_t *tile_ptr;          /* device, ntiles+1 offsets into tile_elems */
+  const int32_t *tile_elems;        /* device, end-start element ids by tile */
+  const int32_t *tile_node_ptr[2];  /* device, ntiles+1 offsets into tile_nodes[m] */
+  const int32_t *tile_nodes[2];     /* device, node ids of each tile, exclusive first */
+  const int32_t *tile_nexcl[2];     /* device, per tile: nodes no other tile has */
+  const uint16_t *elem_local[2];    /* device, (end-start) x arity: local node ids */
+  int32_t max_tile_nodes[2];        /* largest node count of a tile (sizes LDS) */
   /* OP2HIP_ROWGATHER: batches of consecutive block rows */
   int32_t nbatches;
   const int32_t *batch_rows;       /* device, nbatches+1 row boundaries */
@@ -174,6 +177,23 @@ int op2hip_colour_greedy(int32_t nelem, int32_t nmaps,
                          const int32_t *const *maps, const int32_t *arity,
                          const int32_t *target_id, const int32_t *target_size,
                          int32_t ntargets, int32_t *colour, int32_t *ncolours);
+/* Tiles of OP2HIP_STAGED (host arrays).  The elements of [start, end) are
+ * grouped into tiles of at most tile_size elements and, when node_cap > 0,
+ * at most node_cap distinct nodes of map 0; each tile grows breadth first
+ * from the lowest unassigned element over elements sharing a node of map 0,
+ * so tiles are compact regions.  Outputs (capacities for n = end-start):
+ * *ntiles; tile_ptr (n+1: ntiles+1 used) and tile_elems (n element ids by
+ * tile); per map m (nmaps <= 2, map entries >= 0): node_ptr[m] (n+1) and
+ * nodes[m] (n*arity[m]): the distinct nodes of each tile, the nexcl[m][t]
+ * (n) nodes no other tile references first, each part ascending; local[m]
+ * (n*arity[m], tile_elems order): the position of map[e][i] among its
+ * tile's nodes; max_nodes[m]: the largest node count of a tile.
+ * tile_size * arity must stay below 65536. */
+int op2hip_plan_tiles(int32_t start, int32_t end, int32_t nmaps, const int32_t *const *maps,
+                      const int32_t *arity, const int32_t *target_size, int32_t tile_size,
+                      int32_t node_cap, int32_t *ntiles, int32_t *tile_ptr,
+                      int32_t *tile_elems, int32_t *const *node_ptr, int32_t *const *nodes,
+                      int32_t *const *nexcl, uint16_t *const *local, int32_t *max_nodes);
 /* Row batches for OP2HIP_ROWGATHER: consecutive rows whose incidences and
  * blocks stay within max_inc / max_nnz (a single larger row gets its own
  * batch).  Host arrays; batch_rows has room for nrows+1 entries. */

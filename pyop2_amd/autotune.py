@@ -55,6 +55,8 @@ def candidates(executor, parloop):
         if isinstance(parloop.iterset, Subset):
             return ["atomic", "coloured"]
         return ["rowgather_staged", "rowgather_slots", "rowgather_thread", "rowgather", "atomic"]
+    if "staged" in executor.spec and not isinstance(parloop.iterset, Subset):
+        return ["staged", "atomic", "coloured"]
     return ["atomic", "coloured"]
 
 

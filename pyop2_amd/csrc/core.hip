@@ -573,6 +573,141 @@ int op2hip_colour_greedy(int32_t nelem, int32_t nmaps, const int32_t *const *map
   return OP2HIP_OK;
 }
 
+int op2hip_plan_tiles(int32_t start, int32_t end, int32_t nmaps, const int32_t *const *maps,
+                      const int32_t *arity, const int32_t *target_size, int32_t tile_size,
+                      int32_t node_cap, int32_t *ntiles_out, int32_t *tile_ptr,
+                      int32_t *tile_elems, int32_t *const *node_ptr, int32_t *const *nodes,
+                      int32_t *const *nexcl, uint16_t *const *local, int32_t *max_nodes) {
+  OP2HIP_REQUIRE(end >= start && start >= 0 && nmaps >= 1 && nmaps <= 2 && tile_size > 0 &&
+                     ntiles_out && tile_ptr && tile_elems && node_ptr && nodes && nexcl && local &&
+                     max_nodes,
+                 "plan_tiles: bad arguments");
+  OP2HIP_REQUIRE(node_cap <= 0 || node_cap >= arity[0],
+                 "plan_tiles: a node cap of %d cannot hold one element (arity %d)", node_cap,
+                 arity[0]);
+  for (int m = 0; m < nmaps; ++m)
+    OP2HIP_REQUIRE(maps[m] && arity[m] > 0 && (int64_t)tile_size * arity[m] <= 65535,
+                   "plan_tiles: map %d: arity %d, tile size %d (local ids are 16-bit)", m, arity[m],
+                   tile_size);
+  const int32_t n = end - start;
+  // elements around each node of map 0 (the adjacency the regions grow over)
+  const int32_t a0 = arity[0], nt0 = std::max(target_size[0], 1);
+  std::vector<int32_t> adj_ptr(nt0 + 1, 0), adj;
+  for (int32_t k = 0; k < n; ++k)
+    for (int i = 0; i < a0; ++i) {
+      const int32_t v = maps[0][(int64_t)(start + k) * a0 + i];
+      OP2HIP_REQUIRE(v >= 0 && v < target_size[0], "plan_tiles: map 0 entry %d out of range", v);
+      ++adj_ptr[v + 1];
+    }
+  for (int32_t v = 0; v < nt0; ++v) adj_ptr[v + 1] += adj_ptr[v];
+  adj.resize(adj_ptr[nt0]);
+  {
+    std::vector<int32_t> fillp(adj_ptr.begin(), adj_ptr.end() - 1);
+    for (int32_t k = 0; k < n; ++k)
+      for (int i = 0; i < a0; ++i) adj[fillp[maps[0][(int64_t)(start + k) * a0 + i]]++] = k;
+  }
+  // regions: breadth first from the lowest unassigned element; elements queued
+  // but not taken when the tile is full are released for later tiles
+  // (and, with node_cap > 0, at most node_cap distinct nodes of map 0)
+  std::vector<uint8_t> taken(n, 0), queued(n, 0);
+  std::vector<int32_t> queue, mark(nt0, -1);
+  queue.reserve(n);
+  int32_t seed = 0, pos = 0, ntiles = 0;
+  tile_ptr[0] = 0;
+  while (pos < n) {
+    const int32_t t = ntiles++;
+    queue.clear();
+    size_t head = 0;
+    const int32_t want = std::min(tile_size, n - pos);
+    int32_t got = 0, tile_nodes = 0;
+    while (got < want) {
+      if (head == queue.size()) {
+        if (got > 0 && node_cap > 0) break;  // region exhausted: start a new tile
+        while (taken[seed] || queued[seed]) ++seed;
+        queued[seed] = 1;
+        queue.push_back(seed);
+      }
+      const int32_t k = queue[head++];
+      if (node_cap > 0) {
+        int32_t fresh = 0;
+        for (int i = 0; i < a0; ++i) fresh += mark[maps[0][(int64_t)(start + k) * a0 + i]] != t;
+        if (got > 0 && tile_nodes + fresh > node_cap) {
+          queued[k] = 0;
+          continue;
+        }
+        for (int i = 0; i < a0; ++i) {
+          int32_t &mk = mark[maps[0][(int64_t)(start + k) * a0 + i]];
+          tile_nodes += mk != t;
+          mk = t;
+        }
+      }
+      taken[k] = 1;
+      tile_elems[pos++] = start + k;
+      ++got;
+      for (int i = 0; i < a0 && got < want; ++i) {
+        const int32_t v = maps[0][(int64_t)(start + k) * a0 + i];
+        for (int32_t q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q) {
+          const int32_t k2 = adj[q];
+          if (!taken[k2] && !queued[k2]) {
+            queued[k2] = 1;
+            queue.push_back(k2);
+          }
+        }
+      }
+    }
+    for (size_t q = head; q < queue.size(); ++q) queued[queue[q]] = 0;
+    tile_ptr[t + 1] = pos;
+  }
+  *ntiles_out = ntiles;
+  // per map: the distinct nodes of each tile, those of no other tile first
+  for (int m = 0; m < nmaps; ++m) {
+    const int32_t am = arity[m], ntm = std::max(target_size[m], 1);
+    std::vector<int32_t> ntile(ntm, 0), stamp(ntm, -1), slot(ntm, 0);
+    for (int32_t t = 0; t < ntiles; ++t)
+      for (int32_t p = tile_ptr[t]; p < tile_ptr[t + 1]; ++p)
+        for (int i = 0; i < am; ++i) {
+          const int32_t v = maps[m][(int64_t)tile_elems[p] * am + i];
+          OP2HIP_REQUIRE(v >= 0 && v < target_size[m], "plan_tiles: map %d entry %d out of range",
+                         m, v);
+          if (stamp[v] != t) {
+            stamp[v] = t;
+            ++ntile[v];
+          }
+        }
+    std::fill(stamp.begin(), stamp.end(), -1);
+    std::vector<int32_t> own, shared;
+    int32_t total = 0, mx = 0;
+    node_ptr[m][0] = 0;
+    for (int32_t t = 0; t < ntiles; ++t) {
+      own.clear();
+      shared.clear();
+      for (int32_t p = tile_ptr[t]; p < tile_ptr[t + 1]; ++p)
+        for (int i = 0; i < am; ++i) {
+          const int32_t v = maps[m][(int64_t)tile_elems[p] * am + i];
+          if (stamp[v] != t) {
+            stamp[v] = t;
+            (ntile[v] == 1 ? own : shared).push_back(v);
+          }
+        }
+      std::sort(own.begin(), own.end());
+      std::sort(shared.begin(), shared.end());
+      int32_t c = 0;
+      for (int32_t v : own) { nodes[m][total + c] = v; slot[v] = c++; }
+      for (int32_t v : shared) { nodes[m][total + c] = v; slot[v] = c++; }
+      nexcl[m][t] = (int32_t)own.size();
+      for (int32_t p = tile_ptr[t]; p < tile_ptr[t + 1]; ++p)
+        for (int i = 0; i < am; ++i)
+          local[m][(int64_t)p * am + i] =
+              static_cast<uint16_t>(slot[maps[m][(int64_t)tile_elems[p] * am + i]]);
+      total += c;
+      mx = std::max(mx, c);
+      node_ptr[m][t + 1] = total;
+    }
+    max_nodes[m] = mx;
+  }
+  return OP2HIP_OK;
+}
+
 int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr, const int32_t *rowptr,
                             int32_t max_inc, int32_t max_nnz, int32_t *batch_rows,
                             int32_t *nbatches, int32_t *max_batch_inc,

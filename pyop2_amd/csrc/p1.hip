@@ -5,6 +5,7 @@
 // c_i = x_{i+2} - x_{i+1} (cyclic) and det = 2 |T| (signed),
 // K_e[i][j] = (b_i b_j + c_i c_j) / (2 |det|), lumped mass |det| / 6.
 #include "rowgather.hpp"
+#include "staged.hpp"
 
 namespace op2hip {
 
@@ -12,11 +13,7 @@ struct P1Geom {
   double b[3], c[3], s, det;
 };
 
-__device__ __forceinline__ P1Geom p1_geom(const double *__restrict__ X, int32_t v0,
-                                          int32_t v1, int32_t v2) {
-  const double2 p0 = reinterpret_cast<const double2 *>(X)[v0];
-  const double2 p1 = reinterpret_cast<const double2 *>(X)[v1];
-  const double2 p2 = reinterpret_cast<const double2 *>(X)[v2];
+__device__ __forceinline__ P1Geom p1_geom_of(double2 p0, double2 p1, double2 p2) {
   P1Geom g;
   g.b[0] = p1.y - p2.y;
   g.b[1] = p2.y - p0.y;
@@ -28,6 +25,69 @@ __device__ __forceinline__ P1Geom p1_geom(const double *__restrict__ X, int32_t 
   g.s = 1.0 / (2.0 * fabs(g.det));
   return g;
 }
+
+__device__ __forceinline__ P1Geom p1_geom(const double *__restrict__ X, int32_t v0,
+                                          int32_t v1, int32_t v2) {
+  const double2 *X2 = reinterpret_cast<const double2 *>(X);
+  return p1_geom_of(X2[v0], X2[v1], X2[v2]);
+}
+
+// Staged lumped mass: coordinates staged, M incremented (map cell -> vertex).
+struct P1MassStaged {
+  using T = double;
+  static constexpr int kMaps = 1;
+  static constexpr int kAr0 = 3, kIn0 = 2, kOut0 = 1;
+  static constexpr int kAr1 = 0, kIn1 = 0, kOut1 = 0;
+  double *M;
+  const double *X;
+  template <int Mp>
+  __device__ __forceinline__ void stage(int32_t v, double *dst) const {
+    dst[0] = X[2 * (size_t)v];
+    dst[1] = X[2 * (size_t)v + 1];
+  }
+  template <int Mp>
+  __device__ __forceinline__ double *out() const { return M; }
+  __device__ __forceinline__ void element(int32_t, const double (&x)[3][2], const double (&)[1][1],
+                                          double (&y)[3][1], double (&)[1][1]) const {
+    const P1Geom g = p1_geom_of(make_double2(x[0][0], x[0][1]), make_double2(x[1][0], x[1][1]),
+                                make_double2(x[2][0], x[2][1]));
+    const double m = fabs(g.det) / 6.0;
+    y[0][0] = m;
+    y[1][0] = m;
+    y[2][0] = m;
+  }
+};
+
+// Staged Laplace residual: (x, y, u) staged per vertex, R incremented.
+struct P1ResidualStaged {
+  using T = double;
+  static constexpr int kMaps = 1;
+  static constexpr int kAr0 = 3, kIn0 = 3, kOut0 = 1;
+  static constexpr int kAr1 = 0, kIn1 = 0, kOut1 = 0;
+  double *R;
+  const double *X;
+  const double *u;
+  template <int Mp>
+  __device__ __forceinline__ void stage(int32_t v, double *dst) const {
+    dst[0] = X[2 * (size_t)v];
+    dst[1] = X[2 * (size_t)v + 1];
+    dst[2] = u[v];
+  }
+  template <int Mp>
+  __device__ __forceinline__ double *out() const { return R; }
+  __device__ __forceinline__ void element(int32_t, const double (&x)[3][3], const double (&)[1][1],
+                                          double (&y)[3][1], double (&)[1][1]) const {
+    const P1Geom g = p1_geom_of(make_double2(x[0][0], x[0][1]), make_double2(x[1][0], x[1][1]),
+                                make_double2(x[2][0], x[2][1]));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc += (g.b[i] * g.b[j] + g.c[i] * g.c[j]) * g.s * x[j][2];
+      y[i][0] = acc;
+    }
+  }
+};
 
 template <bool kAtomic>
 __device__ __forceinline__ void scatter_add(double *p, double v) {
@@ -160,6 +220,7 @@ extern "C" {
 int op2hip_wrap_p1_lumped_mass(int32_t start, int32_t end, double *M, const double *X,
                                const int32_t *c2v, const op2hip_exec *ex, void *stream) {
   hipStream_t s = stream_of(stream);
+  if (ex && ex->strategy == OP2HIP_STAGED) return launch_staged(start, end, P1MassStaged{M, X}, ex, s);
   const int bs = block_size_of(ex, 256);
   return for_each_launch(ex, start, end, [&](int32_t lo, int32_t hi, const int32_t *list,
                                              bool atomic) -> int {
@@ -174,6 +235,8 @@ int op2hip_wrap_p1_laplace_residual(int32_t start, int32_t end, double *R, const
                                     const double *u, const int32_t *c2v,
                                     const op2hip_exec *ex, void *stream) {
   hipStream_t s = stream_of(stream);
+  if (ex && ex->strategy == OP2HIP_STAGED)
+    return launch_staged(start, end, P1ResidualStaged{R, X, u}, ex, s);
   const int bs = block_size_of(ex, 256);
   return for_each_launch(ex, start, end, [&](int32_t lo, int32_t hi, const int32_t *list,
                                              bool atomic) -> int {

@@ -14,6 +14,7 @@
 // Element matrix block (i, j): with M = sum_q grad phi_i (x) grad phi_j,
 // B = wdet (lam M + mu M^T + mu tr(M) I).
 #include "rowgather.hpp"
+#include "staged.hpp"
 
 namespace op2hip {
 namespace p2 {
@@ -50,17 +51,14 @@ struct Geom {
   T wdet;
 };
 
+// Geometry of the tetrahedron with vertex coordinates xv[v][r].
 template <typename T>
-__device__ __forceinline__ Geom<T> geometry(const T *__restrict__ X,
-                                            const int32_t *__restrict__ c2v, int32_t e) {
-  const int32_t v0 = c2v[4 * e], v1 = c2v[4 * e + 1], v2 = c2v[4 * e + 2], v3 = c2v[4 * e + 3];
-  const T x0[3] = {X[3 * v0], X[3 * v0 + 1], X[3 * v0 + 2]};
+__device__ __forceinline__ Geom<T> geometry_of(const T (&xv)[4][3]) {
   T J[3][3];
-  const int32_t vv[3] = {v1, v2, v3};
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int r = 0; r < 3; ++r) J[r][c] = X[3 * vv[c] + r] - x0[r];
+    for (int r = 0; r < 3; ++r) J[r][c] = xv[c + 1][r] - xv[0][r];
   const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
   const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
   const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
@@ -85,6 +83,19 @@ __device__ __forceinline__ Geom<T> geometry(const T *__restrict__ X,
   }
   g.wdet = T(kQw) * fabs(det);
   return g;
+}
+
+template <typename T>
+__device__ __forceinline__ Geom<T> geometry(const T *__restrict__ X,
+                                            const int32_t *__restrict__ c2v, int32_t e) {
+  T xv[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int32_t n = c2v[4 * e + v];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) xv[v][r] = X[3 * n + r];
+  }
+  return geometry_of(xv);
 }
 
 // Barycentric-form gradient of a P2 field at quadrature point q:
@@ -456,6 +467,66 @@ struct MatrixLanes {
   }
 };
 
+// Staged residual: map 0 = cell -> P2 node (u staged, R incremented), map 1 =
+// cell -> vertex (coordinates staged).
+template <typename TT>
+struct ResidualStaged {
+  using T = TT;
+  static constexpr int kMaps = 2;
+  static constexpr int kAr0 = 10, kIn0 = 3, kOut0 = 3;
+  static constexpr int kAr1 = 4, kIn1 = 3, kOut1 = 0;
+  T *R;
+  const T *X;
+  const T *u;
+  T lam, mu;
+  template <int M>
+  __device__ __forceinline__ void stage(int32_t v, T *dst) const {
+    const T *src = (M == 0 ? u : X) + 3 * (size_t)v;
+    dst[0] = src[0];
+    dst[1] = src[1];
+    dst[2] = src[2];
+  }
+  template <int M>
+  __device__ __forceinline__ T *out() const { return R; }
+  __device__ __forceinline__ void element(int32_t, const T (&uu)[10][3], const T (&xv)[4][3],
+                                          T (&re)[10][3], T (&)[4][1]) const {
+    element_residual(uu, geometry_of(xv), lam, mu, re);
+  }
+};
+
+// Staged adjoint: map 0 = cell -> P2 node (u and Rbar staged, ubar
+// incremented), map 1 = cell -> vertex (X staged, Xbar incremented).
+template <typename TT>
+struct AdjointStaged {
+  using T = TT;
+  static constexpr int kMaps = 2;
+  static constexpr int kAr0 = 10, kIn0 = 6, kOut0 = 3;
+  static constexpr int kAr1 = 4, kIn1 = 3, kOut1 = 3;
+  T *ubar;
+  T *Xbar;
+  const T *Rbar;
+  const T *X;
+  const T *u;
+  T lam, mu;
+  template <int M>
+  __device__ __forceinline__ void stage(int32_t v, T *dst) const {
+    if (M == 0) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        dst[a] = u[3 * (size_t)v + a];
+        dst[3 + a] = Rbar[3 * (size_t)v + a];
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) dst[a] = X[3 * (size_t)v + a];
+    }
+  }
+  template <int M>
+  __device__ __forceinline__ T *out() const { return M == 0 ? ubar : Xbar; }
+  __device__ __forceinline__ void element(int32_t, const T (&ue)[10][6], const T (&xv)[4][3],
+                                          T (&ub)[10][3], T (&xb)[4][3]) const;
+};
+
 // Adjoint of the residual loop for one element (see header of the entry point).
 template <typename T, int Q>
 __device__ __forceinline__ void adjoint_q(const T (&uu)[10][3], const T (&ee)[10][3],
@@ -483,6 +554,55 @@ __device__ __forceinline__ void adjoint_q(const T (&uu)[10][3], const T (&ee)[10
     }
 }
 
+// Adjoint of one element's residual: ub = K_e ee (the residual's transpose
+// applied to ee) and xb = d(ee . R_e)/dX (rows: vertices).
+template <typename T>
+__device__ __forceinline__ void element_adjoint(const T (&uu)[10][3], const T (&ee)[10][3],
+                                                const Geom<T> &g, T lam, T mu, T (&ub)[10][3],
+                                                T (&xb)[4][3]) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ub[i][a] = T(0);
+  T P[3][3] = {};
+  adjoint_q<T, 0>(uu, ee, g, lam, mu, ub, P);
+  adjoint_q<T, 1>(uu, ee, g, lam, mu, ub, P);
+  adjoint_q<T, 2>(uu, ee, g, lam, mu, ub, P);
+  adjoint_q<T, 3>(uu, ee, g, lam, mu, ub, P);
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ub[i][a] *= g.wdet;
+  }
+  // dS/dJ = wdet * P K^T ; X[m+1] gets column m, X[0] minus the row sums.
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    T rowsum = T(0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const T v = g.wdet * (P[l][0] * g.K[m][0] + P[l][1] * g.K[m][1] + P[l][2] * g.K[m][2]);
+      xb[m + 1][l] = v;
+      rowsum += v;
+    }
+    xb[0][l] = -rowsum;
+  }
+}
+
+template <typename TT>
+__device__ __forceinline__ void AdjointStaged<TT>::element(int32_t, const TT (&ue)[10][6],
+                                                           const TT (&xv)[4][3], TT (&ub)[10][3],
+                                                           TT (&xb)[4][3]) const {
+  TT uu[10][3], ee[10][3];
+#pragma unroll
+  for (int j = 0; j < 10; ++j)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      uu[j][a] = ue[j][a];
+      ee[j][a] = ue[j][3 + a];
+    }
+  element_adjoint(uu, ee, geometry_of(xv), lam, mu, ub, xb);
+}
+
 template <typename T, bool kAtomic>
 __global__ void __launch_bounds__(256)
 adjoint_kernel(int32_t lo, int32_t hi, const int32_t *__restrict__ list, T *__restrict__ ubar,
@@ -504,35 +624,10 @@ adjoint_kernel(int32_t lo, int32_t hi, const int32_t *__restrict__ list, T *__re
       uu[j][b] = u[3 * nd[j] + b];
       ee[j][b] = Rbar[3 * nd[j] + b];
     }
-  T ub[10][3];
+  T ub[10][3], xb[4][3];
+  element_adjoint(uu, ee, g, lam, mu, ub, xb);
 #pragma unroll
-  for (int i = 0; i < 10; ++i)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) ub[i][a] = T(0);
-  T P[3][3] = {};
-  adjoint_q<T, 0>(uu, ee, g, lam, mu, ub, P);
-  adjoint_q<T, 1>(uu, ee, g, lam, mu, ub, P);
-  adjoint_q<T, 2>(uu, ee, g, lam, mu, ub, P);
-  adjoint_q<T, 3>(uu, ee, g, lam, mu, ub, P);
-#pragma unroll
-  for (int i = 0; i < 10; ++i) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) ub[i][a] *= g.wdet;
-    add3<T, kAtomic>(ubar + 3 * nd[i], ub[i]);
-  }
-  // dS/dJ = wdet * P K^T ; X[m+1] gets column m, X[0] minus the row sums.
-  T xb[4][3];
-#pragma unroll
-  for (int l = 0; l < 3; ++l) {
-    T rowsum = T(0);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const T v = g.wdet * (P[l][0] * g.K[m][0] + P[l][1] * g.K[m][1] + P[l][2] * g.K[m][2]);
-      xb[m + 1][l] = v;
-      rowsum += v;
-    }
-    xb[0][l] = -rowsum;
-  }
+  for (int i = 0; i < 10; ++i) add3<T, kAtomic>(ubar + 3 * nd[i], ub[i]);
 #pragma unroll
   for (int v = 0; v < 4; ++v) add3<T, kAtomic>(Xbar + 3 * c2v[4 * e + v], xb[v]);
 }
@@ -540,6 +635,8 @@ adjoint_kernel(int32_t lo, int32_t hi, const int32_t *__restrict__ list, T *__re
 template <typename T>
 int residual(int32_t start, int32_t end, T *R, const T *X, const T *u, const int32_t *c2n,
              const int32_t *c2v, double lam, double mu, const op2hip_exec *ex, hipStream_t s) {
+  if (ex && ex->strategy == OP2HIP_STAGED)
+    return launch_staged(start, end, ResidualStaged<T>{R, X, u, T(lam), T(mu)}, ex, s);
   const int bs = block_size_of(ex, 256);
   return for_each_launch(ex, start, end, [&](int32_t lo, int32_t hi, const int32_t *list,
                                              bool atomic) -> int {
@@ -648,6 +745,9 @@ int op2hip_wrap_p2_elasticity_adjoint(int32_t start, int32_t end, double *ubar, 
                                       const int32_t *c2n, const int32_t *c2v, double lam,
                                       double mu, const op2hip_exec *ex, void *stream) {
   hipStream_t s = stream_of(stream);
+  if (ex && ex->strategy == OP2HIP_STAGED)
+    return launch_staged(start, end, p2::AdjointStaged<double>{ubar, Xbar, Rbar, X, u, lam, mu},
+                         ex, s);
   const int bs = block_size_of(ex, 256);
   return for_each_launch(ex, start, end, [&](int32_t lo, int32_t hi, const int32_t *list,
                                              bool atomic) -> int {

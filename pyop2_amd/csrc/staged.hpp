@@ -1,0 +1,180 @@
+// LDS-staged gather / compute / scatter for Dat INC loops (OP2HIP_STAGED).
+//
+// One workgroup per tile of elements (op2hip_plan_tiles: compact regions,
+// per staged map the tile's distinct nodes, those no other tile references
+// first).  The tile's node values of every READ argument are gathered once
+// into LDS, each thread runs the element kernel of one element on values
+// read from LDS through its 16-bit local node ids, and adds its increments
+// into LDS accumulators (one per tile node and component).  The tile then
+// writes each node once: a plain read-modify-write for the nodes only this
+// tile has, a device-scope atomic add for the nodes it shares.  Compared
+// with one atomic per element and node this cuts device atomics by the
+// number of elements around a node, and every gather becomes an LDS read.
+//
+// Elem provides
+//   using T;  static constexpr int kMaps (1 or 2);
+//   static constexpr int kAr0, kIn0, kOut0, kAr1, kIn1, kOut1;
+//     arity, staged values per node (READ arguments on that map, in the
+//     order stage() writes them) and increments per node (INC arguments),
+//     for map 0 and map 1 (kAr1 = 0 when kMaps == 1);
+//   template <int M> void stage(int32_t node, T *dst) const;   // kInM values
+//   template <int M> T *out() const;                             // INC target, kOutM per node
+//   void element(int32_t e, const T (&x0)[A0][I0], const T (&x1)[A1][I1],
+//                T (&y0)[A0][O0], T (&y1)[A1][O1]) const;     // A/I/O >= 1
+#pragma once
+
+#include "launch.hpp"
+
+namespace op2hip {
+
+struct TilePlan {
+  int32_t ntiles;
+  const int32_t *tile_ptr;
+  const int32_t *tile_elems;
+  const int32_t *node_ptr[2];
+  const int32_t *nodes[2];
+  const int32_t *nexcl[2];
+  const uint16_t *local[2];
+  int32_t max_nodes[2];
+};
+
+constexpr int at_least_1(int v) { return v > 0 ? v : 1; }
+
+// LDS: per map, staged inputs (T) then accumulators (double, 8-byte aligned).
+template <class Elem>
+struct TileLayout {
+  using T = typename Elem::T;
+  __host__ __device__ static size_t align8(size_t b) { return (b + 7) & ~(size_t)7; }
+  __host__ __device__ static size_t in_at(int m, const int32_t *mx) {
+    return m == 0 ? 0 : acc_at(0, mx) + (size_t)mx[0] * Elem::kOut0 * sizeof(double);
+  }
+  __host__ __device__ static size_t acc_at(int m, const int32_t *mx) {
+    return align8(in_at(m, mx) + (size_t)mx[m] * (m == 0 ? Elem::kIn0 : Elem::kIn1) * sizeof(T));
+  }
+  static size_t bytes(const int32_t *mx) {
+    return Elem::kMaps == 1 ? acc_at(0, mx) + (size_t)mx[0] * Elem::kOut0 * sizeof(double)
+                            : acc_at(1, mx) + (size_t)mx[1] * Elem::kOut1 * sizeof(double);
+  }
+};
+
+template <class Elem, int M>
+__device__ __forceinline__ void stage_map(const Elem &elem, const TilePlan &P, int32_t b,
+                                          typename Elem::T *in, double *acc) {
+  constexpr int kIn = M == 0 ? Elem::kIn0 : Elem::kIn1;
+  constexpr int kOut = M == 0 ? Elem::kOut0 : Elem::kOut1;
+  const int32_t n0 = P.node_ptr[M][b], nn = P.node_ptr[M][b + 1] - n0;
+  for (int32_t k = threadIdx.x; k < nn; k += blockDim.x) {
+    if constexpr (kIn > 0) elem.template stage<M>(P.nodes[M][n0 + k], in + (size_t)k * kIn);
+#pragma unroll
+    for (int a = 0; a < kOut; ++a) acc[(size_t)k * kOut + a] = 0.0;
+  }
+}
+
+template <class Elem, int M>
+__device__ __forceinline__ void flush_map(const Elem &elem, const TilePlan &P, int32_t b,
+                                          const double *acc) {
+  using T = typename Elem::T;
+  constexpr int kOut = M == 0 ? Elem::kOut0 : Elem::kOut1;
+  if constexpr (kOut > 0) {
+    const int32_t n0 = P.node_ptr[M][b], nn = P.node_ptr[M][b + 1] - n0;
+    const int32_t ne = P.nexcl[M][b];
+    T *out = elem.template out<M>();
+    for (int32_t k = threadIdx.x; k < nn; k += blockDim.x) {
+      T *p = out + (size_t)P.nodes[M][n0 + k] * kOut;
+#pragma unroll
+      for (int a = 0; a < kOut; ++a) {
+        const T v = T(acc[(size_t)k * kOut + a]);
+        if (k < ne) p[a] += v;
+        else atomic_add(p + a, v);
+      }
+    }
+  }
+}
+
+template <class Elem>
+__global__ void __launch_bounds__(256)
+staged_kernel(Elem elem, TilePlan P) {
+  using T = typename Elem::T;
+  using L = TileLayout<Elem>;
+  constexpr int A0 = Elem::kAr0, I0 = at_least_1(Elem::kIn0), O0 = at_least_1(Elem::kOut0);
+  constexpr int A1 = at_least_1(Elem::kAr1), I1 = at_least_1(Elem::kIn1),
+                O1 = at_least_1(Elem::kOut1);
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int32_t b = xcd_contiguous(blockIdx.x, P.ntiles);
+  T *in0 = reinterpret_cast<T *>(smem + L::in_at(0, P.max_nodes));
+  double *acc0 = reinterpret_cast<double *>(smem + L::acc_at(0, P.max_nodes));
+  T *in1 = reinterpret_cast<T *>(smem + (Elem::kMaps > 1 ? L::in_at(1, P.max_nodes) : 0));
+  double *acc1 =
+      reinterpret_cast<double *>(smem + (Elem::kMaps > 1 ? L::acc_at(1, P.max_nodes) : 0));
+
+  stage_map<Elem, 0>(elem, P, b, in0, acc0);
+  if constexpr (Elem::kMaps > 1) stage_map<Elem, 1>(elem, P, b, in1, acc1);
+  __syncthreads();
+
+  const int32_t p0 = P.tile_ptr[b], ne = P.tile_ptr[b + 1] - p0;
+  for (int32_t t = threadIdx.x; t < ne; t += blockDim.x) {
+    const int32_t pos = p0 + t;
+    uint16_t l0[A0], l1[A1];
+#pragma unroll
+    for (int i = 0; i < A0; ++i) l0[i] = P.local[0][(size_t)pos * A0 + i];
+    if constexpr (Elem::kMaps > 1) {
+#pragma unroll
+      for (int i = 0; i < A1; ++i) l1[i] = P.local[1][(size_t)pos * A1 + i];
+    }
+    T x0[A0][I0], x1[A1][I1], y0[A0][O0], y1[A1][O1];
+#pragma unroll
+    for (int i = 0; i < A0; ++i)
+#pragma unroll
+      for (int a = 0; a < I0; ++a) x0[i][a] = Elem::kIn0 > 0 ? in0[(size_t)l0[i] * I0 + a] : T(0);
+#pragma unroll
+    for (int i = 0; i < A1; ++i)
+#pragma unroll
+      for (int a = 0; a < I1; ++a)
+        x1[i][a] = (Elem::kMaps > 1 && Elem::kIn1 > 0) ? in1[(size_t)l1[i] * I1 + a] : T(0);
+    elem.element(P.tile_elems[pos], x0, x1, y0, y1);
+    if constexpr (Elem::kOut0 > 0) {
+#pragma unroll
+      for (int i = 0; i < A0; ++i)
+#pragma unroll
+        for (int a = 0; a < O0; ++a) atomic_add(&acc0[(size_t)l0[i] * O0 + a], double(y0[i][a]));
+    }
+    if constexpr (Elem::kMaps > 1 && Elem::kOut1 > 0) {
+#pragma unroll
+      for (int i = 0; i < A1; ++i)
+#pragma unroll
+        for (int a = 0; a < O1; ++a) atomic_add(&acc1[(size_t)l1[i] * O1 + a], double(y1[i][a]));
+    }
+  }
+  __syncthreads();
+
+  flush_map<Elem, 0>(elem, P, b, acc0);
+  if constexpr (Elem::kMaps > 1) flush_map<Elem, 1>(elem, P, b, acc1);
+}
+
+template <class Elem>
+int launch_staged(int32_t start, int32_t end, const Elem &elem, const op2hip_exec *ex,
+                  hipStream_t s) {
+  OP2HIP_REQUIRE(ex && ex->tile_ptr && ex->tile_elems && ex->ntiles >= 0,
+                 "staged launch without a tile plan");
+  OP2HIP_REQUIRE(!ex->subset, "staged launches do not iterate Subsets");
+  for (int m = 0; m < Elem::kMaps; ++m)
+    OP2HIP_REQUIRE(ex->tile_node_ptr[m] && ex->tile_nodes[m] && ex->tile_nexcl[m] &&
+                       ex->elem_local[m],
+                   "staged launch: tile plan lacks map %d", m);
+  if (end <= start || ex->ntiles == 0) return OP2HIP_OK;
+  TilePlan P{ex->ntiles, ex->tile_ptr, ex->tile_elems,
+             {ex->tile_node_ptr[0], ex->tile_node_ptr[1]},
+             {ex->tile_nodes[0], ex->tile_nodes[1]},
+             {ex->tile_nexcl[0], ex->tile_nexcl[1]},
+             {ex->elem_local[0], ex->elem_local[1]},
+             {ex->max_tile_nodes[0], Elem::kMaps > 1 ? ex->max_tile_nodes[1] : 0}};
+  const size_t lds = TileLayout<Elem>::bytes(P.max_nodes);
+  OP2HIP_REQUIRE(lds <= 64 * 1024, "tile needs %zu bytes of LDS (max 65536)", lds);
+  const int bs = block_size_of(ex, 256);
+  OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % 64 == 0, "staged block size %d", bs);
+  staged_kernel<Elem><<<ex->ntiles, bs, lds, s>>>(elem, P);
+  OP2HIP_LAUNCHED();
+  return OP2HIP_OK;
+}
+
+}  // namespace op2hip

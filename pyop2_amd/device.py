@@ -35,12 +35,14 @@ class ExecPlan(ctypes.Structure):
         ("colour_elems", ctypes.c_void_p),
         ("colour_offsets", ctypes.c_void_p),
         ("ncolours", ctypes.c_int32),
-        ("tile_size", ctypes.c_int32),
         ("ntiles", ctypes.c_int32),
-        ("tile_node_ptr", ctypes.c_void_p),
-        ("tile_nodes", ctypes.c_void_p),
-        ("elem_local", ctypes.c_void_p),
-        ("max_tile_nodes", ctypes.c_int32),
+        ("tile_ptr", ctypes.c_void_p),
+        ("tile_elems", ctypes.c_void_p),
+        ("tile_node_ptr", ctypes.c_void_p * 2),
+        ("tile_nodes", ctypes.c_void_p * 2),
+        ("tile_nexcl", ctypes.c_void_p * 2),
+        ("elem_local", ctypes.c_void_p * 2),
+        ("max_tile_nodes", ctypes.c_int32 * 2),
         ("nbatches", ctypes.c_int32),
         ("batch_rows", ctypes.c_void_p),
         ("batch_max_nnz", ctypes.c_int32),
@@ -99,6 +101,8 @@ _SIGNATURES = {
     "op2hip_sparsity_fill": (ctypes.c_int, [_I, _I, _V, _I, _V, _V, _I, _V, _V, _V]),
     "op2hip_pattern_expand": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _V, _V]),
     "op2hip_colour_greedy": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _I, _V, _i32p]),
+    "op2hip_plan_tiles": (ctypes.c_int, [_I, _I, _I, _V, _V, _V, _I, _I, _i32p, _V, _V, _V, _V, _V,
+                                         _V, _V]),
     "op2hip_plan_row_batches": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _i32p, _i32p, _i32p, _i32p]),
     "op2hip_slot_table_bytes": (ctypes.c_int, [_I, ctypes.POINTER(ctypes.c_size_t)]),
     "op2hip_build_slot_table": (ctypes.c_int, [_I, _V, _V, _V, _V, _V, _V, _I, _I, _V, _V, _V, _V]),

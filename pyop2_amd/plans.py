@@ -11,7 +11,8 @@ import numpy as np
 
 from pyop2_amd import device as dev
 
-__all__ = ["transpose", "colour_plan", "row_batches", "Transpose", "ColourPlan", "RowBatches"]
+__all__ = ["transpose", "colour_plan", "row_batches", "tile_plan", "Transpose", "ColourPlan",
+           "RowBatches", "TilePlan"]
 
 
 @dataclass
@@ -210,5 +211,70 @@ def row_batches(owner, tr, rowptr, max_inc, max_nnz, key):
                                         ctypes.byref(mi), ctypes.byref(mn), ctypes.byref(mr)),
               "plan_row_batches")
     plan = RowBatches(dev.to_device(br[:nb.value + 1]), nb.value, mi.value, mn.value, mr.value)
+    owner._cache[ck] = plan
+    return plan
+
+
+@dataclass
+class TilePlan:
+    """Compact tiles of the elements of [start, end) and, per staged map, each
+    tile's distinct nodes (exclusive ones first) and the elements' local ids."""
+    ntiles: int
+    tile_ptr: object
+    tile_elems: object
+    node_ptr: list
+    nodes: list
+    nexcl: list
+    local: list
+    max_nodes: list
+
+    def exec_fields(self, ex):
+        ex.ntiles = self.ntiles
+        ex.tile_ptr = dev.ptr(self.tile_ptr)
+        ex.tile_elems = dev.ptr(self.tile_elems)
+        for m in range(len(self.nodes)):
+            ex.tile_node_ptr[m] = dev.ptr(self.node_ptr[m])
+            ex.tile_nodes[m] = dev.ptr(self.nodes[m])
+            ex.tile_nexcl[m] = dev.ptr(self.nexcl[m])
+            ex.elem_local[m] = dev.ptr(self.local[m])
+            ex.max_tile_nodes[m] = self.max_nodes[m]
+
+
+def tile_plan(maps, start, end, tile_size, node_cap=0):
+    """Tile plan of the elements [start, end) over ``maps`` (staged map order)
+    with at most ``tile_size`` elements and ``node_cap`` nodes of map 0 (0: no
+    cap) per tile, cached on the first map."""
+    owner = maps[0]
+    ck = ("tiles", tuple(id(m) for m in maps), int(start), int(end), int(tile_size), int(node_cap))
+    plan = owner._cache.get(ck)
+    if plan is not None:
+        return plan
+    L = dev.lib()
+    n = int(end - start)
+    nm = len(maps)
+    vals = [np.ascontiguousarray(m.values_with_halo, dtype=np.int32) for m in maps]
+    ar = np.array([m.arity for m in maps], dtype=np.int32)
+    ts = np.array([m.toset.total_size for m in maps], dtype=np.int32)
+    tp = np.empty(n + 1, dtype=np.int32)
+    te = np.empty(max(n, 1), dtype=np.int32)
+    nps = [np.empty(n + 1, dtype=np.int32) for _ in maps]
+    nds = [np.empty(max(n * m.arity, 1), dtype=np.int32) for m in maps]
+    nxs = [np.empty(max(n, 1), dtype=np.int32) for _ in maps]
+    lcs = [np.empty(max(n * m.arity, 1), dtype=np.uint16) for m in maps]
+    mx = np.zeros(2, dtype=np.int32)
+    nt = ctypes.c_int32()
+
+    def arr(a):
+        return (ctypes.c_void_p * nm)(*[x.ctypes.data for x in a])
+    dev.check(L.op2hip_plan_tiles(int(start), int(end), nm, arr(vals), ar.ctypes.data, ts.ctypes.data,
+                                  int(tile_size), int(node_cap), ctypes.byref(nt), tp.ctypes.data,
+                                  te.ctypes.data, arr(nps), arr(nds), arr(nxs), arr(lcs), mx.ctypes.data),
+              "plan_tiles")
+    nt = nt.value
+    plan = TilePlan(nt, dev.to_device(tp[:nt + 1]), dev.to_device(te),
+                    [dev.to_device(x[:nt + 1]) for x in nps],
+                    [dev.to_device(nds[k][:max(int(nps[k][nt]), 1)]) for k in range(nm)],
+                    [dev.to_device(x[:max(nt, 1)]) for x in nxs],
+                    [dev.to_device(x) for x in lcs], [int(v) for v in mx[:nm]])
     owner._cache[ck] = plan
     return plan

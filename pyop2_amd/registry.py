@@ -37,10 +37,12 @@ def register_element_kernel(code, name, family, params):
 FAMILIES = {
     "p1_lumped_mass": dict(
         args=[("dat", INC, (1,), 3), ("dat", READ, (2,), 3)],
-        maps=[[(0, 0), (1, 0)]], entry={"f64": "op2hip_wrap_p1_lumped_mass"}),
+        maps=[[(0, 0), (1, 0)]], entry={"f64": "op2hip_wrap_p1_lumped_mass"},
+        staged=[0], staged_vals=[(2, 1)]),
     "p1_laplace_residual": dict(
         args=[("dat", INC, (1,), 3), ("dat", READ, (2,), 3), ("dat", READ, (1,), 3)],
-        maps=[[(0, 0), (1, 0), (2, 0)]], entry={"f64": "op2hip_wrap_p1_laplace_residual"}),
+        maps=[[(0, 0), (1, 0), (2, 0)]], entry={"f64": "op2hip_wrap_p1_laplace_residual"},
+        staged=[0], staged_vals=[(3, 1)]),
     "p1_laplace_matrix": dict(
         args=[("mat", INC, (1, 1), (3, 3)), ("dat", READ, (2,), 3)],
         maps=[[(0, 0), (0, 1), (1, 0)]], entry={"f64": "op2hip_wrap_p1_laplace_matrix"}),
@@ -48,7 +50,8 @@ FAMILIES = {
         args=[("dat", INC, (3,), 10), ("dat", READ, (3,), 4), ("dat", READ, (3,), 10)],
         maps=[[(0, 0), (2, 0)]],
         entry={"f64": "op2hip_wrap_p2_elasticity_residual",
-               "f32": "op2hip_wrap_p2_elasticity_residual_f32"}),
+               "f32": "op2hip_wrap_p2_elasticity_residual_f32"},
+        staged=[0, 1], staged_vals=[(3, 3), (3, 0)]),
     "p2_elasticity_matrix": dict(
         args=[("mat", INC, (3, 3), (10, 10)), ("dat", READ, (3,), 4)],
         maps=[[(0, 0), (0, 1)]],
@@ -64,8 +67,23 @@ FAMILIES = {
         args=[("dat", INC, (3,), 10), ("dat", INC, (3,), 4), ("dat", READ, (3,), 10),
               ("dat", READ, (3,), 4), ("dat", READ, (3,), 10)],
         maps=[[(0, 0), (2, 0), (4, 0)], [(1, 0), (3, 0)]],
-        entry={"f64": "op2hip_wrap_p2_elasticity_adjoint"}),
+        entry={"f64": "op2hip_wrap_p2_elasticity_adjoint"},
+        staged=[0, 1], staged_vals=[(6, 3), (3, 3)]),
 }
+
+# "staged": arguments whose maps are the staged maps 0 and 1 of the entry
+# point's OP2HIP_STAGED launch (tiles of _TILE elements); "staged_vals":
+# per staged map, the values staged (Mat dtype) and accumulated (double) per
+# node, which size the launch's LDS.
+_TILE = 256
+_LDS = 64 * 1024
+
+
+def _tile_lds(spec, itemsize, max_nodes):
+    b = 0
+    for (nin, nout), mx in zip(spec["staged_vals"], max_nodes):
+        b = (b + mx * nin * itemsize + 7) // 8 * 8 + mx * nout * 8
+    return b
 
 # Row-batch budgets (max row incidences, max blocks) of the row-gather
 # assembly (LDS <= 64 KiB per workgroup).
@@ -153,13 +171,29 @@ class HandWrittenKernel:
             want = autotune.choice(self, parloop)
         if want.startswith("rowgather") and (not has_mat or subset):
             want = "atomic"
-        if want == "staged":
+        if want == "staged" and (subset or "staged" not in self.spec):
             want = "atomic"
         if has_mat and want.startswith("rowgather"):
             mat = parloop.arguments[0].data
             if mat.block_shape != self.spec["args"][0][2]:
                 want = "atomic"
         return want
+
+    def _tile_plan(self, args, start, end):
+        """Tile plan whose LDS fits a workgroup: the node cap of map 0 shrinks
+        until it does (None if even small tiles do not fit)."""
+        from pyop2_amd import plans
+        maps = [args[k].maps[0] for k in self.spec["staged"]]
+        isz = 8 if self.variant == "f64" else 4
+        nin, nout = self.spec["staged_vals"][0]
+        cap = 0
+        for _ in range(6):
+            tp = plans.tile_plan(maps, start, end, _TILE, cap)
+            if _tile_lds(self.spec, isz, tp.max_nodes) <= _LDS:
+                return tp
+            per0 = nin * isz + nout * 8
+            cap = max(maps[0].arity, int(0.85 * min(tp.max_nodes[0], _LDS // per0)))
+        return None
 
     def _inc_maps(self, parloop):
         out = []
@@ -200,6 +234,14 @@ class HandWrittenKernel:
                                      [(m, id(m.toset)) for m in self._inc_maps(parloop)], key)
             plan.exec_fields(ex)
             keep.append(plan)
+        if strategy == "staged":
+            tp = self._tile_plan(args, start, end)
+            if tp is None:
+                strategy = "atomic"
+                ex.strategy = dev.STRATEGIES[strategy]
+            else:
+                tp.exec_fields(ex)
+                keep.append(tp)
         if self.spec["args"][0][0] == "mat":
             mat = args[0].data
             rmap = args[0].maps[0]

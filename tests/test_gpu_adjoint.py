@@ -15,6 +15,7 @@ import pytest
 from helpers import TetProblem, relerr
 from oracle import seqwrap
 from pyop2_amd import adjoint, device as dev, elements, op2
+from pyop2_amd.configuration import configuration
 from pyop2_amd.meshes import unit_cube_tet_p2, workload_rng
 
 pytestmark = pytest.mark.gpu
@@ -26,7 +27,9 @@ def _jitter(m, rng, amp=0.1):
     return np.ascontiguousarray(X)
 
 
-def test_adjoint_loop_matches_oracle(device):
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged"])
+def test_adjoint_loop_matches_oracle(device, strategy):
+    configuration["strategy"] = strategy
     m = unit_cube_tet_p2(6, rng=workload_rng(4))
     rng = np.random.default_rng([2026, 4, 1])
     X = _jitter(m, rng)

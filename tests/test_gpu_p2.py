@@ -19,7 +19,7 @@ def tet8():
     return unit_cube_tet_p2(8, rng=workload_rng(3))
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged"])
 @pytest.mark.parametrize("split", ["all", "half"])
 def test_p2_residual_f64(device, tet8, strategy, split):
     configuration["strategy"] = strategy
@@ -31,7 +31,9 @@ def test_p2_residual_f64(device, tet8, strategy, split):
     assert relerr(R.data_ro, oracle_p2("p2_elasticity_residual", tet8)) < 1e-12
 
 
-def test_p2_residual_f32(device, tet8):
+@pytest.mark.parametrize("strategy", ["atomic", "staged"])
+def test_p2_residual_f32(device, tet8, strategy):
+    configuration["strategy"] = strategy
     p = TetProblem(tet8, dtype=np.float32)
     R = op2.Dat(p.nodes ** 3, dtype=np.float32, name="R")
     op2.par_loop(elements.p2_elasticity_residual(), p.cells, R(op2.INC, p.c2n), p.X(op2.READ, p.c2v),

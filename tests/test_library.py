@@ -82,6 +82,71 @@ def test_row_batches_cover_rows_within_budget():
             assert ip[hi] - ip[lo] <= 256 and rp[hi] - rp[lo] <= 300
 
 
+def _tiles(maps, sizes, start, end, tile, cap=0):
+    L = dev.lib()
+    n = end - start
+    nm = len(maps)
+    tp, te = np.empty(n + 1, np.int32), np.empty(max(n, 1), np.int32)
+    nps = [np.empty(n + 1, np.int32) for _ in maps]
+    nds = [np.empty(max(n * m.shape[1], 1), np.int32) for m in maps]
+    nxs = [np.empty(max(n, 1), np.int32) for _ in maps]
+    lcs = [np.empty(max(n * m.shape[1], 1), np.uint16) for m in maps]
+    mx = np.zeros(2, np.int32)
+    ar = np.array([m.shape[1] for m in maps], np.int32)
+    ts = np.array(sizes, np.int32)
+    nt = ctypes.c_int32()
+
+    def arr(a):
+        return (ctypes.c_void_p * nm)(*[x.ctypes.data for x in a])
+    rc = L.op2hip_plan_tiles(start, end, nm, arr(maps), ar.ctypes.data, ts.ctypes.data, tile, cap,
+                             ctypes.byref(nt), tp.ctypes.data, te.ctypes.data, arr(nps), arr(nds),
+                             arr(nxs), arr(lcs), mx.ctypes.data)
+    assert rc == 0, dev.lib().op2hip_last_error()
+    nt = nt.value
+    return nt, tp[:nt + 1], te[:n], nps, nds, nxs, lcs, mx
+
+
+@pytest.mark.parametrize("start,end,tile,cap", [(0, 3072, 256, 0), (100, 3000, 128, 0), (0, 37, 64, 0),
+                                               (0, 3072, 256, 300), (5, 2000, 256, 40)])
+def test_tiles_partition_elements_and_renumber_nodes(start, end, tile, cap):
+    """Every element of [start, end) in exactly one tile of at most ``tile``;
+    local ids give back the map entries; a tile's first nexcl nodes occur in
+    no other tile, the others in at least two; node lists strictly increase
+    within each part."""
+    m = unit_cube_tet_p2(8, rng=workload_rng(3))
+    maps = [np.ascontiguousarray(m.cell_node, np.int32), np.ascontiguousarray(m.cell_vertex, np.int32)]
+    nt, tp, te, nps, nds, nxs, lcs, mx = _tiles(maps, [m.nnodes, m.nverts], start, end, tile, cap)
+    if cap:
+        assert mx[0] <= cap
+    assert tp[0] == 0 and tp[-1] == end - start and np.all(np.diff(tp) <= tile) and np.all(np.diff(tp) > 0)
+    assert np.array_equal(np.sort(te), np.arange(start, end))
+    tile_of = np.repeat(np.arange(nt), np.diff(tp))
+    for k, mp in enumerate(maps):
+        loc = lcs[k][:(end - start) * mp.shape[1]].reshape(end - start, -1).astype(np.int64)
+        assert np.array_equal(nds[k][nps[k][tile_of][:, None] + loc], mp[te])
+        owners = {}
+        for t in range(nt):
+            ids = nds[k][nps[k][t]:nps[k][t + 1]]
+            assert len(ids) <= mx[k]
+            ex, sh = ids[:nxs[k][t]], ids[nxs[k][t]:]
+            assert np.all(np.diff(ex) > 0) and np.all(np.diff(sh) > 0)
+            for v in ids:
+                owners.setdefault(int(v), []).append((t, bool(len(ex) and v in set(ex.tolist()))))
+        for v, occ in owners.items():
+            assert all(flag == (len(occ) == 1) for _, flag in occ)
+
+
+def test_tiles_are_compact_regions():
+    """On a structured mesh the grown regions touch far fewer nodes than
+    runs of consecutive elements (a strip of the numbering)."""
+    m = unit_cube_tet_p2(24, rng=workload_rng(3))
+    c2n = np.ascontiguousarray(m.cell_node, np.int32)
+    nt, tp, te, nps, nds, nxs, lcs, mx = _tiles([c2n], [m.nnodes], 0, m.ncells, 256)
+    grown = np.diff(nps[0])[:-1].mean()
+    strips = np.mean([len(np.unique(c2n[a:a + 256])) for a in range(0, m.ncells - 256, 256)])
+    assert grown < 0.8 * strips
+
+
 def test_generated_wrappers_compile_for_gfx950():
     """Every known-answer kernel's generated wrapper compiles with hiprtc."""
     import kat_cases  # noqa: F401  (the kernels are built through pyop2_amd.op2 below)
