@@ -300,6 +300,46 @@ int op2hip_wrap_p2_elasticity_adjoint(int32_t start, int32_t end,
                                       const int32_t *c2v, double lam, double mu,
                                       const op2hip_exec *ex, void *stream);
 
+/* ---- batched loops over many independent meshes (no reference counterpart:
+ * the reference runs one par_loop per mesh) ---- */
+/* One mesh of a batched P1 Laplace launch: CSR assembly of the Laplace
+ * element matrices into A (slot-table row gather: A.inc_ptr/inc_ent from
+ * the mesh's map transpose, batch descriptors and slot table from
+ * op2hip_build_slot_table) and INC of the Laplace residual K u into R
+ * (staged tiles of op2hip_plan_tiles over c2v).  All pointers device. */
+typedef struct op2hip_batch_mesh {
+  op2hip_mat A;
+  const void *batch_desc;
+  const void *slot_table;
+  int32_t nbatches;
+  int32_t ncells;
+  const double *X;                /* vertex coordinates (nverts, 2) */
+  const int32_t *c2v;             /* (ncells, 3) */
+  const double *u;                /* (nverts) */
+  double *R;                      /* (nverts), incremented */
+  int32_t ntiles;
+  int32_t max_tile_nodes;
+  const int32_t *tile_ptr;
+  const int32_t *tile_elems;
+  const int32_t *tile_node_ptr;
+  const int32_t *tile_nodes;
+  const int32_t *tile_nexcl;
+  const uint16_t *elem_local;
+} op2hip_batch_mesh;
+
+/* Work item flag: item (mesh, OP2HIP_BATCH_TILE | t) is residual tile t of
+ * the mesh, item (mesh, b) row batch b of its assembly. */
+#define OP2HIP_BATCH_TILE 0x40000000
+
+/* Runs every work item (items: nitems (mesh, code) int32 pairs, device) of
+ * the meshes (device array of op2hip_batch_mesh) in one launch of grid
+ * workgroups (0: one per item) that stride over the items.  max_batch_nnz /
+ * max_tile_nodes: largest batch / tile over the meshes (size the LDS).
+ * Allocates nothing and synchronises nothing (safe under stream capture). */
+int op2hip_batched_p1_laplace(const op2hip_batch_mesh *meshes, const int32_t *items,
+                              int32_t nitems, int32_t grid, int32_t max_batch_nnz,
+                              int32_t max_tile_nodes, void *stream);
+
 /* ---- generic loops: runtime-compiled gather/compute/scatter template ---- */
 /* Compiles HIP source (a generated wrapper around the user's C kernel) for
  * gfx950 with hiprtc and loads it.  *module receives an opaque handle. */

@@ -5,7 +5,8 @@ combination with ``configuration["strategy"] == "auto"``, every applicable
 strategy is timed with HIP events on scratch copies of the written
 arguments (so the user's data is untouched), and the fastest is kept in
 memory and in ``<cache_dir>/autotune.json``.  With ``autotune`` disabled the
-defaults are ``rowgather`` for Mat assembly and ``atomic`` for Dat INC.
+first candidate is used: ``rowgather_slots`` for Mat assembly, ``staged``
+for Dat INC over a Set.
 """
 import json
 import os
@@ -54,7 +55,7 @@ def candidates(executor, parloop):
     if has_mat:
         if isinstance(parloop.iterset, Subset):
             return ["atomic", "coloured"]
-        return ["rowgather_staged", "rowgather_slots", "rowgather_thread", "rowgather", "atomic"]
+        return ["rowgather_slots", "rowgather_staged", "rowgather_thread", "rowgather", "atomic"]
     if "staged" in executor.spec and not isinstance(parloop.iterset, Subset):
         return ["staged", "atomic", "coloured"]
     return ["atomic", "coloured"]
@@ -87,6 +88,19 @@ def choice(executor, parloop):
         _memo[key] = best
         _save()
     return best
+
+
+def cached(key):
+    """Stored result for ``key`` (a JSON-serialisable list), or None."""
+    with _lock:
+        _load()
+        return _memo.get(json.dumps(key))
+
+
+def store(key, value):
+    with _lock:
+        _memo[json.dumps(key)] = value
+        _save()
 
 
 def tune(executor, parloop, repeats=REPEATS, report=None):

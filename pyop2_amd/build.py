@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libop2hip.so")
 
-SOURCES = ["core.hip", "p1.hip", "p2.hip", "jit.hip"]
+SOURCES = ["core.hip", "p1.hip", "p2.hip", "batch.hip", "jit.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -467,17 +467,17 @@ gather_fill_kernel(int32_t nbatches, BatchDesc *__restrict__ desc,
   }
 }
 
+// One batch of the slot-table row gather (the workgroup's share of a launch,
+// or one work item of a batched launch).
 template <class Elem, int kPre>
-__global__ void __launch_bounds__(256)
-rowgather_slots_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
-                       const BatchDesc *__restrict__ descs, int32_t nbatches,
-                       const SlotEntry<Elem::kCar> *__restrict__ table,
-                       typename Elem::T *__restrict__ rowdat) {
+__device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, int32_t end,
+                                            const Elem &elem, const op2hip_mat &A,
+                                            const SlotEntry<Elem::kCar> *__restrict__ table,
+                                            typename Elem::T *__restrict__ rowdat) {
   using T = typename Elem::T;
   using Acc = typename AccumOf<T>::type;
   constexpr int kBs = Elem::kRd * Elem::kCd;
   extern __shared__ __align__(16) unsigned char smem[];
-  const BatchDesc d = descs[xcd_contiguous(blockIdx.x, nbatches)];
   T *__restrict__ vals = static_cast<T *>(A.values) + (size_t)d.z0 * kBs;
   const int32_t nval = d.nz * kBs;
 
@@ -543,6 +543,16 @@ rowgather_slots_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
     T *rd = rowdat + (size_t)d.r0 * Elem::kRd;
     for (int32_t k = threadIdx.x; k < d.nr * Elem::kRd; k += blockDim.x) rd[k] += T(racc[k]);
   }
+}
+
+template <class Elem, int kPre>
+__global__ void __launch_bounds__(256)
+rowgather_slots_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
+                       const BatchDesc *__restrict__ descs, int32_t nbatches,
+                       const SlotEntry<Elem::kCar> *__restrict__ table,
+                       typename Elem::T *__restrict__ rowdat) {
+  slots_batch<Elem, kPre>(descs[xcd_contiguous(blockIdx.x, nbatches)], start, end, elem, A, table,
+                          rowdat);
 }
 
 // Deepest old-value prefetch of an element (Elem::kMaxPrefetch when given):

@@ -91,16 +91,16 @@ __device__ __forceinline__ void flush_map(const Elem &elem, const TilePlan &P, i
   }
 }
 
+// Tile b of a staged launch (the workgroup's share, or one work item of a
+// batched launch).
 template <class Elem>
-__global__ void __launch_bounds__(256)
-staged_kernel(Elem elem, TilePlan P) {
+__device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P, int32_t b) {
   using T = typename Elem::T;
   using L = TileLayout<Elem>;
   constexpr int A0 = Elem::kAr0, I0 = at_least_1(Elem::kIn0), O0 = at_least_1(Elem::kOut0);
   constexpr int A1 = at_least_1(Elem::kAr1), I1 = at_least_1(Elem::kIn1),
                 O1 = at_least_1(Elem::kOut1);
   extern __shared__ __align__(16) unsigned char smem[];
-  const int32_t b = xcd_contiguous(blockIdx.x, P.ntiles);
   T *in0 = reinterpret_cast<T *>(smem + L::in_at(0, P.max_nodes));
   double *acc0 = reinterpret_cast<double *>(smem + L::acc_at(0, P.max_nodes));
   T *in1 = reinterpret_cast<T *>(smem + (Elem::kMaps > 1 ? L::in_at(1, P.max_nodes) : 0));
@@ -149,6 +149,12 @@ staged_kernel(Elem elem, TilePlan P) {
 
   flush_map<Elem, 0>(elem, P, b, acc0);
   if constexpr (Elem::kMaps > 1) flush_map<Elem, 1>(elem, P, b, acc1);
+}
+
+template <class Elem>
+__global__ void __launch_bounds__(256)
+staged_kernel(Elem elem, TilePlan P) {
+  staged_tile(elem, P, xcd_contiguous(blockIdx.x, P.ntiles));
 }
 
 template <class Elem>

@@ -101,6 +101,7 @@ _SIGNATURES = {
     "op2hip_sparsity_fill": (ctypes.c_int, [_I, _I, _V, _I, _V, _V, _I, _V, _V, _V]),
     "op2hip_pattern_expand": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _V, _V]),
     "op2hip_colour_greedy": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _I, _V, _i32p]),
+    "op2hip_batched_p1_laplace": (ctypes.c_int, [_V, _V, _I, _I, _I, _I, _V]),
     "op2hip_plan_tiles": (ctypes.c_int, [_I, _I, _I, _V, _V, _V, _I, _I, _i32p, _V, _V, _V, _V, _V,
                                          _V, _V]),
     "op2hip_plan_row_batches": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _i32p, _i32p, _i32p, _i32p]),
@@ -162,15 +163,28 @@ def lib():
                 if not os.path.exists(_LIB_PATH):
                     raise DeviceError(f"op2hip library not built: {_LIB_PATH} is missing "
                                       "(run `python -m pyop2_amd.build`)")
-                L = ctypes.CDLL(_LIB_PATH)
-                for name, (res, args) in _SIGNATURES.items():
-                    fn = getattr(L, name)
-                    fn.restype = res
-                    fn.argtypes = args
+                L = _Bound(ctypes.CDLL(_LIB_PATH))
                 if L.op2hip_abi_version() != 1:
                     raise DeviceError("op2hip ABI version mismatch")
                 _lib = L
     return _lib
+
+
+class _Bound:
+    """The library's entry points with their declared argument types only:
+    an undeclared name raises instead of passing 64-bit pointers through
+    ctypes' default int conversion."""
+
+    def __init__(self, cdll):
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(cdll, name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+        self._handle = cdll
+
+    def __getattr__(self, name):
+        raise AttributeError(f"{name}: no ctypes signature declared for this library entry point")
 
 
 def available():
