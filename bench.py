@@ -93,23 +93,24 @@ def measured_traffic(key):
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
+                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
                     return float(parts[2]), os.path.relpath(path, ROOT)
     return None
 
 
 def copy_bandwidth(nbytes=1 << 29, reps=10):
-    """Device-to-device copy of ``nbytes`` (HIP events): measured stream GB/s.
-    Also the calibration kernel of the FETCH_SIZE / WRITE_SIZE profiles."""
+    """Streaming kernel y = -x over ``nbytes`` each way (HIP events): the
+    measured copy bandwidth, and the calibration kernel (torch's neg kernel,
+    known bytes) of the FETCH_SIZE / WRITE_SIZE profiles."""
     from pyop2_amd import device as dev
     torch = dev.torch()
     x = torch.ones(nbytes // 8, dtype=torch.float64, device=dev.device())
     y = torch.empty_like(x)
-    y.copy_(x)
+    torch.neg(x, out=y)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     for a, b in ev:
         a.record()
-        y.copy_(x)
+        torch.neg(x, out=y)
         b.record()
     torch.cuda.synchronize()
     ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
@@ -231,7 +232,7 @@ def run(args):
         out["c5_batched"] = batched(args)
     copy_gbs, copy_ms = copy_bandwidth()
     out["stream_copy"] = dict(bytes_each_way=1 << 29, ms=copy_ms, gbs=copy_gbs)
-    log(f"device copy of 512 MiB: {copy_ms * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)")
+    log(f"streaming y = -x over 512 MiB: {copy_ms * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)")
 
     kname = kernel_name(strategy_used, "op2hip::P1MatrixRow", "op2hip::P1MatrixLanes", "p1_matrix_kernel")
     tkey = f"p1_laplace_matrix:{strategy_used}:{n2}"

@@ -147,6 +147,35 @@ def test_tiles_are_compact_regions():
     assert grown < 0.8 * strips
 
 
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """Sizes and field offsets of op2hip_exec, op2hip_mat and
+    op2hip_batch_mesh as the C compiler lays them out equal the ctypes
+    mirrors (a mismatch would hand the kernels wrong pointers)."""
+    import shutil
+    import subprocess
+    from pyop2_amd.batch import BatchMesh
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"op2hip_exec": dev.ExecPlan, "op2hip_mat": dev.MatDesc, "op2hip_batch_mesh": BatchMesh}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "op2hip.h"', "int main(void) {"]
+    for cname, ct in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in ct._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, ct in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(ct), cname
+        for f in ct._fields_:
+            assert got[(cname, f[0])] == getattr(ct, f[0]).offset, (cname, f[0])
+
+
 def test_generated_wrappers_compile_for_gfx950():
     """Every known-answer kernel's generated wrapper compiles with hiprtc."""
     import kat_cases  # noqa: F401  (the kernels are built through pyop2_amd.op2 below)
