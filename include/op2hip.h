@@ -50,9 +50,18 @@ enum op2hip_strategy {
   OP2HIP_ROWGATHER_THREAD = 4, /* as ROWGATHER, one thread per row incidence */
   OP2HIP_ROWGATHER_SLOTS = 5,  /* as ROWGATHER_THREAD, with a precomputed slot
                                   table (no pattern search in the kernel) */
-  OP2HIP_ROWGATHER_STAGED = 6  /* slot-table plan; element block rows staged in
+  OP2HIP_ROWGATHER_STAGED = 6, /* slot-table plan; element block rows staged in
                                   LDS, each output block summed by one thread
                                   (no atomics, deterministic) */
+  OP2HIP_ROWGATHER_CELLS = 7,  /* row batches; each element's geometry staged in
+                                  LDS once per batch, each output block
+                                  evaluated and summed by one thread
+                                  (op2hip_plan_cell_gather; no atomics) */
+  OP2HIP_ROWGATHER_ROWLIST = 8 /* as ROWGATHER_SLOTS over batches of rows taken
+                                  in an element-local order instead of row-id
+                                  order (op2hip_plan_row_list): the gathers of
+                                  a batch stay in cache when the row numbering
+                                  is scattered */
 };
 
 /* Execution plan of one launch.  Host struct; arrays as annotated. */
@@ -84,6 +93,18 @@ typedef struct op2hip_exec {
   /* OP2HIP_ROWGATHER_STAGED: gather table (op2hip_gather_table_fill) */
   const uint8_t *gather_count;     /* device, per Mat block: element blocks added to it */
   const uint16_t *gather_list;     /* device, per batch: element block positions */
+  /* OP2HIP_ROWGATHER_CELLS (op2hip_plan_cell_gather; gather_count as above) */
+  const int32_t *cell_ptr;         /* device, nbatches+1 offsets into cells */
+  const int32_t *cells;            /* device, the distinct elements of each batch */
+  const uint16_t *block_order;     /* device, per Mat block: batch blocks in thread order */
+  const int32_t *contrib_ptr;      /* device, nbatches+1 offsets into contrib */
+  const uint16_t *contrib;         /* device, (u << 8) | (i << 4) | j per element block */
+  int32_t max_batch_cells;         /* largest cell count of a batch (<= 256) */
+  int32_t max_batch_contrib;       /* largest contribution count of a batch */
+  /* OP2HIP_ROWGATHER_ROWLIST (op2hip_plan_row_list; batch_desc / slot_table
+     then index the batch-ordered incidences) */
+  const int32_t *row_order;        /* device, (row, block offset in batch) in processing order */
+  const int32_t *batch_ent;        /* device, per batch incidence: element * arity + i */
   /* Subset iteration: element n = subset[i] for i in [start,end) (or NULL) */
   const int32_t *subset;           /* device */
 } op2hip_exec;
@@ -219,6 +240,43 @@ int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows,
                             const int32_t *cmap, int32_t rarity, int32_t carity,
                             void *table, void *batch_desc, int32_t *bad,
                             void *stream);
+
+/* Row-list plan of OP2HIP_ROWGATHER_ROWLIST (host arrays).  Rows are taken
+ * in the order of their first incident element (ties by row id) and cut
+ * into batches of at most max_inc incidences and max_nnz blocks (a longer
+ * row is a batch of its own).  Outputs: row_order (2 * nrows int32: per
+ * position, the row and its block offset within its batch); *nbatches;
+ * desc (8 int32 per batch, capacity nrows: {first position in row_order,
+ * rows, 0, blocks, first batch incidence, incidences, 0, 0}); per batch
+ * incidence, in batch order (capacity the incidence count): ent (element *
+ * rarity + i) and a slot entry of op2hip_slot_table_bytes(carity) bytes
+ * (block offset of the row in the batch, row index in the batch, column
+ * slots; 255: no column).  *missing counts map entries not in the pattern;
+ * maxima of the batches go to max_inc_out / max_nnz_out / max_rows_out. */
+int op2hip_plan_row_list(int32_t nrows, const int32_t *rowptr, const int32_t *colidx,
+                         const int32_t *inc_ptr, const int32_t *inc_ent, const int32_t *cmap,
+                         int32_t rarity, int32_t carity, int32_t max_inc, int32_t max_nnz,
+                         int32_t *row_order, int32_t *nbatches, void *desc, int32_t *ent,
+                         void *table, int64_t *missing, int32_t *max_inc_out,
+                         int32_t *max_nnz_out, int32_t *max_rows_out);
+
+/* Cell-gather plan of OP2HIP_ROWGATHER_CELLS (host arrays), from the batch
+ * descriptors and slot table of op2hip_build_slot_table and the map
+ * transpose's inc_ent, all copied to the host.  Per batch b: the distinct
+ * elements of its row incidences, ascending, at cells[cell_ptr[b] ..]; per
+ * Mat block z: gcount[z], the number of element blocks added to it; per
+ * batch its blocks in thread order (order[z0 ..]: block index within the
+ * batch, by count descending, ties ascending) and, in that order, each
+ * block's element blocks at contrib[contrib_ptr[b] ..] as (u << 8) | (i << 4)
+ * | j (u: position in the batch's cell list; i, j: local row and column), in
+ * element order.  Capacities: cell_ptr and contrib_ptr nbatches+1, cells the
+ * total incidence count, contrib that count times carity.  Batches of at
+ * most 256 elements and 65536 blocks, arities <= 16, <= 255 element blocks
+ * per Mat block (else OP2HIP_ERR_ARG). */
+int op2hip_plan_cell_gather(int32_t nbatches, const void *batch_desc, const void *table,
+                            int32_t rarity, int32_t carity, const int32_t *inc_ent,
+                            int32_t *cell_ptr, int32_t *cells, uint8_t *gcount, uint16_t *order,
+                            int32_t *contrib_ptr, uint16_t *contrib);
 
 /* Gather table of OP2HIP_ROWGATHER_STAGED, from a slot table and its batch
  * descriptors (batches of at most 1024 blocks).  Output block z of a batch

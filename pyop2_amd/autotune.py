@@ -55,7 +55,8 @@ def candidates(executor, parloop):
     if has_mat:
         if isinstance(parloop.iterset, Subset):
             return ["atomic", "coloured"]
-        return ["rowgather_slots", "rowgather_staged", "rowgather_thread", "rowgather", "atomic"]
+        return ["rowgather_slots", "rowgather_rows", "rowgather_cells", "rowgather_staged",
+                "rowgather_thread", "rowgather", "atomic"]
     if "staged" in executor.spec and not isinstance(parloop.iterset, Subset):
         return ["staged", "atomic", "coloured"]
     return ["atomic", "coloured"]

@@ -67,7 +67,8 @@ class Configuration(dict):
                                          % (key, valid_type, type(value)))
         if key == "strategy" and value not in ("auto", "atomic", "coloured", "staged", "rowgather",
                                                "rowgather_thread", "rowgather_slots",
-                                               "rowgather_staged"):
+                                               "rowgather_staged", "rowgather_cells",
+                                               "rowgather_rows"):
             raise ConfigurationError(f"unknown strategy {value!r}")
         self._set.add(key)
         super().__setitem__(key, value)

@@ -745,6 +745,156 @@ int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr, const int32_t
 
 #include "rowgather.hpp"
 
+namespace op2hip {
+
+template <int kCar>
+int plan_rowlist(int32_t nrows, const int32_t *rowptr, const int32_t *colidx,
+                 const int32_t *inc_ptr, const int32_t *inc_ent, const int32_t *cmap,
+                 int32_t rarity, int32_t max_inc, int32_t max_nnz, int32_t *row_order,
+                 int32_t *nbatches, BatchDesc *desc, int32_t *ent, SlotEntry<kCar> *table,
+                 int64_t *missing, int32_t *mi_out, int32_t *mn_out, int32_t *mr_out) {
+  // rows by their first (lowest) incident element: incidences are sorted
+  // by element within a row, so that is the row's first incidence
+  std::vector<int32_t> idx(nrows);
+  std::vector<int32_t> key(nrows);
+  for (int32_t r = 0; r < nrows; ++r) {
+    idx[r] = r;
+    key[r] = inc_ptr[r + 1] > inc_ptr[r] ? inc_ent[inc_ptr[r]] / rarity : INT32_MAX;
+  }
+  std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return key[x] < key[y]; });
+  int32_t nb = 0, mi = 0, mn = 0, mr = 0;
+  int64_t pos = 0, miss = 0;
+  int32_t k = 0;
+  while (k < nrows) {
+    const int32_t b0 = k;
+    int32_t inc = 0, nnz = 0;
+    while (k < nrows) {
+      const int32_t r = idx[k];
+      const int32_t di = inc_ptr[r + 1] - inc_ptr[r], dn = rowptr[r + 1] - rowptr[r];
+      if (k > b0 && (inc + di > max_inc || nnz + dn > max_nnz)) break;
+      inc += di;
+      nnz += dn;
+      ++k;
+    }
+    if (nnz > 65535 || k - b0 > 65535) {
+      set_error("row list: a batch of %d blocks / %d rows exceeds 16-bit offsets", nnz, k - b0);
+      return OP2HIP_ERR_ARG;
+    }
+    BatchDesc d;
+    d.r0 = b0;
+    d.nr = k - b0;
+    d.z0 = 0;
+    d.nz = nnz;
+    d.i0 = (int32_t)pos;
+    d.ni = inc;
+    d.pad0 = d.pad1 = 0;
+    desc[nb++] = d;
+    int32_t zoff = 0;
+    for (int32_t q = b0; q < k; ++q) {
+      const int32_t r = idx[q];
+      row_order[2 * q] = r;
+      row_order[2 * q + 1] = zoff;
+      const int32_t zlo = rowptr[r], zhi = rowptr[r + 1];
+      for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
+        const int32_t e = inc_ent[t] / rarity;
+        SlotEntry<kCar> s;
+        s.row_off = static_cast<uint16_t>(zoff);
+        s.lr = static_cast<uint16_t>(q - b0);
+        for (int j = 0; j < kCar; ++j) {
+          const int32_t c = cmap[(int64_t)e * kCar + j];
+          int32_t p = -1;
+          if (c >= 0) {
+            const int32_t *hit = std::lower_bound(colidx + zlo, colidx + zhi, c);
+            if (hit != colidx + zhi && *hit == c) p = int32_t(hit - colidx);
+            else ++miss;
+          }
+          s.slot[j] = (p < 0 || p - zlo > 254) ? uint8_t(255) : static_cast<uint8_t>(p - zlo);
+        }
+        ent[pos] = inc_ent[t];
+        table[pos++] = s;
+      }
+      zoff += zhi - zlo;
+    }
+    mi = std::max(mi, inc);
+    mn = std::max(mn, nnz);
+    mr = std::max(mr, k - b0);
+  }
+  *nbatches = nb;
+  *missing = miss;
+  *mi_out = mi;
+  *mn_out = mn;
+  *mr_out = mr;
+  return OP2HIP_OK;
+}
+
+template <int kCar>
+int plan_cells(int32_t nbatches, const BatchDesc *D, const SlotEntry<kCar> *table, int32_t rarity,
+               const int32_t *inc_ent, int32_t *cell_ptr, int32_t *cells, uint8_t *gcount,
+               uint16_t *order, int32_t *contrib_ptr, uint16_t *contrib) {
+  std::vector<int32_t> es, uniq, cnt, pos, first;
+  std::vector<int32_t> idx;
+  int64_t cp = 0, gp = 0;
+  cell_ptr[0] = 0;
+  contrib_ptr[0] = 0;
+  for (int32_t b = 0; b < nbatches; ++b) {
+    const BatchDesc &d = D[b];
+    es.resize(d.ni);
+    for (int32_t t = 0; t < d.ni; ++t) es[t] = inc_ent[d.i0 + t] / rarity;
+    uniq.assign(es.begin(), es.end());
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    if (uniq.size() > 256) {
+      set_error("cell gather: batch %d has %zu elements (at most 256)", b, uniq.size());
+      return OP2HIP_ERR_ARG;
+    }
+    for (size_t k = 0; k < uniq.size(); ++k) cells[cp + (int64_t)k] = uniq[k];
+    cnt.assign(d.nz, 0);
+    for (int32_t t = 0; t < d.ni; ++t)
+      for (int j = 0; j < kCar; ++j)
+        if (table[d.i0 + t].slot[j] != 255) ++cnt[table[d.i0 + t].row_off + table[d.i0 + t].slot[j]];
+    idx.resize(d.nz);
+    for (int32_t z = 0; z < d.nz; ++z) {
+      if (cnt[z] > 255) {
+        set_error("cell gather: a block takes %d element blocks (at most 255)", cnt[z]);
+        return OP2HIP_ERR_ARG;
+      }
+      gcount[(int64_t)d.z0 + z] = static_cast<uint8_t>(cnt[z]);
+      idx[z] = z;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return cnt[x] > cnt[y]; });
+    first.assign(d.nz, 0);
+    int32_t acc = 0;
+    for (int32_t w = 0; w < d.nz; ++w) {
+      order[(int64_t)d.z0 + w] = static_cast<uint16_t>(idx[w]);
+      first[idx[w]] = acc;
+      acc += cnt[idx[w]];
+    }
+    pos.assign(first.begin(), first.end());
+    for (int32_t t = 0; t < d.ni; ++t) {
+      const int32_t ent = inc_ent[d.i0 + t];
+      const int32_t e = ent / rarity, i = ent - e * rarity;
+      const int32_t u = int32_t(std::lower_bound(uniq.begin(), uniq.end(), e) - uniq.begin());
+      for (int j = 0; j < kCar; ++j) {
+        const uint8_t s = table[d.i0 + t].slot[j];
+        if (s == 255) continue;
+        contrib[gp + pos[table[d.i0 + t].row_off + s]++] =
+            static_cast<uint16_t>((u << 8) | (i << 4) | j);
+      }
+    }
+    cp += (int64_t)uniq.size();
+    gp += acc;
+    if (cp > INT32_MAX || gp > INT32_MAX) {
+      set_error("cell gather: plan exceeds 2^31 entries");
+      return OP2HIP_ERR_OVERFLOW;
+    }
+    cell_ptr[b + 1] = (int32_t)cp;
+    contrib_ptr[b + 1] = (int32_t)gp;
+  }
+  return OP2HIP_OK;
+}
+
+}  // namespace op2hip
+
 extern "C" {
 
 int op2hip_slot_table_bytes(int32_t carity, size_t *entry_bytes) {
@@ -794,6 +944,63 @@ int op2hip_build_slot_table(int32_t nbatches, const int32_t *batch_rows, const i
     OP2HIP_LAUNCHED();
   }
   return OP2HIP_OK;
+}
+
+int op2hip_plan_row_list(int32_t nrows, const int32_t *rowptr, const int32_t *colidx,
+                         const int32_t *inc_ptr, const int32_t *inc_ent, const int32_t *cmap,
+                         int32_t rarity, int32_t carity, int32_t max_inc, int32_t max_nnz,
+                         int32_t *row_order, int32_t *nbatches, void *desc, int32_t *ent,
+                         void *table, int64_t *missing, int32_t *max_inc_out,
+                         int32_t *max_nnz_out, int32_t *max_rows_out) {
+  OP2HIP_REQUIRE(nrows >= 0 && rowptr && colidx && inc_ptr && inc_ent && cmap && rarity > 0 &&
+                     max_inc > 0 && max_nnz > 0 && row_order && nbatches && desc && ent && table &&
+                     missing && max_inc_out && max_nnz_out && max_rows_out,
+                 "plan_row_list: bad arguments");
+  auto *D = static_cast<BatchDesc *>(desc);
+  switch (carity) {
+    case 3:
+      return plan_rowlist<3>(nrows, rowptr, colidx, inc_ptr, inc_ent, cmap, rarity, max_inc,
+                             max_nnz, row_order, nbatches, D, ent,
+                             static_cast<SlotEntry<3> *>(table), missing, max_inc_out,
+                             max_nnz_out, max_rows_out);
+    case 4:
+      return plan_rowlist<4>(nrows, rowptr, colidx, inc_ptr, inc_ent, cmap, rarity, max_inc,
+                             max_nnz, row_order, nbatches, D, ent,
+                             static_cast<SlotEntry<4> *>(table), missing, max_inc_out,
+                             max_nnz_out, max_rows_out);
+    case 10:
+      return plan_rowlist<10>(nrows, rowptr, colidx, inc_ptr, inc_ent, cmap, rarity, max_inc,
+                              max_nnz, row_order, nbatches, D, ent,
+                              static_cast<SlotEntry<10> *>(table), missing, max_inc_out,
+                              max_nnz_out, max_rows_out);
+    default:
+      set_error("row lists exist for column arities 3, 4 and 10 (got %d)", carity);
+      return OP2HIP_ERR_ARG;
+  }
+}
+
+int op2hip_plan_cell_gather(int32_t nbatches, const void *batch_desc, const void *table,
+                            int32_t rarity, int32_t carity, const int32_t *inc_ent,
+                            int32_t *cell_ptr, int32_t *cells, uint8_t *gcount, uint16_t *order,
+                            int32_t *contrib_ptr, uint16_t *contrib) {
+  OP2HIP_REQUIRE(nbatches >= 0 && batch_desc && table && inc_ent && cell_ptr && cells && gcount &&
+                     order && contrib_ptr && contrib && rarity > 0 && rarity <= 16,
+                 "plan_cell_gather: bad arguments");
+  const auto *D = static_cast<const BatchDesc *>(batch_desc);
+  switch (carity) {
+    case 3:
+      return plan_cells<3>(nbatches, D, static_cast<const SlotEntry<3> *>(table), rarity, inc_ent,
+                           cell_ptr, cells, gcount, order, contrib_ptr, contrib);
+    case 4:
+      return plan_cells<4>(nbatches, D, static_cast<const SlotEntry<4> *>(table), rarity, inc_ent,
+                           cell_ptr, cells, gcount, order, contrib_ptr, contrib);
+    case 10:
+      return plan_cells<10>(nbatches, D, static_cast<const SlotEntry<10> *>(table), rarity,
+                            inc_ent, cell_ptr, cells, gcount, order, contrib_ptr, contrib);
+    default:
+      set_error("cell gather plans exist for column arities 3, 4 and 10 (got %d)", carity);
+      return OP2HIP_ERR_ARG;
+  }
 }
 
 int op2hip_gather_table_count(int32_t nbatches, int32_t batch_max_nnz, const void *batch_desc,

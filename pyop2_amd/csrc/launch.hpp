@@ -34,7 +34,8 @@ int for_each_launch(const op2hip_exec *ex, int32_t start, int32_t end, F &&launc
   }
   OP2HIP_REQUIRE(strategy == OP2HIP_ATOMIC || strategy == OP2HIP_STAGED ||
                      strategy == OP2HIP_ROWGATHER || strategy == OP2HIP_ROWGATHER_THREAD ||
-                     strategy == OP2HIP_ROWGATHER_SLOTS || strategy == OP2HIP_ROWGATHER_STAGED,
+                     strategy == OP2HIP_ROWGATHER_SLOTS || strategy == OP2HIP_ROWGATHER_STAGED ||
+                     strategy == OP2HIP_ROWGATHER_CELLS || strategy == OP2HIP_ROWGATHER_ROWLIST,
                  "unknown strategy %d", strategy);
   return launch(start, end, ex ? ex->subset : nullptr, true);
 }

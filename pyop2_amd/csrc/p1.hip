@@ -130,6 +130,14 @@ int op2hip_wrap_p1_laplace_matrix(int32_t start, int32_t end, const op2hip_mat *
     P1MatrixRow elem{X, c2v};
     return launch_rowgather_staged(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
   }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_ROWLIST) {
+    P1MatrixRow elem{X, c2v};
+    return launch_rowgather_rowlist(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_CELLS) {
+    P1MatrixCells elem{X, c2v};
+    return launch_rowgather_cells(start, end, elem, A, ex, static_cast<double *>(nullptr), s);
+  }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_THREAD) {
     P1MatrixRow elem{X, c2v};
     return launch_rowgather(start, end, elem, A, ex, static_cast<double *>(nullptr), s);

@@ -121,6 +121,35 @@ struct P1MatrixRow {
   }
 };
 
+// Cell-gather element: per element (b_0..2, c_0..2, 1 / (2 |det|)), block
+// (i, j) = (b_i b_j + c_i c_j) / (2 |det|).
+struct P1MatrixCells {
+  using T = double;
+  static constexpr int kRar = 3, kCar = 3, kRd = 1, kCd = 1;
+  static constexpr bool kHasRowDat = false;
+  static constexpr int kGeo = 8;
+  const double *X;
+  const int32_t *c2v;
+  __device__ __forceinline__ void geometry(int32_t e, double *geo) const {
+    const P1Geom g = p1_geom(X, c2v[3 * e], c2v[3 * e + 1], c2v[3 * e + 2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      geo[k] = g.b[k];
+      geo[3 + k] = g.c[k];
+    }
+    geo[6] = g.s;
+    geo[7] = 0.0;
+  }
+  __device__ static void no_geometry(double *geo) {
+#pragma unroll
+    for (int k = 0; k < kGeo; ++k) geo[k] = 0.0;
+  }
+  __device__ __forceinline__ void block(const double *geo, int i, int j, double *blk) const {
+    blk[0] = (geo[i] * geo[j] + geo[3 + i] * geo[3 + j]) * geo[6];
+  }
+  __device__ __forceinline__ void rowdat(int32_t, const double *, double *) const {}
+};
+
 // Lane-per-block row-gather element: lane j of a group computes K_e[i][j].
 struct P1MatrixLanes {
   using T = double;

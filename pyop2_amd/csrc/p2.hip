@@ -428,6 +428,77 @@ struct MatrixRow {
   }
 };
 
+// Cell-gather element: per element the barycentric gradients gl[m] (padded
+// to 4), wdet * lam and wdet * mu; block (i, j) evaluated from them with the
+// closed form of row_coefficients / closed_block for runtime i and j.
+template <typename TT, bool kFused>
+struct MatrixCells {
+  using T = TT;
+  static constexpr int kRar = 10, kCar = 10, kRd = 3, kCd = 3;
+  static constexpr bool kHasRowDat = kFused;
+  static constexpr int kGeo = 18;
+  const T *X;
+  const T *u;
+  const int32_t *c2v;
+  T lam, mu;
+  __device__ __forceinline__ void geometry(int32_t e, double *geo) const {
+    const Geom<T> g = p2::geometry(X, c2v, e);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) geo[4 * m + k] = double(g.gl[m][k]);
+      geo[4 * m + 3] = 0.0;
+    }
+    geo[16] = double(g.wdet) * double(lam);
+    geo[17] = double(g.wdet) * double(mu);
+  }
+  __device__ static void no_geometry(double *geo) {
+#pragma unroll
+    for (int k = 0; k < kGeo; ++k) geo[k] = 0.0;
+  }
+  // vertices (a, b) of P2 basis i (a = b = i for a vertex)
+  __device__ static __forceinline__ void ends(int i, int &a, int &b) {
+    const int ei = i - 4;
+    a = i < 4 ? i : (ei < 3 ? 0 : (ei < 5 ? 1 : 2));
+    b = i < 4 ? i : (ei < 3 ? ei + 1 : (ei < 5 ? ei - 1 : 3));
+  }
+  __device__ __forceinline__ void block(const double *geo, int i, int j, T *B) const {
+    int a, b, c, d;
+    ends(i, a, b);
+    ends(j, c, d);
+    const bool ie = i >= 4, jv = j < 4;
+    const double c0 = ie ? 3.2 : -0.8, c1 = ie ? 3.2 : 1.6, cs = ie ? 4.0 : 0.0;
+    double P[3], Q[3], gc[3], gd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double ga = geo[4 * a + k], gb = geo[4 * b + k];
+      const double w = ie ? ga + gb : ga;
+      gc[k] = geo[4 * c + k];
+      gd[k] = geo[4 * d + k];
+      const double g4c = c0 * w + c1 * ((c == b ? ga : 0.0) + (c == a ? gb : 0.0));
+      const double g4d = c0 * w + c1 * ((d == b ? ga : 0.0) + (d == a ? gb : 0.0));
+      P[k] = g4d - (jv ? cs * w : 0.0);
+      Q[k] = jv ? 0.0 : g4c;
+    }
+    double M[3][3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = 0; y < 3; ++y) M[x][y] = P[x] * gc[y] + Q[x] * gd[y];
+    const double lw = geo[16], mw = geo[17];
+    const double tr = mw * (M[0][0] + M[1][1] + M[2][2]);
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = 0; y < 3; ++y) B[3 * x + y] = T(lw * M[x][y] + mw * M[y][x] + (x == y ? tr : 0.0));
+  }
+  __device__ __forceinline__ void rowdat(int32_t col, const T *B, double *rs) const {
+    const double u0 = u[3 * (size_t)col], u1 = u[3 * (size_t)col + 1], u2 = u[3 * (size_t)col + 2];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) rs[x] += double(B[3 * x]) * u0 + double(B[3 * x + 1]) * u1 + double(B[3 * x + 2]) * u2;
+  }
+};
+
 // Lane-per-block row-gather element: lane j of a group computes block (i, j).
 template <typename TT, bool kFused>
 struct MatrixLanes {
@@ -668,6 +739,14 @@ int matrix(int32_t start, int32_t end, const op2hip_mat *A, T *R, const T *X, co
   if (ex && ex->strategy == OP2HIP_ROWGATHER_STAGED) {
     MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
     return launch_rowgather_staged(start, end, elem, A, ex, R, s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_ROWLIST) {
+    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    return launch_rowgather_rowlist(start, end, elem, A, ex, R, s);
+  }
+  if (ex && ex->strategy == OP2HIP_ROWGATHER_CELLS) {
+    MatrixCells<T, kFused> elem{X, u, c2v, T(lam), T(mu)};
+    return launch_rowgather_cells(start, end, elem, A, ex, R, s);
   }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_THREAD) {
     MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};

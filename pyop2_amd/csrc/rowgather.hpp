@@ -812,4 +812,385 @@ int launch_rowgather_staged(int32_t start, int32_t end, const Elem &elem, const 
   return OP2HIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Cell-gather row gather: no atomics, one element geometry per batch.  The
+// batch's distinct elements (op2hip_plan_cell_gather) have their geometry
+// computed once into LDS (Elem::kGeo doubles each); every output block of the
+// batch belongs to one thread, which evaluates each element block (e, i, j)
+// that lands on it from the staged geometry and adds them up in element
+// order; the block is written once.  Blocks are handed to threads by
+// contribution count (descending), so the few heavy blocks (vertex
+// diagonals) share waves instead of stalling many.
+//
+// Elem must additionally provide
+//   static constexpr int kGeo;                          // even
+//   void geometry(int32_t e, double *geo) const;        // kGeo doubles
+//   static void no_geometry(double *geo);               // element adds zero blocks
+//   void block(const double *geo, int i, int j, T *blk) const;  // runtime i, j
+//   void rowdat(int32_t col, const T *blk, Acc *rs) const;      // fused rows only
+
+template <class Elem>
+struct CellsLayout {
+  // region A holds the staged geometry, then (after the blocks are summed) the
+  // batch's new block values for the coalesced flush
+  __host__ __device__ static size_t region_a(int32_t max_cells, int32_t max_nnz) {
+    const size_t g = (size_t)max_cells * Elem::kGeo * sizeof(double);
+    const size_t r = (size_t)max_nnz * Elem::kRd * Elem::kCd * sizeof(double);
+    return ((g > r ? g : r) + 15) & ~(size_t)15;
+  }
+  static size_t bytes(int32_t max_cells, int32_t max_nnz, int32_t max_contrib, int32_t max_rows,
+                      int kOut) {
+    size_t b = region_a(max_cells, max_nnz);
+    b += (((size_t)max_contrib * sizeof(uint16_t)) + 15) & ~(size_t)15;
+    b += (size_t)kOut * 4 * sizeof(int32_t);
+    if (Elem::kHasRowDat)
+      b += (size_t)(max_rows + 2) * sizeof(int32_t) + (size_t)max_rows * Elem::kRd * sizeof(double);
+    return b;
+  }
+};
+
+template <class Elem, int kOut, int kPre>
+__global__ void __launch_bounds__(256)
+rowgather_cells_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
+                       const BatchDesc *__restrict__ descs, int32_t nbatches,
+                       const int32_t *__restrict__ cell_ptr, const int32_t *__restrict__ cells,
+                       const uint8_t *__restrict__ gcount, const uint16_t *__restrict__ order,
+                       const int32_t *__restrict__ contrib_ptr,
+                       const uint16_t *__restrict__ contrib, int32_t max_cells, int32_t max_nnz,
+                       int32_t max_contrib, typename Elem::T *__restrict__ rowdat) {
+  using T = typename Elem::T;
+  using Acc = typename AccumOf<T>::type;
+  constexpr int kBs = Elem::kRd * Elem::kCd;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int32_t b = xcd_contiguous(blockIdx.x, nbatches);
+  const BatchDesc d = descs[b];
+  const int32_t c0 = cell_ptr[b], nc = cell_ptr[b + 1] - c0;
+  const int32_t g0 = contrib_ptr[b], ng = contrib_ptr[b + 1] - g0;
+  T *__restrict__ vals = static_cast<T *>(A.values) + (size_t)d.z0 * kBs;
+  const int32_t nval = d.nz * kBs;
+
+  double *geo = reinterpret_cast<double *>(smem);
+  double *res = geo;  // region A, reused after the sums
+  size_t off = CellsLayout<Elem>::region_a(max_cells, max_nnz);
+  uint16_t *clist = reinterpret_cast<uint16_t *>(smem + off);
+  off += (((size_t)max_contrib * sizeof(uint16_t)) + 15) & ~(size_t)15;
+  int32_t *wtot = reinterpret_cast<int32_t *>(smem + off);
+  off += (size_t)kOut * 4 * sizeof(int32_t);
+  int32_t *srow = reinterpret_cast<int32_t *>(smem + off);
+  double *racc = reinterpret_cast<double *>(
+      smem + off + (((size_t)(d.nr + 1) * sizeof(int32_t) + 7) & ~(size_t)7));
+
+  // old values of the batch, coalesced, in flight during the element work
+  T old[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; ++u) {
+    const int32_t k = threadIdx.x + u * (int32_t)blockDim.x;
+    old[u] = k < nval ? vals[k] : T(0);
+  }
+  int32_t z[kOut], cnt[kOut];
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    const int32_t w = threadIdx.x + o * (int32_t)blockDim.x;
+    z[o] = w < d.nz ? int32_t(order[(size_t)d.z0 + w]) : 0;
+    cnt[o] = w < d.nz ? int32_t(gcount[(size_t)d.z0 + z[o]]) : 0;
+  }
+  for (int32_t k = threadIdx.x; k < ng; k += blockDim.x) clist[k] = contrib[(size_t)g0 + k];
+  if (Elem::kHasRowDat) {
+    for (int32_t k = threadIdx.x; k <= d.nr; k += blockDim.x) srow[k] = A.rowptr[d.r0 + k] - d.z0;
+    for (int32_t k = threadIdx.x; k < d.nr * Elem::kRd; k += blockDim.x) racc[k] = 0.0;
+  }
+  for (int32_t u = threadIdx.x; u < nc; u += blockDim.x) {
+    const int32_t e = cells[(size_t)c0 + u];
+    if (e >= start && e < end) elem.geometry(e, geo + (size_t)u * Elem::kGeo);
+    else Elem::no_geometry(geo + (size_t)u * Elem::kGeo);
+  }
+
+  // exclusive scan of the counts in thread order
+  int32_t incl[kOut];
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    int32_t x = cnt[o];
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+      const int32_t y = __shfl_up(x, dd, 64);
+      if (lane >= dd) x += y;
+    }
+    incl[o] = x;
+    if (lane == 63) wtot[o * 4 + wave] = x;
+  }
+  __syncthreads();
+
+  Acc acc[kOut][kBs];
+  int32_t base = 0;
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    int32_t before = 0, all = 0;
+    for (int w = 0; w < nwaves; ++w) {
+      const int32_t v = wtot[o * 4 + w];
+      before += w < wave ? v : 0;
+      all += v;
+    }
+    const int32_t first = base + before + incl[o] - cnt[o];
+    base += all;
+#pragma unroll
+    for (int k = 0; k < kBs; ++k) acc[o][k] = Acc(0);
+    for (int32_t c = 0; c < cnt[o]; ++c) {
+      const uint32_t code = clist[first + c];
+      T blk[kBs];
+      elem.block(geo + (size_t)(code >> 8) * Elem::kGeo, int((code >> 4) & 15u), int(code & 15u),
+                 blk);
+#pragma unroll
+      for (int k = 0; k < kBs; ++k) acc[o][k] += Acc(blk[k]);
+    }
+  }
+  __syncthreads();  // geometry no longer needed: region A takes the sums
+
+#pragma unroll
+  for (int o = 0; o < kOut; ++o) {
+    const int32_t w = threadIdx.x + o * (int32_t)blockDim.x;
+    if (w >= d.nz) continue;
+#pragma unroll
+    for (int k = 0; k < kBs; ++k) res[(size_t)z[o] * kBs + k] = acc[o][k];
+    if (Elem::kHasRowDat) {
+      // the row's residual gets this block's new contribution times u[col]
+      int32_t lo = 0, hi = d.nr;
+      while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (srow[mid] <= z[o]) lo = mid; else hi = mid;
+      }
+      T add[kBs];
+#pragma unroll
+      for (int k = 0; k < kBs; ++k) add[k] = T(acc[o][k]);
+      Acc rs[Elem::kRd];
+#pragma unroll
+      for (int a = 0; a < Elem::kRd; ++a) rs[a] = Acc(0);
+      elem.rowdat(A.colidx[(size_t)d.z0 + z[o]], add, rs);
+#pragma unroll
+      for (int a = 0; a < Elem::kRd; ++a) atomic_add(&racc[lo * Elem::kRd + a], rs[a]);
+    }
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int u = 0; u < kPre; ++u) {
+    const int32_t k = threadIdx.x + u * (int32_t)blockDim.x;
+    if (k < nval) vals[k] = old[u] + T(res[k]);
+  }
+  for (int32_t k0 = threadIdx.x + kPre * (int32_t)blockDim.x; k0 < nval; k0 += 8 * (int32_t)blockDim.x) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t k = k0 + u * (int32_t)blockDim.x;
+      v[u] = k < nval ? vals[k] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t k = k0 + u * (int32_t)blockDim.x;
+      if (k < nval) vals[k] = v[u] + T(res[k]);
+    }
+  }
+  if (Elem::kHasRowDat) {
+    T *rd = rowdat + (size_t)d.r0 * Elem::kRd;
+    for (int32_t k = threadIdx.x; k < d.nr * Elem::kRd; k += blockDim.x) rd[k] += T(racc[k]);
+  }
+}
+
+template <class Elem>
+int launch_rowgather_cells(int32_t start, int32_t end, const Elem &elem, const op2hip_mat *A,
+                           const op2hip_exec *ex, typename Elem::T *rowdat, hipStream_t s) {
+  OP2HIP_REQUIRE(ex && ex->batch_desc && ex->cell_ptr && ex->cells && ex->gather_count &&
+                     ex->block_order && ex->contrib_ptr && ex->contrib && ex->nbatches >= 0,
+                 "cell-gather row gather without its plan");
+  OP2HIP_REQUIRE(!ex->subset, "row-gather assembly does not iterate Subsets");
+  OP2HIP_REQUIRE(A->rdim == Elem::kRd && A->cdim == Elem::kCd,
+                 "matrix block %dx%d, kernel assembles %dx%d", A->rdim, A->cdim,
+                 Elem::kRd, Elem::kCd);
+  if (end <= start || ex->nbatches == 0) return OP2HIP_OK;
+  const int bs = block_size_of(ex, 256);
+  OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % 64 == 0, "row-gather block size %d", bs);
+  OP2HIP_REQUIRE(ex->max_batch_cells > 0 && ex->max_batch_cells <= 256 && ex->max_batch_contrib >= 0,
+                 "cell-gather plan: bad maxima");
+  const int32_t nout = (ex->batch_max_nnz + bs - 1) / bs;
+  OP2HIP_REQUIRE(nout <= 4, "cell-gather row gather: %d blocks in a batch, at most %d",
+                 ex->batch_max_nnz, 4 * bs);
+  const int kout = nout <= 1 ? 1 : (nout <= 2 ? 2 : 4);
+  const size_t lds = CellsLayout<Elem>::bytes(ex->max_batch_cells, ex->batch_max_nnz,
+                                              ex->max_batch_contrib, ex->batch_max_rows, kout);
+  OP2HIP_REQUIRE(lds <= 64 * 1024, "row batch needs %zu bytes of LDS (max 65536)", lds);
+  const auto *descs = static_cast<const BatchDesc *>(ex->batch_desc);
+  // prefetch depth: the batch's values per thread, at most 24
+  const int64_t per_thread = ((int64_t)ex->batch_max_nnz * Elem::kRd * Elem::kCd + bs - 1) / bs;
+#define OP2HIP_CELLS(K, P)                                                                     \
+  rowgather_cells_kernel<Elem, K, P><<<ex->nbatches, bs, lds, s>>>(                            \
+      start, end, elem, *A, descs, ex->nbatches, ex->cell_ptr, ex->cells, ex->gather_count,    \
+      ex->block_order, ex->contrib_ptr, ex->contrib, ex->max_batch_cells, ex->batch_max_nnz,   \
+      ex->max_batch_contrib, rowdat)
+  if (per_thread <= 8) {
+    if (kout == 1) OP2HIP_CELLS(1, 8);
+    else if (kout == 2) OP2HIP_CELLS(2, 8);
+    else OP2HIP_CELLS(4, 8);
+  } else if (per_thread <= 16) {
+    if (kout == 1) OP2HIP_CELLS(1, 16);
+    else if (kout == 2) OP2HIP_CELLS(2, 16);
+    else OP2HIP_CELLS(4, 16);
+  } else {
+    if (kout == 1) OP2HIP_CELLS(1, 24);
+    else if (kout == 2) OP2HIP_CELLS(2, 24);
+    else OP2HIP_CELLS(4, 24);
+  }
+#undef OP2HIP_CELLS
+  OP2HIP_LAUNCHED();
+  return OP2HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Row-list row gather: the slot-table row gather over batches of rows taken
+// in an element-local order (op2hip_plan_row_list) rather than consecutive
+// row ids.  With a scattered row numbering (a permuted mesh) a batch of
+// consecutive rows touches elements all over the mesh and every coordinate
+// gather misses the cache; rows ordered by their first element keep a
+// batch's elements, and so its gathers, local.  The price is that the
+// batch's rows are scattered in the Mat: the old values are gathered and the
+// new ones written row segment by row segment.
+
+template <class Elem>
+struct RowListLayout {
+  static size_t bytes(int32_t max_nnz, int32_t max_rows) {
+    size_t b = (size_t)max_nnz * Elem::kRd * Elem::kCd * sizeof(double);
+    b += (Elem::kHasRowDat ? (size_t)max_rows * Elem::kRd : 0) * sizeof(double);
+    b += (size_t)(max_rows + 1) * sizeof(int32_t) * 3;
+    return b;
+  }
+};
+
+// Position in the Mat's values of batch value k (binary search of the row
+// segments staged in LDS: seg[lr] first block of row lr in the batch,
+// gbase[lr] its first block in the Mat; seg[nr] = nz).
+template <int kBs>
+__device__ __forceinline__ size_t rowlist_index(int32_t k, const int32_t *seg,
+                                                const int32_t *gbase, int32_t nr) {
+  const int32_t q = k / kBs, comp = k - q * kBs;
+  int32_t lo = 0, hi = nr;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (seg[mid] <= q) lo = mid; else hi = mid;
+  }
+  return (size_t)(gbase[lo] + q - seg[lo]) * kBs + comp;
+}
+
+template <class Elem, int kPre>
+__global__ void __launch_bounds__(256)
+rowgather_rowlist_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
+                         const BatchDesc *__restrict__ descs, int32_t nbatches,
+                         const int2 *__restrict__ row_order, const int32_t *__restrict__ bent,
+                         const SlotEntry<Elem::kCar> *__restrict__ table,
+                         typename Elem::T *__restrict__ rowdat) {
+  using T = typename Elem::T;
+  using Acc = typename AccumOf<T>::type;
+  constexpr int kBs = Elem::kRd * Elem::kCd;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BatchDesc d = descs[xcd_contiguous(blockIdx.x, nbatches)];
+  T *__restrict__ vals = static_cast<T *>(A.values);
+  const int32_t nval = d.nz * kBs;
+  Acc *acc = reinterpret_cast<Acc *>(smem);
+  Acc *racc = acc + nval;
+  int32_t *seg = reinterpret_cast<int32_t *>(racc + (Elem::kHasRowDat ? d.nr * Elem::kRd : 0));
+  int32_t *gbase = seg + d.nr + 1;
+  int32_t *rows = gbase + d.nr + 1;
+
+  for (int32_t lr = threadIdx.x; lr < d.nr; lr += blockDim.x) {
+    const int2 p = row_order[d.r0 + lr];
+    seg[lr] = p.y;
+    gbase[lr] = A.rowptr[p.x];
+    rows[lr] = p.x;
+  }
+  if (threadIdx.x == 0) seg[d.nr] = d.nz;
+  for (int32_t k = threadIdx.x; k < nval; k += blockDim.x) acc[k] = Acc(0);
+  if (Elem::kHasRowDat)
+    for (int32_t k = threadIdx.x; k < d.nr * Elem::kRd; k += blockDim.x) racc[k] = Acc(0);
+  __syncthreads();
+
+  T old[kPre];
+  size_t gidx[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; ++u) {
+    const int32_t k = threadIdx.x + u * (int32_t)blockDim.x;
+    gidx[u] = k < nval ? rowlist_index<kBs>(k, seg, gbase, d.nr) : 0;
+    old[u] = k < nval ? vals[gidx[u]] : T(0);
+  }
+
+  for (int32_t t = threadIdx.x; t < d.ni; t += blockDim.x) {
+    const int32_t ent = bent[d.i0 + t];
+    const SlotEntry<Elem::kCar> se = table[d.i0 + t];
+    const int32_t e = ent / Elem::kRar;
+    if (e < start || e >= end) continue;
+    const int i = ent - e * Elem::kRar;
+    Acc *row = acc + (size_t)se.row_off * kBs;
+    T rsum[Elem::kRd];
+#pragma unroll
+    for (int a = 0; a < Elem::kRd; ++a) rsum[a] = T(0);
+    elem.row(e, i, [&](int j, int32_t col, const T *blk) {
+      const int s = se.slot[j];
+      if (col < 0) return;
+      if (s == 255) {
+        atomicAdd(A.status, 1);
+        return;
+      }
+#pragma unroll
+      for (int k = 0; k < kBs; ++k) atomic_add(&row[s * kBs + k], Acc(blk[k]));
+    }, rsum);
+    if (Elem::kHasRowDat) {
+#pragma unroll
+      for (int a = 0; a < Elem::kRd; ++a) atomic_add(&racc[se.lr * Elem::kRd + a], Acc(rsum[a]));
+    }
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int u = 0; u < kPre; ++u) {
+    const int32_t k = threadIdx.x + u * (int32_t)blockDim.x;
+    if (k < nval) vals[gidx[u]] = old[u] + T(acc[k]);
+  }
+  for (int32_t k = threadIdx.x + kPre * (int32_t)blockDim.x; k < nval; k += blockDim.x) {
+    const size_t g = rowlist_index<kBs>(k, seg, gbase, d.nr);
+    vals[g] += T(acc[k]);
+  }
+  if (Elem::kHasRowDat) {
+    for (int32_t k = threadIdx.x; k < d.nr * Elem::kRd; k += blockDim.x) {
+      const int32_t lr = k / Elem::kRd, a = k - lr * Elem::kRd;
+      rowdat[(size_t)rows[lr] * Elem::kRd + a] += T(racc[k]);
+    }
+  }
+}
+
+template <class Elem>
+int launch_rowgather_rowlist(int32_t start, int32_t end, const Elem &elem, const op2hip_mat *A,
+                             const op2hip_exec *ex, typename Elem::T *rowdat, hipStream_t s) {
+  OP2HIP_REQUIRE(ex && ex->batch_desc && ex->slot_table && ex->row_order && ex->batch_ent &&
+                     ex->nbatches >= 0,
+                 "row-list row gather without its plan");
+  OP2HIP_REQUIRE(!ex->subset, "row-gather assembly does not iterate Subsets");
+  OP2HIP_REQUIRE(A->rdim == Elem::kRd && A->cdim == Elem::kCd,
+                 "matrix block %dx%d, kernel assembles %dx%d", A->rdim, A->cdim,
+                 Elem::kRd, Elem::kCd);
+  if (end <= start || ex->nbatches == 0) return OP2HIP_OK;
+  const size_t lds = RowListLayout<Elem>::bytes(ex->batch_max_nnz, ex->batch_max_rows);
+  OP2HIP_REQUIRE(lds <= 64 * 1024, "row batch needs %zu bytes of LDS (max 65536)", lds);
+  const int bs = block_size_of(ex, 256);
+  OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % 64 == 0, "row-gather block size %d", bs);
+  const auto *descs = static_cast<const BatchDesc *>(ex->batch_desc);
+  const auto *ro = reinterpret_cast<const int2 *>(ex->row_order);
+  const auto *table = static_cast<const SlotEntry<Elem::kCar> *>(ex->slot_table);
+  const int64_t per_thread = ((int64_t)ex->batch_max_nnz * Elem::kRd * Elem::kCd + bs - 1) / bs;
+  if (per_thread <= 8)
+    rowgather_rowlist_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(
+        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, rowdat);
+  else
+    rowgather_rowlist_kernel<Elem, 16><<<ex->nbatches, bs, lds, s>>>(
+        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, rowdat);
+  OP2HIP_LAUNCHED();
+  return OP2HIP_OK;
+}
+
 }  // namespace op2hip

@@ -19,10 +19,12 @@ __all__ = ["lib", "torch", "device", "stream", "check", "ExecPlan", "MatDesc",
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libop2hip.so")
 
 OK, ERR_HIP, ERR_ARG, ERR_WORKSPACE, ERR_OVERFLOW, ERR_JIT, ERR_PATTERN = range(7)
-ATOMIC, COLOURED, STAGED, ROWGATHER, ROWGATHER_THREAD, ROWGATHER_SLOTS, ROWGATHER_STAGED = range(7)
+(ATOMIC, COLOURED, STAGED, ROWGATHER, ROWGATHER_THREAD, ROWGATHER_SLOTS, ROWGATHER_STAGED,
+ ROWGATHER_CELLS, ROWGATHER_ROWLIST) = range(9)
 STRATEGIES = {"atomic": ATOMIC, "coloured": COLOURED, "staged": STAGED, "rowgather": ROWGATHER,
               "rowgather_thread": ROWGATHER_THREAD, "rowgather_slots": ROWGATHER_SLOTS,
-              "rowgather_staged": ROWGATHER_STAGED}
+              "rowgather_staged": ROWGATHER_STAGED, "rowgather_cells": ROWGATHER_CELLS,
+              "rowgather_rows": ROWGATHER_ROWLIST}
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
@@ -52,6 +54,15 @@ class ExecPlan(ctypes.Structure):
         ("slot_table", ctypes.c_void_p),
         ("gather_count", ctypes.c_void_p),
         ("gather_list", ctypes.c_void_p),
+        ("cell_ptr", ctypes.c_void_p),
+        ("cells", ctypes.c_void_p),
+        ("block_order", ctypes.c_void_p),
+        ("contrib_ptr", ctypes.c_void_p),
+        ("contrib", ctypes.c_void_p),
+        ("max_batch_cells", ctypes.c_int32),
+        ("max_batch_contrib", ctypes.c_int32),
+        ("row_order", ctypes.c_void_p),
+        ("batch_ent", ctypes.c_void_p),
         ("subset", ctypes.c_void_p),
     ]
 
@@ -107,6 +118,9 @@ _SIGNATURES = {
     "op2hip_plan_row_batches": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _i32p, _i32p, _i32p, _i32p]),
     "op2hip_slot_table_bytes": (ctypes.c_int, [_I, ctypes.POINTER(ctypes.c_size_t)]),
     "op2hip_build_slot_table": (ctypes.c_int, [_I, _V, _V, _V, _V, _V, _V, _I, _I, _V, _V, _V, _V]),
+    "op2hip_plan_row_list": (ctypes.c_int, [_I, _V, _V, _V, _V, _V, _I, _I, _I, _I, _V, _i32p, _V, _V, _V,
+                                            ctypes.POINTER(ctypes.c_int64), _i32p, _i32p, _i32p]),
+    "op2hip_plan_cell_gather": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _V, _V, _V, _V, _V, _V]),
     "op2hip_gather_table_count": (ctypes.c_int, [_I, _I, _V, _V, _I, _V, _V, _V, _V]),
     "op2hip_gather_table_fill": (ctypes.c_int, [_I, _I, _V, _V, _I, _V, _V, _V, _V, _V]),
     "op2hip_wrap_p1_lumped_mass": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V]),

@@ -114,6 +114,7 @@ class RowBatches:
     max_rows: int
     slots: dict = None   # (cmap id) -> (table, descriptors)
     max_row_nnz: int = None
+    cells_plans: dict = None
 
     def exec_fields(self, ex):
         ex.batch_rows = dev.ptr(self.batch_rows)
@@ -187,6 +188,54 @@ class RowBatches:
         key = id(cmap)
         self.slots[key] = hit = hit + (gt,)
         return gt
+
+    def cell_gather(self, hit, tr, rarity, nblocks):
+        """Cell-gather plan (op2hip_plan_cell_gather) of slot table ``hit``,
+        built once on the host from copies of the device plan."""
+        if self.cells_plans is None:
+            self.cells_plans = {}
+        cmap = hit[2]
+        cg = self.cells_plans.get(id(cmap))
+        if cg is not None:
+            return cg
+        L = dev.lib()
+        nb = self.nbatches
+        desc = np.ascontiguousarray(dev.to_host(hit[1]))
+        table = np.ascontiguousarray(dev.to_host(hit[0]))
+        inc_ent = np.ascontiguousarray(dev.to_host(tr.inc_ent), dtype=np.int32)
+        ninc = int(tr.total)
+        cell_ptr = np.empty(nb + 1, dtype=np.int32)
+        cells = np.empty(max(ninc, 1), dtype=np.int32)
+        gcount = np.zeros(max(nblocks, 1), dtype=np.uint8)
+        order = np.zeros(max(nblocks, 1), dtype=np.uint16)
+        contrib_ptr = np.empty(nb + 1, dtype=np.int32)
+        contrib = np.empty(max(ninc * cmap.arity, 1), dtype=np.uint16)
+        rc = L.op2hip_plan_cell_gather(nb, desc.ctypes.data, table.ctypes.data, rarity, cmap.arity,
+                                       inc_ent.ctypes.data, cell_ptr.ctypes.data, cells.ctypes.data,
+                                       gcount.ctypes.data, order.ctypes.data, contrib_ptr.ctypes.data,
+                                       contrib.ctypes.data)
+        if rc:
+            cg = None
+        else:
+            nc, ng = int(cell_ptr[nb]), int(contrib_ptr[nb])
+            cg = dict(cell_ptr=dev.to_device(cell_ptr), cells=dev.to_device(cells[:max(nc, 1)]),
+                      gcount=dev.to_device(gcount), order=dev.to_device(order),
+                      contrib_ptr=dev.to_device(contrib_ptr), contrib=dev.to_device(contrib[:max(ng, 1)]),
+                      max_cells=int(np.diff(cell_ptr).max(initial=0)),
+                      max_contrib=int(np.diff(contrib_ptr).max(initial=0)))
+        self.cells_plans[id(cmap)] = cg
+        return cg
+
+    @staticmethod
+    def cell_fields(ex, cg):
+        ex.cell_ptr = dev.ptr(cg["cell_ptr"])
+        ex.cells = dev.ptr(cg["cells"])
+        ex.gather_count = dev.ptr(cg["gcount"])
+        ex.block_order = dev.ptr(cg["order"])
+        ex.contrib_ptr = dev.ptr(cg["contrib_ptr"])
+        ex.contrib = dev.ptr(cg["contrib"])
+        ex.max_batch_cells = cg["max_cells"]
+        ex.max_batch_contrib = cg["max_contrib"]
 
     @staticmethod
     def gather_fields(ex, gt):
@@ -277,4 +326,64 @@ def tile_plan(maps, start, end, tile_size, node_cap=0):
                     [dev.to_device(x[:max(nt, 1)]) for x in nxs],
                     [dev.to_device(x) for x in lcs], [int(v) for v in mx[:nm]])
     owner._cache[ck] = plan
+    return plan
+
+
+@dataclass
+class RowList:
+    """Batches of rows in element-local order (op2hip_plan_row_list)."""
+    nbatches: int
+    desc: object        # device, nbatches x 8 int32
+    row_order: object   # device, nrows x 2 int32
+    ent: object         # device, incidences in batch order
+    table: object       # device, slot entries in batch order
+    max_inc: int
+    max_nnz: int
+    max_rows: int
+
+    def exec_fields(self, ex):
+        ex.nbatches = self.nbatches
+        ex.batch_desc = dev.ptr(self.desc)
+        ex.row_order = dev.ptr(self.row_order)
+        ex.batch_ent = dev.ptr(self.ent)
+        ex.slot_table = dev.ptr(self.table)
+        ex.batch_max_inc = self.max_inc
+        ex.batch_max_nnz = self.max_nnz
+        ex.batch_max_rows = self.max_rows
+
+
+def row_list(rmap, cmap, tr, mat, max_inc, max_nnz, key):
+    """Row-list plan of ``mat``'s pattern and the maps, cached on ``rmap``."""
+    ck = ("rowlist", key, id(cmap), int(max_inc), int(max_nnz))
+    plan = rmap._cache.get(ck)
+    if plan is not None:
+        return plan
+    L = dev.lib()
+    nrows = int(mat.sparsity.nrows)
+    rowptr = np.ascontiguousarray(dev.to_host(mat.rowptr), dtype=np.int32)
+    colidx = np.ascontiguousarray(dev.to_host(mat.colidx), dtype=np.int32)
+    inc_ptr = np.ascontiguousarray(dev.to_host(tr.inc_ptr), dtype=np.int32)
+    inc_ent = np.ascontiguousarray(dev.to_host(tr.inc_ent), dtype=np.int32)
+    cvals = np.ascontiguousarray(cmap.values_with_halo, dtype=np.int32)
+    eb = ctypes.c_size_t()
+    dev.check(L.op2hip_slot_table_bytes(cmap.arity, ctypes.byref(eb)), "slot_table_bytes")
+    ninc = int(tr.total)
+    order = np.empty(max(2 * nrows, 2), dtype=np.int32)
+    desc = np.zeros((max(nrows, 1), 8), dtype=np.int32)
+    ent = np.empty(max(ninc, 1), dtype=np.int32)
+    table = np.zeros(max(ninc, 1) * eb.value, dtype=np.uint8)
+    nb, mi, mn, mr = (ctypes.c_int32() for _ in range(4))
+    missing = ctypes.c_int64()
+    dev.check(L.op2hip_plan_row_list(nrows, rowptr.ctypes.data, colidx.ctypes.data, inc_ptr.ctypes.data,
+                                     inc_ent.ctypes.data, cvals.ctypes.data, rmap.arity, cmap.arity,
+                                     int(max_inc), int(max_nnz), order.ctypes.data, ctypes.byref(nb),
+                                     desc.ctypes.data, ent.ctypes.data, table.ctypes.data,
+                                     ctypes.byref(missing), ctypes.byref(mi), ctypes.byref(mn),
+                                     ctypes.byref(mr)), "plan_row_list")
+    if missing.value:
+        from pyop2_amd.exceptions import PatternError
+        raise PatternError(f"{missing.value} map entries are not in the sparsity pattern")
+    plan = RowList(nb.value, dev.to_device(desc[:max(nb.value, 1)]), dev.to_device(order),
+                   dev.to_device(ent), dev.to_device(table), mi.value, mn.value, mr.value)
+    rmap._cache[ck] = plan
     return plan

@@ -94,6 +94,14 @@ _BATCH = {("p1_laplace_matrix", "f64"): (1024, 2048),
           ("p2_elasticity_fused", "f32"): (512, 640)}
 # The staged variant holds one batch's element block rows in LDS: at most
 # 256 / (column groups of the element) incidences per batch.
+# The cell-gather variant stages the geometry of at most 256 elements (one
+# per row incidence at most), then the batch's new block values, in the same
+# LDS (256 P2 elements or 512 3x3 blocks both take 36 KiB).
+_BATCH_CELLS = {("p1_laplace_matrix", "f64"): (256, 512),
+                ("p2_elasticity_matrix", "f64"): (256, 512),
+                ("p2_elasticity_matrix", "f32"): (256, 512),
+                ("p2_elasticity_fused", "f64"): (256, 512),
+                ("p2_elasticity_fused", "f32"): (256, 512)}
 _BATCH_STAGED = {("p1_laplace_matrix", "f64"): (256, 512),
                  ("p2_elasticity_matrix", "f64"): (64, 256),
                  ("p2_elasticity_matrix", "f32"): (64, 256),
@@ -245,13 +253,23 @@ class HandWrittenKernel:
         if self.spec["args"][0][0] == "mat":
             mat = args[0].data
             rmap = args[0].maps[0]
-            if strategy.startswith("rowgather"):
+            if strategy == "rowgather_rows":
+                from pyop2_amd import plans
+                tr = plans.transpose(rmap, mat.sparsity.nrows, rmap.iterset.size)
+                budget = _BATCH[(self.family, self.variant)]
+                rl = plans.row_list(rmap, args[0].maps[1], tr, mat, budget[0], budget[1],
+                                    (id(mat.sparsity), mat.block_shape, self.variant))
+                rl.exec_fields(ex)
+                keep.append(rl)
+                desc = mat.descriptor(tr.inc_ptr, tr.inc_ent)
+            elif strategy.startswith("rowgather"):
                 from pyop2_amd import plans
                 tr = plans.transpose(rmap, mat.sparsity.nrows, rmap.iterset.size)
                 key = (id(mat.sparsity), mat.block_shape, self.variant)
                 rb = None
-                if strategy == "rowgather_staged":
-                    budget = _BATCH_STAGED[(self.family, self.variant)]
+                if strategy in ("rowgather_staged", "rowgather_cells"):
+                    table = _BATCH_STAGED if strategy == "rowgather_staged" else _BATCH_CELLS
+                    budget = table[(self.family, self.variant)]
                     rb = plans.row_batches(rmap, tr, mat.rowptr, budget[0], budget[1], key)
                     if rb.max_inc > budget[0]:
                         # a row with more incidences than one batch can stage
@@ -261,10 +279,18 @@ class HandWrittenKernel:
                     rb = plans.row_batches(rmap, tr, mat.rowptr, budget[0], budget[1], key)
                 rb.exec_fields(ex)
                 keep.append(rb)
-                if strategy in ("rowgather_slots", "rowgather_staged"):
+                if strategy in ("rowgather_slots", "rowgather_staged", "rowgather_cells"):
                     hit = rb.slot_table(tr, mat.rowptr, mat.colidx, args[0].maps[1], rmap.arity)
                     if hit is None:
                         strategy = "rowgather_thread"
+                    elif strategy == "rowgather_cells":
+                        cg = rb.cell_gather(hit, tr, rmap.arity, mat.sparsity.block_nnz)
+                        if cg is None:
+                            strategy = "rowgather_slots"
+                        else:
+                            rb.cell_fields(ex, cg)
+                            keep.append(cg)
+                        rb.slot_fields(ex, hit)
                     elif strategy == "rowgather_staged" and hit[3]:
                         # elements repeating a column need one add per occurrence
                         strategy = "rowgather_thread"

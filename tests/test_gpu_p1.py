@@ -65,7 +65,7 @@ def test_p1_sparsity_bit_exact(device, n, permute):
     assert np.array_equal(sp.nnz, np.diff(rp))
 
 
-@pytest.mark.parametrize("strategy", ["rowgather_staged", "rowgather_slots", "rowgather_thread", "rowgather", "atomic", "coloured"])
+@pytest.mark.parametrize("strategy", ["rowgather_rows", "rowgather_cells", "rowgather_staged", "rowgather_slots", "rowgather_thread", "rowgather", "atomic", "coloured"])
 def test_p1_csr_assembly(device, strategy):
     m = unit_square_tri(128, rng=workload_rng(2), permute=True)
     configuration["strategy"] = strategy

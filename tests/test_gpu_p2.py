@@ -43,7 +43,7 @@ def test_p2_residual_f32(device, tet8, strategy):
     assert relerr(R.data_ro, ref) < 1e-5
 
 
-@pytest.mark.parametrize("strategy", ["rowgather_staged", "rowgather_slots", "rowgather_thread", "rowgather", "atomic", "coloured"])
+@pytest.mark.parametrize("strategy", ["rowgather_rows", "rowgather_cells", "rowgather_staged", "rowgather_slots", "rowgather_thread", "rowgather", "atomic", "coloured"])
 def test_p2_bsr_assembly_f64(device, tet8, strategy):
     configuration["strategy"] = strategy
     p = TetProblem(tet8)
@@ -56,7 +56,7 @@ def test_p2_bsr_assembly_f64(device, tet8, strategy):
     assert relerr(v, ref.values) < 1e-12
 
 
-@pytest.mark.parametrize("strategy", ["rowgather_staged", "rowgather_slots", "rowgather_thread", "atomic"])
+@pytest.mark.parametrize("strategy", ["rowgather_rows", "rowgather_cells", "rowgather_staged", "rowgather_slots", "rowgather_thread", "atomic"])
 def test_p2_fused_f64(device, tet8, strategy):
     configuration["strategy"] = strategy
     p = TetProblem(tet8)
@@ -71,7 +71,7 @@ def test_p2_fused_f64(device, tet8, strategy):
     assert relerr(R.data_ro, oracle_p2("p2_elasticity_residual", tet8)) < 1e-12
 
 
-@pytest.mark.parametrize("strategy", ["rowgather_staged", "rowgather_slots", "rowgather_thread", "atomic"])
+@pytest.mark.parametrize("strategy", ["rowgather_rows", "rowgather_cells", "rowgather_staged", "rowgather_slots", "rowgather_thread", "atomic"])
 def test_p2_bsr_assembly_f32(device, tet8, strategy):
     configuration["strategy"] = strategy
     p = TetProblem(tet8, dtype=np.float32)

@@ -147,6 +147,53 @@ def test_tiles_are_compact_regions():
     assert grown < 0.8 * strips
 
 
+def test_row_list_plan_covers_rows_and_finds_slots():
+    """Row-list plan on a permuted P1 mesh: every row once; batches within
+    budget; rows ordered by first element; each incidence's slots point at
+    the entries of its element's columns in its row."""
+    m = unit_square_tri(24, rng=workload_rng(2), permute=True)
+    cv = np.ascontiguousarray(m.cell_vertex, np.int32)
+    rp, ci = osp.pattern(m.nverts, m.nverts, [(cv, cv, m.ncells)])
+    rp, ci = rp.astype(np.int32), ci.astype(np.int32)
+    ent = np.arange(cv.size, dtype=np.int32)
+    rows = cv.ravel()
+    order = np.lexsort((ent, rows))
+    inc_ent = ent[order].astype(np.int32)
+    inc_ptr = np.zeros(m.nverts + 1, np.int32)
+    np.cumsum(np.bincount(rows, minlength=m.nverts), out=inc_ptr[1:])
+    L = dev.lib()
+    n = m.nverts
+    ro = np.empty(2 * n, np.int32)
+    desc = np.zeros((n, 8), np.int32)
+    bent = np.empty(cv.size, np.int32)
+    tab = np.zeros((cv.size, 8), np.uint8)
+    nb, mi, mn, mr = (ctypes.c_int32() for _ in range(4))
+    miss = ctypes.c_int64()
+    rc = L.op2hip_plan_row_list(n, rp.ctypes.data, ci.ctypes.data, inc_ptr.ctypes.data, inc_ent.ctypes.data,
+                                cv.ctypes.data, 3, 3, 40, 64, ro.ctypes.data, ctypes.byref(nb), desc.ctypes.data,
+                                bent.ctypes.data, tab.ctypes.data, ctypes.byref(miss), ctypes.byref(mi),
+                                ctypes.byref(mn), ctypes.byref(mr))
+    assert rc == 0 and miss.value == 0
+    order_rows = ro[0::2]
+    assert np.array_equal(np.sort(order_rows), np.arange(n))
+    first = np.array([inc_ent[inc_ptr[r]] // 3 for r in order_rows])
+    assert np.all(np.diff(first) >= 0)
+    for b in range(nb.value):
+        r0, nr, _, nz, i0, ni = desc[b, :6]
+        rows_b = order_rows[r0:r0 + nr]
+        lens = rp[rows_b + 1] - rp[rows_b]
+        assert nz == lens.sum() and np.array_equal(ro[1::2][r0:r0 + nr], np.concatenate([[0], np.cumsum(lens)[:-1]]))
+        if nr > 1:
+            assert nz <= 64 and ni <= 40
+        for t in range(i0, i0 + ni):
+            row_off, lr = tab[t, 0] | (tab[t, 1].astype(int) << 8), tab[t, 2] | (tab[t, 3].astype(int) << 8)
+            r = rows_b[lr]
+            e = bent[t] // 3
+            assert r == cv[e, bent[t] % 3]
+            for j in range(3):
+                assert ci[rp[r] + tab[t, 4 + j]] == cv[e, j]
+
+
 def test_ctypes_structs_match_the_c_layout(tmp_path):
     """Sizes and field offsets of op2hip_exec, op2hip_mat and
     op2hip_batch_mesh as the C compiler lays them out equal the ctypes
