@@ -55,8 +55,9 @@ def candidates(executor, parloop):
     if has_mat:
         if isinstance(parloop.iterset, Subset):
             return ["atomic", "coloured"]
-        return ["rowgather_slots", "rowgather_rows", "rowgather_cells", "rowgather_staged",
-                "rowgather_thread", "rowgather", "atomic"]
+        split = ["split"] if "split" in executor.spec else []
+        return split + ["rowgather_slots", "rowgather_rows", "rowgather_cells", "rowgather_staged",
+                        "rowgather_thread", "rowgather", "atomic"]
     if "staged" in executor.spec and not isinstance(parloop.iterset, Subset):
         return ["staged", "atomic", "coloured"]
     return ["atomic", "coloured"]

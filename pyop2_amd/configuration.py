@@ -68,7 +68,7 @@ class Configuration(dict):
         if key == "strategy" and value not in ("auto", "atomic", "coloured", "staged", "rowgather",
                                                "rowgather_thread", "rowgather_slots",
                                                "rowgather_staged", "rowgather_cells",
-                                               "rowgather_rows"):
+                                               "rowgather_rows", "split"):
             raise ConfigurationError(f"unknown strategy {value!r}")
         self._set.add(key)
         super().__setitem__(key, value)

@@ -383,6 +383,7 @@ struct MatrixRow {
   static constexpr bool kHasRowDat = kFused;
   // 24 prefetched doubles would take the fused kernel past 168 registers
   static constexpr int kMaxPrefetch = (kFused && sizeof(TT) == 8) ? 16 : 24;
+  static constexpr bool kSpread = true;
   const T *X;
   const T *u;
   const int32_t *c2n;

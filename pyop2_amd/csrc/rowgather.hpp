@@ -467,6 +467,28 @@ gather_fill_kernel(int32_t nbatches, BatchDesc *__restrict__ desc,
   }
 }
 
+// Incidence handled by thread slot t of a batch of ni incidences: a stride
+// coprime to ni spreads a wave's lanes over the batch's rows, so lanes rarely
+// add into the same block in one LDS instruction (consecutive incidences of a
+// row all hit that row's diagonal block).  Stride 37 or 41 (both prime), or
+// 1 when ni is a multiple of both.
+__device__ __forceinline__ int32_t spread_incidence(int32_t t, int32_t ni) {
+  const int32_t p = (ni % 37) ? 37 : ((ni % 41) ? 41 : 1);
+  return (int32_t)(((int64_t)t * p) % ni);
+}
+
+// Whether an element spreads its incidences over the lanes (Elem::kSpread):
+// worth it when the element work is large (P2), not when neighbouring lanes
+// gain more from sharing their gathers (P1).
+template <class Elem, class = void>
+struct SpreadOf {
+  static constexpr bool value = false;
+};
+template <class Elem>
+struct SpreadOf<Elem, std::void_t<decltype(Elem::kSpread)>> {
+  static constexpr bool value = Elem::kSpread;
+};
+
 // One batch of the slot-table row gather (the workgroup's share of a launch,
 // or one work item of a batched launch).
 template <class Elem, int kPre>
@@ -494,7 +516,8 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
     for (int32_t k = threadIdx.x; k < d.nr * Elem::kRd; k += blockDim.x) racc[k] = Acc(0);
   __syncthreads();
 
-  for (int32_t t = threadIdx.x; t < d.ni; t += blockDim.x) {
+  for (int32_t t0 = threadIdx.x; t0 < d.ni; t0 += blockDim.x) {
+    const int32_t t = SpreadOf<Elem>::value ? spread_incidence(t0, d.ni) : t0;
     const int32_t ent = A.inc_ent[d.i0 + t];
     const SlotEntry<Elem::kCar> se = table[d.i0 + t];
     const int32_t e = ent / Elem::kRar;
@@ -1120,7 +1143,8 @@ rowgather_rowlist_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
     old[u] = k < nval ? vals[gidx[u]] : T(0);
   }
 
-  for (int32_t t = threadIdx.x; t < d.ni; t += blockDim.x) {
+  for (int32_t t0 = threadIdx.x; t0 < d.ni; t0 += blockDim.x) {
+    const int32_t t = SpreadOf<Elem>::value ? spread_incidence(t0, d.ni) : t0;
     const int32_t ent = bent[d.i0 + t];
     const SlotEntry<Elem::kCar> se = table[d.i0 + t];
     const int32_t e = ent / Elem::kRar;

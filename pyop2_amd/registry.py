@@ -62,7 +62,10 @@ FAMILIES = {
               ("dat", READ, (3,), 10)],
         maps=[[(0, 0), (0, 1), (1, 0), (3, 0)]],
         entry={"f64": "op2hip_wrap_p2_elasticity_fused",
-               "f32": "op2hip_wrap_p2_elasticity_fused_f32"}),
+               "f32": "op2hip_wrap_p2_elasticity_fused_f32"},
+        # "split": the same results as the assembly loop (args 0, 2) and the
+        # residual loop (args 1, 2, 3) launched one after the other
+        split=[("p2_elasticity_matrix", [0, 2]), ("p2_elasticity_residual", [1, 2, 3])]),
     "p2_elasticity_adjoint": dict(
         args=[("dat", INC, (3,), 10), ("dat", INC, (3,), 4), ("dat", READ, (3,), 10),
               ("dat", READ, (3,), 4), ("dat", READ, (3,), 10)],
@@ -147,6 +150,14 @@ def lookup(gk):
     return HandWrittenKernel(gk, family, params, variant)
 
 
+class _SubLoop:
+    """The part of a Parloop a split strategy hands to one of its loops."""
+
+    def __init__(self, iterset, arguments):
+        self.iterset = iterset
+        self.arguments = arguments
+
+
 class HandWrittenKernel:
     """Launches a registered element loop through its C entry point."""
 
@@ -180,6 +191,8 @@ class HandWrittenKernel:
         if want.startswith("rowgather") and (not has_mat or subset):
             want = "atomic"
         if want == "staged" and (subset or "staged" not in self.spec):
+            want = "atomic"
+        if want == "split" and "split" not in self.spec:
             want = "atomic"
         if has_mat and want.startswith("rowgather"):
             mat = parloop.arguments[0].data
@@ -219,9 +232,43 @@ class HandWrittenKernel:
             return GeneratedKernel(self.gk)(start, end, ptrs, parloop)
         self.launch(start, end, parloop, self._choose(parloop))
 
+    def _split_parts(self):
+        parts = getattr(self, "_parts", None)
+        if parts is None:
+            parts = self._parts = [(HandWrittenKernel(self.gk, fam, self.params, self.variant), idx)
+                                   for fam, idx in self.spec["split"]]
+        return parts
+
+    def _launch_split(self, start, end, parloop, outputs):
+        """The loop as its split parts: the assembly with its best row gather,
+        the residual with LDS-staged tiles (one element pass each, no
+        redundant gathers of the solution per row incidence)."""
+        for sub, idx in self._split_parts():
+            part = _SubLoop(parloop.iterset, [parloop.arguments[k] for k in idx])
+            sub_out = None
+            if outputs:
+                sub_out = {}
+                for n, k in enumerate(idx):
+                    if k in outputs:
+                        sub_out[n] = outputs[k]
+                if "mat" in outputs:
+                    sub_out["mat"] = outputs["mat"]
+            if sub.spec["args"][0][0] == "mat":
+                same = part.arguments[0].data.block_shape == sub.spec["args"][0][2]
+                strat = "rowgather_slots" if same else "atomic"
+            else:
+                strat = "staged"
+            if isinstance(parloop.iterset, Subset):
+                strat = "atomic"
+            sub.launch(start, end, part, strat, sub_out)
+        self.strategy_used = "split"
+
     def launch(self, start, end, parloop, strategy, outputs=None):
         """Launch with ``strategy``; ``outputs`` optionally replaces the
         written arguments' device pointers (autotuning on scratch copies)."""
+        if strategy == "split":
+            with dev.timed_launch(self.family):
+                return self._launch_split(start, end, parloop, outputs)
         ex = dev.ExecPlan()
         ex.strategy = dev.STRATEGIES[strategy]
         keep = []

@@ -56,7 +56,8 @@ def test_p2_bsr_assembly_f64(device, tet8, strategy):
     assert relerr(v, ref.values) < 1e-12
 
 
-@pytest.mark.parametrize("strategy", ["rowgather_rows", "rowgather_cells", "rowgather_staged", "rowgather_slots", "rowgather_thread", "atomic"])
+@pytest.mark.parametrize("strategy", ["split", "rowgather_rows", "rowgather_cells", "rowgather_staged",
+                                      "rowgather_slots", "rowgather_thread", "atomic"])
 def test_p2_fused_f64(device, tet8, strategy):
     configuration["strategy"] = strategy
     p = TetProblem(tet8)
