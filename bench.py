@@ -73,6 +73,9 @@ def c2_bytes(m, nnz):
 
 
 KERNEL_OF = {
+    "rowgather_rows": "rowgather_rowlist_kernel<{row}, 8>",
+    "rowgather_staged": "rowgather_staged_kernel<{row}, 2>",
+    "rowgather_cells": "rowgather_cells_kernel<{cells}>",
     "rowgather_slots": "rowgather_slots_kernel<{row}, 8>",
     "rowgather_thread": "rowgather_kernel<{row}, 8>",
     "rowgather": "rowgather_lanes_kernel<{lanes}>",
@@ -82,7 +85,8 @@ KERNEL_OF = {
 
 
 def kernel_name(strategy, row, lanes, cell):
-    return KERNEL_OF.get(strategy, "{cell}").format(row=row, lanes=lanes, cell=cell)
+    return KERNEL_OF.get(strategy, "{cell}").format(row=row, lanes=lanes, cell=cell,
+                                                    cells=row.replace("Row", "Cells"))
 
 
 def measured_traffic(key):
@@ -339,6 +343,8 @@ def secondary(args):
                 f"({res[k]['strategy']})")
         if tag == "f64":
             res["c4_adjoint"] = adjoint_ratio(m3, cells, c2v, c2n, X, u, reps)
+            if not args.no_cpu:
+                res["c3_cpu"] = c3_cpu_baseline(m3, res)
         del A, sp
         torch.cuda.empty_cache()
     return res
@@ -431,6 +437,41 @@ def batched(args):
         f"batched {t_batch * 1e3:.3f} ms/pass (kernel {kms * 1e3:.1f} us, grid {batch.grid}, "
         f"{batch.nitems} items): {t_plain / t_batch:.1f}x; batched vs plain rel err {worst:.1e}")
     return res
+
+
+def c3_cpu_baseline(m3, res, sample=24576):
+    """The oracle's sequential wrapper (reference flags, one core) on the
+    first ``sample`` cells of the P2 mesh: residual INC + BSR(3) assembly,
+    scaled to the whole mesh; against the GPU's residual + assembly."""
+    from oracle import seqwrap, sparsity as osp
+    from pyop2_amd import elements
+    n = min(sample, m3.ncells)
+    cn = np.ascontiguousarray(m3.cell_node[:n], dtype=np.int32)
+    cv = np.ascontiguousarray(m3.cell_vertex[:n], dtype=np.int32)
+    R = np.zeros((m3.nnodes, 3))
+    rp, ci = osp.pattern(m3.nnodes, m3.nnodes, [(cn, cn, n)])
+    A = seqwrap.CSRMat(rp, ci, np.zeros((len(ci), 3, 3)), 3, 3, block=True)
+    rcode = elements.kernel_source("p2_elasticity_residual", lam=1.0, mu=0.5)
+    mcode = elements.kernel_source("p2_elasticity_matrix", lam=1.0, mu=0.5)
+    rargs = [seqwrap.DatArg(R, "INC", cn), seqwrap.DatArg(m3.coords, "READ", cv), seqwrap.DatArg(m3.u, "READ", cn)]
+    margs = [seqwrap.MatArg(A, "INC", cn, cn, 3, 3), seqwrap.DatArg(m3.coords, "READ", cv)]
+    seqwrap.run(rcode, "p2_elasticity_residual", rargs, [(0, n)], flavour="fast")
+    seqwrap.run(mcode, "p2_elasticity_matrix", margs, [(0, n)], flavour="fast")
+    t0 = time.perf_counter()
+    seqwrap.run(rcode, "p2_elasticity_residual", rargs, [(0, n)], flavour="fast")
+    t1 = time.perf_counter()
+    seqwrap.run(mcode, "p2_elasticity_matrix", margs, [(0, n)], flavour="fast")
+    t2 = time.perf_counter()
+    scale = m3.ncells / n
+    cpu_s = ((t1 - t0) + (t2 - t1)) * scale
+    gpu_s = (res["c3_residual_f64"]["kernel_us"] + res["c3_assembly_f64"]["kernel_us"]) * 1e-6
+    out = dict(sample_cells=n, residual_s=(t1 - t0) * scale, assembly_s=(t2 - t1) * scale, cpu_s=cpu_s,
+               gpu_s=gpu_s, speedup=cpu_s / gpu_s, cores=1,
+               sample=f"first {n} of {m3.ncells} cells, oracle wrapper with gcc -march=native -O3 -ffast-math "
+                      f"on one core of {cpu_model()}, scaled by {scale:.0f}")
+    log(f"C3 CPU (1 core, {n} cells scaled x{scale:.0f}): residual {out['residual_s']:.2f} s + assembly "
+        f"{out['assembly_s']:.2f} s; GPU residual + assembly {gpu_s * 1e3:.2f} ms: {out['speedup']:.0f}x")
+    return out
 
 
 def adjoint_ratio(m3, cells, c2v, c2n, X, u, reps):
