@@ -381,22 +381,52 @@ struct MatrixRow {
   using T = TT;
   static constexpr int kRar = 10, kCar = 10, kRd = 3, kCd = 3;
   static constexpr bool kHasRowDat = kFused;
-  // 24 prefetched doubles would take the fused kernel past 168 registers
-  static constexpr int kMaxPrefetch = (kFused && sizeof(TT) == 8) ? 16 : 24;
+  // prefetch depth that keeps the float64 kernels at or below 168 registers
+  // (three waves per SIMD, the occupancy their LDS allows)
+  static constexpr int kMaxPrefetch = sizeof(TT) == 8 ? (kFused ? 8 : 16) : 24;
   static constexpr bool kSpread = true;
   const T *X;
   const T *u;
   const int32_t *c2n;
   const int32_t *c2v;
   T lam, mu;
+  // cols: emit the column node ids (the pattern-search row gathers need them;
+  // the slot-table ones do not: the hand-written loops take maps without
+  // negative entries and the slot table marks a column missing from the
+  // pattern), and the fused variant always reads them for u.
+  bool cols = true;
   template <class Emit>
   __device__ __forceinline__ void row(int32_t e, int i, Emit &&emit, T *rsum) const {
     int32_t nd[10];
 #pragma unroll
-    for (int j = 0; j < 10; ++j) nd[j] = c2n[10 * e + j];
+    for (int j = 0; j < 10; ++j) nd[j] = (kFused || cols) ? c2n[10 * e + j] : 0;
     const Geom<T> g = geometry(X, c2v, e);
     const RowCoef<T> rc = row_coefficients(g, i, lam, mu);
     for_blocks<T, 0>(rc, g, [&](int j, const T (&B)[9]) {
+      const int32_t c = nd[j];
+      if (kFused && c >= 0) {
+        const T u0 = u[3 * c], u1 = u[3 * c + 1], u2 = u[3 * c + 2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          rsum[a] += B[3 * a] * u0 + B[3 * a + 1] * u1 + B[3 * a + 2] * u2;
+      }
+      emit(j, c, B);
+    });
+  }
+  // Two lanes per row incidence in the slot-table gather (float64; float32
+  // runs faster with one): half H emits blocks {0,1,4,5,6} or {2,3,7,8,9}
+  // (two vertex and three edge columns each).
+  static constexpr int kHalves = sizeof(TT) == 8 ? 2 : 1;
+  template <int H, class Emit>
+  __device__ __forceinline__ void row_half(int32_t e, int i, Emit &&emit, T *rsum) const {
+    constexpr unsigned kMask = H == 0 ? 0x073u : 0x38Cu;
+    int32_t nd[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+      if ((kMask >> j) & 1u) nd[j] = (kFused || cols) ? c2n[10 * e + j] : 0;
+    const Geom<T> g = geometry(X, c2v, e);
+    const RowCoef<T> rc = row_coefficients(g, i, lam, mu);
+    for_blocks_in<T, kMask>(rc, g, [&](int j, const T (&B)[9]) {
       const int32_t c = nd[j];
       if (kFused && c >= 0) {
         const T u0 = u[3 * c], u1 = u[3 * c + 1], u2 = u[3 * c + 2];
@@ -413,7 +443,7 @@ struct MatrixRow {
     int32_t nd[10];
 #pragma unroll
     for (int j = 0; j < 10; ++j)
-      if ((kColumnGroups[G] >> j) & 1u) nd[j] = c2n[10 * e + j];
+      if ((kColumnGroups[G] >> j) & 1u) nd[j] = (kFused || cols) ? c2n[10 * e + j] : 0;
     const Geom<T> g = geometry(X, c2v, e);
     const RowCoef<T> rc = row_coefficients(g, i, lam, mu);
     for_blocks_in<T, kColumnGroups[G]>(rc, g, [&](int j, const T (&B)[9]) {
@@ -734,15 +764,15 @@ int matrix(int32_t start, int32_t end, const op2hip_mat *A, T *R, const T *X, co
     return launch_rowgather_lanes(start, end, elem, A, ex, R, s);
   }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_SLOTS) {
-    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu), false};
     return launch_rowgather_slots(start, end, elem, A, ex, R, s);
   }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_STAGED) {
-    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu), false};
     return launch_rowgather_staged(start, end, elem, A, ex, R, s);
   }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_ROWLIST) {
-    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu)};
+    MatrixRow<T, kFused> elem{X, u, c2n, c2v, T(lam), T(mu), false};
     return launch_rowgather_rowlist(start, end, elem, A, ex, R, s);
   }
   if (ex && ex->strategy == OP2HIP_ROWGATHER_CELLS) {

@@ -489,6 +489,17 @@ struct SpreadOf<Elem, std::void_t<decltype(Elem::kSpread)>> {
   static constexpr bool value = Elem::kSpread;
 };
 
+// Lanes per row incidence in the slot-table gather (Elem::kHalves = 2 with a
+// row_half<H> member splits an incidence's column blocks between two lanes).
+template <class Elem, class = void>
+struct HalvesOf {
+  static constexpr int value = 1;
+};
+template <class Elem>
+struct HalvesOf<Elem, std::void_t<decltype(Elem::kHalves)>> {
+  static constexpr int value = Elem::kHalves;
+};
+
 // One batch of the slot-table row gather (the workgroup's share of a launch,
 // or one work item of a batched launch).
 template <class Elem, int kPre>
@@ -516,7 +527,11 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
     for (int32_t k = threadIdx.x; k < d.nr * Elem::kRd; k += blockDim.x) racc[k] = Acc(0);
   __syncthreads();
 
-  for (int32_t t0 = threadIdx.x; t0 < d.ni; t0 += blockDim.x) {
+  constexpr int kH = HalvesOf<Elem>::value;
+  for (int32_t tau = threadIdx.x; tau < kH * d.ni; tau += blockDim.x) {
+    // task tau: incidence tau % ni, half tau / ni (uniform over most waves)
+    const int h = kH == 1 ? 0 : int(tau >= d.ni);
+    const int32_t t0 = tau - h * d.ni;
     const int32_t t = SpreadOf<Elem>::value ? spread_incidence(t0, d.ni) : t0;
     const int32_t ent = A.inc_ent[d.i0 + t];
     const SlotEntry<Elem::kCar> se = table[d.i0 + t];
@@ -527,7 +542,7 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
     T rsum[Elem::kRd];
 #pragma unroll
     for (int a = 0; a < Elem::kRd; ++a) rsum[a] = T(0);
-    elem.row(e, i, [&](int j, int32_t col, const T *blk) {
+    auto emit = [&](int j, int32_t col, const T *blk) {
       const int s = se.slot[j];
       if (col < 0) return;
       if (s == 255) {
@@ -536,7 +551,13 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
       }
 #pragma unroll
       for (int k = 0; k < kBs; ++k) atomic_add(&row[s * kBs + k], Acc(blk[k]));
-    }, rsum);
+    };
+    if constexpr (kH == 1) {
+      elem.row(e, i, emit, rsum);
+    } else {
+      if (h == 0) elem.template row_half<0>(e, i, emit, rsum);
+      else elem.template row_half<1>(e, i, emit, rsum);
+    }
     if (Elem::kHasRowDat) {
 #pragma unroll
       for (int a = 0; a < Elem::kRd; ++a) atomic_add(&racc[se.lr * Elem::kRd + a], Acc(rsum[a]));
