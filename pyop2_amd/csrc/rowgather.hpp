@@ -511,6 +511,7 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
   using Acc = typename AccumOf<T>::type;
   constexpr int kBs = Elem::kRd * Elem::kCd;
   extern __shared__ __align__(16) unsigned char smem[];
+  // the batch's values are read once and written once: non-temporal
   T *__restrict__ vals = static_cast<T *>(A.values) + (size_t)d.z0 * kBs;
   const int32_t nval = d.nz * kBs;
 
@@ -518,7 +519,7 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
 #pragma unroll
   for (int u = 0; u < kPre; ++u) {
     const int32_t k = threadIdx.x + u * (int32_t)blockDim.x;
-    old[u] = k < nval ? vals[k] : T(0);
+    old[u] = k < nval ? __builtin_nontemporal_load(&vals[k]) : T(0);
   }
   Acc *acc = reinterpret_cast<Acc *>(smem);
   Acc *racc = acc + nval;
@@ -568,7 +569,7 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
 #pragma unroll
   for (int u = 0; u < kPre; ++u) {
     const int32_t k = threadIdx.x + u * (int32_t)blockDim.x;
-    if (k < nval) vals[k] = old[u] + T(acc[k]);
+    if (k < nval) __builtin_nontemporal_store(old[u] + T(acc[k]), &vals[k]);
   }
   for (int32_t k0 = threadIdx.x + kPre * (int32_t)blockDim.x; k0 < nval; k0 += 8 * (int32_t)blockDim.x) {
     T v[8];
