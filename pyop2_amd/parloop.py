@@ -33,7 +33,7 @@ class ParloopArg(abc.ABC):
     @staticmethod
     def check_map(m):
         if configuration["type_check"]:
-            if m.iterset.total_size > 0 and len(m.values_with_halo) == 0:
+            if m.iterset.total_size > 0 and len(m._table) == 0:
                 raise MapValueError(f"{m} is not initialized")
 
 

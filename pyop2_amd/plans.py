@@ -72,7 +72,7 @@ def greedy_colours(elements, maps_and_targets):
     rows = []
     for m, tkey in maps_and_targets:
         tid = targets.setdefault(tkey, (len(targets), m.toset.total_size))[0]
-        rows.append((np.ascontiguousarray(m.values_with_halo[elements], dtype=np.int32), tid))
+        rows.append((np.ascontiguousarray(m._table[elements], dtype=np.int32), tid))
     nmaps = len(rows)
     ptrs = (ctypes.c_void_p * max(nmaps, 1))(*[r.ctypes.data for r, _ in rows])
     ar = np.array([r.shape[1] for r, _ in rows] or [0], dtype=np.int32)
@@ -301,7 +301,7 @@ def tile_plan(maps, start, end, tile_size, node_cap=0):
     L = dev.lib()
     n = int(end - start)
     nm = len(maps)
-    vals = [np.ascontiguousarray(m.values_with_halo, dtype=np.int32) for m in maps]
+    vals = [np.ascontiguousarray(m._table, dtype=np.int32) for m in maps]
     ar = np.array([m.arity for m in maps], dtype=np.int32)
     ts = np.array([m.toset.total_size for m in maps], dtype=np.int32)
     tp = np.empty(n + 1, dtype=np.int32)
@@ -364,7 +364,7 @@ def row_list(rmap, cmap, tr, mat, max_inc, max_nnz, key):
     colidx = np.ascontiguousarray(dev.to_host(mat.colidx), dtype=np.int32)
     inc_ptr = np.ascontiguousarray(dev.to_host(tr.inc_ptr), dtype=np.int32)
     inc_ent = np.ascontiguousarray(dev.to_host(tr.inc_ent), dtype=np.int32)
-    cvals = np.ascontiguousarray(cmap.values_with_halo, dtype=np.int32)
+    cvals = np.ascontiguousarray(cmap._table, dtype=np.int32)
     eb = ctypes.c_size_t()
     dev.check(L.op2hip_slot_table_bytes(cmap.arity, ctypes.byref(eb)), "slot_table_bytes")
     ninc = int(tr.total)

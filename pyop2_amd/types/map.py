@@ -39,13 +39,21 @@ class Map:
         return MapKernelArg(self.arity, None, None)
 
     @property
+    def _table(self):
+        """The index table the loops gather through: ``(iterset.total_size,
+        arity)`` int32 (a permuted or composed map computes its own); no
+        rows for a map built without values."""
+        v = self._values
+        return v.reshape(self.shape) if v.size == self.shape[0] * self.shape[1] else v.reshape(0, self._arity)
+
+    @property
     def device_values(self):
-        """Device copy (int32) of ``values_with_halo``, range-checked once:
+        """Device copy (int32) of the index table, range-checked once:
         entries must be below ``toset.total_size`` (negative entries mean
         "no target" and are skipped by the loops)."""
         if self._dev is None:
             from pyop2_amd import device as dev
-            v = np.ascontiguousarray(self._values.reshape(self.shape))
+            v = np.ascontiguousarray(self._table, dtype=IntType)
             self._has_negative = bool(v.size and v.min() < 0)
             if v.size and v.max() >= self._toset.total_size:
                 raise ex.MapValueError(f"{self} has entries >= {self._toset.total_size} (target set size)")
@@ -127,3 +135,110 @@ class Map:
 
     def __le__(self, o):
         return self == o
+
+
+class PermutedMap(Map):
+    """A :class:`Map` with a constant permutation of each element's entries
+    (``pyop2/types/map.py:171-213``): element data is staged as
+    ``local[i] = global[map[permutation[i]]]``.  ``values`` are the base
+    map's, as in the reference; the loops gather through the permuted table,
+    which is built once (one gather per entry, no permutation in the loop)."""
+
+    def __init__(self, map_, permutation):
+        if not isinstance(map_, Map):
+            raise TypeError("map_ must be a Map instance")
+        if isinstance(map_, ComposedMap):
+            raise NotImplementedError("PermutedMap of ComposedMap not implemented: simply permute before composing")
+        perm = np.asarray(permutation, dtype=Map.dtype)
+        if not np.array_equal(np.sort(perm), np.arange(map_.arity, dtype=Map.dtype)):
+            raise ValueError(f"permutation {list(perm)} is not a permutation of range({map_.arity})")
+        self.map_ = map_
+        self.permutation = perm
+        self._iterset = map_.iterset
+        self._toset = map_.toset
+        self.comm = map_.comm
+        self._arity = map_.arity
+        self._values = map_._values
+        self.shape = map_.shape
+        self._name = "pmap_#x%x" % id(self)
+        self._offset = map_._offset
+        self._offset_quotient = map_._offset_quotient
+        self._cache = {}
+        self._dev = None
+
+    @utils.cached_property
+    def _table(self):
+        return np.ascontiguousarray(self.map_._table[:, self.permutation])
+
+    def __str__(self):
+        return "OP2 PermutedMap of %s with permutation %s" % (self.map_, list(self.permutation))
+
+    def __repr__(self):
+        return "PermutedMap(%r, %r)" % (self.map_, list(self.permutation))
+
+
+class ComposedMap(Map):
+    """Composition of maps (``pyop2/types/map.py:216-287``): element data is
+    staged as ``local[i] = global[maps_[0][maps_[1][...[i]]]]``; every map but
+    the first has arity 1.  The composed table is built once; a negative entry
+    anywhere along the chain makes the composed entry negative (no target)."""
+
+    def __init__(self, *maps_, name=None):
+        if not all(isinstance(m, Map) for m in maps_):
+            raise TypeError("All maps must be Map instances")
+        for tomap, frommap in zip(maps_[:-1], maps_[1:]):
+            if tomap.iterset is not frommap.toset:
+                raise ex.MapTypeError("tomap.iterset must match frommap.toset")
+            if tomap.comm is not frommap.comm:
+                raise ex.MapTypeError("All maps needs to share a communicator")
+            if frommap.arity != 1:
+                raise ex.MapTypeError("frommap.arity must be 1")
+        self._iterset = maps_[-1].iterset
+        self._toset = maps_[0].toset
+        self.comm = self._toset.comm
+        self._arity = maps_[0].arity
+        self._values = None
+        self.shape = (self._iterset.total_size, self._arity)
+        self._name = name or "cmap_#x%x" % id(self)
+        self._offset = maps_[0]._offset
+        self._offset_quotient = None
+        self._cache = {}
+        self._dev = None
+        self.maps_ = tuple(maps_)
+
+    @utils.cached_property
+    def _table(self):
+        if any(len(m._table) != m.iterset.total_size for m in self.maps_):
+            return np.zeros((0, self._arity), dtype=Map.dtype)
+        idx = np.asarray(self.maps_[-1]._table[:, 0], dtype=np.int64)
+        for m in reversed(self.maps_[1:-1]):
+            t = m._table[:, 0]
+            idx = np.where(idx >= 0, t[np.clip(idx, 0, None)], -1)
+        first = self.maps_[0]._table
+        out = first[np.clip(idx, 0, None)].copy() if len(first) else np.zeros(self.shape, Map.dtype)
+        out[idx < 0] = -1
+        return np.ascontiguousarray(out, dtype=Map.dtype).reshape(self.shape)
+
+    @utils.cached_property
+    def values(self):
+        raise RuntimeError("ComposedMap does not store values directly")
+
+    @utils.cached_property
+    def values_with_halo(self):
+        raise RuntimeError("ComposedMap does not store values directly")
+
+    @utils.cached_property
+    def flattened_maps(self):
+        out = []
+        for m in self.maps_:
+            out.extend(m.flattened_maps)
+        return tuple(out)
+
+    def __str__(self):
+        return "OP2 ComposedMap of Maps: [%s]" % ",".join([str(m) for m in self.maps_])
+
+    def __repr__(self):
+        return "ComposedMap(%s)" % ",".join([repr(m) for m in self.maps_])
+
+    def __le__(self, o):
+        raise NotImplementedError("__le__ not implemented for ComposedMap")

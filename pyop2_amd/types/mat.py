@@ -80,7 +80,7 @@ class Sparsity(caching.ObjectCached):
                 for m in (rmap, cmap):
                     if not isinstance(m, Map):
                         raise ex.MapTypeError("All maps must be of type map, not type %r" % type(m))
-                    if len(m.values_with_halo) == 0 and m.iterset.total_size > 0:
+                    if len(m._table) == 0 and m.iterset.total_size > 0:
                         raise ex.MapValueError("Unpopulated map values when trying to build sparsity.")
                 if rmap.toset is not dsets[0].set or cmap.toset is not dsets[1].set:
                     raise RuntimeError("Map toset must be the same as DataSet set")

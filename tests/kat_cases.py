@@ -338,3 +338,348 @@ OTHER_CASES = [
     ("write_vs_inc_matrix", case_write_vs_inc_matrix),
     ("sparsity_nnz", case_sparsity_nnz),
 ]
+
+
+# ---- permuted and composed maps (test_indirect_loop.py:277-377) -------------
+COPY4_INT = "static void copy4(int *dst, const int *src) { for (int k = 0; k < 4; ++k) dst[k] = src[k]; }"
+COPY2_DBL = "static void copy2(double *dst, const double *src) { for (int k = 0; k < 2; ++k) dst[k] = src[k]; }"
+NODAL_A = np.array([0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7])
+
+
+def case_permuted_map(op2, split=None):
+    """test_indirect_loop.py:277-293 -- WRITE through PermutedMap(m, [3,2,0,1]),
+    READ through m = [1,2,3,0]: dst[perm-table[k]] = src[m[k]] gives [1,3,0,2]."""
+    one, four = op2.Set(1), op2.Set(4)
+    src = op2.Dat(op2.DataSet(four, 1), dtype=np.int32)
+    dst = op2.Dat(op2.DataSet(four, 1), dtype=np.int32)
+    src.data[:] = np.arange(4, dtype=np.int32)
+    m = op2.Map(one, four, 4, values=[1, 2, 3, 0])
+    pm = op2.PermutedMap(m, [3, 2, 0, 1])
+    op2.par_loop(op2.Kernel(COPY4_INT, "copy4"), one, dst(op2.WRITE, pm), src(op2.READ, m))
+    assert (np.asarray(src.data) == np.arange(4)).all()
+    assert list(np.asarray(dst.data)) == [1, 3, 0, 2]
+    assert list(np.asarray(pm.values).ravel()) == [1, 2, 3, 0]      # values are the base map's
+
+
+def case_permuted_map_both(op2, split=None):
+    """test_indirect_loop.py:295-313 -- both arguments permuted: [2,1,3,0]."""
+    one, four = op2.Set(1), op2.Set(4)
+    src = op2.Dat(op2.DataSet(four, 1), dtype=np.int32)
+    dst = op2.Dat(op2.DataSet(four, 1), dtype=np.int32)
+    src.data[:] = np.arange(4, dtype=np.int32)
+    m = op2.Map(one, four, 4, values=[0, 2, 1, 3])
+    op2.par_loop(op2.Kernel(COPY4_INT, "copy4"), one, dst(op2.WRITE, op2.PermutedMap(m, [3, 2, 1, 0])),
+                 src(op2.READ, op2.PermutedMap(m, [0, 2, 3, 1])))
+    assert list(np.asarray(dst.data)) == [2, 1, 3, 0]
+
+
+def case_composed_two(op2, split=None, permuted=False):
+    """test_indirect_loop.py:316-344 -- ComposedMap(A0, A1): B-element b reads
+    the nodes of A-element A1[b]; with A0 pre-permuted the pairs swap."""
+    setB, nodesB = op2.Set(3), op2.Set(6)
+    datB = op2.Dat(op2.DataSet(nodesB, 1), dtype=np.float64)
+    mapB = op2.Map(setB, nodesB, 2, values=[[0, 1], [2, 3], [4, 5]])
+    setA, nodesA = op2.Set(5), op2.Set(8)
+    datA = op2.Dat(op2.DataSet(nodesA, 1), dtype=np.float64)
+    datA.data[:] = NODAL_A
+    mapA0 = op2.Map(setA, nodesA, 2, values=[[0, 1], [2, 3], [4, 5], [6, 7], [0, 1]])
+    if permuted:
+        mapA0 = op2.PermutedMap(mapA0, [1, 0])
+    mapA = op2.ComposedMap(mapA0, op2.Map(setB, setA, 1, values=[3, 1, 2]))
+    op2.par_loop(op2.Kernel(COPY2_DBL, "copy2"), setB, datB(op2.WRITE, mapB), datA(op2.READ, mapA))
+    want = [0.7, 0.6, 0.3, 0.2, 0.5, 0.4] if permuted else [0.6, 0.7, 0.2, 0.3, 0.4, 0.5]
+    assert list(np.asarray(datB.data)) == want
+
+
+def case_composed_three(op2, split=None, nested="none", subset=False):
+    """test_indirect_loop.py:347-386 -- three maps, flat or nested either way,
+    over the whole set or the subset [1]."""
+    setC, nodesC = op2.Set(2), op2.Set(4)
+    datC = op2.Dat(op2.DataSet(nodesC, 1), dtype=np.float64)
+    mapC = op2.Map(setC, nodesC, 2, values=[[0, 1], [2, 3]])
+    setB, setA, nodesA = op2.Set(3), op2.Set(5), op2.Set(8)
+    datA = op2.Dat(op2.DataSet(nodesA, 1), dtype=np.float64)
+    datA.data[:] = NODAL_A
+    a0 = op2.Map(setA, nodesA, 2, values=[[0, 1], [2, 3], [4, 5], [6, 7], [0, 1]])
+    a1 = op2.Map(setB, setA, 1, values=[3, 1, 2])
+    a2 = op2.Map(setC, setB, 1, values=[2, 0])
+    mapA = {"none": lambda: op2.ComposedMap(a0, a1, a2),
+            "first": lambda: op2.ComposedMap(op2.ComposedMap(a0, a1), a2),
+            "last": lambda: op2.ComposedMap(a0, op2.ComposedMap(a1, a2))}[nested]()
+    it = op2.Subset(setC, np.array([1], dtype=np.int32)) if subset else setC
+    op2.par_loop(op2.Kernel(COPY2_DBL, "copy2"), it, datC(op2.WRITE, mapC), datA(op2.READ, mapA))
+    want = [0.0, 0.0, 0.6, 0.7] if subset else [0.4, 0.5, 0.6, 0.7]
+    assert list(np.asarray(datC.data)) == want
+
+
+def case_composed_errors(op2, split=None):
+    """map.py:216-260 -- composition checks, and values are not stored."""
+    import pytest
+    s1, s2, s3 = op2.Set(3), op2.Set(4), op2.Set(5)
+    m = op2.Map(s2, s3, 2, values=np.zeros((4, 2), dtype=np.int32))
+    with pytest.raises(op2.MapTypeError):
+        op2.ComposedMap(m, op2.Map(s1, s3, 1, values=[0, 1, 2]))        # iterset/toset mismatch
+    with pytest.raises(op2.MapTypeError):
+        op2.ComposedMap(m, op2.Map(s1, s2, 2, values=np.zeros((3, 2), dtype=np.int32)))   # arity 2
+    cm = op2.ComposedMap(m, op2.Map(s1, s2, 1, values=[0, 1, 2]))
+    assert (cm.iterset, cm.toset, cm.arity) == (s1, s3, 2)
+    with pytest.raises(RuntimeError):
+        cm.values
+
+
+MAP_CASES = [
+    ("permuted_map", case_permuted_map),
+    ("permuted_map_both", case_permuted_map_both),
+    ("composed_two_none", lambda op2: case_composed_two(op2, permuted=False)),
+    ("composed_two_pre", lambda op2: case_composed_two(op2, permuted=True)),
+] + [(f"composed_three_{n}_{'subset' if s else 'all'}",
+      (lambda n, s: lambda op2: case_composed_three(op2, nested=n, subset=s))(n, s))
+     for n in ("none", "first", "last") for s in (False, True)] + [
+    ("composed_errors", case_composed_errors),
+]
+
+
+# ---- direct loops (test_direct_loop.py:44-200) -------------------------------
+DIRECT_SPLITS = SPLITS + [(0, N // 2, N)]          # test_direct_loop.py:46-50 (the last has a halo)
+
+
+def _direct(op2, split, dim=1):
+    elems = op2.Set(split, "elems")
+    ds = op2.DataSet(elems, dim)
+    init = np.arange(N, dtype=np.uint32)
+    x = op2.Dat(ds, init if dim == 1 else np.stack([init, init], axis=1), np.uint32)
+    return elems, x
+
+
+def case_direct_wo(op2, split):
+    """test_direct_loop.py:90-95 -- WRITE 42 everywhere owned."""
+    elems, x = _direct(op2, split)
+    op2.par_loop(op2.Kernel("static void wo(unsigned int *v) { *v = 42u; }", "wo"), elems, x(op2.WRITE))
+    assert (np.asarray(x.data) == 42).all()
+
+
+def case_direct_rw(op2, split):
+    """test_direct_loop.py:107-115 -- RW +1: sum of the owned part n(n+1)/2."""
+    elems, x = _direct(op2, split)
+    op2.par_loop(op2.Kernel("static void plus1(unsigned int *v) { *v = *v + 1u; }", "plus1"), elems, x(op2.RW))
+    n = elems.size
+    assert int(np.asarray(x.data_ro, dtype=np.int64).sum()) == n * (n + 1) // 2
+
+
+def case_direct_global_inc(op2, split):
+    """test_direct_loop.py:117-125 -- RW and Global INC in one loop."""
+    elems, x = _direct(op2, split)
+    g = op2.Global(1, 0, np.uint32, "g")
+    op2.par_loop(op2.Kernel("static void acc(unsigned int *v, unsigned int *s) { *v += 1u; *s += *v; }", "acc"),
+                 elems, x(op2.RW), g(op2.INC))
+    n = elems.size
+    assert int(g.data[0]) == n * (n + 1) // 2
+
+
+def case_direct_global_inc_init(op2, split):
+    """test_direct_loop.py:127-132 -- INC into a Global that starts at 10."""
+    elems, _ = _direct(op2, split)
+    g = op2.Global(1, 0, np.uint32, "g")
+    g.data[0] = 10
+    op2.par_loop(op2.Kernel("static void count(unsigned int *s) { *s += 1u; }", "count"), elems, g(op2.INC))
+    assert int(g.data[0]) == elems.size + 10
+
+
+def case_direct_global_max(op2, split):
+    """test_direct_loop.py:134-158 -- MAX: from 0 reduces to the Dat's max;
+    from 2N (larger than every entry) stays 2N."""
+    elems, x = _direct(op2, split)
+    g = op2.Global(1, 0, np.uint32, "g")
+    op2.par_loop(op2.Kernel("static void mx(unsigned int *m, const unsigned int *v) { if (*m < *v) *m = *v; }",
+                            "mx"), elems, g(op2.MAX), x(op2.READ))
+    assert int(g.data[0]) == (int(np.asarray(x.data).max()) if elems.size else 0)
+    h = op2.Global(1, 2 * N, np.uint32, "h")
+    op2.par_loop(op2.Kernel("static void mx2(const unsigned int *v, unsigned int *m) { if (*m < *v) *m = *v; }",
+                            "mx2"), elems, x(op2.READ), h(op2.MAX))
+    assert int(h.data[0]) == 2 * N
+
+
+def case_direct_global_min(op2, split):
+    """test_direct_loop.py:160-185 -- MIN: from 1000 reduces to the Dat's min;
+    from 10 over values 11 stays 10."""
+    elems, x = _direct(op2, split)
+    g = op2.Global(1, 1000, np.uint32, "g")
+    op2.par_loop(op2.Kernel("static void mn(unsigned int *m, const unsigned int *v) { if (*m > *v) *m = *v; }",
+                            "mn"), elems, g(op2.MIN), x(op2.READ))
+    assert int(g.data[0]) == (int(np.asarray(x.data).min()) if elems.size else 1000)
+    x.data[:] = 11
+    h = op2.Global(1, 10, np.uint32, "h")
+    op2.par_loop(op2.Kernel("static void mn2(const unsigned int *v, unsigned int *m) { if (*m > *v) *m = *v; }",
+                            "mn2"), elems, x(op2.READ), h(op2.MIN))
+    assert int(h.data[0]) == 10
+
+
+def case_direct_global_read(op2, split):
+    """test_direct_loop.py:187-196 -- add a Global 1 to every entry."""
+    elems, x = _direct(op2, split)
+    h = op2.Global(1, 1, np.uint32, "h")
+    op2.par_loop(op2.Kernel("static void addg(unsigned int *v, const unsigned int *c) { *v += *c; }", "addg"),
+                 elems, x(op2.RW), h(op2.READ))
+    n = elems.size
+    assert int(np.asarray(x.data_ro, dtype=np.int64).sum()) == n * (n + 1) // 2
+
+
+def case_direct_2d(op2, split):
+    """test_direct_loop.py:198-204 -- WRITE both components of a dim-2 Dat."""
+    elems, y = _direct(op2, split, dim=2)
+    op2.par_loop(op2.Kernel("static void two(unsigned int *v) { v[0] = 42u; v[1] = 43u; }", "two"),
+                 elems, y(op2.WRITE))
+    d = np.asarray(y.data)
+    assert (d[:, 0] == 42).all() and (d[:, 1] == 43).all()
+
+
+def case_direct_host_write(op2, split):
+    """test_direct_loop.py:206-223 -- host writes between loops are seen:
+    INC of ones gives n, then of twos 2n (argument order swapped)."""
+    elems, x = _direct(op2, split)
+    g = op2.Global(1, 0, np.uint32, "g")
+    x.data[:] = 1
+    g.data[:] = 0
+    op2.par_loop(op2.Kernel("static void s1(unsigned int *s, const unsigned int *v) { *s += *v; }", "s1"),
+                 elems, g(op2.INC), x(op2.READ))
+    assert int(g.data[0]) == elems.size
+    x.data[:] = 2
+    g.data[:] = 0
+    op2.par_loop(op2.Kernel("static void s2(const unsigned int *v, unsigned int *s) { *s += *v; }", "s2"),
+                 elems, x(op2.READ), g(op2.INC))
+    assert int(g.data[0]) == 2 * elems.size
+
+
+DIRECT_CASES = [case_direct_wo, case_direct_rw, case_direct_global_inc, case_direct_global_inc_init,
+                case_direct_global_max, case_direct_global_min, case_direct_global_read, case_direct_2d,
+                case_direct_host_write]
+
+
+# ---- Global reductions (test_global_reduction.py:44-470) ---------------------
+def _typed(dtype):
+    return {np.uint32: "unsigned int", np.int32: "int", np.float32: "float", np.float64: "double"}[dtype]
+
+
+def case_reduce_typed(op2, split, dtype, op, start, fill, want):
+    """test_global_reduction.py:172-258 -- MIN/MAX of a constant Dat against a
+    Global initial value, per scalar type."""
+    s = op2.Set(split, "set")
+    d = op2.Dat(op2.DataSet(s, 1), [fill] * N, dtype, "d")
+    t = _typed(dtype)
+    cmp = "<" if op == "MIN" else ">"
+    k = op2.Kernel(f"static void red({t} *g, const {t} *x) {{ if (*x {cmp} *g) *g = *x; }}", "red")
+    g = op2.Global(1, start, dtype, "g")
+    op2.par_loop(k, s, g(getattr(op2, op)), d(op2.READ))
+    assert np.asarray(g.data)[0] == dtype(want if s.size else start)
+
+
+TYPED_REDUCTIONS = [
+    ("min_uint32", np.uint32, "MIN", 8, 12, 8),
+    ("min_int32", np.int32, "MIN", 8, -12, -12),
+    ("max_int32", np.int32, "MAX", -42, -12, -12),
+    ("min_float", np.float32, "MIN", -0.8, -12.0, -12.0),
+    ("max_float", np.float32, "MAX", -42.8, -12.0, -12.0),
+    ("min_double", np.float64, "MIN", -0.8, -12.0, -12.0),
+    ("max_double", np.float64, "MAX", -42.8, -12.0, -12.0),
+]
+
+
+def _seq(op2, split, dim):
+    s = op2.Set(split, "set")
+    d = op2.Dat(op2.DataSet(s, dim), np.arange(dim * N, dtype=np.uint32) + 1, np.uint32)
+    return s, d
+
+
+K1 = {"INC": "static void r1(unsigned int *g, const unsigned int *x) { *g += *x; }",
+      "MIN": "static void r1(unsigned int *g, const unsigned int *x) { if (*x < *g) *g = *x; }",
+      "MAX": "static void r1(unsigned int *g, const unsigned int *x) { if (*x > *g) *g = *x; }"}
+K2 = {"INC": "static void r2(unsigned int *g, const unsigned int *x) { g[0] += x[0]; g[1] += x[1]; }",
+      "MIN": "static void r2(unsigned int *g, const unsigned int *x)"
+             " { for (int c = 0; c < 2; ++c) if (x[c] < g[c]) g[c] = x[c]; }",
+      "MAX": "static void r2(unsigned int *g, const unsigned int *x)"
+             " { for (int c = 0; c < 2; ++c) if (x[c] > g[c]) g[c] = x[c]; }"}
+
+
+def case_reduce_1d(op2, split):
+    """test_global_reduction.py:260-336 -- READ (with and without initial
+    data), INC, and MIN/MAX with the Dat or the Global holding the extreme."""
+    s, d = _seq(op2, split, 1)
+    n = s.size
+    own = np.asarray(d.data).astype(np.int64)
+    g = op2.Global(1, 1, dtype=np.uint32)
+    op2.par_loop(op2.Kernel("static void w(unsigned int *x, const unsigned int *g) { *x = *g; }", "w"),
+                 s, d(op2.WRITE), g(op2.READ))
+    assert (np.asarray(d.data) == 1).all()
+    g0 = op2.Global(1, dtype=np.uint32)
+    d.data[:] = 100
+    op2.par_loop(op2.Kernel("static void w(unsigned int *x, const unsigned int *g) { *x = *g; }", "w"),
+                 s, d(op2.WRITE), g0(op2.READ))
+    assert (np.asarray(g0.data) == 0).all() and (np.asarray(d.data) == 0).all()
+    d.data[:] = own
+    for start in (0, None):
+        gi = op2.Global(1, start, dtype=np.uint32)
+        op2.par_loop(op2.Kernel(K1["INC"], "r1"), s, gi(op2.INC), d(op2.READ))
+        assert int(gi.data[0]) == own.sum()
+    if n:
+        for op, start, want in (("MIN", own.min() + 1, own.min()), ("MAX", own.max() - 1, own.max()),
+                                ("MIN", own.min() - 1, own.min() - 1), ("MAX", own.max() + 1, own.max() + 1)):
+            gm = op2.Global(1, start, dtype=np.uint32)
+            op2.par_loop(op2.Kernel(K1[op], "r1"), s, gm(getattr(op2, op)), d(op2.READ))
+            assert int(gm.data[0]) == want, (op, start)
+
+
+def case_reduce_2d(op2, split):
+    """test_global_reduction.py:270-277, 338-397 -- dim-2 READ, INC, MIN, MAX."""
+    s, d = _seq(op2, split, 2)
+    own = np.asarray(d.data).astype(np.int64)
+    x = op2.Dat(op2.DataSet(s, 1), np.zeros(N, dtype=np.uint32), np.uint32)
+    g = op2.Global(2, (1, 2), dtype=np.uint32)
+    op2.par_loop(op2.Kernel("static void w2(unsigned int *x, const unsigned int *g) { *x = g[0] + g[1]; }", "w2"),
+                 s, x(op2.WRITE), g(op2.READ))
+    assert (np.asarray(x.data) == 3).all()
+    gi = op2.Global(2, (0, 0), dtype=np.uint32)
+    op2.par_loop(op2.Kernel(K2["INC"], "r2"), s, gi(op2.INC), d(op2.READ))
+    assert list(np.asarray(gi.data).astype(np.int64)) == list(own.sum(axis=0))
+    if s.size:
+        for op, delta, red in (("MIN", 1, own.min(axis=0)), ("MAX", -1, own.max(axis=0))):
+            gm = op2.Global(2, tuple(int(v) + delta for v in red), dtype=np.uint32)
+            op2.par_loop(op2.Kernel(K2[op], "r2"), s, gm(getattr(op2, op)), d(op2.READ))
+            assert list(np.asarray(gm.data).astype(np.int64)) == list(red), op
+        for op, delta, red in (("MIN", -1, own.min(axis=0)), ("MAX", 1, own.max(axis=0))):
+            start = tuple(int(v) + delta for v in red)
+            gm = op2.Global(2, start, dtype=np.uint32)
+            op2.par_loop(op2.Kernel(K2[op], "r2"), s, gm(getattr(op2, op)), d(op2.READ))
+            assert tuple(int(v) for v in np.asarray(gm.data)) == start, op
+
+
+def case_reduce_repeated(op2, split):
+    """test_global_reduction.py:399-470 -- INC twice accumulates; setting the
+    Global in between is seen; two Globals of different types; a ParLoop
+    object computed repeatedly, with zero() in between."""
+    s, d = _seq(op2, split, 1)
+    n = s.size
+    total = int(np.asarray(d.data).astype(np.int64).sum())
+    g = op2.Global(1, 0, dtype=np.uint32)
+    k = op2.Kernel(K1["INC"], "r1")
+    op2.par_loop(k, s, g(op2.INC), d(op2.READ))
+    op2.par_loop(k, s, g(op2.INC), d(op2.READ))
+    assert int(g.data[0]) == 2 * total
+    g.data = 10
+    op2.par_loop(k, s, g(op2.INC), d(op2.READ))
+    assert int(g.data[0]) == total + 10
+    gu = op2.Global(1, [0], np.uint32, "gu")
+    gd = op2.Global(1, [0.0], np.float64, "gd")
+    op2.par_loop(op2.Kernel("static void both(unsigned int *i, double *x) { *i += 1u; *x += 1.0; }", "both"),
+                 s, gu(op2.INC), gd(op2.INC))
+    assert int(gu.data[0]) == n and float(gd.data[0]) == float(n)
+    c = op2.Global(1, 0, dtype=np.uint32)
+    loop = op2.ParLoop(op2.Kernel("static void one(unsigned int *g) { *g += 1u; }", "one"), s, c(op2.INC))
+    loop.compute()
+    loop.compute()
+    assert int(c.data[0]) == 2 * n
+    c.zero()
+    loop.compute()
+    assert int(c.data[0]) == n
+
+
+REDUCTION_CASES = [case_reduce_1d, case_reduce_2d, case_reduce_repeated]

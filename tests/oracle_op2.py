@@ -9,6 +9,10 @@ from oracle import seqwrap, sparsity as osp
 READ, WRITE, RW, INC, MIN, MAX = "READ", "WRITE", "RW", "INC", "MIN", "MAX"
 
 
+class MapTypeError(TypeError):
+    pass
+
+
 class Set:
     def __init__(self, size, name=None):
         if isinstance(size, (int, np.integer)):
@@ -20,17 +24,81 @@ class Set:
         return DataSet(self, dim)
 
 
+class Subset(Set):
+    """Elements ``indices`` of ``superset``; part sizes count the indices
+    below the superset's part boundaries (``pyop2/types/set.py:338-372``)."""
+
+    def __init__(self, superset, indices):
+        self.superset = superset
+        self.indices = np.unique(np.asarray(indices, dtype=np.int32))
+        super().__init__(tuple(int((self.indices < b).sum())
+                               for b in (superset.core_size, superset.size, superset.total_size)))
+
+
 class DataSet:
-    def __init__(self, set_, dim):
+    def __init__(self, set_, dim, name=None):
         self.set = set_
         self.dim = tuple(dim) if isinstance(dim, (tuple, list)) else (int(dim),)
         self.cdim = int(np.prod(self.dim))
 
 
 class Map:
+    """``table`` is what a loop gathers through: entry ``k`` of element ``n``
+    names the target ``table[n, k]``."""
+
     def __init__(self, iterset, toset, arity, values=None, name=None):
         self.iterset, self.toset, self.arity = iterset, toset, arity
         self.values = np.ascontiguousarray(np.asarray(values, dtype=np.int32).reshape(iterset.total_size, arity))
+
+    @property
+    def table(self):
+        return self.values
+
+
+class PermutedMap(Map):
+    """Entry ``k`` of the permuted map is entry ``permutation[k]`` of the
+    base map (``pyop2/codegen/builder.py:145-175``, ``PMap._permute``)."""
+
+    def __init__(self, map_, permutation):
+        self.map_, self.permutation = map_, np.asarray(permutation, dtype=np.int32)
+        self.iterset, self.toset, self.arity = map_.iterset, map_.toset, map_.arity
+        self.values = map_.values
+
+    @property
+    def table(self):
+        base = self.map_.table
+        return np.array([[base[n][self.permutation[k]] for k in range(self.arity)]
+                         for n in range(len(base))], dtype=np.int32).reshape(-1, self.arity)
+
+
+class ComposedMap(Map):
+    """Element ``n`` is carried through ``maps_[-1]``, ..., ``maps_[1]`` (entry
+    0 each) and then indexes ``maps_[0]`` (``pyop2/codegen/builder.py:178-194``,
+    ``CMap.indexed``; the checks follow ``pyop2/types/map.py:216-260``)."""
+
+    def __init__(self, *maps_, name=None):
+        for tomap, frommap in zip(maps_[:-1], maps_[1:]):
+            if tomap.iterset is not frommap.toset:
+                raise MapTypeError("tomap.iterset must match frommap.toset")
+            if frommap.arity != 1:
+                raise MapTypeError("frommap.arity must be 1")
+        self.maps_ = maps_
+        self.iterset, self.toset, self.arity = maps_[-1].iterset, maps_[0].toset, maps_[0].arity
+
+    @property
+    def values(self):
+        raise RuntimeError("ComposedMap does not store values directly")
+
+    @property
+    def table(self):
+        tables = [m.table for m in self.maps_]
+        out = []
+        for n in range(self.iterset.total_size):
+            e = n
+            for t in reversed(tables[1:]):
+                e = int(t[e][0])
+            out.append([int(v) for v in tables[0][e]])
+        return np.array(out, dtype=np.int32).reshape(-1, self.arity)
 
 
 class Dat:
@@ -49,6 +117,9 @@ class Dat:
 
     data_ro = data
 
+    def zero(self):
+        self._a[...] = 0
+
     def __call__(self, access, path=None):
         return ("dat", self, access, path)
 
@@ -63,7 +134,14 @@ class Global:
     def data(self):
         return self._a
 
+    @data.setter
+    def data(self, value):
+        self._a[...] = value
+
     data_ro = data
+
+    def zero(self):
+        self._a[...] = 0
 
     def __call__(self, access):
         return ("glob", self, access, None)
@@ -77,7 +155,7 @@ class Sparsity:
         self.pairs = [(r, c) for r, c, _ in pairs]
         diag = rset is cset or (rset.set is cset.set and rset.dim == cset.dim)
         self.rowptr, self.colidx = osp.pattern(rset.set.total_size, cset.set.total_size,
-                                               [(r.values, c.values, r.iterset.size) for r, c in self.pairs],
+                                               [(r.table, c.table, r.iterset.size) for r, c in self.pairs],
                                                set_diag=diag)
         self.block = block_sparse and self.rdim == self.cdim and self.rdim > 1
         self.nnz = osp.dof_nnz(self.rowptr, self.rdim, self.cdim, block_sparse)
@@ -117,12 +195,20 @@ class Kernel:
         self.code, self.name = code, name
 
 
+class ParLoop:
+    def __init__(self, kernel, iterset, *args):
+        self.kernel, self.iterset, self.args = kernel, iterset, args
+
+    def compute(self):
+        par_loop(self.kernel, self.iterset, *self.args)
+
+
 def par_loop(kernel, iterset, *args):
     sargs = []
     temps = []
     for kind, obj, access, path in args:
         if kind == "dat":
-            sargs.append(seqwrap.DatArg(obj._a, access, path.values if path is not None else None))
+            sargs.append(seqwrap.DatArg(obj._a, access, path.table if path is not None else None))
         elif kind == "glob":
             if access == INC:
                 tmp = np.zeros_like(obj._a)
@@ -134,9 +220,10 @@ def par_loop(kernel, iterset, *args):
             r, c = path
             s = obj.sparsity
             rd, cd = s.rdim, s.cdim
-            sargs.append(seqwrap.MatArg(obj.csr, access, r.values, c.values, rd, cd))
+            sargs.append(seqwrap.MatArg(obj.csr, access, r.table, c.table, rd, cd))
     parts = [(0, iterset.core_size), (iterset.core_size, iterset.size)]
-    err = seqwrap.run(kernel.code, kernel.name, sargs, parts)
+    subset = iterset.indices if isinstance(iterset, Subset) else None
+    err = seqwrap.run(kernel.code, kernel.name, sargs, parts, subset=subset)
     assert err == 0, "insertion outside the sparsity pattern"
     for g, tmp in temps:
         g._a += tmp

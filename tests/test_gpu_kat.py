@@ -18,6 +18,46 @@ def test_loop_cases(device, case, split, strategy):
     case(op2, split)
 
 
-@pytest.mark.parametrize("name,case", kat_cases.OTHER_CASES, ids=[n for n, _ in kat_cases.OTHER_CASES])
+@pytest.mark.parametrize("name,case", kat_cases.OTHER_CASES + kat_cases.MAP_CASES,
+                         ids=[n for n, _ in kat_cases.OTHER_CASES + kat_cases.MAP_CASES])
 def test_other_cases(device, name, case):
     case(op2)
+
+
+@pytest.mark.parametrize("split", kat_cases.DIRECT_SPLITS, ids=["all", "nocore", "half", "halo"])
+@pytest.mark.parametrize("case", kat_cases.DIRECT_CASES, ids=lambda c: c.__name__)
+def test_direct_cases(device, case, split):
+    case(op2, split)
+
+
+@pytest.mark.parametrize("split", kat_cases.SPLITS, ids=["all", "nocore", "half"])
+@pytest.mark.parametrize("case", kat_cases.REDUCTION_CASES, ids=lambda c: c.__name__)
+def test_reduction_cases(device, case, split):
+    case(op2, split)
+
+
+@pytest.mark.parametrize("name,dtype,op,start,fill,want", kat_cases.TYPED_REDUCTIONS,
+                         ids=[c[0] for c in kat_cases.TYPED_REDUCTIONS])
+def test_typed_reductions(device, name, dtype, op, start, fill, want):
+    kat_cases.case_reduce_typed(op2, kat_cases.SPLITS[2], dtype, op, start, fill, want)
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+def test_permuted_composed_under_strategies(device, strategy):
+    """INC through permuted and composed maps under both conflict strategies:
+    the device result equals the table-level restatement."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    configuration["strategy"] = strategy
+    cells, mid, nodes = op2.Set(3000), op2.Set(2000), op2.Set(900)
+    base = op2.Map(mid, nodes, 3, rng.integers(0, 900, (2000, 3)).astype(np.int32))
+    c2m = op2.Map(cells, mid, 1, rng.integers(0, 2000, (3000, 1)).astype(np.int32))
+    cm = op2.ComposedMap(op2.PermutedMap(base, [2, 0, 1]), c2m)
+    w = op2.Dat(cells ** 3, rng.random((3000, 3)), np.float64)
+    r = op2.Dat(nodes, np.zeros(900), np.float64)
+    k = op2.Kernel("static void sc(double *r, const double *w) { for (int k = 0; k < 3; ++k) r[k] += w[k]; }", "sc")
+    op2.par_loop(k, cells, r(op2.INC, cm), w(op2.READ))
+    tab = base.values[:, [2, 0, 1]][c2m.values[:, 0]]
+    want = np.zeros(900)
+    np.add.at(want, tab.ravel(), np.asarray(w.data).ravel())
+    assert np.allclose(np.asarray(r.data), want, rtol=1e-12, atol=1e-12)

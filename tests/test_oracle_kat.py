@@ -3,6 +3,7 @@
 Runs every case of ``kat_cases`` through ``oracle_op2`` -- the oracle's
 sequential wrapper (``oracle/seqwrap.py``) behind the op2 call forms -- over
 the reference's core/owned split parametrisation."""
+import numpy as np
 import pytest
 
 import kat_cases
@@ -15,22 +16,59 @@ def test_loop_cases(case, split):
     case(oracle_op2, split)
 
 
-@pytest.mark.parametrize("name,case", kat_cases.OTHER_CASES, ids=[n for n, _ in kat_cases.OTHER_CASES])
+@pytest.mark.parametrize("name,case", kat_cases.OTHER_CASES + kat_cases.MAP_CASES,
+                         ids=[n for n, _ in kat_cases.OTHER_CASES + kat_cases.MAP_CASES])
 def test_other_cases(name, case):
     case(oracle_op2)
 
 
-def test_planted_mistake_is_caught():
-    """A wrapper that forgets to zero INC buffers must fail the INC case."""
-    from oracle import seqwrap
-    orig = seqwrap._generate
+@pytest.mark.parametrize("split", kat_cases.DIRECT_SPLITS, ids=["all", "nocore", "half", "halo"])
+@pytest.mark.parametrize("case", kat_cases.DIRECT_CASES, ids=lambda c: c.__name__)
+def test_direct_cases(case, split):
+    case(oracle_op2, split)
 
-    def broken(code, name, args, ids, subset, headers=()):
-        src = orig(code, name, args, ids, subset, headers)
-        return src.replace("buf0[t_] = 0;", "buf0[t_] = 1;")
-    seqwrap._generate = broken
+
+@pytest.mark.parametrize("split", kat_cases.SPLITS, ids=["all", "nocore", "half"])
+@pytest.mark.parametrize("case", kat_cases.REDUCTION_CASES, ids=lambda c: c.__name__)
+def test_reduction_cases(case, split):
+    case(oracle_op2, split)
+
+
+@pytest.mark.parametrize("name,dtype,op,start,fill,want", kat_cases.TYPED_REDUCTIONS,
+                         ids=[c[0] for c in kat_cases.TYPED_REDUCTIONS])
+def test_typed_reductions(name, dtype, op, start, fill, want):
+    kat_cases.case_reduce_typed(oracle_op2, kat_cases.SPLITS[2], dtype, op, start, fill, want)
+
+
+def test_planted_composition_mistake_is_caught():
+    """Dropping a middle map of the chain must fail the three-map case."""
+    orig = oracle_op2.ComposedMap.table
+
+    def wrong(self):
+        tables = [m.table for m in self.maps_]
+        out = []
+        for n in range(self.iterset.total_size):
+            e = int(tables[-1][n][0])
+            out.append([int(v) for v in tables[0][e]])
+        return np.array(out, dtype=np.int32).reshape(-1, self.arity)
+    oracle_op2.ComposedMap.table = property(wrong)
     try:
         with pytest.raises(AssertionError):
-            kat_cases.case_inc_unitset(oracle_op2, kat_cases.SPLITS[0])
+            kat_cases.case_composed_three(oracle_op2, nested="none")
     finally:
-        seqwrap._generate = orig
+        oracle_op2.ComposedMap.table = orig
+
+
+def test_planted_permutation_mistake_is_caught():
+    """Applying the inverse permutation must fail the permuted-map case."""
+    orig = oracle_op2.PermutedMap.table
+
+    def inverse(self):
+        inv = np.argsort(self.permutation)
+        return np.ascontiguousarray(self.map_.table[:, inv])
+    oracle_op2.PermutedMap.table = property(inverse)
+    try:
+        with pytest.raises(AssertionError):
+            kat_cases.case_permuted_map(oracle_op2)
+    finally:
+        oracle_op2.PermutedMap.table = orig
