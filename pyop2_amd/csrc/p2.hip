@@ -437,6 +437,20 @@ struct MatrixRow {
       emit(j, c, B);
     });
   }
+  // split gather (slot-table launches of the assembly alone): vertex ids,
+  // then every column block of row i from the gathered coordinates
+  static constexpr bool kSplitGather = !kFused;
+  static constexpr int kNv = 4, kGdim = 3;
+  __device__ __forceinline__ void vertices(int32_t e, int32_t (&v)[4]) const {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) v[m] = c2v[4 * e + m];
+  }
+  template <class Emit>
+  __device__ __forceinline__ void row_at(const T (&xv)[4][3], int i, Emit &&emit) const {
+    const Geom<T> g = geometry_of(xv);
+    const RowCoef<T> rc = row_coefficients(g, i, lam, mu);
+    for_blocks<T, 0>(rc, g, [&](int j, const T (&B)[9]) { emit(j, 0, B); });
+  }
   static constexpr int kGroups = 4;
   template <int G, class Emit>
   __device__ __forceinline__ void row_group(int32_t e, int i, Emit &&emit, T *rsum) const {

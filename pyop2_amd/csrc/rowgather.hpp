@@ -500,6 +500,149 @@ struct HalvesOf<Elem, std::void_t<decltype(Elem::kHalves)>> {
   static constexpr int value = Elem::kHalves;
 };
 
+// Elements whose row evaluation splits into a gather of the element's vertex
+// coordinates and arithmetic on them (Elem::kSplitGather = true) provide
+//   static constexpr int kNv, kGdim;          vertices per element, coordinates
+//   const T *X;                                coordinates, kGdim per vertex
+//   void vertices(int32_t e, int32_t (&v)[kNv]) const;
+//   template <class Emit> void row_at(const T (&xv)[kNv][kGdim], int i, Emit &&) const;
+// where row_at emits every column block of row i (column id argument unused).
+template <class Elem, class = void>
+struct SplitGatherOf {
+  static constexpr bool value = false;
+};
+template <class Elem>
+struct SplitGatherOf<Elem, std::void_t<decltype(Elem::kSplitGather)>> {
+  static constexpr bool value = Elem::kSplitGather;
+};
+
+// Slot-table batch of a split-gather element.  The batch's row incidences
+// are taken in rounds of kRound per thread; a round loads the incidence
+// entries, slot entries and vertex ids of all its incidences before the first
+// element is evaluated, so the dependent loads of one incidence do not wait
+// behind the arithmetic of the previous one.  float64 batches start their
+// LDS accumulators from the old matrix values -- all loads of the batch are
+// issued before the first round waits, and the batch is written back from
+// LDS with no load after the element work -- while float32 batches keep the
+// old values of their first kPre * blockDim entries in registers (their LDS
+// accumulators, in float64, start from zero).
+template <class Elem, int kPre>
+__device__ __forceinline__ void slots_batch_split(const BatchDesc &d, int32_t start, int32_t end,
+                                                  const Elem &elem, const op2hip_mat &A,
+                                                  const SlotEntry<Elem::kCar> *__restrict__ table) {
+  using T = typename Elem::T;
+  constexpr int kBs = Elem::kRd * Elem::kCd, kNv = Elem::kNv, kG = Elem::kGdim;
+  constexpr int kRound = 2;
+  constexpr bool kInit = sizeof(T) == 8;
+  constexpr int kInitUnroll = 24;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *__restrict__ vals = static_cast<T *>(A.values) + (size_t)d.z0 * kBs;
+  const int32_t nval = d.nz * kBs;
+  double *acc = reinterpret_cast<double *>(smem);
+  const int32_t bd = (int32_t)blockDim.x;
+
+  int32_t ent[kRound], vv[kRound][kNv];
+  SlotEntry<Elem::kCar> se[kRound];
+  bool ok[kRound];
+  auto load_round = [&](int32_t base) {
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      const int32_t t0 = base + u * bd;
+      ok[u] = t0 < d.ni;
+      const int32_t t = spread_incidence(ok[u] ? t0 : 0, d.ni);
+      ent[u] = ok[u] ? A.inc_ent[d.i0 + t] : 0;
+      if (ok[u]) se[u] = table[d.i0 + t];
+    }
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      const int32_t e = ent[u] / Elem::kRar;
+      ok[u] = ok[u] && e >= start && e < end;
+      if (ok[u]) {
+        elem.vertices(e, vv[u]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < kNv; ++m) vv[u][m] = 0;
+      }
+    }
+  };
+  T old[kInit ? 1 : kPre];
+  if constexpr (!kInit) {
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+      const int32_t k = threadIdx.x + u * bd;
+      old[u] = k < nval ? __builtin_nontemporal_load(&vals[k]) : T(0);
+    }
+  }
+  load_round(threadIdx.x);
+  if constexpr (kInit) {
+    for (int32_t k0 = threadIdx.x; k0 < nval; k0 += kInitUnroll * bd) {
+      T v[kInitUnroll];
+#pragma unroll
+      for (int u = 0; u < kInitUnroll; ++u) {
+        const int32_t k = k0 + u * bd;
+        v[u] = k < nval ? __builtin_nontemporal_load(&vals[k]) : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kInitUnroll; ++u) {
+        const int32_t k = k0 + u * bd;
+        if (k < nval) acc[k] = double(v[u]);
+      }
+    }
+  } else {
+    for (int32_t k = threadIdx.x; k < nval; k += bd) acc[k] = 0.0;
+  }
+  __syncthreads();
+
+  for (int32_t base = threadIdx.x; base < d.ni; base += kRound * bd) {
+    if (base != (int32_t)threadIdx.x) load_round(base);
+#pragma unroll
+    for (int u = 0; u < kRound; ++u) {
+      if (!ok[u]) continue;
+      T xv[kNv][kG];
+#pragma unroll
+      for (int m = 0; m < kNv; ++m)
+#pragma unroll
+        for (int k = 0; k < kG; ++k) xv[m][k] = elem.X[(size_t)kG * vv[u][m] + k];
+      const int32_t e = ent[u] / Elem::kRar;
+      const int i = ent[u] - e * Elem::kRar;
+      double *row = acc + (size_t)se[u].row_off * kBs;
+      elem.row_at(xv, i, [&](int j, int32_t, const T *blk) {
+        const int sl = se[u].slot[j];
+        if (sl == 255) {
+          atomicAdd(A.status, 1);
+          return;
+        }
+#pragma unroll
+        for (int k = 0; k < kBs; ++k) atomic_add(&row[sl * kBs + k], double(blk[k]));
+      });
+    }
+  }
+  __syncthreads();
+
+  if constexpr (kInit) {
+    for (int32_t k = threadIdx.x; k < nval; k += bd) __builtin_nontemporal_store(T(acc[k]), &vals[k]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+      const int32_t k = threadIdx.x + u * bd;
+      if (k < nval) __builtin_nontemporal_store(old[u] + T(acc[k]), &vals[k]);
+    }
+    for (int32_t k0 = threadIdx.x + kPre * bd; k0 < nval; k0 += 8 * bd) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int32_t k = k0 + u * bd;
+        v[u] = k < nval ? vals[k] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int32_t k = k0 + u * bd;
+        if (k < nval) vals[k] = v[u] + T(acc[k]);
+      }
+    }
+  }
+}
+
 // One batch of the slot-table row gather (the workgroup's share of a launch,
 // or one work item of a batched launch).
 template <class Elem, int kPre>
@@ -514,6 +657,10 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
   // the batch's values are read once and written once: non-temporal
   T *__restrict__ vals = static_cast<T *>(A.values) + (size_t)d.z0 * kBs;
   const int32_t nval = d.nz * kBs;
+  if constexpr (SplitGatherOf<Elem>::value && !Elem::kHasRowDat) {
+    slots_batch_split<Elem, kPre>(d, start, end, elem, A, table);
+    return;
+  }
 
   T old[kPre];
 #pragma unroll
@@ -634,15 +781,21 @@ int launch_rowgather_slots(int32_t start, int32_t end, const Elem &elem, const o
   const auto *table = static_cast<const SlotEntry<Elem::kCar> *>(ex->slot_table);
   const int64_t per_thread = std::min<int64_t>(
       ((int64_t)ex->batch_max_nnz * Elem::kRd * Elem::kCd + bs - 1) / bs, MaxPrefetch<Elem>::value);
-  if (per_thread <= 8)
+  if constexpr (SplitGatherOf<Elem>::value && !Elem::kHasRowDat) {
+    (void)per_thread;
     rowgather_slots_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
                                                                    ex->nbatches, table, rowdat);
-  else if (per_thread <= 16)
-    rowgather_slots_kernel<Elem, 16><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
-                                                                    ex->nbatches, table, rowdat);
-  else
-    rowgather_slots_kernel<Elem, 24><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
-                                                                    ex->nbatches, table, rowdat);
+  } else {
+    if (per_thread <= 8)
+      rowgather_slots_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
+                                                                     ex->nbatches, table, rowdat);
+    else if (per_thread <= 16)
+      rowgather_slots_kernel<Elem, 16><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
+                                                                      ex->nbatches, table, rowdat);
+    else
+      rowgather_slots_kernel<Elem, 24><<<ex->nbatches, bs, lds, s>>>(start, end, elem, *A, descs,
+                                                                      ex->nbatches, table, rowdat);
+  }
   OP2HIP_LAUNCHED();
   return OP2HIP_OK;
 }

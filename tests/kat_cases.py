@@ -683,3 +683,105 @@ def case_reduce_repeated(op2, split):
 
 
 REDUCTION_CASES = [case_reduce_1d, case_reduce_2d, case_reduce_repeated]
+
+
+# ---- subsets (test_subset.py:42-258) -----------------------------------------
+NS = 32
+SUBSET_SPLITS = [(NS, NS, NS), (0, NS, NS), (NS // 2, NS, NS)]     # test_subset.py:43-48
+INC1 = "static void inc1(unsigned int *v) { *v += 1u; }"
+
+
+def case_subset_direct(op2, split):
+    """test_subset.py:56-74 -- RW +1 over the even elements touches exactly
+    them; over an empty subset touches nothing."""
+    it = op2.Set(split, "iterset")
+    even = np.arange(0, NS, 2, dtype=np.int32)
+    d = op2.Dat(it ** 1, data=None, dtype=np.uint32)
+    op2.par_loop(op2.Kernel(INC1, "inc1"), op2.Subset(it, even), d(op2.RW))
+    assert list(np.nonzero(np.asarray(d.data))[0]) == list(even)
+    e = op2.Dat(it ** 1, data=None, dtype=np.uint32)
+    op2.par_loop(op2.Kernel(INC1, "inc1"), op2.Subset(it, []), e(op2.RW))
+    assert not np.asarray(e.data).any()
+
+
+def case_subset_complementary(op2, split):
+    """test_subset.py:76-102, 184-207 -- even then odd subsets (built both
+    ways) cover every element once, direct and through a map."""
+    it = op2.Set(split, "iterset")
+    even, odd = np.arange(0, NS, 2, dtype=np.int32), np.arange(1, NS, 2, dtype=np.int32)
+    for sse, sso in ((op2.Subset(it, even), op2.Subset(it, odd)), (it(even), it(odd))):
+        d = op2.Dat(it ** 1, data=None, dtype=np.uint32)
+        op2.par_loop(op2.Kernel(INC1, "inc1"), sse, d(op2.RW))
+        op2.par_loop(op2.Kernel(INC1, "inc1"), sso, d(op2.RW))
+        assert (np.asarray(d.data) == 1).all()
+    ind = op2.Set(NS, "indset")
+    m = op2.Map(it, ind, 1, np.arange(NS, dtype=np.int32))
+    d1 = op2.Dat(it ** 1, data=None, dtype=np.uint32)
+    d2 = op2.Dat(ind ** 1, data=None, dtype=np.uint32)
+    k = op2.Kernel("static void both(unsigned int *a, unsigned int *b) { *a += 1u; *b += 1u; }", "both")
+    op2.par_loop(k, op2.Subset(it, even), d1(op2.RW), d2(op2.INC, m))
+    op2.par_loop(k, op2.Subset(it, odd), d1(op2.RW), d2(op2.INC, m))
+    assert int(np.asarray(d1.data).sum()) == NS and int(np.asarray(d2.data).sum()) == NS
+
+
+def case_subset_nested(op2, split):
+    """test_subset.py:104-136 -- a subset of a subset (every other of the
+    evens) equals the subset of multiples of 4, built both ways."""
+    it = op2.Set(split, "iterset")
+    k = op2.Kernel(INC1, "inc1")
+    ref = op2.Dat(it ** 1, data=None, dtype=np.uint32)
+    op2.par_loop(k, op2.Subset(it, np.arange(0, NS, 4, dtype=np.int32)), ref(op2.RW))
+    inner = np.arange(0, NS // 2, 2, dtype=np.int32)
+    for sss in (op2.Subset(op2.Subset(it, np.arange(0, NS, 2, dtype=np.int32)), inner),
+                it(np.arange(0, NS, 2, dtype=np.int32))(inner)):
+        d = op2.Dat(it ** 1, data=None, dtype=np.uint32)
+        op2.par_loop(k, sss, d(op2.RW))
+        assert (np.asarray(d.data) == np.asarray(ref.data)).all()
+
+
+def case_subset_indirect(op2, split):
+    """test_subset.py:138-182 -- INC through a map over the evens: half the
+    elements hit target 0; empty subset leaves zeros; a direct READ Dat is
+    read at the subset's elements."""
+    it = op2.Set(split, "iterset")
+    even = np.arange(0, NS, 2, dtype=np.int32)
+    ind = op2.Set(2, "indset")
+    m = op2.Map(it, ind, 1, (np.arange(NS) % 2).astype(np.int32))
+    d = op2.Dat(ind ** 1, data=None, dtype=np.uint32)
+    op2.par_loop(op2.Kernel(INC1, "inc1"), op2.Subset(it, even), d(op2.INC, m))
+    assert int(d.data[0]) == NS // 2
+    e = op2.Dat(ind ** 1, data=None, dtype=np.uint32)
+    op2.par_loop(op2.Kernel(INC1, "inc1"), op2.Subset(it, []), e(op2.INC, m))
+    assert not np.asarray(e.data).any()
+    vals = np.full(NS, 2976579765, dtype=np.uint32)      # odd entries must never be read
+    vals[::2] = np.arange(NS)[::2] // 2
+    src = op2.Dat(it ** 1, data=vals, dtype=np.uint32)
+    dst = op2.Dat(ind ** 1, data=None, dtype=np.uint32)
+    op2.par_loop(op2.Kernel("static void add(unsigned int *d, const unsigned int *s) { *d += *s; }", "add"),
+                 op2.Subset(it, even), dst(op2.INC, m), src(op2.READ))
+    assert int(dst.data[0]) == int(vals[::2].astype(np.int64).sum())
+
+
+def case_subset_matrix(op2, split=None):
+    """test_subset.py:209-252 -- Mat INC over the Set equals INC over the
+    subsets [0, 1] and [1, 0] (bitwise: the same additions)."""
+    it, idset, ind = op2.Set(2), op2.Set(2), op2.Set(4)
+    dat = op2.Dat(idset ** 1, data=[0, 1], dtype=np.float64)
+    m = op2.Map(it, ind, 4, [0, 1, 2, 3, 0, 1, 2, 3])
+    idm = op2.Map(it, idset, 1, [0, 1])
+    sp = op2.Sparsity((ind ** 1, ind ** 1), [(m, m, None)])
+    k = op2.Kernel("static void uid(double a[4][4], const double *d)"
+                   " { for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) a[i][j] += d[0] * 16 + i * 4 + j; }",
+                   "uid")
+    out = []
+    for iters in (it, op2.Subset(it, [0, 1]), op2.Subset(it, [1, 0])):
+        A = op2.Mat(sp, np.float64)
+        A.zero()
+        op2.par_loop(k, iters, A(op2.INC, (m, m)), dat(op2.READ, idm))
+        A.assemble()
+        out.append(np.asarray(A.values))
+    want = np.array([[16.0 + 2 * (4 * i + j) for j in range(4)] for i in range(4)])
+    assert (out[0] == want).all() and (out[1] == out[0]).all() and (out[2] == out[0]).all()
+
+
+SUBSET_CASES = [case_subset_direct, case_subset_complementary, case_subset_nested, case_subset_indirect]

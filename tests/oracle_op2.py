@@ -23,14 +23,21 @@ class Set:
     def __pow__(self, dim):
         return DataSet(self, dim)
 
+    def __call__(self, indices):
+        return Subset(self, indices)
+
 
 class Subset(Set):
     """Elements ``indices`` of ``superset``; part sizes count the indices
-    below the superset's part boundaries (``pyop2/types/set.py:338-372``)."""
+    below the superset's part boundaries (``pyop2/types/set.py:394-450``).
+    A subset of a subset is a subset of the outer set."""
 
     def __init__(self, superset, indices):
+        indices = np.unique(np.asarray(indices, dtype=np.int32))
+        if isinstance(superset, Subset):
+            indices, superset = superset.indices[indices], superset.superset
         self.superset = superset
-        self.indices = np.unique(np.asarray(indices, dtype=np.int32))
+        self.indices = indices
         super().__init__(tuple(int((self.indices < b).sum())
                                for b in (superset.core_size, superset.size, superset.total_size)))
 

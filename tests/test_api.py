@@ -31,6 +31,47 @@ def test_subset():
         op2.Subset(s, [11])
 
 
+def test_set_operations():
+    """test_subset.py:261-330: union/intersection/difference/symmetric
+    difference between Sets and Subsets."""
+    a = op2.Set(10)
+    assert a.union(a) is a and a.intersection(a) is a
+    assert a.difference(a)._indices.size == 0 and a.symmetric_difference(a)._indices.size == 0
+    b = op2.Subset(a, np.array([2, 3, 5, 7], dtype=np.int32))
+    assert a.union(b) is a and a.intersection(b) is b
+    assert list(a.difference(b)._indices) == [0, 1, 4, 6, 8, 9]
+    assert list(a.symmetric_difference(b)._indices) == [0, 1, 4, 6, 8, 9]
+    assert b.union(a) is a and b.intersection(a) is b
+    assert b.difference(a)._indices.size == 0
+    assert list(b.symmetric_difference(a)._indices) == [0, 1, 4, 6, 8, 9]
+    c = op2.Subset(a, np.array([2, 4, 6, 8], dtype=np.int32))
+    assert list(b.union(c)._indices) == [2, 3, 4, 5, 6, 7, 8]
+    assert list(b.intersection(c)._indices) == [2]
+    assert list(b.difference(c)._indices) == [3, 5, 7]
+    assert list(b.symmetric_difference(c)._indices) == [3, 4, 5, 6, 7, 8]
+
+
+def test_permuted_composed_map_api():
+    """map.py:171-287: a PermutedMap keeps its base map's values and checks
+    the permutation; a ComposedMap checks its chain and stores no values."""
+    s1, s2, s3 = op2.Set(3), op2.Set(4), op2.Set(5)
+    m = op2.Map(s2, s3, 2, values=[[0, 1], [1, 2], [2, 3], [3, 4]])
+    pm = op2.PermutedMap(m, [1, 0])
+    assert (pm.values == m.values).all() and list(pm.permutation) == [1, 0]
+    assert (pm.iterset, pm.toset, pm.arity) == (s2, s3, 2)
+    with pytest.raises(ValueError):
+        op2.PermutedMap(m, [0, 0])
+    c = op2.ComposedMap(m, op2.Map(s1, s2, 1, values=[3, -1, 0]))
+    assert c.flattened_maps == (m, c.maps_[1])
+    assert c._table.tolist() == [[3, 4], [-1, -1], [0, 1]]      # a -1 anywhere masks the entry
+    with pytest.raises(RuntimeError):
+        c.values_with_halo
+    with pytest.raises(ex.MapTypeError):
+        op2.ComposedMap(m, op2.Map(s1, s3, 1, values=[0, 1, 2]))
+    with pytest.raises(NotImplementedError):
+        op2.PermutedMap(c, [1, 0])
+
+
 def test_dataset_identity_cache():
     s = op2.Set(4)
     assert s ** 2 is s ** 2

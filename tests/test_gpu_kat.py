@@ -61,3 +61,13 @@ def test_permuted_composed_under_strategies(device, strategy):
     want = np.zeros(900)
     np.add.at(want, tab.ravel(), np.asarray(w.data).ravel())
     assert np.allclose(np.asarray(r.data), want, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("split", kat_cases.SUBSET_SPLITS, ids=["all", "nocore", "half"])
+@pytest.mark.parametrize("case", kat_cases.SUBSET_CASES, ids=lambda c: c.__name__)
+def test_subset_cases(device, case, split):
+    case(op2, split)
+
+
+def test_subset_matrix(device):
+    kat_cases.case_subset_matrix(op2)

@@ -72,3 +72,13 @@ def test_planted_permutation_mistake_is_caught():
             kat_cases.case_permuted_map(oracle_op2)
     finally:
         oracle_op2.PermutedMap.table = orig
+
+
+@pytest.mark.parametrize("split", kat_cases.SUBSET_SPLITS, ids=["all", "nocore", "half"])
+@pytest.mark.parametrize("case", kat_cases.SUBSET_CASES, ids=lambda c: c.__name__)
+def test_subset_cases(case, split):
+    case(oracle_op2, split)
+
+
+def test_subset_matrix():
+    kat_cases.case_subset_matrix(oracle_op2)
