@@ -82,6 +82,18 @@ typedef struct op2hip_exec {
   const int32_t *tile_nexcl[2];     /* device, per tile: nodes no other tile has */
   const uint16_t *elem_local[2];    /* device, (end-start) x arity: local node ids */
   int32_t max_tile_nodes[2];        /* largest node count of a tile (sizes LDS) */
+  /* Optional, per staged map with increments: the nodes a tile shares with
+     other tiles go through partial sums instead of device atomics
+     (op2hip_plan_tile_reduce).  Each tile writes its shared nodes'
+     accumulators to partials; a second pass adds, for every shared node, its
+     partials in tile order to the Dat.  NULL tile_shared_off[m]: device
+     atomic adds. */
+  const int32_t *tile_shared_off[2]; /* device, ntiles+1: first partial slot of each tile */
+  int32_t nshared[2];                /* distinct shared nodes */
+  const int32_t *shared_nodes[2];    /* device, nshared node ids */
+  const int32_t *shared_ptr[2];      /* device, nshared+1 offsets into shared_slots */
+  const int32_t *shared_slots[2];    /* device, partial slots of each shared node, tile order */
+  double *partials[2];               /* device workspace, tile_shared_off[m][ntiles] x increments per node */
   /* OP2HIP_ROWGATHER: batches of consecutive block rows */
   int32_t nbatches;
   const int32_t *batch_rows;       /* device, nbatches+1 row boundaries */
@@ -215,6 +227,19 @@ int op2hip_plan_tiles(int32_t start, int32_t end, int32_t nmaps, const int32_t *
                       int32_t node_cap, int32_t *ntiles, int32_t *tile_ptr,
                       int32_t *tile_elems, int32_t *const *node_ptr, int32_t *const *nodes,
                       int32_t *const *nexcl, uint16_t *const *local, int32_t *max_nodes);
+/* Shared-node reduction plan of one staged map (op2hip_exec.tile_shared_off
+ * and the arrays after it), from op2hip_plan_tiles' node_ptr / nodes / nexcl
+ * of that map (host arrays).  The shared nodes of tile b (positions
+ * nexcl[b].. of its node list) get partial slots shared_off[b] + k - nexcl[b];
+ * shared_nodes / shared_ptr / shared_slots list, per distinct shared node in
+ * increasing id order, its slots in tile order.  Capacities: shared_off
+ * ntiles+1; shared_nodes, shared_slots: shared_off[ntiles] (the total of
+ * node_ptr[b+1] - node_ptr[b] - nexcl[b]); shared_ptr one more.  No
+ * counterpart in the reference, whose loop is sequential. */
+int op2hip_plan_tile_reduce(int32_t ntiles, const int32_t *node_ptr, const int32_t *nodes,
+                            const int32_t *nexcl, int32_t nnodes, int32_t *shared_off,
+                            int32_t *nshared, int32_t *shared_nodes, int32_t *shared_ptr,
+                            int32_t *shared_slots);
 /* Row batches for OP2HIP_ROWGATHER: consecutive rows whose incidences and
  * blocks stay within max_inc / max_nnz (a single larger row gets its own
  * batch).  Host arrays; batch_rows has room for nrows+1 entries. */

@@ -28,6 +28,9 @@ class Configuration(dict):
         "strategy": ("PYOP2_AMD_STRATEGY", str, "auto"),
         "autotune": ("PYOP2_AMD_AUTOTUNE", bool, True),
         "device": ("PYOP2_AMD_DEVICE", int, 0),
+        # staged Dat INC loops: nodes shared between tiles through partial
+        # sums and a second pass (False: device atomic adds)
+        "staged_partials": ("PYOP2_AMD_STAGED_PARTIALS", bool, True),
     }
 
     def __init__(self):

@@ -708,6 +708,51 @@ int op2hip_plan_tiles(int32_t start, int32_t end, int32_t nmaps, const int32_t *
   return OP2HIP_OK;
 }
 
+int op2hip_plan_tile_reduce(int32_t ntiles, const int32_t *node_ptr, const int32_t *nodes,
+                            const int32_t *nexcl, int32_t nnodes, int32_t *shared_off,
+                            int32_t *nshared, int32_t *shared_nodes, int32_t *shared_ptr,
+                            int32_t *shared_slots) {
+  OP2HIP_REQUIRE(ntiles >= 0 && nnodes >= 0 && node_ptr && nodes && nexcl && shared_off &&
+                     nshared && shared_nodes && shared_ptr && shared_slots,
+                 "plan_tile_reduce: bad arguments");
+  // partial slots: the shared nodes of each tile, tiles in order
+  shared_off[0] = 0;
+  for (int32_t b = 0; b < ntiles; ++b) {
+    const int32_t cnt = node_ptr[b + 1] - node_ptr[b];
+    OP2HIP_REQUIRE(cnt >= 0 && nexcl[b] >= 0 && nexcl[b] <= cnt,
+                   "plan_tile_reduce: tile %d has %d nodes, %d exclusive", b, cnt, nexcl[b]);
+    shared_off[b + 1] = shared_off[b] + (cnt - nexcl[b]);
+  }
+  const int32_t nslots = shared_off[ntiles];
+  // counting sort of the slots by node id (stable: tile order within a node)
+  std::vector<int32_t> count((size_t)nnodes + 1, 0);
+  for (int32_t b = 0; b < ntiles; ++b)
+    for (int32_t k = node_ptr[b] + nexcl[b]; k < node_ptr[b + 1]; ++k) {
+      const int32_t v = nodes[k];
+      OP2HIP_REQUIRE(v >= 0 && v < nnodes, "plan_tile_reduce: node %d out of range", v);
+      ++count[(size_t)v + 1];
+    }
+  int32_t ns = 0;
+  std::vector<int32_t> first((size_t)nnodes, -1);
+  for (int32_t v = 0; v < nnodes; ++v)
+    if (count[(size_t)v + 1] > 0) {
+      first[v] = ns;
+      shared_nodes[ns] = v;
+      ++ns;
+    }
+  shared_ptr[0] = 0;
+  for (int32_t s = 0; s < ns; ++s) shared_ptr[s + 1] = shared_ptr[s] + count[(size_t)shared_nodes[s] + 1];
+  std::vector<int32_t> fill(shared_ptr, shared_ptr + ns);
+  for (int32_t b = 0; b < ntiles; ++b)
+    for (int32_t k = node_ptr[b] + nexcl[b]; k < node_ptr[b + 1]; ++k) {
+      const int32_t s = first[nodes[k]];
+      shared_slots[fill[s]++] = shared_off[b] + (k - node_ptr[b] - nexcl[b]);
+    }
+  OP2HIP_REQUIRE(shared_ptr[ns] == nslots, "plan_tile_reduce: internal slot count mismatch");
+  *nshared = ns;
+  return OP2HIP_OK;
+}
+
 int op2hip_plan_row_batches(int32_t nrows, const int32_t *inc_ptr, const int32_t *rowptr,
                             int32_t max_inc, int32_t max_nnz, int32_t *batch_rows,
                             int32_t *nbatches, int32_t *max_batch_inc,

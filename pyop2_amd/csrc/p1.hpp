@@ -49,7 +49,7 @@ struct P1MassStaged {
     dst[1] = X[2 * (size_t)v + 1];
   }
   template <int Mp>
-  __device__ __forceinline__ double *out() const { return M; }
+  __host__ __device__ __forceinline__ double *out() const { return M; }
   __device__ __forceinline__ void element(int32_t, const double (&x)[3][2], const double (&)[1][1],
                                           double (&y)[3][1], double (&)[1][1]) const {
     const P1Geom g = p1_geom_of(make_double2(x[0][0], x[0][1]), make_double2(x[1][0], x[1][1]),
@@ -77,7 +77,7 @@ struct P1ResidualStaged {
     dst[2] = u[v];
   }
   template <int Mp>
-  __device__ __forceinline__ double *out() const { return R; }
+  __host__ __device__ __forceinline__ double *out() const { return R; }
   __device__ __forceinline__ void element(int32_t, const double (&x)[3][3], const double (&)[1][1],
                                           double (&y)[3][1], double (&)[1][1]) const {
     const P1Geom g = p1_geom_of(make_double2(x[0][0], x[0][1]), make_double2(x[1][0], x[1][1]),

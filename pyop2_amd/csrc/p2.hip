@@ -603,7 +603,7 @@ struct ResidualStaged {
     dst[2] = src[2];
   }
   template <int M>
-  __device__ __forceinline__ T *out() const { return R; }
+  __host__ __device__ __forceinline__ T *out() const { return R; }
   __device__ __forceinline__ void element(int32_t, const T (&uu)[10][3], const T (&xv)[4][3],
                                           T (&re)[10][3], T (&)[4][1]) const {
     element_residual(uu, geometry_of(xv), lam, mu, re);
@@ -638,7 +638,7 @@ struct AdjointStaged {
     }
   }
   template <int M>
-  __device__ __forceinline__ T *out() const { return M == 0 ? ubar : Xbar; }
+  __host__ __device__ __forceinline__ T *out() const { return M == 0 ? ubar : Xbar; }
   __device__ __forceinline__ void element(int32_t, const T (&ue)[10][6], const T (&xv)[4][3],
                                           T (&ub)[10][3], T (&xb)[4][3]) const;
 };

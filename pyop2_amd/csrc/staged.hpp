@@ -36,6 +36,9 @@ struct TilePlan {
   const int32_t *nexcl[2];
   const uint16_t *local[2];
   int32_t max_nodes[2];
+  // shared-node partials (null shared_off[m]: device atomics)
+  const int32_t *shared_off[2];
+  double *partials[2];
 };
 
 constexpr int at_least_1(int v) { return v > 0 ? v : 1; }
@@ -57,42 +60,115 @@ struct TileLayout {
   }
 };
 
+// Node slots per thread whose ids stay in registers from staging to flush.
+constexpr int kStageSlots = 2;
+
+// Stage map M of tile b: inputs into LDS, and the accumulators start from the
+// old output values of the nodes only this tile has (no other tile of the
+// launch reads or writes them, so reading them here instead of at the flush
+// changes nothing) and from zero for shared nodes.  All loads of a thread's
+// node slots are issued together.  ids[] keeps the node ids of the thread's
+// first kStageSlots slots for the flush.
 template <class Elem, int M>
 __device__ __forceinline__ void stage_map(const Elem &elem, const TilePlan &P, int32_t b,
-                                          typename Elem::T *in, double *acc) {
+                                          typename Elem::T *in, double *acc,
+                                          int32_t (&ids)[kStageSlots]) {
+  using T = typename Elem::T;
   constexpr int kIn = M == 0 ? Elem::kIn0 : Elem::kIn1;
   constexpr int kOut = M == 0 ? Elem::kOut0 : Elem::kOut1;
   const int32_t n0 = P.node_ptr[M][b], nn = P.node_ptr[M][b + 1] - n0;
-  for (int32_t k = threadIdx.x; k < nn; k += blockDim.x) {
-    if constexpr (kIn > 0) elem.template stage<M>(P.nodes[M][n0 + k], in + (size_t)k * kIn);
+  const int32_t ne = kOut > 0 ? P.nexcl[M][b] : 0;
+  const int32_t bd = (int32_t)blockDim.x;
+  const T *out = nullptr;
+  if constexpr (kOut > 0) out = elem.template out<M>();
+  for (int32_t k0 = threadIdx.x; k0 < nn; k0 += kStageSlots * bd) {
+    int32_t id[kStageSlots];
 #pragma unroll
-    for (int a = 0; a < kOut; ++a) acc[(size_t)k * kOut + a] = 0.0;
+    for (int u = 0; u < kStageSlots; ++u) {
+      const int32_t k = k0 + u * bd;
+      id[u] = k < nn ? P.nodes[M][n0 + k] : 0;
+    }
+    if (k0 == (int32_t)threadIdx.x) {
+#pragma unroll
+      for (int u = 0; u < kStageSlots; ++u) ids[u] = id[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kStageSlots; ++u) {
+      const int32_t k = k0 + u * bd;
+      if (k >= nn) continue;
+      if constexpr (kIn > 0) elem.template stage<M>(id[u], in + (size_t)k * kIn);
+#pragma unroll
+      for (int a = 0; a < kOut; ++a)
+        acc[(size_t)k * kOut + a] = k < ne ? double(out[(size_t)id[u] * kOut + a]) : 0.0;
+    }
   }
 }
 
+// Flush map M of tile b: exclusive nodes take their accumulator (which
+// started from the old value); shared nodes write it to their partial slot
+// (contiguous per tile) when the plan has partials, else add it with a
+// device atomic.  No load waits.
 template <class Elem, int M>
 __device__ __forceinline__ void flush_map(const Elem &elem, const TilePlan &P, int32_t b,
-                                          const double *acc) {
+                                          const double *acc, const int32_t (&ids)[kStageSlots]) {
   using T = typename Elem::T;
   constexpr int kOut = M == 0 ? Elem::kOut0 : Elem::kOut1;
   if constexpr (kOut > 0) {
     const int32_t n0 = P.node_ptr[M][b], nn = P.node_ptr[M][b + 1] - n0;
     const int32_t ne = P.nexcl[M][b];
+    const int32_t bd = (int32_t)blockDim.x;
     T *out = elem.template out<M>();
-    for (int32_t k = threadIdx.x; k < nn; k += blockDim.x) {
-      T *p = out + (size_t)P.nodes[M][n0 + k] * kOut;
+    double *part = nullptr;
+    if (P.shared_off[M]) part = P.partials[M] + ((size_t)P.shared_off[M][b] - ne) * kOut;
+    for (int32_t k0 = threadIdx.x; k0 < nn; k0 += kStageSlots * bd) {
 #pragma unroll
-      for (int a = 0; a < kOut; ++a) {
-        const T v = T(acc[(size_t)k * kOut + a]);
-        if (k < ne) p[a] += v;
-        else atomic_add(p + a, v);
+      for (int u = 0; u < kStageSlots; ++u) {
+        const int32_t k = k0 + u * bd;
+        if (k >= nn) continue;
+        if (k >= ne && part) {
+#pragma unroll
+          for (int a = 0; a < kOut; ++a) part[(size_t)k * kOut + a] = acc[(size_t)k * kOut + a];
+          continue;
+        }
+        const int32_t id = k0 == (int32_t)threadIdx.x ? ids[u] : P.nodes[M][n0 + k];
+        T *p = out + (size_t)id * kOut;
+#pragma unroll
+        for (int a = 0; a < kOut; ++a) {
+          const T v = T(acc[(size_t)k * kOut + a]);
+          if (k < ne) p[a] = v;
+          else atomic_add(p + a, v);
+        }
       }
     }
   }
 }
 
+// Second pass of a partials plan: every shared node adds its partials, in
+// tile order, to the Dat (one thread per node; the order of the additions
+// does not depend on scheduling).
+template <typename T, int kOut>
+__global__ void __launch_bounds__(256)
+staged_reduce_kernel(int32_t nshared, const int32_t *__restrict__ nodes,
+                     const int32_t *__restrict__ ptr, const int32_t *__restrict__ slots,
+                     const double *__restrict__ partials, T *__restrict__ out) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nshared) return;
+  double sum[kOut];
+#pragma unroll
+  for (int a = 0; a < kOut; ++a) sum[a] = 0.0;
+  for (int32_t j = ptr[s]; j < ptr[s + 1]; ++j) {
+    const double *p = partials + (size_t)slots[j] * kOut;
+#pragma unroll
+    for (int a = 0; a < kOut; ++a) sum[a] += p[a];
+  }
+  T *o = out + (size_t)nodes[s] * kOut;
+#pragma unroll
+  for (int a = 0; a < kOut; ++a) o[a] = T(double(o[a]) + sum[a]);
+}
+
 // Tile b of a staged launch (the workgroup's share, or one work item of a
-// batched launch).
+// batched launch).  The local node ids of a thread's first element are
+// loaded before the staging barrier.
 template <class Elem>
 __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P, int32_t b) {
   using T = typename Elem::T;
@@ -107,20 +183,27 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
   double *acc1 =
       reinterpret_cast<double *>(smem + (Elem::kMaps > 1 ? L::acc_at(1, P.max_nodes) : 0));
 
-  stage_map<Elem, 0>(elem, P, b, in0, acc0);
-  if constexpr (Elem::kMaps > 1) stage_map<Elem, 1>(elem, P, b, in1, acc1);
-  __syncthreads();
-
   const int32_t p0 = P.tile_ptr[b], ne = P.tile_ptr[b + 1] - p0;
-  for (int32_t t = threadIdx.x; t < ne; t += blockDim.x) {
-    const int32_t pos = p0 + t;
-    uint16_t l0[A0], l1[A1];
+  uint16_t l0[A0], l1[A1];
+  int32_t elem_id = 0;
+  auto load_locals = [&](int32_t pos) {
 #pragma unroll
     for (int i = 0; i < A0; ++i) l0[i] = P.local[0][(size_t)pos * A0 + i];
     if constexpr (Elem::kMaps > 1) {
 #pragma unroll
       for (int i = 0; i < A1; ++i) l1[i] = P.local[1][(size_t)pos * A1 + i];
     }
+    elem_id = P.tile_elems[pos];
+  };
+  if ((int32_t)threadIdx.x < ne) load_locals(p0 + threadIdx.x);
+
+  int32_t ids0[kStageSlots], ids1[kStageSlots];
+  stage_map<Elem, 0>(elem, P, b, in0, acc0, ids0);
+  if constexpr (Elem::kMaps > 1) stage_map<Elem, 1>(elem, P, b, in1, acc1, ids1);
+  __syncthreads();
+
+  for (int32_t t = threadIdx.x; t < ne; t += blockDim.x) {
+    if (t != (int32_t)threadIdx.x) load_locals(p0 + t);
     T x0[A0][I0], x1[A1][I1], y0[A0][O0], y1[A1][O1];
 #pragma unroll
     for (int i = 0; i < A0; ++i)
@@ -131,7 +214,7 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
 #pragma unroll
       for (int a = 0; a < I1; ++a)
         x1[i][a] = (Elem::kMaps > 1 && Elem::kIn1 > 0) ? in1[(size_t)l1[i] * I1 + a] : T(0);
-    elem.element(P.tile_elems[pos], x0, x1, y0, y1);
+    elem.element(elem_id, x0, x1, y0, y1);
     if constexpr (Elem::kOut0 > 0) {
 #pragma unroll
       for (int i = 0; i < A0; ++i)
@@ -147,8 +230,8 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
   }
   __syncthreads();
 
-  flush_map<Elem, 0>(elem, P, b, acc0);
-  if constexpr (Elem::kMaps > 1) flush_map<Elem, 1>(elem, P, b, acc1);
+  flush_map<Elem, 0>(elem, P, b, acc0, ids0);
+  if constexpr (Elem::kMaps > 1) flush_map<Elem, 1>(elem, P, b, acc1, ids1);
 }
 
 template <class Elem>
@@ -168,17 +251,38 @@ int launch_staged(int32_t start, int32_t end, const Elem &elem, const op2hip_exe
                        ex->elem_local[m],
                    "staged launch: tile plan lacks map %d", m);
   if (end <= start || ex->ntiles == 0) return OP2HIP_OK;
+  for (int m = 0; m < Elem::kMaps; ++m)
+    OP2HIP_REQUIRE(!ex->tile_shared_off[m] ||
+                       (ex->partials[m] && ex->shared_nodes[m] && ex->shared_ptr[m] &&
+                        ex->shared_slots[m] && ex->nshared[m] >= 0),
+                   "staged launch: map %d has a partial-slot plan without its arrays", m);
   TilePlan P{ex->ntiles, ex->tile_ptr, ex->tile_elems,
              {ex->tile_node_ptr[0], ex->tile_node_ptr[1]},
              {ex->tile_nodes[0], ex->tile_nodes[1]},
              {ex->tile_nexcl[0], ex->tile_nexcl[1]},
              {ex->elem_local[0], ex->elem_local[1]},
-             {ex->max_tile_nodes[0], Elem::kMaps > 1 ? ex->max_tile_nodes[1] : 0}};
+             {ex->max_tile_nodes[0], Elem::kMaps > 1 ? ex->max_tile_nodes[1] : 0},
+             {Elem::kOut0 > 0 ? ex->tile_shared_off[0] : nullptr,
+              Elem::kMaps > 1 && Elem::kOut1 > 0 ? ex->tile_shared_off[1] : nullptr},
+             {ex->partials[0], Elem::kMaps > 1 ? ex->partials[1] : nullptr}};
   const size_t lds = TileLayout<Elem>::bytes(P.max_nodes);
   OP2HIP_REQUIRE(lds <= 64 * 1024, "tile needs %zu bytes of LDS (max 65536)", lds);
   const int bs = block_size_of(ex, 256);
   OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % 64 == 0, "staged block size %d", bs);
   staged_kernel<Elem><<<ex->ntiles, bs, lds, s>>>(elem, P);
+  OP2HIP_LAUNCHED();
+  if constexpr (Elem::kOut0 > 0) {
+    if (P.shared_off[0] && ex->nshared[0] > 0)
+      staged_reduce_kernel<typename Elem::T, Elem::kOut0><<<(ex->nshared[0] + 255) / 256, 256, 0, s>>>(
+          ex->nshared[0], ex->shared_nodes[0], ex->shared_ptr[0], ex->shared_slots[0],
+          ex->partials[0], elem.template out<0>());
+  }
+  if constexpr (Elem::kMaps > 1 && Elem::kOut1 > 0) {
+    if (P.shared_off[1] && ex->nshared[1] > 0)
+      staged_reduce_kernel<typename Elem::T, Elem::kOut1><<<(ex->nshared[1] + 255) / 256, 256, 0, s>>>(
+          ex->nshared[1], ex->shared_nodes[1], ex->shared_ptr[1], ex->shared_slots[1],
+          ex->partials[1], elem.template out<1>());
+  }
   OP2HIP_LAUNCHED();
   return OP2HIP_OK;
 }

@@ -45,6 +45,12 @@ class ExecPlan(ctypes.Structure):
         ("tile_nexcl", ctypes.c_void_p * 2),
         ("elem_local", ctypes.c_void_p * 2),
         ("max_tile_nodes", ctypes.c_int32 * 2),
+        ("tile_shared_off", ctypes.c_void_p * 2),
+        ("nshared", ctypes.c_int32 * 2),
+        ("shared_nodes", ctypes.c_void_p * 2),
+        ("shared_ptr", ctypes.c_void_p * 2),
+        ("shared_slots", ctypes.c_void_p * 2),
+        ("partials", ctypes.c_void_p * 2),
         ("nbatches", ctypes.c_int32),
         ("batch_rows", ctypes.c_void_p),
         ("batch_max_nnz", ctypes.c_int32),
@@ -115,6 +121,7 @@ _SIGNATURES = {
     "op2hip_batched_p1_laplace": (ctypes.c_int, [_V, _V, _I, _I, _I, _I, _V]),
     "op2hip_plan_tiles": (ctypes.c_int, [_I, _I, _I, _V, _V, _V, _I, _I, _i32p, _V, _V, _V, _V, _V,
                                          _V, _V]),
+    "op2hip_plan_tile_reduce": (ctypes.c_int, [_I, _V, _V, _V, _I, _V, _i32p, _V, _V, _V]),
     "op2hip_plan_row_batches": (ctypes.c_int, [_I, _V, _V, _I, _I, _V, _i32p, _i32p, _i32p, _i32p]),
     "op2hip_slot_table_bytes": (ctypes.c_int, [_I, ctypes.POINTER(ctypes.c_size_t)]),
     "op2hip_build_slot_table": (ctypes.c_int, [_I, _V, _V, _V, _V, _V, _V, _I, _I, _V, _V, _V, _V]),

@@ -5,7 +5,7 @@ cached on the Map (``Map._cache``), the way the reference caches objects
 built on top of Sets and Maps.
 """
 import ctypes
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -267,7 +267,9 @@ def row_batches(owner, tr, rowptr, max_inc, max_nnz, key):
 @dataclass
 class TilePlan:
     """Compact tiles of the elements of [start, end) and, per staged map, each
-    tile's distinct nodes (exclusive ones first) and the elements' local ids."""
+    tile's distinct nodes (exclusive ones first) and the elements' local ids;
+    per map, the shared-node reduction plan (partial slot offsets per tile,
+    distinct shared nodes, their slots) of ``op2hip_plan_tile_reduce``."""
     ntiles: int
     tile_ptr: object
     tile_elems: object
@@ -276,8 +278,27 @@ class TilePlan:
     nexcl: list
     local: list
     max_nodes: list
+    shared_off: list = None
+    nshared: list = None
+    shared_nodes: list = None
+    shared_ptr: list = None
+    shared_slots: list = None
+    nslot_total: list = None
+    workspaces: dict = field(default_factory=dict)
 
-    def exec_fields(self, ex):
+    def partials(self, m, nout):
+        """Device workspace of map m's partial slots (nout float64 each),
+        allocated on first use and kept with the plan."""
+        buf = self.workspaces.get((m, nout))
+        if buf is None:
+            buf = dev.zeros((max(int(self.nslot_total[m]) * nout, 1),), np.float64)
+            self.workspaces[(m, nout)] = buf
+        return buf
+
+    def exec_fields(self, ex, nouts=None):
+        """Fill the tile fields of ``ex``; with ``nouts`` (increments per node
+        of each staged map), also the partial-slot plan of the maps that have
+        increments."""
         ex.ntiles = self.ntiles
         ex.tile_ptr = dev.ptr(self.tile_ptr)
         ex.tile_elems = dev.ptr(self.tile_elems)
@@ -287,6 +308,32 @@ class TilePlan:
             ex.tile_nexcl[m] = dev.ptr(self.nexcl[m])
             ex.elem_local[m] = dev.ptr(self.local[m])
             ex.max_tile_nodes[m] = self.max_nodes[m]
+            ex.tile_shared_off[m] = None
+            if nouts is not None and nouts[m] > 0 and self.shared_off is not None:
+                ex.tile_shared_off[m] = dev.ptr(self.shared_off[m])
+                ex.nshared[m] = self.nshared[m]
+                ex.shared_nodes[m] = dev.ptr(self.shared_nodes[m])
+                ex.shared_ptr[m] = dev.ptr(self.shared_ptr[m])
+                ex.shared_slots[m] = dev.ptr(self.shared_slots[m])
+                ex.partials[m] = dev.ptr(self.partials(m, nouts[m]))
+
+
+def tile_reduce_plan(L, ntiles, node_ptr, nodes, nexcl, nnodes):
+    """Host arrays of ``op2hip_plan_tile_reduce`` for one staged map:
+    (shared_off, nshared, shared_nodes, shared_ptr, shared_slots)."""
+    cnt = np.diff(node_ptr[:ntiles + 1]).astype(np.int64) - nexcl[:ntiles]
+    nslots = int(cnt.sum()) if ntiles else 0
+    off = np.empty(ntiles + 1, dtype=np.int32)
+    sn = np.empty(max(nslots, 1), dtype=np.int32)
+    sp = np.empty(nslots + 1, dtype=np.int32)
+    ss = np.empty(max(nslots, 1), dtype=np.int32)
+    ns = ctypes.c_int32()
+    dev.check(L.op2hip_plan_tile_reduce(int(ntiles), node_ptr.ctypes.data, nodes.ctypes.data,
+                                        nexcl.ctypes.data, int(nnodes), off.ctypes.data,
+                                        ctypes.byref(ns), sn.ctypes.data, sp.ctypes.data,
+                                        ss.ctypes.data), "plan_tile_reduce")
+    ns = ns.value
+    return off, ns, sn[:max(ns, 1)], sp[:ns + 1], ss, nslots
 
 
 def tile_plan(maps, start, end, tile_size, node_cap=0):
@@ -320,11 +367,15 @@ def tile_plan(maps, start, end, tile_size, node_cap=0):
                                   te.ctypes.data, arr(nps), arr(nds), arr(nxs), arr(lcs), mx.ctypes.data),
               "plan_tiles")
     nt = nt.value
+    red = [tile_reduce_plan(L, nt, nps[k], nds[k], nxs[k], ts[k]) for k in range(nm)]
     plan = TilePlan(nt, dev.to_device(tp[:nt + 1]), dev.to_device(te),
                     [dev.to_device(x[:nt + 1]) for x in nps],
                     [dev.to_device(nds[k][:max(int(nps[k][nt]), 1)]) for k in range(nm)],
                     [dev.to_device(x[:max(nt, 1)]) for x in nxs],
-                    [dev.to_device(x) for x in lcs], [int(v) for v in mx[:nm]])
+                    [dev.to_device(x) for x in lcs], [int(v) for v in mx[:nm]],
+                    [dev.to_device(r[0]) for r in red], [r[1] for r in red],
+                    [dev.to_device(r[2]) for r in red], [dev.to_device(r[3]) for r in red],
+                    [dev.to_device(r[4]) for r in red], [r[5] for r in red])
     owner._cache[ck] = plan
     return plan
 

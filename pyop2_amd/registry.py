@@ -295,7 +295,8 @@ class HandWrittenKernel:
                 strategy = "atomic"
                 ex.strategy = dev.STRATEGIES[strategy]
             else:
-                tp.exec_fields(ex)
+                nouts = [nout for _, nout in self.spec["staged_vals"]]
+                tp.exec_fields(ex, nouts if configuration["staged_partials"] else None)
                 keep.append(tp)
         if self.spec["args"][0][0] == "mat":
             mat = args[0].data

@@ -6,6 +6,14 @@ from oracle import seqwrap, sparsity as osp
 from pyop2_amd import elements, op2
 
 
+def use_strategy(name):
+    """Select a par_loop strategy; "staged_atomic" is the staged strategy with
+    device atomics for the nodes tiles share (no partial-sum pass)."""
+    from pyop2_amd.configuration import configuration
+    configuration["staged_partials"] = name != "staged_atomic"
+    configuration["strategy"] = "staged" if name == "staged_atomic" else name
+
+
 def relerr(a, b):
     """Normwise relative error ||a - b|| / ||b||."""
     a = np.asarray(a, dtype=np.float64).ravel()

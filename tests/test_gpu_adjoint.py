@@ -12,7 +12,7 @@
 import numpy as np
 import pytest
 
-from helpers import TetProblem, relerr
+from helpers import TetProblem, relerr, use_strategy
 from oracle import seqwrap
 from pyop2_amd import adjoint, device as dev, elements, op2
 from pyop2_amd.configuration import configuration
@@ -27,9 +27,9 @@ def _jitter(m, rng, amp=0.1):
     return np.ascontiguousarray(X)
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged"])
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic"])
 def test_adjoint_loop_matches_oracle(device, strategy):
-    configuration["strategy"] = strategy
+    use_strategy(strategy)
     m = unit_cube_tet_p2(6, rng=workload_rng(4))
     rng = np.random.default_rng([2026, 4, 1])
     X = _jitter(m, rng)

@@ -7,7 +7,7 @@ reduced size (parity) and at full size (pattern count and row-sum property).
 import numpy as np
 import pytest
 
-from helpers import TriProblem, oracle_p1, relerr
+from helpers import TriProblem, oracle_p1, relerr, use_strategy
 from oracle import sparsity as osp
 from pyop2_amd import elements, op2
 from pyop2_amd.configuration import configuration
@@ -23,12 +23,12 @@ def mesh256():
     return unit_square_tri(256, rng=workload_rng(1))
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "auto"])
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic", "auto"])
 @pytest.mark.parametrize("split", ["all", "nocore", "half"])
 def test_p1_residual_and_mass(device, mesh256, strategy, split):
     n = mesh256.ncells
     core = {"all": None, "nocore": 0, "half": n // 2}[split]
-    configuration["strategy"] = strategy
+    use_strategy(strategy)
     p = TriProblem(mesh256, core=core)
     R = op2.Dat(p.verts, name="R")
     M = op2.Dat(p.verts, name="M")

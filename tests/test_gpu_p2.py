@@ -6,7 +6,7 @@ reference Mat is FP64-only, so FP32 has no reference counterpart).
 import numpy as np
 import pytest
 
-from helpers import TetProblem, oracle_p2, relerr
+from helpers import TetProblem, oracle_p2, relerr, use_strategy
 from pyop2_amd import elements, op2
 from pyop2_amd.configuration import configuration
 from pyop2_amd.meshes import unit_cube_tet_p2, workload_rng
@@ -19,10 +19,10 @@ def tet8():
     return unit_cube_tet_p2(8, rng=workload_rng(3))
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged"])
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic"])
 @pytest.mark.parametrize("split", ["all", "half"])
 def test_p2_residual_f64(device, tet8, strategy, split):
-    configuration["strategy"] = strategy
+    use_strategy(strategy)
     core = None if split == "all" else tet8.ncells // 2
     p = TetProblem(tet8, core=core)
     R = op2.Dat(p.nodes ** 3, name="R")
@@ -31,9 +31,9 @@ def test_p2_residual_f64(device, tet8, strategy, split):
     assert relerr(R.data_ro, oracle_p2("p2_elasticity_residual", tet8)) < 1e-12
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "staged"])
+@pytest.mark.parametrize("strategy", ["atomic", "staged", "staged_atomic"])
 def test_p2_residual_f32(device, tet8, strategy):
-    configuration["strategy"] = strategy
+    use_strategy(strategy)
     p = TetProblem(tet8, dtype=np.float32)
     R = op2.Dat(p.nodes ** 3, dtype=np.float32, name="R")
     op2.par_loop(elements.p2_elasticity_residual(), p.cells, R(op2.INC, p.c2n), p.X(op2.READ, p.c2v),

@@ -136,6 +136,32 @@ def test_tiles_partition_elements_and_renumber_nodes(start, end, tile, cap):
             assert all(flag == (len(occ) == 1) for _, flag in occ)
 
 
+@pytest.mark.parametrize("start,end,tile", [(0, 3072, 256), (7, 1500, 96)])
+def test_tile_reduce_plan_lists_shared_slots_in_tile_order(start, end, tile):
+    """Shared-node reduction plan: tile b's shared nodes (positions nexcl..)
+    get consecutive slots from shared_off[b]; each shared node lists exactly
+    the slots of its occurrences, in tile order; nodes in increasing order.
+    Checked against a dictionary built from the tile lists."""
+    from pyop2_amd import plans
+    m = unit_cube_tet_p2(8, rng=workload_rng(3))
+    c2n = np.ascontiguousarray(m.cell_node, np.int32)
+    nt, tp, te, nps, nds, nxs, lcs, mx = _tiles([c2n], [m.nnodes], start, end, tile)
+    off, ns, sn, sp, ss, nslots = plans.tile_reduce_plan(dev.lib(), nt, nps[0], nds[0], nxs[0], m.nnodes)
+    want = {}
+    slot = 0
+    for b in range(nt):
+        assert off[b] == slot
+        ids = nds[0][nps[0][b]:nps[0][b + 1]]
+        for v in ids[nxs[0][b]:]:
+            want.setdefault(int(v), []).append(slot)
+            slot += 1
+    assert off[nt] == slot == nslots and ns == len(want)
+    assert list(sn[:ns]) == sorted(want)
+    for s_, v in enumerate(sn[:ns]):
+        assert list(ss[sp[s_]:sp[s_ + 1]]) == want[int(v)]
+        assert len(want[int(v)]) >= 2            # a shared node is in two tiles at least
+
+
 def test_tiles_are_compact_regions():
     """On a structured mesh the grown regions touch far fewer nodes than
     runs of consecutive elements (a strip of the numbering)."""
