@@ -516,23 +516,98 @@ struct SplitGatherOf<Elem, std::void_t<decltype(Elem::kSplitGather)>> {
   static constexpr bool value = Elem::kSplitGather;
 };
 
-// Slot-table batch of a split-gather element.  The batch's row incidences
-// are taken in rounds of kRound per thread; a round loads the incidence
-// entries, slot entries and vertex ids of all its incidences before the first
-// element is evaluated, so the dependent loads of one incidence do not wait
-// behind the arithmetic of the previous one.  float64 batches start their
-// LDS accumulators from the old matrix values -- all loads of the batch are
-// issued before the first round waits, and the batch is written back from
-// LDS with no load after the element work -- while float32 batches keep the
-// old values of their first kPre * blockDim entries in registers (their LDS
-// accumulators, in float64, start from zero).
+// Row incidences per thread and round of a split-gather element
+// (Elem::kRound, default 2).
+template <class Elem, class = void>
+struct RoundOf {
+  static constexpr int value = 2;
+};
+template <class Elem>
+struct RoundOf<Elem, std::void_t<decltype(Elem::kRound)>> {
+  static constexpr int value = Elem::kRound;
+};
+
+// One round of a split-gather batch: kR row incidences per thread.  load()
+// issues the incidence entries, slot entries and vertex ids of all of them
+// before any element is evaluated, so the dependent loads of one incidence do
+// not wait behind the arithmetic of the previous one; eval() gathers each
+// incidence's coordinates, evaluates its block row and adds it into the
+// batch's LDS accumulators.
+template <class Elem>
+struct SplitRound {
+  using T = typename Elem::T;
+  static constexpr int kR = RoundOf<Elem>::value;
+  static constexpr int kBs = Elem::kRd * Elem::kCd;
+  int32_t ent[kR], vv[kR][Elem::kNv];
+  SlotEntry<Elem::kCar> se[kR];
+  bool ok[kR];
+
+  __device__ __forceinline__ void load(const Elem &elem, int32_t base, int32_t ni,
+                                       const int32_t *__restrict__ ents,
+                                       const SlotEntry<Elem::kCar> *__restrict__ table,
+                                       int32_t start, int32_t end) {
+    const int32_t bd = (int32_t)blockDim.x;
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int32_t t0 = base + u * bd;
+      ok[u] = t0 < ni;
+      const int32_t t = SpreadOf<Elem>::value ? spread_incidence(ok[u] ? t0 : 0, ni) : t0;
+      ent[u] = ok[u] ? ents[t] : 0;
+      if (ok[u]) se[u] = table[t];
+    }
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int32_t e = ent[u] / Elem::kRar;
+      ok[u] = ok[u] && e >= start && e < end;
+      if (ok[u]) {
+        elem.vertices(e, vv[u]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < Elem::kNv; ++m) vv[u][m] = 0;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void eval(const Elem &elem, double *acc, int32_t *status) const {
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      if (!ok[u]) continue;
+      T xv[Elem::kNv][Elem::kGdim];
+#pragma unroll
+      for (int m = 0; m < Elem::kNv; ++m)
+#pragma unroll
+        for (int k = 0; k < Elem::kGdim; ++k)
+          xv[m][k] = elem.X[(size_t)Elem::kGdim * vv[u][m] + k];
+      const int32_t e = ent[u] / Elem::kRar;
+      const int i = ent[u] - e * Elem::kRar;
+      double *row = acc + (size_t)se[u].row_off * kBs;
+      const SlotEntry<Elem::kCar> &s = se[u];
+      elem.row_at(xv, i, [&](int j, int32_t, const T *blk) {
+        const int sl = s.slot[j];
+        if (sl == 255) {
+          atomicAdd(status, 1);
+          return;
+        }
+#pragma unroll
+        for (int k = 0; k < kBs; ++k) atomic_add(&row[sl * kBs + k], double(blk[k]));
+      });
+    }
+  }
+};
+
+// Slot-table batch of a split-gather element, in rounds of SplitRound.
+// float64 batches start their LDS accumulators from the old matrix values
+// -- all loads of the batch are issued before the first round waits, and the
+// batch is written back from LDS with no load after the element work --
+// while float32 batches keep the old values of their first kPre * blockDim
+// entries in registers (their LDS accumulators, in float64, start from zero).
 template <class Elem, int kPre>
 __device__ __forceinline__ void slots_batch_split(const BatchDesc &d, int32_t start, int32_t end,
                                                   const Elem &elem, const op2hip_mat &A,
                                                   const SlotEntry<Elem::kCar> *__restrict__ table) {
   using T = typename Elem::T;
-  constexpr int kBs = Elem::kRd * Elem::kCd, kNv = Elem::kNv, kG = Elem::kGdim;
-  constexpr int kRound = 2;
+  using R = SplitRound<Elem>;
+  constexpr int kBs = R::kBs;
   constexpr bool kInit = sizeof(T) == 8;
   constexpr int kInitUnroll = 24;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -541,30 +616,7 @@ __device__ __forceinline__ void slots_batch_split(const BatchDesc &d, int32_t st
   double *acc = reinterpret_cast<double *>(smem);
   const int32_t bd = (int32_t)blockDim.x;
 
-  int32_t ent[kRound], vv[kRound][kNv];
-  SlotEntry<Elem::kCar> se[kRound];
-  bool ok[kRound];
-  auto load_round = [&](int32_t base) {
-#pragma unroll
-    for (int u = 0; u < kRound; ++u) {
-      const int32_t t0 = base + u * bd;
-      ok[u] = t0 < d.ni;
-      const int32_t t = spread_incidence(ok[u] ? t0 : 0, d.ni);
-      ent[u] = ok[u] ? A.inc_ent[d.i0 + t] : 0;
-      if (ok[u]) se[u] = table[d.i0 + t];
-    }
-#pragma unroll
-    for (int u = 0; u < kRound; ++u) {
-      const int32_t e = ent[u] / Elem::kRar;
-      ok[u] = ok[u] && e >= start && e < end;
-      if (ok[u]) {
-        elem.vertices(e, vv[u]);
-      } else {
-#pragma unroll
-        for (int m = 0; m < kNv; ++m) vv[u][m] = 0;
-      }
-    }
-  };
+  R round;
   T old[kInit ? 1 : kPre];
   if constexpr (!kInit) {
 #pragma unroll
@@ -573,7 +625,7 @@ __device__ __forceinline__ void slots_batch_split(const BatchDesc &d, int32_t st
       old[u] = k < nval ? __builtin_nontemporal_load(&vals[k]) : T(0);
     }
   }
-  load_round(threadIdx.x);
+  round.load(elem, threadIdx.x, d.ni, A.inc_ent + d.i0, table + d.i0, start, end);
   if constexpr (kInit) {
     for (int32_t k0 = threadIdx.x; k0 < nval; k0 += kInitUnroll * bd) {
       T v[kInitUnroll];
@@ -593,29 +645,10 @@ __device__ __forceinline__ void slots_batch_split(const BatchDesc &d, int32_t st
   }
   __syncthreads();
 
-  for (int32_t base = threadIdx.x; base < d.ni; base += kRound * bd) {
-    if (base != (int32_t)threadIdx.x) load_round(base);
-#pragma unroll
-    for (int u = 0; u < kRound; ++u) {
-      if (!ok[u]) continue;
-      T xv[kNv][kG];
-#pragma unroll
-      for (int m = 0; m < kNv; ++m)
-#pragma unroll
-        for (int k = 0; k < kG; ++k) xv[m][k] = elem.X[(size_t)kG * vv[u][m] + k];
-      const int32_t e = ent[u] / Elem::kRar;
-      const int i = ent[u] - e * Elem::kRar;
-      double *row = acc + (size_t)se[u].row_off * kBs;
-      elem.row_at(xv, i, [&](int j, int32_t, const T *blk) {
-        const int sl = se[u].slot[j];
-        if (sl == 255) {
-          atomicAdd(A.status, 1);
-          return;
-        }
-#pragma unroll
-        for (int k = 0; k < kBs; ++k) atomic_add(&row[sl * kBs + k], double(blk[k]));
-      });
-    }
+  for (int32_t base = threadIdx.x; base < d.ni; base += R::kR * bd) {
+    if (base != (int32_t)threadIdx.x)
+      round.load(elem, base, d.ni, A.inc_ent + d.i0, table + d.i0, start, end);
+    round.eval(elem, acc, A.status);
   }
   __syncthreads();
 

@@ -63,9 +63,12 @@ def test_p2_fused_f64(device, tet8, strategy):
     p = TetProblem(tet8)
     A = op2.Mat(p.sparsity(), np.float64)
     R = op2.Dat(p.nodes ** 3, name="R")
-    op2.par_loop(elements.p2_elasticity_fused(), p.cells, A(op2.INC, (p.c2n, p.c2n)), R(op2.INC, p.c2n),
-                 p.X(op2.READ, p.c2v), p.u(op2.READ, p.c2n))
+    pl = op2.ParLoop(elements.p2_elasticity_fused(), p.cells, A(op2.INC, (p.c2n, p.c2n)), R(op2.INC, p.c2n),
+                     p.X(op2.READ, p.c2v), p.u(op2.READ, p.c2n))
+    pl.compute()
     A.assemble()
+    if strategy == "split":
+        assert pl.global_kernel.compile().strategy_used == "split"
     ref = oracle_p2("p2_elasticity_matrix", tet8)
     _, _, v = A._host_csr()
     assert relerr(v, ref.values) < 1e-12
