@@ -285,3 +285,13 @@ def test_generated_wrappers_compile_for_gfx950():
         rc = L.op2hip_jit_code_object(src.encode(), copts, len(opts), None, ctypes.byref(size), log, len(log))
         assert rc == 0, log.value.decode()
         assert size.value > 1000
+
+
+def test_integration_stub_struct_matches_the_abi():
+    """The ctypes op2hip_exec in the integration guide's binding stub lists
+    the fields of the C struct in order (the same list the package binds)."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "INTEGRATION.md")
+    text = open(path).read()
+    stub = text[text.index("class op2hip_exec"):text.index("_f = _lib")]
+    assert re.findall(r'\("(\w+)"', stub) == [f[0] for f in dev.ExecPlan._fields_]
