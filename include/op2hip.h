@@ -117,6 +117,8 @@ typedef struct op2hip_exec {
      then index the batch-ordered incidences) */
   const int32_t *row_order;        /* device, (row, block offset in batch) in processing order */
   const int32_t *batch_ent;        /* device, per batch incidence: element * arity + i */
+  const int32_t *row_base;         /* device, optional: rowptr[row] per row_order entry
+                                      (NULL: read from the Mat's rowptr) */
   /* Subset iteration: element n = subset[i] for i in [start,end) (or NULL) */
   const int32_t *subset;           /* device */
 } op2hip_exec;

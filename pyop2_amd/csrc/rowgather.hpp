@@ -1316,7 +1316,7 @@ rowgather_rowlist_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
                          const BatchDesc *__restrict__ descs, int32_t nbatches,
                          const int2 *__restrict__ row_order, const int32_t *__restrict__ bent,
                          const SlotEntry<Elem::kCar> *__restrict__ table,
-                         typename Elem::T *__restrict__ rowdat) {
+                         const int32_t *__restrict__ rbase, typename Elem::T *__restrict__ rowdat) {
   using T = typename Elem::T;
   using Acc = typename AccumOf<T>::type;
   constexpr int kBs = Elem::kRd * Elem::kCd;
@@ -1330,10 +1330,11 @@ rowgather_rowlist_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
   int32_t *gbase = seg + d.nr + 1;
   int32_t *rows = gbase + d.nr + 1;
 
+  // the plan's row bases are contiguous per batch (no scattered rowptr read)
   for (int32_t lr = threadIdx.x; lr < d.nr; lr += blockDim.x) {
     const int2 p = row_order[d.r0 + lr];
     seg[lr] = p.y;
-    gbase[lr] = A.rowptr[p.x];
+    gbase[lr] = rbase ? rbase[d.r0 + lr] : A.rowptr[p.x];
     rows[lr] = p.x;
   }
   if (threadIdx.x == 0) seg[d.nr] = d.nz;
@@ -1417,10 +1418,10 @@ int launch_rowgather_rowlist(int32_t start, int32_t end, const Elem &elem, const
   const int64_t per_thread = ((int64_t)ex->batch_max_nnz * Elem::kRd * Elem::kCd + bs - 1) / bs;
   if (per_thread <= 8)
     rowgather_rowlist_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(
-        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, rowdat);
+        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base, rowdat);
   else
     rowgather_rowlist_kernel<Elem, 16><<<ex->nbatches, bs, lds, s>>>(
-        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, rowdat);
+        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base, rowdat);
   OP2HIP_LAUNCHED();
   return OP2HIP_OK;
 }

@@ -69,6 +69,7 @@ class ExecPlan(ctypes.Structure):
         ("max_batch_contrib", ctypes.c_int32),
         ("row_order", ctypes.c_void_p),
         ("batch_ent", ctypes.c_void_p),
+        ("row_base", ctypes.c_void_p),
         ("subset", ctypes.c_void_p),
     ]
 

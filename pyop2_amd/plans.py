@@ -391,12 +391,14 @@ class RowList:
     max_inc: int
     max_nnz: int
     max_rows: int
+    row_base: object = None   # device, rowptr[row] per row_order entry
 
     def exec_fields(self, ex):
         ex.nbatches = self.nbatches
         ex.batch_desc = dev.ptr(self.desc)
         ex.row_order = dev.ptr(self.row_order)
         ex.batch_ent = dev.ptr(self.ent)
+        ex.row_base = dev.ptr(self.row_base) if self.row_base is not None else None
         ex.slot_table = dev.ptr(self.table)
         ex.batch_max_inc = self.max_inc
         ex.batch_max_nnz = self.max_nnz
@@ -434,7 +436,9 @@ def row_list(rmap, cmap, tr, mat, max_inc, max_nnz, key):
     if missing.value:
         from pyop2_amd.exceptions import PatternError
         raise PatternError(f"{missing.value} map entries are not in the sparsity pattern")
+    base = rowptr[order[0:2 * nrows:2]] if nrows else np.zeros(1, dtype=np.int32)
     plan = RowList(nb.value, dev.to_device(desc[:max(nb.value, 1)]), dev.to_device(order),
-                   dev.to_device(ent), dev.to_device(table), mi.value, mn.value, mr.value)
+                   dev.to_device(ent), dev.to_device(table), mi.value, mn.value, mr.value,
+                   dev.to_device(np.ascontiguousarray(base, dtype=np.int32)))
     rmap._cache[ck] = plan
     return plan
