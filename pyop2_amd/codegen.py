@@ -36,7 +36,8 @@ from pyop2_amd import device as dev
 from pyop2_amd.configuration import configuration
 from pyop2_amd.datatypes import as_cstr
 from pyop2_amd.exceptions import CompilationError
-from pyop2_amd.global_kernel import DatKernelArg, GlobalKernelArg, MatKernelArg
+from pyop2_amd.global_kernel import (DatKernelArg, GlobalKernelArg, MatKernelArg, MixedDatKernelArg,
+                                     MixedMatKernelArg)
 from pyop2_amd.types.access import INC, MAX, MIN, READ, RW, WRITE
 from pyop2_amd.types.set import Subset
 
@@ -155,6 +156,18 @@ def _strip_headers(text):
     return _STD_HEADER.sub("", text)
 
 
+def _dat_store(acc, tgt, val):
+    """Statement storing a Dat pack entry ``val`` into ``tgt`` for ``acc``
+    (atomic when the launch is not coloured)."""
+    if acc == INC:
+        return f"if (atomic) op2_atomic_add(&{tgt}, {val}); else {tgt} += {val};"
+    if acc == MIN:
+        return f"if (atomic) op2_atomic_min(&{tgt}, {val}); else {tgt} = op2_min({tgt}, {val});"
+    if acc == MAX:
+        return f"if (atomic) op2_atomic_max(&{tgt}, {val}); else {tgt} = op2_max({tgt}, {val});"
+    return f"{tgt} = {val};"
+
+
 def generate_source(gk, map_index, block_flags=None):
     """HIP source of the wrapper kernel ``op2_wrap`` for GlobalKernel ``gk``.
 
@@ -193,16 +206,8 @@ def generate_source(gk, map_index, block_flags=None):
                 continue
             tgt = f"arg{k}[(int64_t)mi{k}[i] * {dim} + j]"
             val = f"buf{k}[i * {dim} + j]"
-            if acc == INC:
-                op = f"if (atomic) op2_atomic_add(&{tgt}, {val}); else {tgt} += {val};"
-            elif acc == MIN:
-                op = f"if (atomic) op2_atomic_min(&{tgt}, {val}); else {tgt} = op2_min({tgt}, {val});"
-            elif acc == MAX:
-                op = f"if (atomic) op2_atomic_max(&{tgt}, {val}); else {tgt} = op2_max({tgt}, {val});"
-            else:
-                op = f"{tgt} = {val};"
             post.append(f"    for (int i = 0; i < {ar}; ++i) {{ if (mi{k}[i] < 0) continue;"
-                        f" for (int j = 0; j < {dim}; ++j) {{ {op} }} }}")
+                        f" for (int j = 0; j < {dim}; ++j) {{ {_dat_store(acc, tgt, val)} }} }}")
         elif isinstance(garg, GlobalKernelArg):
             dim = int(np.prod(garg.dim)) if len(garg.dim) else 1
             params.append(f"{t} *__restrict__ arg{k}")
@@ -223,6 +228,71 @@ def generate_source(gk, map_index, block_flags=None):
             aop = {INC: "op2_atomic_add", MIN: "op2_atomic_min", MAX: "op2_atomic_max"}[acc]
             red.append(("post", f"  for (int q = 0; q < {dim}; ++q) {{ {t} v = op2_wave_{fn}(buf{k}[q]);"
                                 f" if (lane == 0) {aop}(&arg{k}[q], v); }}"))
+        elif isinstance(garg, MixedDatKernelArg):
+            # one buffer: the packs of the components one after the other
+            comps = garg.arguments
+            total = sum(c.map_.arity * int(np.prod(c.dim)) for c in comps)
+            pre.append(f"    {t} buf{k}[{total}];")
+            if acc in (INC, WRITE):
+                pre.append(f"    for (int q = 0; q < {total}; ++q) buf{k}[q] = ({t})0;")
+            off = 0
+            for c, comp in enumerate(comps):
+                dim, ar, m = int(np.prod(comp.dim)), comp.map_.arity, map_index[(k, c)]
+                a_, mi_ = f"arg{k}_{c}", f"mi{k}_{c}"
+                params.append(f"{t} *__restrict__ {a_}")
+                pre.append(f"    int32_t {mi_}[{ar}];")
+                pre.append(f"    for (int i = 0; i < {ar}; ++i) {mi_}[i] = map{m}[(int64_t)n * {ar} + i];")
+                if acc not in (INC, WRITE):
+                    pre.append(f"    for (int i = 0; i < {ar}; ++i) for (int j = 0; j < {dim}; ++j)"
+                               f" buf{k}[{off} + i * {dim} + j] = {mi_}[i] >= 0 ?"
+                               f" {a_}[(int64_t){mi_}[i] * {dim} + j] : ({t})0;")
+                if acc != READ:
+                    tgt = f"{a_}[(int64_t){mi_}[i] * {dim} + j]"
+                    val = f"buf{k}[{off} + i * {dim} + j]"
+                    post.append(f"    for (int i = 0; i < {ar}; ++i) {{ if ({mi_}[i] < 0) continue;"
+                                f" for (int j = 0; j < {dim}; ++j) {{ {_dat_store(acc, tgt, val)} }} }}")
+                off += ar * dim
+            call.append(f"op2_ptr_{{(void *)buf{k}}}")
+        elif isinstance(garg, MixedMatKernelArg):
+            # one element matrix; block (i, j) is rows of row component i
+            # times columns of column component j, added into block Mat (i, j)
+            nr, nc = garg.shape
+            blocks = garg.arguments
+            rds = [blocks[i * nc].dims[0][0][0] for i in range(nr)]
+            cds = [blocks[j].dims[0][0][1] for j in range(nc)]
+            rars = [blocks[i * nc].maps[0].arity for i in range(nr)]
+            cars = [blocks[j].maps[1].arity for j in range(nc)]
+            roff = [sum(rars[q] * rds[q] for q in range(i)) for i in range(nr)]
+            coff = [sum(cars[q] * cds[q] for q in range(j)) for j in range(nc)]
+            NR, NC = roff[-1] + rars[-1] * rds[-1], coff[-1] + cars[-1] * cds[-1]
+            pre.append(f"    {t} buf{k}[{NR * NC}];")
+            pre.append(f"    for (int q = 0; q < {NR * NC}; ++q) buf{k}[q] = ({t})0;")
+            call.append(f"op2_ptr_{{(void *)buf{k}}}")
+            flags = block_flags.get(k) or (False,) * (nr * nc)
+            store = ("if (atomic) op2_atomic_add(&{tgt}, {val}); else {tgt} += {val};" if acc == INC
+                     else "{tgt} = {val};")
+            for i in range(nr):
+                for j in range(nc):
+                    b = i * nc + j
+                    rd, cd, rar, car = rds[i], cds[j], rars[i], cars[j]
+                    rm, cm = map_index[(k, i)], map_index[(k, nr + j)]
+                    params += [f"const int32_t *__restrict__ rowptr{k}_{b}",
+                               f"const int32_t *__restrict__ colidx{k}_{b}", f"{t} *__restrict__ arg{k}_{b}"]
+                    val = f"buf{k}[({roff[i]} + ii * {rd} + ri) * {NC} + {coff[j]} + jj * {cd} + cj]"
+                    if flags[b]:
+                        tgt = f"arg{k}_{b}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj]"
+                        post.append(
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"        const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, r_, c_); if (s_ < 0) {{ atomicAdd(status, 1); continue; }}\n"
+                            f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj) {{ {store.format(tgt=tgt, val=val)} }} }} }}")
+                    else:
+                        tgt = f"arg{k}_{b}[s_]"
+                        post.append(
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
+                            f"      for (int ri = 0; ri < {rd}; ++ri) for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, r_ * {rd} + ri, c_ * {cd} + cj);"
+                            f" if (s_ < 0) {{ atomicAdd(status, 1); continue; }} {store.format(tgt=tgt, val=val)} }} }} }}")
         elif isinstance(garg, MatKernelArg):
             rd, cd = garg.dims[0][0]
             params += [f"const int32_t *__restrict__ rowptr{k}", f"const int32_t *__restrict__ colidx{k}",
@@ -361,8 +431,13 @@ class GeneratedKernel:
         return seen, index
 
     def _flags(self, parloop):
-        return tuple((k, a.data._block) for k, (garg, a) in enumerate(zip(self.gk.arguments, parloop.arguments))
-                     if isinstance(garg, MatKernelArg))
+        out = []
+        for k, (garg, a) in enumerate(zip(self.gk.arguments, parloop.arguments)):
+            if isinstance(garg, MatKernelArg):
+                out.append((k, a.data._block))
+            elif isinstance(garg, MixedMatKernelArg):
+                out.append((k, tuple(b._block for b in a.data)))
+        return tuple(out)
 
     def source(self, parloop):
         flags = self._flags(parloop)
@@ -382,10 +457,19 @@ class GeneratedKernel:
                     need = True
                 if larg.access in (RW, WRITE, INC, MIN, MAX):
                     out.append(a.maps[0])
+            elif isinstance(garg, MixedDatKernelArg):
+                if larg.access in (RW, WRITE):
+                    need = True
+                if larg.access in (RW, WRITE, INC, MIN, MAX):
+                    out.extend(a.maps)
             elif isinstance(garg, MatKernelArg):
                 if larg.access == WRITE:
                     need = True
                 out.append(a.maps[0])
+            elif isinstance(garg, MixedMatKernelArg):
+                if larg.access == WRITE:
+                    need = True
+                out.extend(a.maps[:garg.shape[0]])
         return (out if need else []), need
 
     def __call__(self, start, end, ptrs, parloop):
@@ -403,6 +487,10 @@ class GeneratedKernel:
                 if (isinstance(garg, DatKernelArg) and garg.map_ is not None and larg.access != READ) \
                         or isinstance(garg, MatKernelArg):
                     conflict.append(a.maps[0])
+                elif isinstance(garg, MixedDatKernelArg) and larg.access != READ:
+                    conflict.extend(a.maps)
+                elif isinstance(garg, MixedMatKernelArg):
+                    conflict.extend(a.maps[:garg.shape[0]])
             need_colour = bool(conflict)
         vals = []
         mat_status = []
@@ -412,6 +500,13 @@ class GeneratedKernel:
                 vals += [ctypes.c_void_p(dev.ptr(mat.rowptr)), ctypes.c_void_p(dev.ptr(mat.colidx)),
                          ctypes.c_void_p(dev.ptr(mat.device_values))]
                 mat_status.append(mat._status)
+            elif isinstance(garg, MixedMatKernelArg):
+                for blk in a.data:
+                    vals += [ctypes.c_void_p(dev.ptr(blk.rowptr)), ctypes.c_void_p(dev.ptr(blk.colidx)),
+                             ctypes.c_void_p(dev.ptr(blk.device_values))]
+                mat_status.append(a.data[0, 0]._status)
+            elif isinstance(garg, MixedDatKernelArg):
+                vals += [ctypes.c_void_p(dev.ptr(d._device_array())) for d in a.data]
             else:
                 vals.append(ctypes.c_void_p(dev.ptr(a.data._device_array())))
         vals += [ctypes.c_void_p(dev.ptr(m.device_values)) for m in maps]

@@ -85,6 +85,57 @@ class MatKernelArg:
         return type(self), self.dims, tuple(m.cache_key for m in self.maps), self.unroll
 
 
+@dataclass(frozen=True)
+class MixedDatKernelArg:
+    """A MixedDat argument: one DatKernelArg per component, packed one after
+    the other into the kernel's buffer (``global_kernel.py:141-180``)."""
+    arguments: Tuple[DatKernelArg, ...]
+
+    @property
+    def cache_key(self):
+        return tuple(a.cache_key for a in self.arguments)
+
+    @property
+    def maps(self):
+        return tuple(m for a in self.arguments for m in a.maps)
+
+    @property
+    def is_direct(self):
+        return False
+
+    @property
+    def is_indirect(self):
+        return True
+
+    @property
+    def split(self):
+        return self.arguments
+
+
+@dataclass(frozen=True)
+class MixedMatKernelArg:
+    """A MixedMat argument: the MatKernelArg of every block, row-major over
+    ``shape`` = (row components, column components) (``global_kernel.py:
+    209-247``).  ``maps`` lists the row maps, then the column maps."""
+    arguments: Tuple[MatKernelArg, ...]
+    shape: Tuple[int, int]
+
+    @property
+    def cache_key(self):
+        return type(self), tuple(a.cache_key for a in self.arguments), self.shape
+
+    @property
+    def maps(self):
+        nr, nc = self.shape
+        rows = tuple(self.arguments[i * nc].maps[0] for i in range(nr))
+        cols = tuple(self.arguments[j].maps[1] for j in range(nc))
+        return rows + cols
+
+    @property
+    def split(self):
+        return self.arguments
+
+
 class PassthroughKernelArg:
     @property
     def cache_key(self):

@@ -8,10 +8,12 @@ from pyop2_amd.parloop import (GlobalParloopArg, DatParloopArg, MatParloopArg,  
                                Parloop, ParLoop, par_loop, parloop, PassthroughArg)
 from pyop2_amd.types import (Access, READ, WRITE, RW, INC, MIN, MAX,  # noqa: F401
                              Set, Subset, SetPartition, MixedSet, DataSet, Map, Dat,
-                             Global, Constant, Sparsity, Mat, PermutedMap, ComposedMap)
+                             Global, Constant, Sparsity, Mat, PermutedMap, ComposedMap,
+                             MixedDataSet, MixedMap, MixedDat)
 
 __all__ = ['configuration', 'READ', 'WRITE', 'RW', 'INC', 'MIN', 'MAX',
-           'Set', 'Subset', 'DataSet', 'MixedSet', 'Map', 'PermutedMap', 'ComposedMap',
+           'Set', 'Subset', 'DataSet', 'MixedSet', 'MixedDataSet', 'Map', 'MixedMap',
+           'PermutedMap', 'ComposedMap', 'MixedDat',
            'Dat', 'Global', 'Constant',
            'Kernel', 'Sparsity', 'Mat', 'par_loop', 'parloop', 'ParLoop', 'Parloop',
            'GlobalKernel', 'DatKernelArg', 'MatKernelArg', 'GlobalKernelArg', 'MapKernelArg',

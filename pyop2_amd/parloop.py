@@ -16,7 +16,7 @@ import numpy as np
 from pyop2_amd.configuration import configuration
 from pyop2_amd.exceptions import KernelTypeError, MapValueError, SetTypeError
 from pyop2_amd.global_kernel import (DatKernelArg, GlobalKernel, GlobalKernelArg, MatKernelArg,
-                                     PassthroughKernelArg)
+                                     MixedDatKernelArg, MixedMatKernelArg, PassthroughKernelArg)
 from pyop2_amd.local_kernel import CStringLocalKernel, LocalKernel
 from pyop2_amd.types.access import Access
 from pyop2_amd.types.dat import AbstractDat, Dat
@@ -94,6 +94,55 @@ class MatParloopArg(ParloopArg):
     def map_kernel_args(self):
         rmap, cmap = self.maps
         return tuple(rmap._kernel_args_) + tuple(cmap._kernel_args_)
+
+
+@dataclass
+class MixedDatParloopArg(ParloopArg):
+    """A MixedDat through a MixedMap: component data pointers, then the
+    component maps (``parloop.py:90-117``)."""
+    data: Any
+    map_: Any
+
+    def __post_init__(self):
+        for m in self.map_.split:
+            self.check_map(m)
+
+    @property
+    def _kernel_args_(self):
+        return self.data._kernel_args_
+
+    @property
+    def map_kernel_args(self):
+        return self.map_._kernel_args_
+
+    @property
+    def maps(self):
+        return tuple(self.map_.split)
+
+
+@dataclass
+class MixedMatParloopArg(ParloopArg):
+    """A MixedMat through a pair of MixedMaps; ``maps`` lists the row
+    component maps, then the column component maps."""
+    data: Any
+    mixed_maps: Tuple[Any, Any]
+
+    def __post_init__(self):
+        for m in self.maps:
+            self.check_map(m)
+
+    @property
+    def _kernel_args_(self):
+        return self.data._kernel_args_
+
+    @property
+    def map_kernel_args(self):
+        return sum((m._kernel_args_ for m in self.maps), ())
+
+    @property
+    def maps(self):
+        rmap, cmap = self.mixed_maps
+        return tuple(rmap.split) + tuple(cmap.split)
 
 
 @dataclass
@@ -199,7 +248,7 @@ class Parloop:
 
     def replace_lgmaps(self):
         for lk_arg, gk_arg, pl_arg in self.zipped_arguments:
-            if isinstance(gk_arg, MatKernelArg):
+            if isinstance(gk_arg, (MatKernelArg, MixedMatKernelArg)):
                 new_state = {Access.INC: Mat.ADD_VALUES, Access.WRITE: Mat.INSERT_VALUES}[lk_arg.access]
                 pl_arg.data.change_assembly_state(new_state)
 
@@ -207,7 +256,8 @@ class Parloop:
         seen = set()
         out = []
         for i, (lk, gk, pl) in enumerate(self.zipped_arguments):
-            if isinstance(gk, DatKernelArg) and gk.is_indirect and pl.data not in seen and lk.access in modes:
+            if isinstance(gk, (DatKernelArg, MixedDatKernelArg)) and gk.is_indirect and pl.data not in seen \
+                    and lk.access in modes:
                 out.append(i)
                 seen.add(pl.data)
         return out
@@ -253,13 +303,15 @@ class Parloop:
             access = self.accesses[i]
             if access is Access.READ:
                 continue
-            if isinstance(wrapper_arg, DatKernelArg):
+            if isinstance(wrapper_arg, (DatKernelArg, MixedDatKernelArg)):
                 d.data.halo_valid = False
                 d.data._device_modified()
             elif isinstance(wrapper_arg, GlobalKernelArg):
                 d.data._device_modified()
             elif isinstance(wrapper_arg, MatKernelArg):
                 d.data.assembly_state = {Access.WRITE: Mat.INSERT_VALUES, Access.INC: Mat.ADD_VALUES}[access]
+            elif isinstance(wrapper_arg, MixedMatKernelArg):
+                d.data.change_assembly_state({Access.WRITE: Mat.INSERT_VALUES, Access.INC: Mat.ADD_VALUES}[access])
 
     @classmethod
     def check_iterset(cls, iterset, global_knl, arguments):
@@ -375,6 +427,49 @@ class MatLegacyArg(LegacyArg):
     @property
     def parloop_arg(self):
         return MatParloopArg(self.data, self.maps, self.lgmaps)
+
+
+@dataclass
+class MixedDatLegacyArg(LegacyArg):
+    data: Any
+    map_: Any
+    access: Access
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    @property
+    def global_kernel_arg(self):
+        return MixedDatKernelArg(tuple(DatKernelArg(d.dataset.dim, m._global_kernel_arg)
+                                       for d, m in zip(self.data.split, self.map_.split)))
+
+    @property
+    def parloop_arg(self):
+        return MixedDatParloopArg(self.data, self.map_)
+
+
+@dataclass
+class MixedMatLegacyArg(LegacyArg):
+    data: Any
+    maps: Tuple[Any, Any]
+    access: Access
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    @property
+    def global_kernel_arg(self):
+        rmap, cmap = self.maps
+        nr, nc = len(rmap), len(cmap)
+        blocks = tuple(MatKernelArg(self.data[i, j].dims, (rmap[i]._global_kernel_arg, cmap[j]._global_kernel_arg))
+                       for i in range(nr) for j in range(nc))
+        return MixedMatKernelArg(blocks, (nr, nc))
+
+    @property
+    def parloop_arg(self):
+        return MixedMatParloopArg(self.data, self.maps)
 
 
 @dataclass

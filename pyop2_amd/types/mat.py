@@ -24,6 +24,7 @@ from pyop2_amd.types.access import Access
 from pyop2_amd.types.data_carrier import DataCarrier
 from pyop2_amd.types.dataset import DataSet
 from pyop2_amd.types.map import Map
+from pyop2_amd.types.mixed import MixedDataSet, MixedMap
 from pyop2_amd.types.set import MixedSet, Subset
 
 
@@ -45,12 +46,22 @@ class Sparsity(caching.ObjectCached):
         rset, cset = self.dsets
         self.comm = rset.comm
         self._has_diagonal = (rset == cset) and diagonal_block
-        self._dims = (((rset.cdim, cset.cdim),),)
         self._name = name or "sparsity_#x%x" % id(self)
-        self._nested = False
-        self._blocks = [[self]]
         self._pattern = None
         self._dof_pattern = None
+        if isinstance(rset, MixedDataSet):
+            # one block Sparsity per (row, column) component (mat.py:86-114)
+            self._nested = True
+            self._blocks = [[Sparsity((rd, cd), sorted(maps_and_regions.get((i, j), ()),
+                                                       key=lambda t: (id(t[0]), id(t[1]))),
+                                      name="%s_%d_%d" % (self._name, i, j), block_sparse=block_sparse,
+                                      diagonal_block=diagonal_block and i == j)
+                             for j, cd in enumerate(cset.split)] for i, rd in enumerate(rset.split)]
+            self._dims = tuple(tuple((rd.cdim, cd.cdim) for cd in cset.split) for rd in rset.split)
+        else:
+            self._nested = False
+            self._blocks = [[self]]
+            self._dims = (((rset.cdim, cset.cdim),),)
         self._initialized = True
 
     _cache = {}
@@ -61,20 +72,23 @@ class Sparsity(caching.ObjectCached):
                       diagonal_block=True):
         if len(dsets) != 2:
             raise RuntimeError(f"dsets must be a tuple of two DataSets: got {dsets}")
+        mixed = any(isinstance(d, MixedDataSet) for d in dsets)
+        if mixed:
+            dsets = tuple(d if isinstance(d, MixedDataSet) else MixedDataSet((d,)) for d in dsets)
         for dset in dsets:
-            if not isinstance(dset, DataSet) and dset is not None:
+            if not isinstance(dset, (DataSet, MixedDataSet)) and dset is not None:
                 raise ex.DataSetTypeError("All data sets must be of type DataSet, not type %r" % type(dset))
-            if isinstance(getattr(dset, "set", None), MixedSet):
-                raise NotImplementedError("mixed sparsities are not on the device path")
         if isinstance(maps_and_regions, Sequence):
             maps_and_regions = {(0, 0): maps_and_regions}
         elif not isinstance(maps_and_regions, dict):
             raise TypeError(f"maps_and_regions must be dict or Sequence: got {type(maps_and_regions)}")
+        nr, nc = (len(dsets[0]), len(dsets[1])) if mixed else (1, 1)
         processed = {(0, 0): frozenset()}
         for key, val in maps_and_regions.items():
             i, j = key
-            if i >= 1 or j >= 1:
-                raise RuntimeError(f"(i, j) must be < (1, 1): got {(i, j)}")
+            if i >= nr or j >= nc:
+                raise RuntimeError(f"(i, j) must be < {(nr, nc)}: got {(i, j)}")
+            rds, cds = (dsets[0][i], dsets[1][j]) if mixed else dsets
             pv = set()
             for rmap, cmap, iteration_regions in set(val):
                 for m in (rmap, cmap):
@@ -82,7 +96,7 @@ class Sparsity(caching.ObjectCached):
                         raise ex.MapTypeError("All maps must be of type map, not type %r" % type(m))
                     if len(m._table) == 0 and m.iterset.total_size > 0:
                         raise ex.MapValueError("Unpopulated map values when trying to build sparsity.")
-                if rmap.toset is not dsets[0].set or cmap.toset is not dsets[1].set:
+                if rmap.toset is not rds.set or cmap.toset is not cds.set:
                     raise RuntimeError("Map toset must be the same as DataSet set")
                 if rmap.iterset is not cmap.iterset:
                     raise RuntimeError("Iterset of both maps in a pair must be the same")
@@ -130,7 +144,7 @@ class Sparsity(caching.ObjectCached):
 
     @utils.cached_property
     def shape(self):
-        return (1, 1)
+        return (len(self._blocks), len(self._blocks[0]))
 
     @property
     def nested(self):
@@ -153,7 +167,7 @@ class Sparsity(caching.ObjectCached):
 
     def __contains__(self, other):
         rm, cm = other[0], other[1]
-        return any(r is rm and c is cm for r, c in self.rcmaps[(0, 0)])
+        return any(r is rm and c is cm for r, c in self.rcmaps.get((0, 0), ()))
 
     # -- the pattern -----------------------------------------------------------
     @property
@@ -259,6 +273,11 @@ class Mat(DataCarrier):
     ADD_VALUES = "ADD_VALUES"
 
     _modes = [Access.WRITE, Access.INC]
+
+    def __new__(cls, sparsity=None, *args, **kwargs):
+        if cls is Mat and getattr(sparsity, "nested", False):
+            cls = MixedMat
+        return super().__new__(cls)
 
     @utils.validate_type(('sparsity', Sparsity, ex.SparsityTypeError),
                          ('name', str, ex.NameTypeError))
@@ -558,3 +577,115 @@ class Mat(DataCarrier):
 
     def __repr__(self):
         return "Mat(%r, %r, %r)" % (self._sparsity, self._datatype, self._name)
+
+
+class MixedMat(Mat):
+    """Matrix on a mixed Sparsity: one block Mat per (row, column)
+    component (the reference's MatNest)."""
+
+    def __init__(self, sparsity, dtype=None, name=None, mat_type=None):
+        self._sparsity = sparsity
+        self.comm = sparsity.comm
+        self._datatype = np.dtype(dtype or np.float64)
+        self._name = name or "mat_#x%x" % id(self)
+        self.mat_type = mat_type
+        nr, nc = sparsity.shape
+        self._blocks = [[Mat(sparsity[i, j], self._datatype, "%s_%d_%d" % (self._name, i, j))
+                         for j in range(nc)] for i in range(nr)]
+        self._block = False
+        self._values = None
+        self._status = None
+        self._desc = None
+        self.assembly_state = Mat.ASSEMBLED
+
+    def _init(self):
+        for b in self:
+            b._init()
+
+    @property
+    def _kernel_args_(self):
+        return sum((b._kernel_args_ for b in self), ())
+
+    @utils.validate_in(('access', Mat._modes, ex.ModeValueError))
+    def __call__(self, access, path, lgmaps=None, unroll_map=False):
+        from pyop2_amd.parloop import MixedMatLegacyArg
+        if lgmaps is not None:
+            raise NotImplementedError("lgmaps are not supported on the device path")
+        rmap, cmap = path
+        if not (isinstance(rmap, MixedMap) and isinstance(cmap, MixedMap)):
+            raise ex.MapTypeError("A MixedMat is accessed through a pair of MixedMaps")
+        nr, nc = self._sparsity.shape
+        if len(rmap) != nr or len(cmap) != nc:
+            raise ex.MapValueError("Path maps do not match the blocks of the Mat")
+        if conf.configuration["type_check"]:
+            for i in range(nr):
+                for j in range(nc):
+                    if (rmap[i], cmap[j]) not in self._sparsity[i, j]:
+                        raise ex.MapValueError("Path maps not in sparsity maps")
+        return MixedMatLegacyArg(self, (rmap, cmap), access)
+
+    @property
+    def nblocks(self):
+        nr, nc = self._sparsity.shape
+        return nr * nc
+
+    @property
+    def is_mixed(self):
+        return True
+
+    @property
+    def nrows(self):
+        return sum(row[0].nrows for row in self._blocks)
+
+    @property
+    def ncols(self):
+        return sum(b.ncols for b in self._blocks[0])
+
+    @property
+    def nbytes(self):
+        return sum(b.nbytes for b in self)
+
+    def __iter__(self):
+        for row in self._blocks:
+            for b in row:
+                yield b
+
+    def __getitem__(self, idx):
+        try:
+            i, j = idx
+            return self._blocks[i][j]
+        except TypeError:
+            return self._blocks[idx]
+
+    @property
+    def blocks(self):
+        return self._blocks
+
+    def change_assembly_state(self, new_state):
+        for b in self:
+            b.change_assembly_state(new_state)
+        self.assembly_state = new_state
+
+    def assemble(self):
+        for b in self:
+            b.assemble()
+        self.assembly_state = Mat.ASSEMBLED
+
+    def zero(self):
+        for b in self:
+            b.zero()
+
+    @property
+    def values(self):
+        """Dense values of the whole matrix (blocks stacked)."""
+        return np.block([[b.values for b in row] for row in self._blocks])
+
+    @property
+    def dat_version(self):
+        return sum(b.dat_version for b in self)
+
+    def __str__(self):
+        return "OP2 MixedMat %s on %s" % (self._name, self._sparsity)
+
+    def __repr__(self):
+        return "MixedMat(%r, %r, %r)" % (self._sparsity, self._datatype, self._name)

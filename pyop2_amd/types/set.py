@@ -337,7 +337,9 @@ class MixedSet(Set, caching.ObjectCached):
         return len(self._sets)
 
     def __pow__(self, e):
-        raise NotImplementedError("MixedDataSet is not on the device path")
+        """A MixedDataSet with dimension ``e`` on every component."""
+        from pyop2_amd.types.mixed import MixedDataSet
+        return MixedDataSet(self, e)
 
     def __eq__(self, other):
         return type(self) is type(other) and self._sets == other._sets

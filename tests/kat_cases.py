@@ -785,3 +785,105 @@ def case_subset_matrix(op2, split=None):
 
 
 SUBSET_CASES = [case_subset_direct, case_subset_complementary, case_subset_nested, case_subset_indirect]
+
+
+# ---- mixed Dats and Mats (test_matrices.py:502-531, 864-981) ----------------
+# off-diagonal and lower-right blocks of the mixed mass-like matrix
+# (test_matrices.py:869-876)
+MIXED_OD = np.array([[1.0, 2.0, 0.0, 0.0], [0.0, 4.0, 6.0, 0.0], [0.0, 0.0, 9.0, 12.0]])
+MIXED_LL = np.diag([1.0, 8.0, 18.0, 16.0]) + np.diag([2.0, 6.0, 12.0], -1) + np.diag([2.0, 6.0, 12.0], 1)
+
+
+def _mixed(op2, dim=1):
+    """test_matrices.py:502-524: 3 elements, 4 nodes on a line; element e
+    maps to itself and to nodes (e, e+1); data 1..n on each component."""
+    mset = op2.MixedSet((op2.Set(3), op2.Set(4)))
+    elem, node = mset
+    mmap = op2.MixedMap((op2.Map(elem, elem, 1, [0, 1, 2]), op2.Map(elem, node, 2, [0, 1, 1, 2, 2, 3])))
+    if dim == 1:
+        mdat = op2.MixedDat(op2.Dat(s, np.arange(1, s.size + 1, dtype=np.float64)) for s in mset)
+    else:
+        mdat = op2.MixedDat(op2.Dat(s ** 2, np.repeat(np.arange(1, s.size + 1, dtype=np.float64), 2).reshape(-1, 2))
+                            for s in mset)
+    return mset, mmap, mdat
+
+
+def case_mixed_mat(op2, split=None):
+    """test_matrices.py:878-916: INC of d d^T into a Mat on the 2x2 mixed
+    sparsity; each block against the reference's closed form."""
+    mset, mmap, mdat = _mixed(op2)
+    sp = op2.Sparsity((mset ** 1, mset ** 1), {(i, j): [(rm, cm, None)] for i, rm in enumerate(mmap)
+                                               for j, cm in enumerate(mmap)})
+    mat = op2.Mat(sp)
+    k = op2.Kernel("static void outer3(double v[9], const double d[3])"
+                   " { for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) v[i * 3 + j] += d[i] * d[j]; }",
+                   "outer3")
+    op2.par_loop(k, mmap.iterset, mat(op2.INC, (mmap, mmap)), mdat(op2.READ, mmap))
+    mat.assemble()
+    assert np.allclose(mat[0, 0].values, np.diag([1.0, 4.0, 9.0]), rtol=1e-12, atol=0)
+    assert np.allclose(mat[0, 1].values, MIXED_OD, rtol=1e-12, atol=0)
+    assert np.allclose(mat[1, 0].values, MIXED_OD.T, rtol=1e-12, atol=0)
+    assert np.allclose(mat[1, 1].values, MIXED_LL, rtol=1e-12, atol=0)
+    assert np.allclose(mat.values, np.block([[np.diag([1.0, 4.0, 9.0]), MIXED_OD], [MIXED_OD.T, MIXED_LL]]))
+
+
+def case_mixed_rhs(op2, split=None):
+    """test_matrices.py:895-922: INC of the element's packs into a MixedDat."""
+    mset, mmap, mdat = _mixed(op2)
+    dat = op2.MixedDat(mset)
+    k = op2.Kernel("static void add3(double v[3], const double d[3]) { for (int i = 0; i < 3; ++i) v[i] += d[i]; }",
+                   "add3")
+    op2.par_loop(k, mmap.iterset, dat(op2.INC, mmap), mdat(op2.READ, mmap))
+    assert np.allclose(np.asarray(dat[0].data_ro), [1.0, 2.0, 3.0], rtol=1e-12, atol=0)
+    assert np.allclose(np.asarray(dat[1].data_ro), [1.0, 4.0, 6.0, 4.0], rtol=1e-12, atol=0)
+
+
+def case_mixed_rhs_vector(op2, split=None):
+    """test_matrices.py:924-942: the same on dim-2 components."""
+    mset, mmap, mvdat = _mixed(op2, dim=2)
+    dat = op2.MixedDat(mset ** 2)
+    k = op2.Kernel("static void add3v(double v[6], const double d[3][2])"
+                   " { for (int i = 0; i < 3; ++i) { v[i * 2] += d[i][0]; v[i * 2 + 1] += d[i][1]; } }", "add3v")
+    op2.par_loop(k, mmap.iterset, dat(op2.INC, mmap), mvdat(op2.READ, mmap))
+    assert np.allclose(np.asarray(dat[0].data_ro), np.kron(np.array([[1.0], [2.0], [3.0]]), np.ones(2)), rtol=1e-12)
+    assert np.allclose(np.asarray(dat[1].data_ro), np.kron(np.array([[1.0], [4.0], [6.0], [4.0]]), np.ones(2)),
+                       rtol=1e-12)
+
+
+def case_mixed_rw(op2, split=None):
+    """RW through a MixedMap (coloured launch): every element adds 1 to its
+    own entry and to each of its two nodes; nodes shared by two elements
+    get 2.  Closed form, in the style of test_indirect_loop.py:138-143."""
+    mset, mmap, _ = _mixed(op2)
+    dat = op2.MixedDat(mset)
+    k = op2.Kernel("static void bump3(double v[3]) { for (int i = 0; i < 3; ++i) v[i] = v[i] + 1.0; }", "bump3")
+    op2.par_loop(k, mmap.iterset, dat(op2.RW, mmap))
+    assert list(np.asarray(dat[0].data_ro)) == [1.0, 1.0, 1.0]
+    assert list(np.asarray(dat[1].data_ro)) == [1.0, 2.0, 2.0, 1.0]
+
+
+def case_mixed_blockwise_sparsity(op2, split=None):
+    """test_matrices.py:945-981: a mixed sparsity given block by block, with
+    ComposedMaps (one with a -1 entry, which adds nothing): per-block nnz."""
+    setA, nodesetA, setB, nodesetB = op2.Set(3), op2.Set(4), op2.Set(2), op2.Set(3)
+    nodesetAB = op2.MixedSet((nodesetA, nodesetB))
+    dsAB = nodesetAB ** 1
+    mapA = op2.Map(setA, nodesetA, 2, values=[[0, 1], [1, 2], [2, 3]])
+    mapB = op2.Map(setB, nodesetB, 2, values=[[0, 1], [1, 2]])
+    mapBA = op2.Map(setB, setA, 1, values=[1, 2])
+    mapAB = op2.Map(setA, setB, 1, values=[-1, 0, 1])
+    s = op2.Sparsity((dsAB, dsAB), {(1, 0): [(mapB, op2.ComposedMap(mapA, mapBA), None)],
+                                    (0, 1): [(mapA, op2.ComposedMap(mapB, mapAB), None)]})
+    assert list(s._blocks[0][0].nnz) == [1, 1, 1, 1]
+    assert list(s._blocks[0][1].nnz) == [0, 2, 3, 2]
+    assert list(s._blocks[1][0].nnz) == [2, 3, 2]
+    assert list(s._blocks[1][1].nnz) == [1, 1, 1]
+
+
+MIXED_CASES = [
+    ("mixed_mat", case_mixed_mat),
+    ("mixed_rhs", case_mixed_rhs),
+    ("mixed_rhs_vector", case_mixed_rhs_vector),
+    ("mixed_rw", case_mixed_rw),
+    ("mixed_blockwise_sparsity", case_mixed_blockwise_sparsity),
+]

@@ -72,6 +72,33 @@ def test_permuted_composed_map_api():
         op2.PermutedMap(c, [1, 0])
 
 
+def test_mixed_types_api():
+    """types/dataset.py:166-312, map.py:297-437, dat.py:646-880, mat.py:86-114:
+    components, sizes, block structure and the errors of mixed objects."""
+    mset = op2.MixedSet((op2.Set(3), op2.Set(4)))
+    elem, node = mset
+    mds = mset ** 2
+    assert [d.cdim for d in mds] == [2, 2] and mds.cdim == 4 and mds.set is mset
+    assert op2.MixedDataSet(mset, (1, 3)).dim == ((1,), (3,))
+    mmap = op2.MixedMap((op2.Map(elem, elem, 1, [0, 1, 2]), op2.Map(elem, node, 2, [0, 1, 1, 2, 2, 3])))
+    assert mmap.arity == 3 and mmap.arities == (1, 2) and mmap.arange == (0, 1, 3)
+    assert mmap.values.tolist() == [[0, 0, 1], [1, 1, 2], [2, 2, 3]]
+    assert mmap.iterset is elem and mmap.toset == mset
+    with pytest.raises(ex.MapTypeError):
+        op2.MixedMap((op2.Map(elem, node, 1, [0, 1, 2]), op2.Map(node, node, 1, [0, 1, 2, 3])))
+    md = op2.MixedDat(mset ** 2)
+    assert [d.data_ro.shape for d in md] == [(3, 2), (4, 2)] and len(md) == 2
+    with pytest.raises(ex.MapValueError):      # component tosets differ from the Dat's sets
+        md(op2.READ, op2.MixedMap((mmap[1], mmap[0])))
+    sp = op2.Sparsity((mset ** 1, mset ** 1), {(i, j): [(rm, cm, None)] for i, rm in enumerate(mmap)
+                                               for j, cm in enumerate(mmap)})
+    assert sp.shape == (2, 2) and sp.nested and sp[1, 0].dsets == (node ** 1, elem ** 1)
+    A = op2.Mat(sp)
+    assert type(A).__name__ == "MixedMat" and A[0, 1].sparsity is sp[0, 1] and len(list(A)) == 4
+    with pytest.raises(RuntimeError):
+        op2.Sparsity((mset ** 1, mset ** 1), {(2, 0): [(mmap[0], mmap[0], None)]})
+
+
 def test_dataset_identity_cache():
     s = op2.Set(4)
     assert s ** 2 is s ** 2

@@ -71,3 +71,10 @@ def test_subset_cases(device, case, split):
 
 def test_subset_matrix(device):
     kat_cases.case_subset_matrix(op2)
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+@pytest.mark.parametrize("name,case", kat_cases.MIXED_CASES, ids=[n for n, _ in kat_cases.MIXED_CASES])
+def test_mixed_cases(device, name, case, strategy):
+    configuration["strategy"] = strategy
+    case(op2)
