@@ -21,6 +21,12 @@ iteration element ``n`` in ``[start, end)`` (``pyop2/codegen/builder.py``):
    a sorted-row binary search of each block's column followed by the add or
    the store, block by block in row-major order.
 
+Mixed arguments (``builder.py:413-498``, ``609-680``): a MixedDat is one
+buffer holding its components' packs one after the other, each gathered and
+scattered through its own map; a MixedMat is one element matrix whose rows
+are the row components' packs and whose columns are the column components'
+packs, block (i, j) inserted into block Mat (i, j) as above.
+
 A Subset iteration set indirects ``n`` through ``subset_indices``
 (``builder.py:725-733``).  Map entries below zero are skipped, as PETSc skips
 negative indices in ``MatSetValues``.
@@ -39,7 +45,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-__all__ = ["DatArg", "GlobalArg", "MatArg", "CSRMat", "run", "compile_wrapper",
+__all__ = ["DatArg", "GlobalArg", "MatArg", "MixedDatArg", "MixedMatArg", "CSRMat", "run", "compile_wrapper",
            "READ", "WRITE", "RW", "INC", "MIN", "MAX"]
 
 READ, WRITE, RW, INC, MIN, MAX = "READ", "WRITE", "RW", "INC", "MIN", "MAX"
@@ -122,6 +128,22 @@ class MatArg:
     cdim: int = 1
 
 
+@dataclass
+class MixedDatArg:
+    """A MixedDat argument: its components (DatArgs with maps, one access)."""
+    parts: Sequence[DatArg]
+    access: str
+
+
+@dataclass
+class MixedMatArg:
+    """A MixedMat argument: the MatArgs of its blocks, row-major over
+    ``shape`` (row components, column components)."""
+    blocks: Sequence[MatArg]
+    shape: tuple
+    access: str
+
+
 def _ctype(dtype):
     try:
         return _CTYPES[np.dtype(dtype)]
@@ -156,6 +178,12 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
         if isinstance(a, MatArg):
             vt = _ctype(a.mat.values.dtype)
             params += [f"const int32_t *rowptr{k}", f"const int32_t *colidx{k}", f"{vt} *arg{k}"]
+        elif isinstance(a, MixedMatArg):
+            for b, blk in enumerate(a.blocks):
+                vt = _ctype(blk.mat.values.dtype)
+                params += [f"const int32_t *rowptr{k}_{b}", f"const int32_t *colidx{k}_{b}", f"{vt} *arg{k}_{b}"]
+        elif isinstance(a, MixedDatArg):
+            params += [f"{_ctype(p.data.dtype)} *arg{k}_{c}" for c, p in enumerate(a.parts)]
         else:
             params.append(f"{_ctype(a.data.dtype)} *arg{k}")
     nmaps = len(set(nmaps_ids.values()))
@@ -211,6 +239,66 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                   MIN: "if (buf{k}[t_] < arg{k}[t_]) arg{k}[t_] = buf{k}[t_];",
                   MAX: "if (buf{k}[t_] > arg{k}[t_]) arg{k}[t_] = buf{k}[t_];"}[a.access]
             unpack.append(f"    for (int t_ = 0; t_ < {dim}; ++t_) {{ {op.format(k=k)} }}")
+        elif isinstance(a, MixedDatArg):
+            t = _ctype(a.parts[0].data.dtype)
+            total = sum(p.map_.shape[1] * p.cdim for p in a.parts)
+            body.append(f"    {t} buf{k}[{total}];")
+            if a.access in (INC, WRITE):
+                body.append(f"    for (int t_ = 0; t_ < {total}; ++t_) buf{k}[t_] = 0;")
+            off = 0
+            for c, p in enumerate(a.parts):
+                m, ar, dim = nmaps_ids[(k, c)], p.map_.shape[1], p.cdim
+                if a.access not in (INC, WRITE):
+                    body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = map{m}[(int64_t)n * {ar} + i];"
+                                f" for (int j = 0; j < {dim}; ++j) buf{k}[{off} + i * {dim} + j] ="
+                                f" m_ >= 0 ? arg{k}_{c}[(int64_t)m_ * {dim} + j] : 0; }}")
+                if a.access != READ:
+                    src_ = f"buf{k}[{off} + i * {dim} + j]"
+                    op = {INC: f"arg{k}_{c}[o_] += {src_};", WRITE: f"arg{k}_{c}[o_] = {src_};",
+                          RW: f"arg{k}_{c}[o_] = {src_};",
+                          MIN: f"{{ const {t} v_ = {src_}; if (v_ < arg{k}_{c}[o_]) arg{k}_{c}[o_] = v_; }}",
+                          MAX: f"{{ const {t} v_ = {src_}; if (v_ > arg{k}_{c}[o_]) arg{k}_{c}[o_] = v_; }}"}[a.access]
+                    unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = map{m}[(int64_t)n * {ar} + i];"
+                                  f" if (m_ < 0) continue; for (int j = 0; j < {dim}; ++j) {{"
+                                  f" const int64_t o_ = (int64_t)m_ * {dim} + j; {op} }} }}")
+                off += ar * dim
+            call.append(f"(void *)buf{k}")
+        elif isinstance(a, MixedMatArg):
+            nr, nc = a.shape
+            t = _ctype(a.blocks[0].mat.values.dtype)
+            rds = [a.blocks[i * nc].rdim for i in range(nr)]
+            cds = [a.blocks[j].cdim for j in range(nc)]
+            rars = [a.blocks[i * nc].rmap.shape[1] for i in range(nr)]
+            cars = [a.blocks[j].cmap.shape[1] for j in range(nc)]
+            roff = [sum(rars[q] * rds[q] for q in range(i)) for i in range(nr)]
+            coff = [sum(cars[q] * cds[q] for q in range(j)) for j in range(nc)]
+            NR, NC = roff[-1] + rars[-1] * rds[-1], coff[-1] + cars[-1] * cds[-1]
+            body.append(f"    {t} buf{k}[{NR * NC}];")
+            body.append(f"    for (int t_ = 0; t_ < {NR * NC}; ++t_) buf{k}[t_] = 0;")
+            call.append(f"(void *)buf{k}")
+            assign = "+=" if a.access == INC else "="
+            for i in range(nr):
+                for j in range(nc):
+                    b = i * nc + j
+                    blk = a.blocks[b]
+                    rm, cm = nmaps_ids[(k, i)], nmaps_ids[(k, nr + j)]
+                    rar, car, rd, cd = rars[i], cars[j], rds[i], cds[j]
+                    val = f"buf{k}[({roff[i]} + ii * {rd} + ri) * {NC} + {coff[j]} + jj * {cd} + cj]"
+                    if blk.mat.block:
+                        unpack.append(
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"        const int32_t s_ = op2o_slot(rowptr{k}_{b}, colidx{k}_{b}, r_, c_); if (s_ < 0) {{ ++err; continue; }}\n"
+                            f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj)\n"
+                            f"          arg{k}_{b}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj] {assign} {val}; }} }}")
+                    else:
+                        unpack.append(
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
+                            f"      for (int ri = 0; ri < {rd}; ++ri)\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"        for (int cj = 0; cj < {cd}; ++cj) {{\n"
+                            f"          const int32_t s_ = op2o_slot(rowptr{k}_{b}, colidx{k}_{b}, r_ * {rd} + ri, c_ * {cd} + cj); if (s_ < 0) {{ ++err; continue; }}\n"
+                            f"          arg{k}_{b}[s_] {assign} {val}; }} }} }}")
         else:  # MatArg
             t = _ctype(a.mat.values.dtype)
             rm, cm = nmaps_ids[(k, 0)], nmaps_ids[(k, 1)]
@@ -289,6 +377,15 @@ def _map_ids(args):
         elif isinstance(a, MatArg):
             ids[(k, 0)] = idx(a.rmap)
             ids[(k, 1)] = idx(a.cmap)
+        elif isinstance(a, MixedDatArg):
+            for c, p in enumerate(a.parts):
+                ids[(k, c)] = idx(p.map_)
+        elif isinstance(a, MixedMatArg):
+            nr, nc = a.shape
+            for i in range(nr):
+                ids[(k, i)] = idx(a.blocks[i * nc].rmap)
+            for j in range(nc):
+                ids[(k, nr + j)] = idx(a.blocks[j].cmap)
     return ids, seen
 
 
@@ -310,6 +407,15 @@ def run(code, name, args: Sequence, parts, subset=None, flavour="strict", header
             for arr in (a.mat.rowptr, a.mat.colidx, a.mat.values):
                 assert arr.flags.c_contiguous
                 cargs.append(arr.ctypes.data_as(ctypes.c_void_p))
+        elif isinstance(a, MixedMatArg):
+            for blk in a.blocks:
+                for arr in (blk.mat.rowptr, blk.mat.colidx, blk.mat.values):
+                    assert arr.flags.c_contiguous
+                    cargs.append(arr.ctypes.data_as(ctypes.c_void_p))
+        elif isinstance(a, MixedDatArg):
+            for p in a.parts:
+                assert p.data.flags.c_contiguous
+                cargs.append(p.data.ctypes.data_as(ctypes.c_void_p))
         else:
             assert a.data.flags.c_contiguous
             cargs.append(a.data.ctypes.data_as(ctypes.c_void_p))

@@ -42,6 +42,38 @@ class Subset(Set):
                                for b in (superset.core_size, superset.size, superset.total_size)))
 
 
+class MixedSet:
+    def __init__(self, sets):
+        self.sets = tuple(sets)
+
+    def __iter__(self):
+        return iter(self.sets)
+
+    def __getitem__(self, i):
+        return self.sets[i]
+
+    def __len__(self):
+        return len(self.sets)
+
+    def __pow__(self, dim):
+        return MixedDataSet([s ** dim for s in self.sets])
+
+
+class MixedDataSet:
+    def __init__(self, dsets):
+        self.dsets = tuple(dsets)
+        self.set = MixedSet(d.set for d in self.dsets)
+
+    def __iter__(self):
+        return iter(self.dsets)
+
+    def __getitem__(self, i):
+        return self.dsets[i]
+
+    def __len__(self):
+        return len(self.dsets)
+
+
 class DataSet:
     def __init__(self, set_, dim, name=None):
         self.set = set_
@@ -108,6 +140,40 @@ class ComposedMap(Map):
         return np.array(out, dtype=np.int32).reshape(-1, self.arity)
 
 
+class MixedMap:
+    def __init__(self, maps):
+        self.maps = tuple(maps)
+        self.iterset = self.maps[0].iterset
+
+    def __iter__(self):
+        return iter(self.maps)
+
+    def __getitem__(self, i):
+        return self.maps[i]
+
+    def __len__(self):
+        return len(self.maps)
+
+
+class MixedDat:
+    def __init__(self, arg):
+        if isinstance(arg, MixedSet):
+            arg = arg ** 1
+        if isinstance(arg, MixedDataSet):
+            self.dats = tuple(Dat(d) for d in arg)
+        else:
+            self.dats = tuple(arg)
+
+    def __getitem__(self, i):
+        return self.dats[i]
+
+    def __iter__(self):
+        return iter(self.dats)
+
+    def __call__(self, access, path):
+        return ("mdat", self, access, path)
+
+
 class Dat:
     def __init__(self, dataset, data=None, dtype=None, name=None):
         if isinstance(dataset, Set):
@@ -155,6 +221,11 @@ class Global:
 
 
 class Sparsity:
+    def __new__(cls, dsets, pairs, name=None, block_sparse=True):
+        if isinstance(dsets[0], MixedDataSet):
+            return MixedSparsity(dsets, pairs, block_sparse)
+        return super().__new__(cls)
+
     def __init__(self, dsets, pairs, name=None, block_sparse=True):
         self.dsets = dsets
         rset, cset = dsets
@@ -168,7 +239,50 @@ class Sparsity:
         self.nnz = osp.dof_nnz(self.rowptr, self.rdim, self.cdim, block_sparse)
 
 
+class MixedSparsity:
+    """One block Sparsity per (row, column) component; blocks not given
+    hold only their diagonal when on the diagonal."""
+
+    def __init__(self, dsets, blocks, block_sparse=True):
+        self.dsets = dsets
+        rd, cd = dsets
+        self.shape = (len(rd), len(cd))
+        self._blocks = [[Sparsity((rd[i], cd[j]), blocks.get((i, j), []), block_sparse=block_sparse)
+                         if (i, j) in blocks or i == j else None
+                         for j in range(len(cd))] for i in range(len(rd))]
+
+    def __getitem__(self, idx):
+        i, j = idx
+        return self._blocks[i][j]
+
+
+class MixedMat:
+    def __init__(self, sparsity, dtype=np.float64, name=None):
+        self.sparsity = sparsity
+        nr, nc = sparsity.shape
+        self.blocks = [[Mat(sparsity[i, j], dtype) for j in range(nc)] for i in range(nr)]
+
+    def __getitem__(self, idx):
+        i, j = idx
+        return self.blocks[i][j]
+
+    def __call__(self, access, maps):
+        return ("mmat", self, access, maps)
+
+    def assemble(self):
+        pass
+
+    @property
+    def values(self):
+        return np.block([[b.values for b in row] for row in self.blocks])
+
+
 class Mat:
+    def __new__(cls, sparsity, dtype=np.float64, name=None):
+        if isinstance(sparsity, MixedSparsity):
+            return MixedMat(sparsity, dtype, name)
+        return super().__new__(cls)
+
     def __init__(self, sparsity, dtype=np.float64, name=None):
         self.sparsity = sparsity
         s = sparsity
@@ -216,6 +330,18 @@ def par_loop(kernel, iterset, *args):
     for kind, obj, access, path in args:
         if kind == "dat":
             sargs.append(seqwrap.DatArg(obj._a, access, path.table if path is not None else None))
+        elif kind == "mdat":
+            sargs.append(seqwrap.MixedDatArg([seqwrap.DatArg(d._a, access, m.table) for d, m in zip(obj, path)],
+                                             access))
+        elif kind == "mmat":
+            rmaps, cmaps = path
+            blocks = []
+            for i, r in enumerate(rmaps):
+                for j, c in enumerate(cmaps):
+                    b = obj[i, j]
+                    blocks.append(seqwrap.MatArg(b.csr, access, r.table, c.table, b.sparsity.rdim,
+                                                 b.sparsity.cdim))
+            sargs.append(seqwrap.MixedMatArg(blocks, (len(rmaps), len(cmaps)), access))
         elif kind == "glob":
             if access == INC:
                 tmp = np.zeros_like(obj._a)

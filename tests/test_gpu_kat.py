@@ -78,3 +78,38 @@ def test_subset_matrix(device):
 def test_mixed_cases(device, name, case, strategy):
     configuration["strategy"] = strategy
     case(op2)
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+def test_mixed_random_against_oracle(device, strategy):
+    """A mixed (cell, vertex) space on a 16x16 triangle mesh: MixedMat and
+    MixedDat INC of a non-symmetric element kernel, device against the
+    oracle's sequential wrapper on the same inputs (1e-12)."""
+    import numpy as np
+    import oracle_op2
+    from pyop2_amd.meshes import unit_square_tri, workload_rng
+    configuration["strategy"] = strategy
+    m = unit_square_tri(16, rng=workload_rng(1))
+    rng = np.random.default_rng(11)
+    cval, vval = rng.random(m.ncells), rng.random(m.nverts)
+    code = ("static void mk(double A[16], double b[4], const double d[4]) {"
+            " for (int i = 0; i < 4; ++i) { b[i] += d[i] * (i + 1);"
+            " for (int j = 0; j < 4; ++j) A[i * 4 + j] += d[i] * d[j] + 0.5 * i - 0.25 * j; } }")
+    out = []
+    for ns in (op2, oracle_op2):
+        cells, verts = ns.Set(m.ncells), ns.Set(m.nverts)
+        ms = ns.MixedSet((cells, verts))
+        mmap = ns.MixedMap((ns.Map(cells, cells, 1, np.arange(m.ncells, dtype=np.int32)),
+                            ns.Map(cells, verts, 3, m.cell_vertex.astype(np.int32))))
+        d = ns.MixedDat([ns.Dat(cells ** 1, cval.copy(), np.float64), ns.Dat(verts ** 1, vval.copy(), np.float64)])
+        b = ns.MixedDat(ms)
+        sp = ns.Sparsity((ms ** 1, ms ** 1), {(i, j): [(rm, cm, None)] for i, rm in enumerate(mmap)
+                                              for j, cm in enumerate(mmap)})
+        A = ns.Mat(sp)
+        ns.par_loop(ns.Kernel(code, "mk"), cells, A(ns.INC, (mmap, mmap)), b(ns.INC, mmap), d(ns.READ, mmap))
+        A.assemble()
+        out.append(([np.asarray(A[i, j].values) for i in range(2) for j in range(2)],
+                    [np.asarray(x.data_ro) for x in b]))
+    (Ad, bd), (Ao, bo) = out
+    for x, y in zip(Ad + bd, Ao + bo):
+        assert np.linalg.norm(x - y) <= 1e-12 * np.linalg.norm(y)

@@ -82,3 +82,9 @@ def test_subset_cases(case, split):
 
 def test_subset_matrix():
     kat_cases.case_subset_matrix(oracle_op2)
+
+
+@pytest.mark.parametrize("name,case", [c for c in kat_cases.MIXED_CASES if c[0] != "mixed_blockwise_sparsity"],
+                         ids=[n for n, _ in kat_cases.MIXED_CASES if n != "mixed_blockwise_sparsity"])
+def test_mixed_cases(name, case):
+    case(oracle_op2)
