@@ -151,8 +151,18 @@ def _ctype(dtype):
         raise TypeError(f"unsupported dtype {dtype}") from None
 
 
-def _generate(code, name, args, nmaps_ids, subset, headers=()):
-    """C source of the sequential wrapper (see module docstring)."""
+def _generate(code, name, args, nmaps_ids, subset, headers=(), extruded=None):
+    """C source of the sequential wrapper (see module docstring).
+
+    ``extruded``: None, or ``(constant_layers, pass_layer, offsets)`` where
+    ``offsets[m]`` is unique map m's per-layer offset (or None)."""
+    offsets = extruded[2] if extruded else {}
+
+    def mv(m, ar, i):
+        base = f"map{m}[(int64_t)n * {ar} + {i}]"
+        if offsets.get(m) is not None:
+            return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * (layer - bottom) : -1)"
+        return base
     lines = ["#include <math.h>", "#include <stdint.h>", "#include <string.h>",
              "#include <stdlib.h>"]
     lines += [h for h in headers]
@@ -189,10 +199,21 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
     nmaps = len(set(nmaps_ids.values()))
     for m in range(nmaps):
         params.append(f"const int32_t *map{m}")
+    if extruded:
+        params.append("const int32_t *layers")
     body = []
     body.append("  int32_t err = 0;")
     body.append("  for (int32_t i_ = start; i_ < end; ++i_) {")
     body.append("    const int32_t n = %s;" % ("subset_indices[i_]" if subset else "i_"))
+    if extruded:
+        # builder.py:683-800: cells bottom..top-1 of column n, maps offset by
+        # the layer above the column's bottom (builder.py:81-129)
+        for m, o in sorted(offsets.items()):
+            if o is not None:
+                body.append(f"    const int32_t off{m}[{len(o)}] = {{{', '.join(str(int(v)) for v in o)}}};")
+        row = "0" if extruded[0] else "n"
+        body.append(f"    const int32_t bottom = layers[2 * {row}], top = layers[2 * {row} + 1] - 1;")
+        body.append("    for (int32_t layer = bottom; layer < top; ++layer) {")
     call = []
     unpack = []
     for k, a in enumerate(args):
@@ -208,7 +229,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
             if a.access in (INC, WRITE):
                 body.append(f"    for (int t_ = 0; t_ < {ar * dim}; ++t_) buf{k}[t_] = 0;")
             else:
-                body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = map{m}[(int64_t)n * {ar} + i];"
+                body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
                             f" for (int j = 0; j < {dim}; ++j) buf{k}[i * {dim} + j] = m_ >= 0 ? arg{k}[(int64_t)m_ * {dim} + j] : 0; }}")
             call.append(f"(void *)buf{k}")
             if a.access == READ:
@@ -219,7 +240,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                   MIN: "{{ const {t} v_ = buf{k}[i * {dim} + j]; if (v_ < arg{k}[o_]) arg{k}[o_] = v_; }}",
                   MAX: "{{ const {t} v_ = buf{k}[i * {dim} + j]; if (v_ > arg{k}[o_]) arg{k}[o_] = v_; }}"}[a.access]
             op = op.format(k=k, dim=dim, t=t)
-            unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = map{m}[(int64_t)n * {ar} + i];"
+            unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
                           f" if (m_ < 0) continue; for (int j = 0; j < {dim}; ++j) {{ const int64_t o_ = (int64_t)m_ * {dim} + j; {op} }} }}")
         elif isinstance(a, GlobalArg):
             t = _ctype(a.data.dtype)
@@ -249,7 +270,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
             for c, p in enumerate(a.parts):
                 m, ar, dim = nmaps_ids[(k, c)], p.map_.shape[1], p.cdim
                 if a.access not in (INC, WRITE):
-                    body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = map{m}[(int64_t)n * {ar} + i];"
+                    body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
                                 f" for (int j = 0; j < {dim}; ++j) buf{k}[{off} + i * {dim} + j] ="
                                 f" m_ >= 0 ? arg{k}_{c}[(int64_t)m_ * {dim} + j] : 0; }}")
                 if a.access != READ:
@@ -258,7 +279,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                           RW: f"arg{k}_{c}[o_] = {src_};",
                           MIN: f"{{ const {t} v_ = {src_}; if (v_ < arg{k}_{c}[o_]) arg{k}_{c}[o_] = v_; }}",
                           MAX: f"{{ const {t} v_ = {src_}; if (v_ > arg{k}_{c}[o_]) arg{k}_{c}[o_] = v_; }}"}[a.access]
-                    unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = map{m}[(int64_t)n * {ar} + i];"
+                    unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
                                   f" if (m_ < 0) continue; for (int j = 0; j < {dim}; ++j) {{"
                                   f" const int64_t o_ = (int64_t)m_ * {dim} + j; {op} }} }}")
                 off += ar * dim
@@ -286,16 +307,16 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                     val = f"buf{k}[({roff[i]} + ii * {rd} + ri) * {NC} + {coff[j]} + jj * {cd} + cj]"
                     if blk.mat.block:
                         unpack.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
-                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
                             f"        const int32_t s_ = op2o_slot(rowptr{k}_{b}, colidx{k}_{b}, r_, c_); if (s_ < 0) {{ ++err; continue; }}\n"
                             f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj)\n"
                             f"          arg{k}_{b}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj] {assign} {val}; }} }}")
                     else:
                         unpack.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
                             f"      for (int ri = 0; ri < {rd}; ++ri)\n"
-                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
                             f"        for (int cj = 0; cj < {cd}; ++cj) {{\n"
                             f"          const int32_t s_ = op2o_slot(rowptr{k}_{b}, colidx{k}_{b}, r_ * {rd} + ri, c_ * {cd} + cj); if (s_ < 0) {{ ++err; continue; }}\n"
                             f"          arg{k}_{b}[s_] {assign} {val}; }} }} }}")
@@ -311,21 +332,25 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
             assign = "+=" if a.access == INC else "="
             if a.mat.block:
                 unpack.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + i]; if (r_ < 0) continue;\n"
-                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + j]; if (c_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
+                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
                     f"        const int32_t s_ = op2o_slot(rowptr{k}, colidx{k}, r_, c_); if (s_ < 0) {{ ++err; continue; }}\n"
                     f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj)\n"
                     f"          arg{k}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj] {assign} buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]; }} }}")
             else:
                 unpack.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + i]; if (r_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
                     f"      for (int ri = 0; ri < {rd}; ++ri)\n"
-                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + j]; if (c_ < 0) continue;\n"
+                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
                     f"        for (int cj = 0; cj < {cd}; ++cj) {{\n"
                     f"          const int32_t s_ = op2o_slot(rowptr{k}, colidx{k}, r_ * {rd} + ri, c_ * {cd} + cj); if (s_ < 0) {{ ++err; continue; }}\n"
                     f"          arg{k}[s_] {assign} buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]; }} }} }}")
+    if extruded and extruded[1]:
+        call.append("layer")
     body.append(f"    {name}({', '.join(call)});")
     body += unpack
+    if extruded:
+        body.append("    }")
     body.append("  }")
     body.append("  return err;")
     src = "\n".join(lines) + "\nint op2o_wrap(" + ", ".join(params) + ")\n{\n" + "\n".join(body) + "\n}\n"
@@ -389,14 +414,27 @@ def _map_ids(args):
     return ids, seen
 
 
-def run(code, name, args: Sequence, parts, subset=None, flavour="strict", headers=()):
+def run(code, name, args: Sequence, parts, subset=None, flavour="strict", headers=(), layers=None,
+        offsets=None, pass_layer=False):
     """Execute the sequential wrapper over each ``(start, end)`` in ``parts``.
 
     Arrays are modified in place (C-contiguous numpy arrays required).
     Returns the number of Mat insertions that missed the pattern.
+    Extruded loops: ``layers`` the (columns, 2) or (1, 2) int32 [bottom, top)
+    layers, ``offsets`` (map array, per-layer offsets) pairs (maps matched by
+    identity), ``pass_layer`` to pass the layer as the kernel's last argument.
     """
     ids, maps = _map_ids(args)
-    src = _generate(code, name, args, ids, subset is not None, headers)
+    ext = None
+    if layers is not None:
+        layers = np.ascontiguousarray(layers, dtype=np.int32)
+        offs = {}
+        for q, m in enumerate(maps):
+            for key, o in (offsets or ()):
+                if key is m:
+                    offs[q] = o
+        ext = (layers.shape[0] == 1, pass_layer, offs)
+    src = _generate(code, name, args, ids, subset is not None, headers, ext)
     fn = compile_wrapper(src, flavour)
     cargs = []
     if subset is not None:
@@ -422,6 +460,8 @@ def run(code, name, args: Sequence, parts, subset=None, flavour="strict", header
     for m in maps:
         assert m.dtype == np.int32 and m.flags.c_contiguous
         cargs.append(m.ctypes.data_as(ctypes.c_void_p))
+    if layers is not None:
+        cargs.append(layers.ctypes.data_as(ctypes.c_void_p))
     err = 0
     for start, end in parts:
         err += fn(ctypes.c_int32(start), ctypes.c_int32(end), *cargs)

@@ -156,6 +156,18 @@ def _strip_headers(text):
     return _STD_HEADER.sub("", text)
 
 
+def _kernel_maps(garg):
+    """MapKernelArgs of a kernel argument in the order of its parloop
+    argument's ``maps``."""
+    if isinstance(garg, DatKernelArg):
+        return (garg.map_,) if garg.map_ is not None else ()
+    if isinstance(garg, MatKernelArg):
+        return tuple(garg.maps)
+    if isinstance(garg, (MixedDatKernelArg, MixedMatKernelArg)):
+        return tuple(garg.maps)
+    return ()
+
+
 def _dat_store(acc, tgt, val):
     """Statement storing a Dat pack entry ``val`` into ``tgt`` for ``acc``
     (atomic when the launch is not coloured)."""
@@ -177,6 +189,20 @@ def generate_source(gk, map_index, block_flags=None):
     """
     block_flags = block_flags or {}
     lk = gk.local_kernel
+    extruded = getattr(gk, "_extruded", False)
+    # map offsets per unique map (extruded maps step by offset[i] per layer)
+    offsets = {}
+    for k, garg in enumerate(gk.arguments):
+        for j, mk in enumerate(_kernel_maps(garg)):
+            if mk.offset is not None and extruded:
+                offsets[map_index[(k, j)]] = mk.offset
+
+    def _mv(m, ar, i):
+        """Entry ``i`` of unique map ``m`` for element ``n`` (and layer)."""
+        base = f"map{m}[(int64_t)n * {ar} + {i}]"
+        if m in offsets:
+            return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * lyr_ : -1)"
+        return base
     headers = "\n".join(_strip_headers(h) for h in lk.headers)
     code = _strip_headers(lk.code)
     params = ["int32_t lo", "int32_t hi", "const int32_t *__restrict__ list", "int32_t atomic"]
@@ -194,7 +220,7 @@ def generate_source(gk, map_index, block_flags=None):
             m = map_index[(k, 0)]
             ar = garg.map_.arity
             pre.append(f"    int32_t mi{k}[{ar}];")
-            pre.append(f"    for (int i = 0; i < {ar}; ++i) mi{k}[i] = map{m}[(int64_t)n * {ar} + i];")
+            pre.append(f"    for (int i = 0; i < {ar}; ++i) mi{k}[i] = {_mv(m, ar, 'i')};")
             pre.append(f"    {t} buf{k}[{ar * dim}];")
             if acc in (INC, WRITE):
                 pre.append(f"    for (int q = 0; q < {ar * dim}; ++q) buf{k}[q] = ({t})0;")
@@ -241,7 +267,7 @@ def generate_source(gk, map_index, block_flags=None):
                 a_, mi_ = f"arg{k}_{c}", f"mi{k}_{c}"
                 params.append(f"{t} *__restrict__ {a_}")
                 pre.append(f"    int32_t {mi_}[{ar}];")
-                pre.append(f"    for (int i = 0; i < {ar}; ++i) {mi_}[i] = map{m}[(int64_t)n * {ar} + i];")
+                pre.append(f"    for (int i = 0; i < {ar}; ++i) {mi_}[i] = {_mv(m, ar, 'i')};")
                 if acc not in (INC, WRITE):
                     pre.append(f"    for (int i = 0; i < {ar}; ++i) for (int j = 0; j < {dim}; ++j)"
                                f" buf{k}[{off} + i * {dim} + j] = {mi_}[i] >= 0 ?"
@@ -282,15 +308,15 @@ def generate_source(gk, map_index, block_flags=None):
                     if flags[b]:
                         tgt = f"arg{k}_{b}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj]"
                         post.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
-                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {_mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
                             f"        const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, r_, c_); if (s_ < 0) {{ atomicAdd(status, 1); continue; }}\n"
                             f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj) {{ {store.format(tgt=tgt, val=val)} }} }} }}")
                     else:
                         tgt = f"arg{k}_{b}[s_]"
                         post.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + ii]; if (r_ < 0) continue;\n"
-                            f"      for (int ri = 0; ri < {rd}; ++ri) for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + jj]; if (c_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
+                            f"      for (int ri = 0; ri < {rd}; ++ri) for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {_mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
                             f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, r_ * {rd} + ri, c_ * {cd} + cj);"
                             f" if (s_ < 0) {{ atomicAdd(status, 1); continue; }} {store.format(tgt=tgt, val=val)} }} }} }}")
         elif isinstance(garg, MatKernelArg):
@@ -312,16 +338,16 @@ def generate_source(gk, map_index, block_flags=None):
                 tgt = f"arg{k}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj]"
                 val = f"buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]"
                 post.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + i]; if (r_ < 0) continue;\n"
-                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + j]; if (c_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {_mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
+                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {_mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
                     f"        const int32_t s_ = op2_find_slot(rowptr{k}, colidx{k}, r_, c_); if (s_ < 0) {{ atomicAdd(status, 1); continue; }}\n"
                     f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj) {{ {store.format(tgt=tgt, val=val)} }} }} }}")
             else:
                 tgt = f"arg{k}[s_]"
                 val = f"buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]"
                 post.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = map{rm}[(int64_t)n * {rar} + i]; if (r_ < 0) continue;\n"
-                    f"      for (int ri = 0; ri < {rd}; ++ri) for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = map{cm}[(int64_t)n * {car} + j]; if (c_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {_mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
+                    f"      for (int ri = 0; ri < {rd}; ++ri) for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {_mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
                     f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}, colidx{k}, r_ * {rd} + ri, c_ * {cd} + cj);"
                     f" if (s_ < 0) {{ atomicAdd(status, 1); continue; }} {store.format(tgt=tgt, val=val)} }} }} }}")
         else:
@@ -329,6 +355,10 @@ def generate_source(gk, map_index, block_flags=None):
     for m in range(nmaps):
         params.append(f"const int32_t *__restrict__ map{m}")
     params.append("int32_t *__restrict__ status")
+    if extruded:
+        params.append("const int32_t *__restrict__ layers")
+    offdecl = [f"    const int32_t off{m}[{len(o)}] = {{{', '.join(str(int(v)) for v in o)}}};"
+               for m, o in sorted(offsets.items())]
     decls = "\n".join(s for kind, s in red if kind == "decl")
     reds = "\n".join(s for kind, s in red if kind == "post")
     src = [_PRELUDE,
@@ -345,9 +375,17 @@ def generate_source(gk, map_index, block_flags=None):
            decls,
            "  if (active) {",
            "    const int32_t n = list ? list[t_] : t_;",
+           *([] if not extruded else [
+               *offdecl,
+               "    const int32_t lay_ = %s;" % ("0" if gk._constant_layers else "n"),
+               "    const int32_t bottom_ = layers[2 * lay_], top_ = layers[2 * lay_ + 1] - 1;",
+               "    for (int32_t layer_ = bottom_; layer_ < top_; ++layer_) {",
+               "    const int32_t lyr_ = layer_ - bottom_;",
+               "    (void)lyr_;"]),
            *pre,
-           f"    {lk.name}({', '.join(call)});",
+           f"    {lk.name}({', '.join(call + (['layer_'] if getattr(gk, '_pass_layer_arg', False) else []))});",
            *post,
+           *(["    }"] if extruded else []),
            "  }",
            reds,
            "}"]
@@ -513,6 +551,9 @@ class GeneratedKernel:
         status = mat_status[0] if mat_status else _dummy_status()
         vals.append(ctypes.c_void_p(dev.ptr(status)))
         iterset = parloop.iterset
+        extruded = getattr(self.gk, "_extruded", False)
+        if extruded:
+            vals.append(ctypes.c_void_p(dev.ptr(iterset.device_layers)))
         L = dev.lib()
         s = dev.stream()
 
@@ -531,7 +572,15 @@ class GeneratedKernel:
             else:
                 elems = np.arange(start, end, dtype=np.int32)
             key = (id(iterset), start, end, tuple(id(m) for m in conflict))
-            plan = plans.colour_plan(conflict[0], elems, [(m, id(m.toset)) for m in conflict], key)
+            if extruded:
+                # a column's targets over all its layers (a thread runs the
+                # layers of its column one after the other)
+                targets = [(_ColumnTargets(m, iterset, start, end), id(m.toset)) for m in conflict]
+                elems = np.arange(end - start, dtype=np.int32)
+                plan = plans.colour_plan(conflict[0], elems, targets, ("extruded",) + key)
+                plan = _shifted_plan(plan, start, conflict[0], ("extruded-shift",) + key)
+            else:
+                plan = plans.colour_plan(conflict[0], elems, [(m, id(m.toset)) for m in conflict], key)
             ep = dev.ptr(plan.elems)
             for c in range(plan.ncolours):
                 lo, hi = int(plan.offsets[c]), int(plan.offsets[c + 1])
@@ -542,6 +591,26 @@ class GeneratedKernel:
             lst = dev.ptr(iterset.device_indices) if isinstance(iterset, Subset) else 0
             launch(start, end, lst, 1)
             self.strategy_used = "atomic"
+
+
+class _ColumnTargets:
+    """Duck-typed map for colouring the columns [start, end) of an extruded
+    loop: row c lists every target column start + c touches."""
+
+    def __init__(self, m, iterset, start, end):
+        self._table = m.column_table(iterset, np.arange(start, end))
+        self.toset = m.toset
+
+
+def _shifted_plan(plan, start, owner, key):
+    """A colour plan over local column positions turned into column ids."""
+    from pyop2_amd import plans
+    got = owner._cache.get(key)
+    if got is None:
+        elems = dev.to_host(plan.elems).astype(np.int32) + np.int32(start)
+        got = plans.ColourPlan(plan.colours, dev.to_device(elems), plan.offsets, plan.ncolours)
+        owner._cache[key] = got
+    return got
 
 
 _STATUS = None

@@ -153,8 +153,10 @@ class GlobalKernel:
                  constant_layers=False, subset=False, iteration_region=None, pass_layer_arg=False):
         if not len(local_kernel.accesses) == len(arguments):
             raise ValueError("Number of arguments passed to the local and global kernels do not match")
-        if extruded or extruded_periodic or constant_layers or iteration_region is not None or pass_layer_arg:
-            raise NotImplementedError("extruded iteration is not on the device path")
+        if extruded_periodic or iteration_region is not None:
+            raise NotImplementedError("periodic extrusion and iteration regions are not on the device path")
+        if pass_layer_arg and not extruded:
+            raise ValueError("Cannot request layer argument for non-extruded iteration")
         from pyop2_amd.types.glob import Constant
         if any(isinstance(garg, Constant) and larg.access is not READ
                for larg, garg in zip(local_kernel.arguments, arguments)):
@@ -171,6 +173,9 @@ class GlobalKernel:
         self.local_kernel = local_kernel
         self.arguments = arguments
         self._subset = subset
+        self._extruded = bool(extruded)
+        self._constant_layers = bool(constant_layers)
+        self._pass_layer_arg = bool(pass_layer_arg)
         self._executor = None
 
     def __call__(self, comm, *args, parloop=None):

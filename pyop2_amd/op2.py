@@ -7,12 +7,12 @@ from pyop2_amd.global_kernel import (GlobalKernel, GlobalKernelArg, DatKernelArg
 from pyop2_amd.parloop import (GlobalParloopArg, DatParloopArg, MatParloopArg,  # noqa: F401
                                Parloop, ParLoop, par_loop, parloop, PassthroughArg)
 from pyop2_amd.types import (Access, READ, WRITE, RW, INC, MIN, MAX,  # noqa: F401
-                             Set, Subset, SetPartition, MixedSet, DataSet, Map, Dat,
+                             Set, Subset, SetPartition, MixedSet, ExtrudedSet, DataSet, Map, Dat,
                              Global, Constant, Sparsity, Mat, PermutedMap, ComposedMap,
                              MixedDataSet, MixedMap, MixedDat)
 
 __all__ = ['configuration', 'READ', 'WRITE', 'RW', 'INC', 'MIN', 'MAX',
-           'Set', 'Subset', 'DataSet', 'MixedSet', 'MixedDataSet', 'Map', 'MixedMap',
+           'Set', 'Subset', 'ExtrudedSet', 'DataSet', 'MixedSet', 'MixedDataSet', 'Map', 'MixedMap',
            'PermutedMap', 'ComposedMap', 'MixedDat',
            'Dat', 'Global', 'Constant',
            'Kernel', 'Sparsity', 'Mat', 'par_loop', 'parloop', 'ParLoop', 'Parloop',

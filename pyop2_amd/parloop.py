@@ -24,7 +24,7 @@ from pyop2_amd.types.data_carrier import DataCarrier
 from pyop2_amd.types.glob import Constant, Global
 from pyop2_amd.types.map import Map
 from pyop2_amd.types.mat import Mat
-from pyop2_amd.types.set import MixedSet, Set, Subset
+from pyop2_amd.types.set import ExtrudedSet, MixedSet, Set, Subset
 from pyop2_amd.utils import cached_property
 
 
@@ -325,7 +325,8 @@ class Parloop:
             iterset = iterset.superset
         for i, (lk_arg, gk_arg, pl_arg) in enumerate(cls.zip_arguments(global_knl, arguments)):
             if isinstance(gk_arg, DatKernelArg) and gk_arg.is_direct:
-                if pl_arg.data.dataset.set != iterset:
+                _iterset = iterset.parent if isinstance(iterset, ExtrudedSet) else iterset
+                if pl_arg.data.dataset.set != _iterset:
                     raise MapValueError(f"Iterset of direct arg {i} does not match parloop iterset")
             for j, m in enumerate(pl_arg.maps):
                 if m.iterset != iterset and m.iterset not in iterset:
@@ -516,6 +517,8 @@ def LegacyParloop(local_knl, iterset, *args, **kwargs):
         local_knl.dtypes = tuple(a.dtype for a in args)
     global_knl_args = tuple(a.global_kernel_arg for a in args)
     subset = isinstance(iterset, Subset)
+    extruded = getattr(iterset, "_extruded", False)
+    kwargs = dict(kwargs, extruded=extruded, constant_layers=extruded and iterset.constant_layers)
     global_knl = _cached_global_kernel(local_knl, global_knl_args, subset, kwargs)
     parloop_args = tuple(a.parloop_arg for a in args)
     return Parloop(global_knl, iterset, parloop_args)

@@ -125,7 +125,7 @@ def lookup(gk):
     """Executor for ``gk`` if a hand-written entry point implements it."""
     lk = gk.local_kernel
     hit = _REGISTERED.get(code_digest(lk.code, lk.name))
-    if hit is None:
+    if hit is None or getattr(gk, "_extruded", False):
         return None
     family, params = hit
     spec = FAMILIES[family]

@@ -1,5 +1,5 @@
 from pyop2_amd.types.access import Access, READ, WRITE, RW, INC, MIN, MAX  # noqa: F401
-from pyop2_amd.types.set import Set, Subset, SetPartition, MixedSet  # noqa: F401
+from pyop2_amd.types.set import Set, Subset, SetPartition, MixedSet, ExtrudedSet  # noqa: F401
 from pyop2_amd.types.dataset import DataSet  # noqa: F401
 from pyop2_amd.types.map import Map, PermutedMap, ComposedMap  # noqa: F401
 from pyop2_amd.types.mixed import MixedDataSet, MixedMap, MixedDat  # noqa: F401

@@ -26,9 +26,16 @@ class Map:
         self._values = utils.verify_reshape(values, IntType, (iterset.total_size, arity), allow_none=True)
         self.shape = (iterset.total_size, arity)
         self._name = name or "map_#x%x" % id(self)
-        if offset is not None and len(offset) > 0:
-            raise NotImplementedError("extruded maps (offset) are not on the device path")
-        self._offset = None
+        # extruded maps: entry i steps by offset[i] per layer (map.py:33-60)
+        if offset is None or len(offset) == 0:
+            self._offset = None
+        else:
+            off = np.asarray(offset, dtype=IntType).reshape(-1)
+            self._offset = np.broadcast_to(off, (self._arity,)).copy() if off.size == 1 else off
+            if self._offset.shape != (self._arity,):
+                raise ex.DataValueError("Map offset must have one entry per map entry")
+        if offset_quotient is not None and len(offset_quotient) > 0:
+            raise NotImplementedError("periodic extruded maps (offset_quotient) are not on the device path")
         self._offset_quotient = None
         self._cache = {}
         self._dev = None
@@ -36,7 +43,38 @@ class Map:
     @utils.cached_property
     def _global_kernel_arg(self):
         from pyop2_amd.global_kernel import MapKernelArg
-        return MapKernelArg(self.arity, None, None)
+        off = tuple(int(x) for x in self._offset) if self._offset is not None else None
+        return MapKernelArg(self.arity, off, None)
+
+    def column_table(self, iterset, columns):
+        """Entries of ``columns`` (host ids) of an extruded loop over all their
+        layers: ``(len(columns), nlayers_max * arity)``, entry (n, l, i) =
+        ``table[n][i] + offset[i] * l`` for the column's cells l, -1 past
+        its top (the targets a column touches, for colouring and patterns)."""
+        cols = np.asarray(columns, dtype=np.int64)
+        lay = iterset.column_layers(cols)
+        ncell = (lay[:, 1] - lay[:, 0] - 1) if len(cols) else np.zeros(0, dtype=np.int64)
+        nl = int(ncell.max()) if len(cols) else 0
+        base = self._table[cols].astype(np.int64)
+        off = self._offset if self._offset is not None else np.zeros(self._arity, dtype=IntType)
+        steps = np.arange(nl, dtype=np.int64)
+        out = base[:, None, :] + steps[None, :, None] * off[None, None, :]
+        out = np.where((steps[None, :] < ncell[:, None])[:, :, None] & (base[:, None, :] >= 0), out, -1)
+        return np.ascontiguousarray(out.reshape(len(cols), nl * self._arity), dtype=IntType)
+
+    def cell_table(self, iterset, columns):
+        """Entries of every cell of ``columns`` of an extruded loop, one row
+        per cell (columns in order, cells bottom to top)."""
+        t = self.column_table(iterset, columns).reshape(len(columns), -1, self._arity)
+        return np.ascontiguousarray(t[self.column_cells_mask(iterset, columns)], dtype=IntType)
+
+    @staticmethod
+    def column_cells_mask(iterset, columns):
+        cols = np.asarray(columns, dtype=np.int64)
+        lay = iterset.column_layers(cols)
+        ncell = lay[:, 1] - lay[:, 0] - 1
+        nl = int(ncell.max()) if len(cols) else 0
+        return np.arange(nl)[None, :] < ncell[:, None]
 
     @property
     def _table(self):

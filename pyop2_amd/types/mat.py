@@ -28,6 +28,21 @@ from pyop2_amd.types.mixed import MixedDataSet, MixedMap
 from pyop2_amd.types.set import MixedSet, Subset
 
 
+def _cells_map(m):
+    """A map of an extruded loop as a plain map over its owned cells (columns
+    in order, cells bottom to top), cached on ``m``: the sparsity of an
+    extruded pair is the union over cells (``sparsity.pyx:285-393``)."""
+    from pyop2_amd.types.set import Set
+    key = ("cells_map",)
+    f = m._cache.get(key)
+    if f is None:
+        it = m.iterset
+        tab = m.cell_table(it, np.arange(it.size))
+        f = Map(Set(len(tab)), m.toset, m.arity, tab)
+        m._cache[key] = f
+    return f
+
+
 class Sparsity(caching.ObjectCached):
     """Non-zero structure of a matrix from pairs of Maps.
 
@@ -90,7 +105,11 @@ class Sparsity(caching.ObjectCached):
                 raise RuntimeError(f"(i, j) must be < {(nr, nc)}: got {(i, j)}")
             rds, cds = (dsets[0][i], dsets[1][j]) if mixed else dsets
             pv = set()
-            for rmap, cmap, iteration_regions in set(val):
+            for entry in set(val):
+                rmap, cmap = entry[0], entry[1]
+                iteration_regions = entry[2] if len(entry) > 2 else None
+                if len(entry) > 3 and entry[3] is not None:
+                    raise NotImplementedError("extra sparsity map data is not on the device path")
                 for m in (rmap, cmap):
                     if not isinstance(m, Map):
                         raise ex.MapTypeError("All maps must be of type map, not type %r" % type(m))
@@ -190,6 +209,8 @@ class Sparsity(caching.ObjectCached):
         arr = (dev.MapPair * max(len(pairs), 1))()
         incp, ince = [], []
         for k, (rmap, cmap) in enumerate(pairs):
+            if getattr(rmap.iterset, "_extruded", False):
+                rmap, cmap = _cells_map(rmap), _cells_map(cmap)
             size = rmap.iterset.size
             tr = plans.transpose(rmap, nrows, size)
             incp.append(tr.inc_ptr)

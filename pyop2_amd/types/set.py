@@ -168,6 +168,107 @@ class Set:
         return self.difference(other)
 
 
+class ExtrudedSet(Set):
+    """A Set extruded into columns of cells (``pyop2/types/set.py:305-391``).
+
+    ``layers`` is the number of layers of every column (``layers - 1`` cells
+    each) or an ``(parent.total_size, 2)`` array of [bottom, top) layers per
+    column.  A par_loop over it runs, for every column of the parent's
+    partition, the cells of the column bottom to top; map entries step by
+    the map's ``offset`` per layer.
+    """
+
+    _extruded = True
+
+    @utils.validate_type(('parent', Set, TypeError))
+    def __init__(self, parent, layers, extruded_periodic=False):
+        if extruded_periodic:
+            raise NotImplementedError("periodic extrusion is not on the device path")
+        if isinstance(parent, (ExtrudedSet, Subset)):
+            raise NotImplementedError("only a plain Set can be extruded on the device path")
+        self._parent = parent
+        self.comm = parent.comm
+        arr = np.asarray(layers)
+        if arr.ndim == 2:
+            arr = utils.verify_reshape(layers, IntType, (parent.total_size, 2))
+            if arr.size and arr.min() < 0:
+                raise ex.SizeTypeError("Bottom of layers must be >= 0")
+            if np.any(arr[:, 1] - arr[:, 0] < 1):
+                raise ex.SizeTypeError("Number of layers must be >= 0")
+            self.constant_layers = False
+        else:
+            if arr.shape:
+                raise ex.SizeTypeError("Specifying layers per entity, but provided %s, needed (%d, 2)"
+                                       % (arr.shape, parent.total_size))
+            if int(layers) < 2:
+                raise ex.SizeTypeError("Need at least two layers, not %d" % int(layers))
+            arr = np.asarray([[0, int(layers)]], dtype=IntType)
+            self.constant_layers = True
+        self._layers = np.ascontiguousarray(arr, dtype=IntType)
+        self._extruded_periodic = False
+        self._cache = {}
+        self._dev_layers = None
+        self._partition_size = 1024
+        self._constrained_size = 0
+
+    @utils.cached_property
+    def parent(self):
+        return self._parent
+
+    @utils.cached_property
+    def layers(self):
+        """The number of layers (constant layers only)."""
+        if self.constant_layers:
+            return int(self._layers[0, 1])
+        raise ValueError("No single layer, use layers_array attribute")
+
+    @utils.cached_property
+    def layers_array(self):
+        return self._layers
+
+    @property
+    def device_layers(self):
+        """Device copy of ``layers_array``."""
+        if self._dev_layers is None:
+            from pyop2_amd import device as dev
+            self._dev_layers = dev.to_device(self._layers)
+        return self._dev_layers
+
+    def column_layers(self, columns):
+        """(bottom, top) layer of each of the host column ids ``columns``."""
+        cols = np.asarray(columns, dtype=np.int64)
+        if self.constant_layers:
+            return np.broadcast_to(self._layers[0], (len(cols), 2))
+        return self._layers[cols]
+
+    @property
+    def _sizes(self):
+        return self._parent.sizes
+
+    @property
+    def _name(self):
+        return self._parent.name
+
+    @property
+    def _halo(self):
+        return self._parent.halo
+
+    def __contains__(self, set):
+        return set is self._parent
+
+    def __eq__(self, other):
+        return self is other
+
+    def __hash__(self):
+        return id(self)
+
+    def __str__(self):
+        return "OP2 ExtrudedSet: %s with size %s (%s layers)" % (self._name, self.size, self._layers)
+
+    def __repr__(self):
+        return "ExtrudedSet(%r, %r)" % (self._parent, self._layers)
+
+
 class Subset(Set):
     """Elements ``indices`` (sorted, de-duplicated) of ``superset``."""
 

@@ -887,3 +887,168 @@ MIXED_CASES = [
     ("mixed_rw", case_mixed_rw),
     ("mixed_blockwise_sparsity", case_mixed_blockwise_sparsity),
 ]
+
+
+# ---- extruded sets (test_extrusion.py:40-449) --------------------------------
+XN = 32                     # base elements of the extruded strip
+XLAYERS = 11
+XWEDGES = XLAYERS - 1
+XNODES = XN + 2
+
+
+def _strip(op2, layers=XLAYERS):
+    """test_extrusion.py:80-271: a strip of XN triangles (element i has
+    nodes i, i+1, i+2, at (k // 2, k % 2)) extruded into XLAYERS layers.
+    Node k of layer l is dof k * XLAYERS + l of a 2-vector Dat; the cell
+    field of element i and cell l is entry i * XWEDGES + l; both maps step
+    by 1 per layer."""
+    base = op2.Set(XN)
+    elements = op2.ExtrudedSet(base, layers=layers)
+    node_set = op2.Set(XNODES * XLAYERS)
+    elem_set = op2.Set(XN * XWEDGES)
+    cvals = np.array([[m * XLAYERS + d for m in (i, i + 1, i + 2) for d in (0, 1)] for i in range(XN)],
+                     dtype=np.int32)
+    coords_map = op2.Map(elements, node_set, 6, cvals, "elem_dofs", np.ones(6, dtype=np.int32))
+    field_map = op2.Map(elements, elem_set, 1, (np.arange(XN) * XWEDGES).astype(np.int32), "elem_elem",
+                        np.ones(1, dtype=np.int32))
+    xy = np.array([[k // 2, k % 2] for k in range(XNODES) for _ in range(XLAYERS)], dtype=np.float64)
+    coords = op2.Dat(node_set ** 2, xy, np.float64, "coords")
+    field = op2.Dat(elem_set ** 1, np.ones(XN * XWEDGES), np.float64, "field")
+    return base, elements, node_set, elem_set, coords_map, field_map, coords, field
+
+
+def case_extruded_volume(op2, split=None):
+    """test_extrusion.py:340-356: Global INC of 0.05 per cell (triangle area
+    1/2, height 0.1, field 1): 16 over 32 x 10 cells."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    g = op2.Global(1, 0.0, np.float64, "g")
+    k = op2.Kernel("""
+static void vol(double A[1], const double x[6][2], const double y[1])
+{
+  double a = x[0][0] * (x[2][1] - x[4][1]) + x[2][0] * (x[4][1] - x[0][1]) + x[4][0] * (x[0][1] - x[2][1]);
+  if (a < 0) a = -a;
+  A[0] += 0.5 * a * 0.1 * y[0];
+}""", "vol")
+    op2.par_loop(k, elements, g(op2.INC), coords(op2.READ, cmap), field(op2.READ, fmap))
+    assert abs(float(g.data[0]) - XWEDGES * 0.1 * (XN // 2)) < 1e-9
+
+
+def case_extruded_direct_inc(op2, split=None):
+    """test_extrusion.py:362-369: a direct Dat of the base set is incremented
+    once per cell of its column: 9 cells of 10 layers."""
+    base = op2.Set(XN)
+    dat = op2.Dat(base ** 1, np.zeros(XN), np.float64)
+    op2.par_loop(op2.Kernel("static void one(double *x) { *x += 1.0; }", "one"),
+                 op2.ExtrudedSet(base, layers=10), dat(op2.INC))
+    assert np.allclose(np.asarray(dat.data), 9.0)
+
+
+def case_extruded_layer_arg(op2, split=None):
+    """test_extrusion.py:371-387: the layer is passed to the kernel; cell l of
+    column n writes l into entry n * XWEDGES + l."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    f = op2.Dat(elem_set ** 1, -np.ones(XN * XWEDGES), np.float64)
+    op2.par_loop(op2.Kernel("static void lay(double *x, int layer) { x[0] = layer; }", "lay"),
+                 elements, f(op2.WRITE, fmap), pass_layer_arg=True)
+    assert list(np.asarray(f.data)) == list(np.tile(np.arange(XWEDGES, dtype=np.float64), XN))
+
+
+def case_extruded_write(op2, split=None):
+    """test_extrusion.py:389-410: WRITE through the field and the coordinate
+    maps reaches every cell entry and every node of every layer."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    f = op2.Dat(elem_set ** 1, -np.ones(XN * XWEDGES), np.float64)
+    op2.par_loop(op2.Kernel("static void wo(double *x) { x[0] = 42.0; }", "wo"), elements, f(op2.WRITE, fmap))
+    assert (np.asarray(f.data) == 42).all()
+    c = op2.Dat(node_set ** 2, np.zeros((XNODES * XLAYERS, 2)), np.float64)
+    op2.par_loop(op2.Kernel("static void wo6(double x[6][2]) { for (int i = 0; i < 6; ++i) x[i][0] = x[i][1] = 42.0; }",
+                            "wo6"), elements, c(op2.WRITE, cmap))
+    assert (np.asarray(c.data) == 42).all()
+
+
+def case_extruded_neighbours(op2, split=None):
+    """test_extrusion.py:412-426: cell l of element i writes the sum of its
+    six coordinate pairs: 2 * ((i + i + 1 + i + 2) // 2 terms) -- computed
+    here from the node positions (k // 2) + (k % 2) of nodes i, i+1, i+2,
+    each counted twice (layers l and l + 1)."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    f = op2.Dat(elem_set ** 1, -np.ones(XN * XWEDGES), np.float64)
+    k = op2.Kernel("static void wtf(double *y, const double x[6][2])"
+                   " { double s = 0.0; for (int i = 0; i < 6; ++i) s += x[i][0] + x[i][1]; y[0] = s; }", "wtf")
+    op2.par_loop(k, elements, f(op2.WRITE, fmap), coords(op2.READ, cmap))
+    want = [2.0 * sum(kk // 2 + kk % 2 for kk in (i, i + 1, i + 2)) for i in range(XN) for _ in range(XWEDGES)]
+    assert list(np.asarray(f.data)) == want
+
+
+def case_extruded_rw(op2, split=None):
+    """test_extrusion.py:428-446: RW through the coordinate map sets every
+    still-zero pair to 1 (cells in order within a column): every pair 1."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    c = op2.Dat(node_set ** 2, np.zeros((XNODES * XLAYERS, 2)), np.float64)
+    k = op2.Kernel("""
+static void inc(double y[6][2], const double x[6][2])
+{
+  for (int i = 0; i < 6; i++)
+    if (y[i][0] == 0) { y[i][0] += 1; y[i][1] += 1; }
+}""", "inc")
+    op2.par_loop(k, elements, c(op2.RW, cmap), coords(op2.READ, cmap))
+    assert float(np.asarray(c.data).sum()) == XNODES * XLAYERS * 2
+
+
+def case_extruded_variable_layers(op2, split=None):
+    """Variable layers ([bottom, top) per column): direct INC counts each
+    column's cells, and an INC through a map stepping by 1 per layer covers
+    exactly the cells above each column's bottom (set.py:305-346)."""
+    base = op2.Set(4)
+    lay = np.array([[0, 3], [1, 5], [2, 3], [0, 6]], dtype=np.int32)
+    ext = op2.ExtrudedSet(base, lay)
+    cnt = op2.Dat(base ** 1, np.zeros(4), np.float64)
+    op2.par_loop(op2.Kernel("static void one(double *x) { *x += 1.0; }", "one"), ext, cnt(op2.INC))
+    assert list(np.asarray(cnt.data)) == [2.0, 3.0, 0.0, 5.0]
+    tgt = op2.Set(40)
+    m = op2.Map(ext, tgt, 1, np.array([0, 10, 20, 30], dtype=np.int32), offset=np.ones(1, dtype=np.int32))
+    d = op2.Dat(tgt ** 1, np.zeros(40), np.float64)
+    op2.par_loop(op2.Kernel("static void tag(double *x, int layer) { x[0] += 100.0 + layer; }", "tag"),
+                 ext, d(op2.INC, m), pass_layer_arg=True)
+    want = np.zeros(40)
+    for n, (b, t) in enumerate(lay):
+        for lyr in range(t - b - 1):
+            want[10 * n + lyr] += 100.0 + b + lyr
+    assert list(np.asarray(d.data)) == list(want)
+
+
+def case_extruded_matrix(op2, split=None):
+    """An extruded Mat: the 6x6 coordinate-dof block per cell, INC of
+    (1 + i) * (1 + j) into the extruded sparsity; each pattern entry holds
+    the sum over the cells that couple its two dofs (counted here from the
+    cell lists)."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2, layers=4)
+    sp = op2.Sparsity((node_set ** 1, node_set ** 1), [(cmap, cmap, None)])
+    A = op2.Mat(sp, np.float64)
+    k = op2.Kernel("static void blk(double a[6][6])"
+                   " { for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) a[i][j] += (1 + i) * (1 + j); }",
+                   "blk")
+    op2.par_loop(k, elements, A(op2.INC, (cmap, cmap)))
+    A.assemble()
+    want = np.zeros((XNODES * XLAYERS, XNODES * XLAYERS))
+    cv = np.array([[m * XLAYERS + d for m in (i, i + 1, i + 2) for d in (0, 1)] for i in range(XN)])
+    for i in range(XN):
+        for lyr in range(3):
+            dofs = cv[i] + lyr
+            for a in range(6):
+                for b in range(6):
+                    want[dofs[a], dofs[b]] += (1 + a) * (1 + b)
+    assert np.array_equal(np.asarray(A.values), want)
+    assert int(np.sum(sp.nnz)) == int(np.count_nonzero(want + np.eye(len(want))))
+
+
+EXTRUDED_CASES = [
+    ("extruded_volume", case_extruded_volume),
+    ("extruded_direct_inc", case_extruded_direct_inc),
+    ("extruded_layer_arg", case_extruded_layer_arg),
+    ("extruded_write", case_extruded_write),
+    ("extruded_neighbours", case_extruded_neighbours),
+    ("extruded_rw", case_extruded_rw),
+    ("extruded_variable_layers", case_extruded_variable_layers),
+    ("extruded_matrix", case_extruded_matrix),
+]

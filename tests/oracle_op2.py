@@ -81,13 +81,40 @@ class DataSet:
         self.cdim = int(np.prod(self.dim))
 
 
+class ExtrudedSet(Set):
+    """Columns of cells over ``parent`` (constant ``layers`` or a
+    (parent.total_size, 2) array of [bottom, top) layers)."""
+
+    def __init__(self, parent, layers):
+        super().__init__((parent.core_size, parent.size, parent.total_size))
+        self.parent = parent
+        arr = np.asarray(layers, dtype=np.int32)
+        self.layers_array = arr if arr.ndim == 2 else np.array([[0, int(layers)]], dtype=np.int32)
+
+    def cells(self, n):
+        a, b = self.layers_array[0 if len(self.layers_array) == 1 else n]
+        return range(b - a - 1)
+
+
 class Map:
     """``table`` is what a loop gathers through: entry ``k`` of element ``n``
-    names the target ``table[n, k]``."""
+    names the target ``table[n, k]``; an extruded map's entry k of layer l
+    above the column's bottom is ``table[n, k] + offset[k] * l``."""
 
-    def __init__(self, iterset, toset, arity, values=None, name=None):
+    def __init__(self, iterset, toset, arity, values=None, name=None, offset=None):
         self.iterset, self.toset, self.arity = iterset, toset, arity
         self.values = np.ascontiguousarray(np.asarray(values, dtype=np.int32).reshape(iterset.total_size, arity))
+        self.offset = None if offset is None else np.broadcast_to(np.asarray(offset, dtype=np.int32),
+                                                                  (arity,)).copy()
+
+    def cell_rows(self):
+        """One row of entries per cell of an extruded iteration (owned columns)."""
+        rows = []
+        for n in range(self.iterset.size):
+            for lyr in self.iterset.cells(n):
+                rows.append([int(v) + int(self.offset[k]) * lyr if v >= 0 else -1
+                             for k, v in enumerate(self.table[n])])
+        return np.array(rows, dtype=np.int32).reshape(-1, self.arity)
 
     @property
     def table(self):
@@ -230,11 +257,16 @@ class Sparsity:
         self.dsets = dsets
         rset, cset = dsets
         self.rdim, self.cdim = rset.cdim, cset.cdim
-        self.pairs = [(r, c) for r, c, _ in pairs]
+        self.pairs = [(p[0], p[1]) for p in pairs]
         diag = rset is cset or (rset.set is cset.set and rset.dim == cset.dim)
-        self.rowptr, self.colidx = osp.pattern(rset.set.total_size, cset.set.total_size,
-                                               [(r.table, c.table, r.iterset.size) for r, c in self.pairs],
-                                               set_diag=diag)
+        tabs = []
+        for r, c in self.pairs:
+            if isinstance(r.iterset, ExtrudedSet):
+                rt, ct = r.cell_rows(), c.cell_rows()
+                tabs.append((rt, ct, len(rt)))
+            else:
+                tabs.append((r.table, c.table, r.iterset.size))
+        self.rowptr, self.colidx = osp.pattern(rset.set.total_size, cset.set.total_size, tabs, set_diag=diag)
         self.block = block_sparse and self.rdim == self.cdim and self.rdim > 1
         self.nnz = osp.dof_nnz(self.rowptr, self.rdim, self.cdim, block_sparse)
 
@@ -324,7 +356,7 @@ class ParLoop:
         par_loop(self.kernel, self.iterset, *self.args)
 
 
-def par_loop(kernel, iterset, *args):
+def par_loop(kernel, iterset, *args, pass_layer_arg=False):
     sargs = []
     temps = []
     for kind, obj, access, path in args:
@@ -356,7 +388,15 @@ def par_loop(kernel, iterset, *args):
             sargs.append(seqwrap.MatArg(obj.csr, access, r.table, c.table, rd, cd))
     parts = [(0, iterset.core_size), (iterset.core_size, iterset.size)]
     subset = iterset.indices if isinstance(iterset, Subset) else None
-    err = seqwrap.run(kernel.code, kernel.name, sargs, parts, subset=subset)
+    ext = {}
+    if isinstance(iterset, ExtrudedSet):
+        offs = []
+        for kind, obj, access, path in args:
+            for m in (path if isinstance(path, tuple) else (path,)):
+                if isinstance(m, Map) and m.offset is not None:
+                    offs.append((m.table, m.offset))
+        ext = dict(layers=iterset.layers_array, offsets=offs, pass_layer=pass_layer_arg)
+    err = seqwrap.run(kernel.code, kernel.name, sargs, parts, subset=subset, **ext)
     assert err == 0, "insertion outside the sparsity pattern"
     for g, tmp in temps:
         g._a += tmp

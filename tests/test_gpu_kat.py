@@ -113,3 +113,10 @@ def test_mixed_random_against_oracle(device, strategy):
     (Ad, bd), (Ao, bo) = out
     for x, y in zip(Ad + bd, Ao + bo):
         assert np.linalg.norm(x - y) <= 1e-12 * np.linalg.norm(y)
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+@pytest.mark.parametrize("name,case", kat_cases.EXTRUDED_CASES, ids=[n for n, _ in kat_cases.EXTRUDED_CASES])
+def test_extruded_cases(device, name, case, strategy):
+    configuration["strategy"] = strategy
+    case(op2)

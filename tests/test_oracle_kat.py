@@ -88,3 +88,39 @@ def test_subset_matrix():
                          ids=[n for n, _ in kat_cases.MIXED_CASES if n != "mixed_blockwise_sparsity"])
 def test_mixed_cases(name, case):
     case(oracle_op2)
+
+
+@pytest.mark.parametrize("name,case", kat_cases.EXTRUDED_CASES, ids=[n for n, _ in kat_cases.EXTRUDED_CASES])
+def test_extruded_cases(name, case):
+    case(oracle_op2)
+
+
+def test_planted_extrusion_mistake_is_caught():
+    """Offsetting maps by the absolute layer instead of the layer above the
+    column's bottom must fail the variable-layers case."""
+    from oracle import seqwrap
+    orig = seqwrap._generate
+
+    def broken(*a, **k):
+        return orig(*a, **k).replace("* (layer - bottom)", "* layer")
+    seqwrap._generate = broken
+    try:
+        with pytest.raises(AssertionError):
+            kat_cases.case_extruded_variable_layers(oracle_op2)
+    finally:
+        seqwrap._generate = orig
+
+
+def test_planted_mistake_is_caught():
+    """A wrapper that forgets to zero INC buffers must fail the INC case."""
+    from oracle import seqwrap
+    orig = seqwrap._generate
+
+    def broken(*a, **k):
+        return orig(*a, **k).replace("buf0[t_] = 0;", "buf0[t_] = 1;")
+    seqwrap._generate = broken
+    try:
+        with pytest.raises(AssertionError):
+            kat_cases.case_inc_unitset(oracle_op2, kat_cases.SPLITS[0])
+    finally:
+        seqwrap._generate = orig
