@@ -401,11 +401,22 @@ def batched(args):
         for _, pr, _, _ in plain:
             pr()
     reps = max(2, min(args.reps, 5))
-    t1 = time.perf_counter()
-    for _ in range(reps):
+    # the faster of the two shared-node treatments of the staged residual
+    # (partial sums add a launch per mesh; atomics do not)
+    t_plain, plain_partials = None, None
+    saved_partials = configuration["staged_partials"]
+    for partials in (True, False):
+        configuration["staged_partials"] = partials
         plain_pass()
-    torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t1) / reps
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            plain_pass()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t1) / reps
+        if t_plain is None or t < t_plain:
+            t_plain, plain_partials = t, partials
+    configuration["staged_partials"] = saved_partials
     configuration["strategy"] = saved
     tb = time.time()
     batch = BatchedLaplace([(pm, pr) for pm, pr, _, _ in pairs])
@@ -420,8 +431,8 @@ def batched(args):
         t_batch = (time.perf_counter() - t1) / (reps * 4)
     kms = kernel_ms(tm, "batched_p1_laplace")
     # parity: batched and plain results of every mesh (batched ran 1 + 4 reps
-    # times, plain 1 + reps times)
-    nb, npl = 1 + 4 * reps, 1 + reps
+    # times, plain 1 + 2 x (1 + reps) times)
+    nb, npl = 1 + 4 * reps, 3 + 2 * reps
     worst = 0.0
     for (_, _, A0, R0), (_, _, A1, R1) in zip(plain, pairs):
         v0, v1 = A0.device_values / npl, A1.device_values / nb
@@ -429,6 +440,7 @@ def batched(args):
         r0, r1 = R0._device_array() / npl, R1._device_array() / nb
         worst = max(worst, float((r1 - r0).norm() / max(float(r0.norm()), 1e-300)))
     res = dict(meshes=count, cells=ncells, nnz=nnz, plain_ms_per_pass=t_plain * 1e3,
+               plain_staged_partials=plain_partials,
                batched_ms_per_pass=t_batch * 1e3, batched_kernel_us=kms * 1e3,
                speedup=t_plain / t_batch, cells_per_s=ncells / t_batch, grid=batch.grid,
                items=batch.nitems, relerr_batched_vs_plain=worst, setup_s=time.time() - t0,
