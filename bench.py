@@ -186,10 +186,13 @@ def run(args):
     achieved = alg / (kms * 1e-3) / 1e9
     log(f"C2 assembly: {ms_per_step:.3f} ms/step, kernel {kms * 1e3:.1f} us ({strategy_used}), "
         f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
-    kms_cold = cold_kernel_ms(step, max(3, min(args.steps, 10)), "p1_laplace_matrix")
-    log(f"C2 assembly, last-level cache flushed before each launch: kernel {kms_cold * 1e3:.1f} us, "
-        f"{alg / (kms_cold * 1e-3) / 1e9:.0f} GB/s algorithmic")
-    nrun_extra = 1 + max(3, min(args.steps, 10))
+    if args.no_cold:
+        kms_cold, nrun_extra = float("nan"), 0
+    else:
+        kms_cold = cold_kernel_ms(step, max(3, min(args.steps, 10)), "p1_laplace_matrix")
+        log(f"C2 assembly, last-level cache flushed before each launch: kernel {kms_cold * 1e3:.1f} us, "
+            f"{alg / (kms_cold * 1e-3) / 1e9:.0f} GB/s algorithmic")
+        nrun_extra = 1 + max(3, min(args.steps, 10))
 
     # parity of the timed Mat against the oracle (values scale with the step count)
     from oracle import seqwrap, sparsity as osp
@@ -518,6 +521,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="small meshes (smoke of the harness)")
     ap.add_argument("--primary-only", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cold", action="store_true",
+                    help="skip the cache-flushed C2 variant (profiles of the warm kernels)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

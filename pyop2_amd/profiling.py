@@ -52,7 +52,9 @@ def kernel_stats(dirs):
     out = {}
     for name, ds in acc.items():
         gx, wx, vg, ag, lds = meta[name]
-        out[name] = dict(calls=len(ds), avg_us=sum(ds) / len(ds) / 1e3, min_us=min(ds) / 1e3,
+        srt = sorted(ds)
+        out[name] = dict(calls=len(ds), avg_us=sum(ds) / len(ds) / 1e3, med_us=srt[len(srt) // 2] / 1e3,
+                         min_us=min(ds) / 1e3,
                          max_us=max(ds) / 1e3, total_us=sum(ds) / 1e3, grid=gx, wg=wx, vgpr=vg,
                          agpr=ag, lds=lds)
     return out
@@ -116,6 +118,8 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("dirs", nargs="+")
     s.add_argument("--top", type=int, default=25)
+    s.add_argument("--match", action="append", default=[],
+                   help="also list the kernels whose name contains this (repeatable)")
     c = sub.add_parser("counters")
     c.add_argument("dirs", nargs="+")
     c.add_argument("--kernel", default="")
@@ -129,11 +133,12 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         st = kernel_stats(a.dirs)
-        rows = sorted(st.items(), key=lambda kv: -kv[1]["total_us"])[:a.top]
-        print(f"{'calls':>6} {'avg us':>10} {'min us':>10} {'max us':>10} {'grid':>9} {'wg':>4} "
+        ranked = sorted(st.items(), key=lambda kv: -kv[1]["total_us"])
+        rows = ranked[:a.top] + [kv for kv in ranked[a.top:] if any(m in kv[0] for m in a.match)]
+        print(f"{'calls':>6} {'avg us':>10} {'med us':>10} {'min us':>10} {'max us':>10} {'grid':>9} {'wg':>4} "
               f"{'vgpr':>4} {'lds':>6}  kernel")
         for name, r in rows:
-            print(f"{r['calls']:6d} {r['avg_us']:10.2f} {r['min_us']:10.2f} {r['max_us']:10.2f} "
+            print(f"{r['calls']:6d} {r['avg_us']:10.2f} {r['med_us']:10.2f} {r['min_us']:10.2f} {r['max_us']:10.2f} "
                   f"{r['grid']:9d} {r['wg']:4d} {r['vgpr']:4d} {r['lds']:6d}  {_short(name)}")
     elif a.cmd == "counters":
         for name, ctrs in sorted(counter_means(a.dirs).items()):
