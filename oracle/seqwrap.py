@@ -154,15 +154,24 @@ def _ctype(dtype):
 def _generate(code, name, args, nmaps_ids, subset, headers=(), extruded=None):
     """C source of the sequential wrapper (see module docstring).
 
-    ``extruded``: None, or ``(constant_layers, pass_layer, offsets)`` where
-    ``offsets[m]`` is unique map m's per-layer offset (or None)."""
+    ``extruded``: None, or ``(constant_layers, pass_layer, offsets, region)``
+    where ``offsets[m]`` is unique map m's per-layer offset (or None) and
+    ``region`` one of "all", "bottom", "top", "interior"."""
     offsets = extruded[2] if extruded else {}
+    region = extruded[3] if extruded else "all"
+    # interior facets (builder.py:81-129, 825, 904): every map entry is taken
+    # for the cell below (f = 0) and the cell above (f = 1), f outermost
+    nf = 2 if region == "interior" else 1
 
     def mv(m, ar, i):
-        base = f"map{m}[(int64_t)n * {ar} + {i}]"
-        if offsets.get(m) is not None:
-            return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * (layer - bottom) : -1)"
-        return base
+        if nf == 1:
+            base = f"map{m}[(int64_t)n * {ar} + {i}]"
+            if offsets.get(m) is not None:
+                return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * (layer - bottom) : -1)"
+            return base
+        assert offsets.get(m) is not None, "interior facets need maps with offsets"
+        base = f"map{m}[(int64_t)n * {ar} + ({i}) % {ar}]"
+        return f"(({base}) >= 0 ? ({base}) + off{m}[({i}) % {ar}] * (layer - bottom + ({i}) / {ar}) : -1)"
     lines = ["#include <math.h>", "#include <stdint.h>", "#include <string.h>",
              "#include <stdlib.h>"]
     lines += [h for h in headers]
@@ -211,9 +220,14 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
         for m, o in sorted(offsets.items()):
             if o is not None:
                 body.append(f"    const int32_t off{m}[{len(o)}] = {{{', '.join(str(int(v)) for v in o)}}};")
+        # builder.py:771-795: the region's layers; a column with no cells
+        # runs no top cell
         row = "0" if extruded[0] else "n"
-        body.append(f"    const int32_t bottom = layers[2 * {row}], top = layers[2 * {row} + 1] - 1;")
-        body.append("    for (int32_t layer = bottom; layer < top; ++layer) {")
+        body.append(f"    const int32_t bottom = layers[2 * {row}], cend = layers[2 * {row} + 1] - 1;")
+        lo, hi = {"all": ("bottom", "cend"), "bottom": ("bottom", "bottom + 1"),
+                  "top": ("(cend - 1 > bottom ? cend - 1 : bottom)", "cend"),
+                  "interior": ("bottom", "cend - 1")}[region]
+        body.append(f"    for (int32_t layer = {lo}; layer < {hi}; ++layer) {{")
     call = []
     unpack = []
     for k, a in enumerate(args):
@@ -224,12 +238,13 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                 call.append(f"(void *)(arg{k} + (int64_t)n * {dim})")
                 continue
             m = nmaps_ids[k]
-            ar = a.map_.shape[1]
+            bar = a.map_.shape[1]
+            ar = nf * bar
             body.append(f"    {t} buf{k}[{ar * dim}];")
             if a.access in (INC, WRITE):
                 body.append(f"    for (int t_ = 0; t_ < {ar * dim}; ++t_) buf{k}[t_] = 0;")
             else:
-                body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
+                body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, bar, 'i')};"
                             f" for (int j = 0; j < {dim}; ++j) buf{k}[i * {dim} + j] = m_ >= 0 ? arg{k}[(int64_t)m_ * {dim} + j] : 0; }}")
             call.append(f"(void *)buf{k}")
             if a.access == READ:
@@ -240,7 +255,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                   MIN: "{{ const {t} v_ = buf{k}[i * {dim} + j]; if (v_ < arg{k}[o_]) arg{k}[o_] = v_; }}",
                   MAX: "{{ const {t} v_ = buf{k}[i * {dim} + j]; if (v_ > arg{k}[o_]) arg{k}[o_] = v_; }}"}[a.access]
             op = op.format(k=k, dim=dim, t=t)
-            unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
+            unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, bar, 'i')};"
                           f" if (m_ < 0) continue; for (int j = 0; j < {dim}; ++j) {{ const int64_t o_ = (int64_t)m_ * {dim} + j; {op} }} }}")
         elif isinstance(a, GlobalArg):
             t = _ctype(a.data.dtype)
@@ -262,15 +277,16 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
             unpack.append(f"    for (int t_ = 0; t_ < {dim}; ++t_) {{ {op.format(k=k)} }}")
         elif isinstance(a, MixedDatArg):
             t = _ctype(a.parts[0].data.dtype)
-            total = sum(p.map_.shape[1] * p.cdim for p in a.parts)
+            total = sum(nf * p.map_.shape[1] * p.cdim for p in a.parts)
             body.append(f"    {t} buf{k}[{total}];")
             if a.access in (INC, WRITE):
                 body.append(f"    for (int t_ = 0; t_ < {total}; ++t_) buf{k}[t_] = 0;")
             off = 0
             for c, p in enumerate(a.parts):
-                m, ar, dim = nmaps_ids[(k, c)], p.map_.shape[1], p.cdim
+                m, bar, dim = nmaps_ids[(k, c)], p.map_.shape[1], p.cdim
+                ar = nf * bar
                 if a.access not in (INC, WRITE):
-                    body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
+                    body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, bar, 'i')};"
                                 f" for (int j = 0; j < {dim}; ++j) buf{k}[{off} + i * {dim} + j] ="
                                 f" m_ >= 0 ? arg{k}_{c}[(int64_t)m_ * {dim} + j] : 0; }}")
                 if a.access != READ:
@@ -279,7 +295,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                           RW: f"arg{k}_{c}[o_] = {src_};",
                           MIN: f"{{ const {t} v_ = {src_}; if (v_ < arg{k}_{c}[o_]) arg{k}_{c}[o_] = v_; }}",
                           MAX: f"{{ const {t} v_ = {src_}; if (v_ > arg{k}_{c}[o_]) arg{k}_{c}[o_] = v_; }}"}[a.access]
-                    unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, ar, 'i')};"
+                    unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, bar, 'i')};"
                                   f" if (m_ < 0) continue; for (int j = 0; j < {dim}; ++j) {{"
                                   f" const int64_t o_ = (int64_t)m_ * {dim} + j; {op} }} }}")
                 off += ar * dim
@@ -289,8 +305,8 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
             t = _ctype(a.blocks[0].mat.values.dtype)
             rds = [a.blocks[i * nc].rdim for i in range(nr)]
             cds = [a.blocks[j].cdim for j in range(nc)]
-            rars = [a.blocks[i * nc].rmap.shape[1] for i in range(nr)]
-            cars = [a.blocks[j].cmap.shape[1] for j in range(nc)]
+            rars = [nf * a.blocks[i * nc].rmap.shape[1] for i in range(nr)]
+            cars = [nf * a.blocks[j].cmap.shape[1] for j in range(nc)]
             roff = [sum(rars[q] * rds[q] for q in range(i)) for i in range(nr)]
             coff = [sum(cars[q] * cds[q] for q in range(j)) for j in range(nc)]
             NR, NC = roff[-1] + rars[-1] * rds[-1], coff[-1] + cars[-1] * cds[-1]
@@ -307,23 +323,23 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                     val = f"buf{k}[({roff[i]} + ii * {rd} + ri) * {NC} + {coff[j]} + jj * {cd} + cj]"
                     if blk.mat.block:
                         unpack.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
-                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {mv(rm, rar // nf, 'ii')}; if (r_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {mv(cm, car // nf, 'jj')}; if (c_ < 0) continue;\n"
                             f"        const int32_t s_ = op2o_slot(rowptr{k}_{b}, colidx{k}_{b}, r_, c_); if (s_ < 0) {{ ++err; continue; }}\n"
                             f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj)\n"
                             f"          arg{k}_{b}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj] {assign} {val}; }} }}")
                     else:
                         unpack.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {mv(rm, rar // nf, 'ii')}; if (r_ < 0) continue;\n"
                             f"      for (int ri = 0; ri < {rd}; ++ri)\n"
-                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {mv(cm, car // nf, 'jj')}; if (c_ < 0) continue;\n"
                             f"        for (int cj = 0; cj < {cd}; ++cj) {{\n"
                             f"          const int32_t s_ = op2o_slot(rowptr{k}_{b}, colidx{k}_{b}, r_ * {rd} + ri, c_ * {cd} + cj); if (s_ < 0) {{ ++err; continue; }}\n"
                             f"          arg{k}_{b}[s_] {assign} {val}; }} }} }}")
         else:  # MatArg
             t = _ctype(a.mat.values.dtype)
             rm, cm = nmaps_ids[(k, 0)], nmaps_ids[(k, 1)]
-            rar, car = a.rmap.shape[1], a.cmap.shape[1]
+            rar, car = nf * a.rmap.shape[1], nf * a.cmap.shape[1]
             rd, cd = a.rdim, a.cdim
             ncol = car * cd
             body.append(f"    {t} buf{k}[{rar * rd * ncol}];")
@@ -332,16 +348,16 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
             assign = "+=" if a.access == INC else "="
             if a.mat.block:
                 unpack.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
-                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {mv(rm, rar // nf, 'i')}; if (r_ < 0) continue;\n"
+                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {mv(cm, car // nf, 'j')}; if (c_ < 0) continue;\n"
                     f"        const int32_t s_ = op2o_slot(rowptr{k}, colidx{k}, r_, c_); if (s_ < 0) {{ ++err; continue; }}\n"
                     f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj)\n"
                     f"          arg{k}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj] {assign} buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]; }} }}")
             else:
                 unpack.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {mv(rm, rar // nf, 'i')}; if (r_ < 0) continue;\n"
                     f"      for (int ri = 0; ri < {rd}; ++ri)\n"
-                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
+                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {mv(cm, car // nf, 'j')}; if (c_ < 0) continue;\n"
                     f"        for (int cj = 0; cj < {cd}; ++cj) {{\n"
                     f"          const int32_t s_ = op2o_slot(rowptr{k}, colidx{k}, r_ * {rd} + ri, c_ * {cd} + cj); if (s_ < 0) {{ ++err; continue; }}\n"
                     f"          arg{k}[s_] {assign} buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]; }} }} }}")
@@ -415,14 +431,15 @@ def _map_ids(args):
 
 
 def run(code, name, args: Sequence, parts, subset=None, flavour="strict", headers=(), layers=None,
-        offsets=None, pass_layer=False):
+        offsets=None, pass_layer=False, region="all"):
     """Execute the sequential wrapper over each ``(start, end)`` in ``parts``.
 
     Arrays are modified in place (C-contiguous numpy arrays required).
     Returns the number of Mat insertions that missed the pattern.
     Extruded loops: ``layers`` the (columns, 2) or (1, 2) int32 [bottom, top)
     layers, ``offsets`` (map array, per-layer offsets) pairs (maps matched by
-    identity), ``pass_layer`` to pass the layer as the kernel's last argument.
+    identity), ``pass_layer`` to pass the layer as the kernel's last argument,
+    ``region`` the iteration region ("all", "bottom", "top", "interior").
     """
     ids, maps = _map_ids(args)
     ext = None
@@ -433,7 +450,7 @@ def run(code, name, args: Sequence, parts, subset=None, flavour="strict", header
             for key, o in (offsets or ()):
                 if key is m:
                     offs[q] = o
-        ext = (layers.shape[0] == 1, pass_layer, offs)
+        ext = (layers.shape[0] == 1, pass_layer, offs, region)
     src = _generate(code, name, args, ids, subset is not None, headers, ext)
     fn = compile_wrapper(src, flavour)
     cargs = []

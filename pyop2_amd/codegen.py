@@ -38,6 +38,7 @@ from pyop2_amd.datatypes import as_cstr
 from pyop2_amd.exceptions import CompilationError
 from pyop2_amd.global_kernel import (DatKernelArg, GlobalKernelArg, MatKernelArg, MixedDatKernelArg,
                                      MixedMatKernelArg)
+from pyop2_amd.types import region as reg
 from pyop2_amd.types.access import INC, MAX, MIN, READ, RW, WRITE
 from pyop2_amd.types.set import Subset
 
@@ -180,6 +181,20 @@ def _dat_store(acc, tgt, val):
     return f"{tgt} = {val};"
 
 
+def _region_bounds(region):
+    """C expressions of the first and one-past-last layer a column runs in
+    ``region``, from its ``bottom_`` and cell end ``cend_``
+    (``builder.py:771-795``; a column with no cells runs no top cell)."""
+    region = reg.normalise(region)
+    if region == reg.ON_BOTTOM:
+        return "bottom_", "bottom_ + 1"
+    if region == reg.ON_TOP:
+        return "(cend_ - 1 > bottom_ ? cend_ - 1 : bottom_)", "cend_"
+    if region == reg.ON_INTERIOR_FACETS:
+        return "bottom_", "cend_ - 1"
+    return "bottom_", "cend_"
+
+
 def generate_source(gk, map_index, block_flags=None):
     """HIP source of the wrapper kernel ``op2_wrap`` for GlobalKernel ``gk``.
 
@@ -190,6 +205,9 @@ def generate_source(gk, map_index, block_flags=None):
     block_flags = block_flags or {}
     lk = gk.local_kernel
     extruded = getattr(gk, "_extruded", False)
+    region = getattr(gk, "_iteration_region", None) if extruded else None
+    # interior facets: every map names the cell below, then the cell above
+    nf = reg.facet_factor(region)
     # map offsets per unique map (extruded maps step by offset[i] per layer)
     offsets = {}
     for k, garg in enumerate(gk.arguments):
@@ -198,11 +216,18 @@ def generate_source(gk, map_index, block_flags=None):
                 offsets[map_index[(k, j)]] = mk.offset
 
     def _mv(m, ar, i):
-        """Entry ``i`` of unique map ``m`` for element ``n`` (and layer)."""
-        base = f"map{m}[(int64_t)n * {ar} + {i}]"
-        if m in offsets:
-            return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * lyr_ : -1)"
-        return base
+        """Entry ``i`` of unique map ``m`` (arity ``ar``) for element ``n``
+        (and layer); with interior facets ``i`` runs over ``2 * ar`` entries,
+        the cell below first (``builder.py:81-129``)."""
+        if nf == 1:
+            base = f"map{m}[(int64_t)n * {ar} + {i}]"
+            if m in offsets:
+                return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * lyr_ : -1)"
+            return base
+        if m not in offsets:
+            raise ValueError("ON_INTERIOR_FACETS needs extruded maps (maps with an offset)")
+        base = f"map{m}[(int64_t)n * {ar} + ({i}) % {ar}]"
+        return f"(({base}) >= 0 ? ({base}) + off{m}[({i}) % {ar}] * (lyr_ + ({i}) / {ar}) : -1)"
     headers = "\n".join(_strip_headers(h) for h in lk.headers)
     code = _strip_headers(lk.code)
     params = ["int32_t lo", "int32_t hi", "const int32_t *__restrict__ list", "int32_t atomic"]
@@ -219,20 +244,21 @@ def generate_source(gk, map_index, block_flags=None):
                 continue
             m = map_index[(k, 0)]
             ar = garg.map_.arity
-            pre.append(f"    int32_t mi{k}[{ar}];")
-            pre.append(f"    for (int i = 0; i < {ar}; ++i) mi{k}[i] = {_mv(m, ar, 'i')};")
-            pre.append(f"    {t} buf{k}[{ar * dim}];")
+            ear = nf * ar
+            pre.append(f"    int32_t mi{k}[{ear}];")
+            pre.append(f"    for (int i = 0; i < {ear}; ++i) mi{k}[i] = {_mv(m, ar, 'i')};")
+            pre.append(f"    {t} buf{k}[{ear * dim}];")
             if acc in (INC, WRITE):
-                pre.append(f"    for (int q = 0; q < {ar * dim}; ++q) buf{k}[q] = ({t})0;")
+                pre.append(f"    for (int q = 0; q < {ear * dim}; ++q) buf{k}[q] = ({t})0;")
             else:
-                pre.append(f"    for (int i = 0; i < {ar}; ++i) for (int j = 0; j < {dim}; ++j)"
+                pre.append(f"    for (int i = 0; i < {ear}; ++i) for (int j = 0; j < {dim}; ++j)"
                            f" buf{k}[i * {dim} + j] = mi{k}[i] >= 0 ? arg{k}[(int64_t)mi{k}[i] * {dim} + j] : ({t})0;")
             call.append(f"op2_ptr_{{(void *)buf{k}}}")
             if acc == READ:
                 continue
             tgt = f"arg{k}[(int64_t)mi{k}[i] * {dim} + j]"
             val = f"buf{k}[i * {dim} + j]"
-            post.append(f"    for (int i = 0; i < {ar}; ++i) {{ if (mi{k}[i] < 0) continue;"
+            post.append(f"    for (int i = 0; i < {ear}; ++i) {{ if (mi{k}[i] < 0) continue;"
                         f" for (int j = 0; j < {dim}; ++j) {{ {_dat_store(acc, tgt, val)} }} }}")
         elif isinstance(garg, GlobalKernelArg):
             dim = int(np.prod(garg.dim)) if len(garg.dim) else 1
@@ -257,27 +283,28 @@ def generate_source(gk, map_index, block_flags=None):
         elif isinstance(garg, MixedDatKernelArg):
             # one buffer: the packs of the components one after the other
             comps = garg.arguments
-            total = sum(c.map_.arity * int(np.prod(c.dim)) for c in comps)
+            total = sum(nf * c.map_.arity * int(np.prod(c.dim)) for c in comps)
             pre.append(f"    {t} buf{k}[{total}];")
             if acc in (INC, WRITE):
                 pre.append(f"    for (int q = 0; q < {total}; ++q) buf{k}[q] = ({t})0;")
             off = 0
             for c, comp in enumerate(comps):
                 dim, ar, m = int(np.prod(comp.dim)), comp.map_.arity, map_index[(k, c)]
+                ear = nf * ar
                 a_, mi_ = f"arg{k}_{c}", f"mi{k}_{c}"
                 params.append(f"{t} *__restrict__ {a_}")
-                pre.append(f"    int32_t {mi_}[{ar}];")
-                pre.append(f"    for (int i = 0; i < {ar}; ++i) {mi_}[i] = {_mv(m, ar, 'i')};")
+                pre.append(f"    int32_t {mi_}[{ear}];")
+                pre.append(f"    for (int i = 0; i < {ear}; ++i) {mi_}[i] = {_mv(m, ar, 'i')};")
                 if acc not in (INC, WRITE):
-                    pre.append(f"    for (int i = 0; i < {ar}; ++i) for (int j = 0; j < {dim}; ++j)"
+                    pre.append(f"    for (int i = 0; i < {ear}; ++i) for (int j = 0; j < {dim}; ++j)"
                                f" buf{k}[{off} + i * {dim} + j] = {mi_}[i] >= 0 ?"
                                f" {a_}[(int64_t){mi_}[i] * {dim} + j] : ({t})0;")
                 if acc != READ:
                     tgt = f"{a_}[(int64_t){mi_}[i] * {dim} + j]"
                     val = f"buf{k}[{off} + i * {dim} + j]"
-                    post.append(f"    for (int i = 0; i < {ar}; ++i) {{ if ({mi_}[i] < 0) continue;"
+                    post.append(f"    for (int i = 0; i < {ear}; ++i) {{ if ({mi_}[i] < 0) continue;"
                                 f" for (int j = 0; j < {dim}; ++j) {{ {_dat_store(acc, tgt, val)} }} }}")
-                off += ar * dim
+                off += ear * dim
             call.append(f"op2_ptr_{{(void *)buf{k}}}")
         elif isinstance(garg, MixedMatKernelArg):
             # one element matrix; block (i, j) is rows of row component i
@@ -286,8 +313,8 @@ def generate_source(gk, map_index, block_flags=None):
             blocks = garg.arguments
             rds = [blocks[i * nc].dims[0][0][0] for i in range(nr)]
             cds = [blocks[j].dims[0][0][1] for j in range(nc)]
-            rars = [blocks[i * nc].maps[0].arity for i in range(nr)]
-            cars = [blocks[j].maps[1].arity for j in range(nc)]
+            rars = [nf * blocks[i * nc].maps[0].arity for i in range(nr)]
+            cars = [nf * blocks[j].maps[1].arity for j in range(nc)]
             roff = [sum(rars[q] * rds[q] for q in range(i)) for i in range(nr)]
             coff = [sum(cars[q] * cds[q] for q in range(j)) for j in range(nc)]
             NR, NC = roff[-1] + rars[-1] * rds[-1], coff[-1] + cars[-1] * cds[-1]
@@ -308,15 +335,15 @@ def generate_source(gk, map_index, block_flags=None):
                     if flags[b]:
                         tgt = f"arg{k}_{b}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj]"
                         post.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
-                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {_mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar // nf, 'ii')}; if (r_ < 0) continue;\n"
+                            f"      for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {_mv(cm, car // nf, 'jj')}; if (c_ < 0) continue;\n"
                             f"        const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, r_, c_); if (s_ < 0) {{ atomicAdd(status, 1); continue; }}\n"
                             f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj) {{ {store.format(tgt=tgt, val=val)} }} }} }}")
                     else:
                         tgt = f"arg{k}_{b}[s_]"
                         post.append(
-                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar, 'ii')}; if (r_ < 0) continue;\n"
-                            f"      for (int ri = 0; ri < {rd}; ++ri) for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {_mv(cm, car, 'jj')}; if (c_ < 0) continue;\n"
+                            f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar // nf, 'ii')}; if (r_ < 0) continue;\n"
+                            f"      for (int ri = 0; ri < {rd}; ++ri) for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {_mv(cm, car // nf, 'jj')}; if (c_ < 0) continue;\n"
                             f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, r_ * {rd} + ri, c_ * {cd} + cj);"
                             f" if (s_ < 0) {{ atomicAdd(status, 1); continue; }} {store.format(tgt=tgt, val=val)} }} }} }}")
         elif isinstance(garg, MatKernelArg):
@@ -324,7 +351,7 @@ def generate_source(gk, map_index, block_flags=None):
             params += [f"const int32_t *__restrict__ rowptr{k}", f"const int32_t *__restrict__ colidx{k}",
                        f"{t} *__restrict__ arg{k}"]
             rm, cm = map_index[(k, 0)], map_index[(k, 1)]
-            rar, car = garg.maps[0].arity, garg.maps[1].arity
+            rar, car = nf * garg.maps[0].arity, nf * garg.maps[1].arity
             ncol = car * cd
             pre.append(f"    {t} buf{k}[{rar * rd * ncol}];")
             pre.append(f"    for (int q = 0; q < {rar * rd * ncol}; ++q) buf{k}[q] = ({t})0;")
@@ -338,16 +365,16 @@ def generate_source(gk, map_index, block_flags=None):
                 tgt = f"arg{k}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj]"
                 val = f"buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]"
                 post.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {_mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
-                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {_mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {_mv(rm, rar // nf, 'i')}; if (r_ < 0) continue;\n"
+                    f"      for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {_mv(cm, car // nf, 'j')}; if (c_ < 0) continue;\n"
                     f"        const int32_t s_ = op2_find_slot(rowptr{k}, colidx{k}, r_, c_); if (s_ < 0) {{ atomicAdd(status, 1); continue; }}\n"
                     f"        for (int ri = 0; ri < {rd}; ++ri) for (int cj = 0; cj < {cd}; ++cj) {{ {store.format(tgt=tgt, val=val)} }} }} }}")
             else:
                 tgt = f"arg{k}[s_]"
                 val = f"buf{k}[(i * {rd} + ri) * {ncol} + j * {cd} + cj]"
                 post.append(
-                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {_mv(rm, rar, 'i')}; if (r_ < 0) continue;\n"
-                    f"      for (int ri = 0; ri < {rd}; ++ri) for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {_mv(cm, car, 'j')}; if (c_ < 0) continue;\n"
+                    f"    for (int i = 0; i < {rar}; ++i) {{ const int32_t r_ = {_mv(rm, rar // nf, 'i')}; if (r_ < 0) continue;\n"
+                    f"      for (int ri = 0; ri < {rd}; ++ri) for (int j = 0; j < {car}; ++j) {{ const int32_t c_ = {_mv(cm, car // nf, 'j')}; if (c_ < 0) continue;\n"
                     f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}, colidx{k}, r_ * {rd} + ri, c_ * {cd} + cj);"
                     f" if (s_ < 0) {{ atomicAdd(status, 1); continue; }} {store.format(tgt=tgt, val=val)} }} }} }}")
         else:
@@ -378,8 +405,9 @@ def generate_source(gk, map_index, block_flags=None):
            *([] if not extruded else [
                *offdecl,
                "    const int32_t lay_ = %s;" % ("0" if gk._constant_layers else "n"),
-               "    const int32_t bottom_ = layers[2 * lay_], top_ = layers[2 * lay_ + 1] - 1;",
-               "    for (int32_t layer_ = bottom_; layer_ < top_; ++layer_) {",
+               "    const int32_t bottom_ = layers[2 * lay_], cend_ = layers[2 * lay_ + 1] - 1;",
+               "    const int32_t lo_ = %s, hi_ = %s;" % _region_bounds(region),
+               "    for (int32_t layer_ = lo_; layer_ < hi_; ++layer_) {",
                "    const int32_t lyr_ = layer_ - bottom_;",
                "    (void)lyr_;"]),
            *pre,
@@ -553,6 +581,10 @@ class GeneratedKernel:
         iterset = parloop.iterset
         extruded = getattr(self.gk, "_extruded", False)
         if extruded:
+            region = getattr(self.gk, "_iteration_region", None)
+            for m in maps:
+                if getattr(m, "offset", None) is not None:
+                    m.check_extruded(iterset, region)
             vals.append(ctypes.c_void_p(dev.ptr(iterset.device_layers)))
         L = dev.lib()
         s = dev.stream()
@@ -571,11 +603,13 @@ class GeneratedKernel:
                 elems = iterset.indices[start:end]
             else:
                 elems = np.arange(start, end, dtype=np.int32)
-            key = (id(iterset), start, end, tuple(id(m) for m in conflict))
+            key = (id(iterset), start, end, tuple(id(m) for m in conflict),
+                   int(getattr(self.gk, "_iteration_region", None) or 0))
             if extruded:
                 # a column's targets over all its layers (a thread runs the
                 # layers of its column one after the other)
-                targets = [(_ColumnTargets(m, iterset, start, end), id(m.toset)) for m in conflict]
+                region = getattr(self.gk, "_iteration_region", None)
+                targets = [(_ColumnTargets(m, iterset, start, end, region), id(m.toset)) for m in conflict]
                 elems = np.arange(end - start, dtype=np.int32)
                 plan = plans.colour_plan(conflict[0], elems, targets, ("extruded",) + key)
                 plan = _shifted_plan(plan, start, conflict[0], ("extruded-shift",) + key)
@@ -597,8 +631,8 @@ class _ColumnTargets:
     """Duck-typed map for colouring the columns [start, end) of an extruded
     loop: row c lists every target column start + c touches."""
 
-    def __init__(self, m, iterset, start, end):
-        self._table = m.column_table(iterset, np.arange(start, end))
+    def __init__(self, m, iterset, start, end, region=None):
+        self._table = m.column_table(iterset, np.arange(start, end), region)
         self.toset = m.toset
 
 

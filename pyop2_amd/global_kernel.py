@@ -153,8 +153,13 @@ class GlobalKernel:
                  constant_layers=False, subset=False, iteration_region=None, pass_layer_arg=False):
         if not len(local_kernel.accesses) == len(arguments):
             raise ValueError("Number of arguments passed to the local and global kernels do not match")
-        if extruded_periodic or iteration_region is not None:
-            raise NotImplementedError("periodic extrusion and iteration regions are not on the device path")
+        if extruded_periodic:
+            raise NotImplementedError("periodic extrusion is not on the device path")
+        from pyop2_amd.types import region as reg
+        if iteration_region is not None:
+            iteration_region = reg.normalise(iteration_region)
+            if iteration_region == reg.ON_INTERIOR_FACETS and not extruded:
+                raise ValueError("ON_INTERIOR_FACETS needs an extruded iteration set")
         if pass_layer_arg and not extruded:
             raise ValueError("Cannot request layer argument for non-extruded iteration")
         from pyop2_amd.types.glob import Constant
@@ -176,6 +181,9 @@ class GlobalKernel:
         self._extruded = bool(extruded)
         self._constant_layers = bool(constant_layers)
         self._pass_layer_arg = bool(pass_layer_arg)
+        # the layers a column runs (codegen/builder.py:771-795); no effect on
+        # a loop that is not extruded
+        self._iteration_region = iteration_region if extruded else None
         self._executor = None
 
     def __call__(self, comm, *args, parloop=None):

@@ -9,11 +9,13 @@ from pyop2_amd.parloop import (GlobalParloopArg, DatParloopArg, MatParloopArg,  
 from pyop2_amd.types import (Access, READ, WRITE, RW, INC, MIN, MAX,  # noqa: F401
                              Set, Subset, SetPartition, MixedSet, ExtrudedSet, DataSet, Map, Dat,
                              Global, Constant, Sparsity, Mat, PermutedMap, ComposedMap,
-                             MixedDataSet, MixedMap, MixedDat)
+                             MixedDataSet, MixedMap, MixedDat,
+                             IterationRegion, ON_BOTTOM, ON_TOP, ON_INTERIOR_FACETS, ALL)
 
 __all__ = ['configuration', 'READ', 'WRITE', 'RW', 'INC', 'MIN', 'MAX',
            'Set', 'Subset', 'ExtrudedSet', 'DataSet', 'MixedSet', 'MixedDataSet', 'Map', 'MixedMap',
            'PermutedMap', 'ComposedMap', 'MixedDat',
+           'ON_BOTTOM', 'ON_TOP', 'ON_INTERIOR_FACETS', 'ALL',
            'Dat', 'Global', 'Constant',
            'Kernel', 'Sparsity', 'Mat', 'par_loop', 'parloop', 'ParLoop', 'Parloop',
            'GlobalKernel', 'DatKernelArg', 'MatKernelArg', 'GlobalKernelArg', 'MapKernelArg',

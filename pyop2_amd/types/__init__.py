@@ -6,3 +6,5 @@ from pyop2_amd.types.mixed import MixedDataSet, MixedMap, MixedDat  # noqa: F401
 from pyop2_amd.types.dat import AbstractDat, Dat  # noqa: F401
 from pyop2_amd.types.glob import Global, Constant  # noqa: F401
 from pyop2_amd.types.mat import Sparsity, Mat  # noqa: F401
+from pyop2_amd.types.region import (IterationRegion, ON_BOTTOM, ON_TOP,  # noqa: F401
+                                     ON_INTERIOR_FACETS, ALL)

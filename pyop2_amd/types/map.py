@@ -46,35 +46,72 @@ class Map:
         off = tuple(int(x) for x in self._offset) if self._offset is not None else None
         return MapKernelArg(self.arity, off, None)
 
-    def column_table(self, iterset, columns):
-        """Entries of ``columns`` (host ids) of an extruded loop over all their
-        layers: ``(len(columns), nlayers_max * arity)``, entry (n, l, i) =
-        ``table[n][i] + offset[i] * l`` for the column's cells l, -1 past
-        its top (the targets a column touches, for colouring and patterns)."""
+    def column_table(self, iterset, columns, region=None):
+        """Entries of ``columns`` (host ids) of an extruded loop over the
+        layers ``region`` runs (all cells by default): ``(len(columns),
+        nsteps * nf * arity)``; entry (c, s, f, i) = ``table[c][i] +
+        offset[i] * (lo_c + s + f)`` for the column's steps s, -1 past them
+        (``nf`` = 2 for interior facets: the cell below, then the one above).
+        These are the targets a column touches, for colouring and patterns."""
+        from pyop2_amd.types import region as reg
         cols = np.asarray(columns, dtype=np.int64)
         lay = iterset.column_layers(cols)
         ncell = (lay[:, 1] - lay[:, 0] - 1) if len(cols) else np.zeros(0, dtype=np.int64)
-        nl = int(ncell.max()) if len(cols) else 0
+        lo, hi = reg.layer_steps(ncell, region)
+        nsteps = hi - lo
+        ns = int(nsteps.max()) if len(cols) else 0
+        nf = reg.facet_factor(region)
         base = self._table[cols].astype(np.int64)
         off = self._offset if self._offset is not None else np.zeros(self._arity, dtype=IntType)
-        steps = np.arange(nl, dtype=np.int64)
-        out = base[:, None, :] + steps[None, :, None] * off[None, None, :]
-        out = np.where((steps[None, :] < ncell[:, None])[:, :, None] & (base[:, None, :] >= 0), out, -1)
-        return np.ascontiguousarray(out.reshape(len(cols), nl * self._arity), dtype=IntType)
+        step = lo[:, None, None] + np.arange(ns, dtype=np.int64)[None, :, None] + np.arange(nf)[None, None, :]
+        out = base[:, None, None, :] + step[..., None] * off[None, None, None, :]
+        live = (np.arange(ns)[None, :] < nsteps[:, None])[:, :, None, None] & (base[:, None, None, :] >= 0)
+        out = np.where(live, out, -1)
+        return np.ascontiguousarray(out.reshape(len(cols), ns * nf * self._arity), dtype=IntType)
 
-    def cell_table(self, iterset, columns):
-        """Entries of every cell of ``columns`` of an extruded loop, one row
-        per cell (columns in order, cells bottom to top)."""
-        t = self.column_table(iterset, columns).reshape(len(columns), -1, self._arity)
-        return np.ascontiguousarray(t[self.column_cells_mask(iterset, columns)], dtype=IntType)
+    def cell_table(self, iterset, columns, region=None):
+        """Entries of every iteration of ``columns`` of an extruded loop over
+        ``region``, one row per (column, layer), columns in order, layers
+        bottom to top (``nf * arity`` entries a row)."""
+        from pyop2_amd.types import region as reg
+        nf = reg.facet_factor(region)
+        t = self.column_table(iterset, columns, region).reshape(len(columns), -1, nf * self._arity)
+        return np.ascontiguousarray(t[self.column_cells_mask(iterset, columns, region)], dtype=IntType)
+
+    def check_extruded(self, iterset, region=None):
+        """Raise MapValueError unless every entry an extruded loop over
+        ``region`` of ``iterset`` reaches through this map lies in the
+        target set (checked once per iterset and region).  Entries are
+        linear in the layer, so the column's first and last steps bound
+        them."""
+        from pyop2_amd.types import region as reg
+        key = ("extruded-checked", id(iterset), int(reg.normalise(region)))
+        if self._offset is None or key in self._cache:
+            return
+        cols = np.arange(iterset.total_size, dtype=np.int64)
+        lay = iterset.column_layers(cols)
+        lo, hi = reg.layer_steps(lay[:, 1] - lay[:, 0] - 1, region)
+        base = self._table[cols].astype(np.int64)
+        live = (hi > lo)[:, None] & (base >= 0)
+        off = self._offset.astype(np.int64)[None, :]
+        first = base + off * lo[:, None]
+        last = base + off * (hi - 1 + reg.facet_factor(region) - 1)[:, None]
+        n = self._toset.total_size
+        if live.any() and (min(first[live].min(), last[live].min()) < 0
+                           or max(first[live].max(), last[live].max()) >= n):
+            raise ex.MapValueError(f"{self} with offsets reaches entries outside [0, {n}) "
+                                   f"in the layers of {iterset}")
+        self._cache[key] = True
 
     @staticmethod
-    def column_cells_mask(iterset, columns):
+    def column_cells_mask(iterset, columns, region=None):
+        from pyop2_amd.types import region as reg
         cols = np.asarray(columns, dtype=np.int64)
         lay = iterset.column_layers(cols)
-        ncell = lay[:, 1] - lay[:, 0] - 1
-        nl = int(ncell.max()) if len(cols) else 0
-        return np.arange(nl)[None, :] < ncell[:, None]
+        lo, hi = reg.layer_steps(lay[:, 1] - lay[:, 0] - 1, region)
+        nsteps = hi - lo
+        ns = int(nsteps.max()) if len(cols) else 0
+        return np.arange(ns)[None, :] < nsteps[:, None]
 
     @property
     def _table(self):

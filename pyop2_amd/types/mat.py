@@ -23,22 +23,27 @@ from pyop2_amd import caching, configuration as conf, exceptions as ex, utils
 from pyop2_amd.types.access import Access
 from pyop2_amd.types.data_carrier import DataCarrier
 from pyop2_amd.types.dataset import DataSet
+from pyop2_amd.types import region as reg
 from pyop2_amd.types.map import Map
+from pyop2_amd.types.region import ALL
 from pyop2_amd.types.mixed import MixedDataSet, MixedMap
 from pyop2_amd.types.set import MixedSet, Subset
 
 
-def _cells_map(m):
-    """A map of an extruded loop as a plain map over its owned cells (columns
-    in order, cells bottom to top), cached on ``m``: the sparsity of an
-    extruded pair is the union over cells (``sparsity.pyx:285-393``)."""
+def _cells_map(m, region=None):
+    """A map of an extruded loop over ``region`` as a plain map over its
+    iterations (owned columns in order, layers bottom to top; twice the
+    arity for interior facets), cached on ``m``: the sparsity of an extruded
+    pair is the union over the iterations (``sparsity.pyx:285-393``)."""
+    from pyop2_amd.types import region as reg
     from pyop2_amd.types.set import Set
-    key = ("cells_map",)
+    region = reg.normalise(region)
+    key = ("cells_map", int(region))
     f = m._cache.get(key)
     if f is None:
         it = m.iterset
-        tab = m.cell_table(it, np.arange(it.size))
-        f = Map(Set(len(tab)), m.toset, m.arity, tab)
+        tab = m.cell_table(it, np.arange(it.size), region)
+        f = Map(Set(len(tab)), m.toset, reg.facet_factor(region) * m.arity, tab)
         m._cache[key] = f
     return f
 
@@ -119,9 +124,11 @@ class Sparsity(caching.ObjectCached):
                     raise RuntimeError("Map toset must be the same as DataSet set")
                 if rmap.iterset is not cmap.iterset:
                     raise RuntimeError("Iterset of both maps in a pair must be the same")
-                if iteration_regions is not None:
-                    raise NotImplementedError("iteration regions (extruded) are not on the device path")
-                pv.add((rmap, cmap, None))
+                if iteration_regions is None:
+                    iteration_regions = (ALL,)
+                else:
+                    iteration_regions = tuple(sorted(reg.normalise(r) for r in iteration_regions))
+                pv.add((rmap, cmap, iteration_regions))
             if pv:
                 processed[key] = frozenset(pv)
         cache = dsets[0].set
@@ -155,7 +162,7 @@ class Sparsity(caching.ObjectCached):
 
     @utils.cached_property
     def iteration_regions(self):
-        return {key: [reg for _, _, reg in val] for key, val in self._maps_and_regions.items()}
+        return {key: [regions for _, _, regions in val] for key, val in self._maps_and_regions.items()}
 
     @property
     def dims(self):
@@ -204,13 +211,17 @@ class Sparsity(caching.ObjectCached):
         from pyop2_amd import device as dev, plans
         L = dev.lib()
         s = dev.stream()
-        pairs = self.rcmaps[(0, 0)]
+        pairs = []
+        for rmap, cmap, regions in sorted(self._maps_and_regions[(0, 0)],
+                                          key=lambda t: (id(t[0]), id(t[1]))):
+            if getattr(rmap.iterset, "_extruded", False):
+                pairs += [(_cells_map(rmap, r), _cells_map(cmap, r)) for r in regions]
+            else:
+                pairs.append((rmap, cmap))
         nrows, ncols = self.nrows, self.ncols
         arr = (dev.MapPair * max(len(pairs), 1))()
         incp, ince = [], []
         for k, (rmap, cmap) in enumerate(pairs):
-            if getattr(rmap.iterset, "_extruded", False):
-                rmap, cmap = _cells_map(rmap), _cells_map(cmap)
             size = rmap.iterset.size
             tr = plans.transpose(rmap, nrows, size)
             incp.append(tr.inc_ptr)

@@ -1042,6 +1042,131 @@ def case_extruded_matrix(op2, split=None):
     assert int(np.sum(sp.nnz)) == int(np.count_nonzero(want + np.eye(len(want))))
 
 
+def case_extruded_bottom_top(op2, split=None):
+    """Iteration regions (builder.py:771-795).  An INC of 1 through the
+    6-entry dof map over the bottom cells reaches layers 0 and 1 of every
+    node, over the top cells layers XLAYERS - 2 and XLAYERS - 1, each as
+    often as base elements share the node (elements k - 2 .. k that exist);
+    a direct INC counts one cell per column; a WRITE through the cell map on
+    the top cells sets only the top cell of every column, and the layer
+    argument there is XWEDGES - 1."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    share = np.array([min(k, XN - 1) - max(k - 2, 0) + 1 for k in range(XNODES)], dtype=np.float64)
+    one6 = op2.Kernel("static void one6(double x[6]) { for (int i = 0; i < 6; ++i) x[i] += 1.0; }", "one6")
+    one = op2.Kernel("static void one(double *x) { *x += 1.0; }", "one")
+    for region, lays in ((op2.ON_BOTTOM, (0, 1)), (op2.ON_TOP, (XLAYERS - 2, XLAYERS - 1))):
+        cnt = op2.Dat(node_set ** 1, np.zeros(XNODES * XLAYERS), np.float64)
+        op2.par_loop(one6, elements, cnt(op2.INC, cmap), iteration_region=region)
+        want = np.zeros((XNODES, XLAYERS))
+        for lay in lays:
+            want[:, lay] = share
+        assert np.array_equal(np.asarray(cnt.data), want.reshape(-1)), region
+        col = op2.Dat(base ** 1, np.zeros(XN), np.float64)
+        op2.par_loop(one, elements, col(op2.INC), iteration_region=region)
+        assert np.array_equal(np.asarray(col.data), np.ones(XN)), region
+    f = op2.Dat(elem_set ** 1, -np.ones(XN * XWEDGES), np.float64)
+    k = op2.Kernel("static void top(double *y, int layer) { y[0] = 100.0 + layer; }", "top")
+    op2.par_loop(k, elements, f(op2.WRITE, fmap), iteration_region=op2.ON_TOP, pass_layer_arg=True)
+    want = [100.0 + lyr if lyr == XWEDGES - 1 else -1.0 for _ in range(XN) for lyr in range(XWEDGES)]
+    assert list(np.asarray(f.data)) == want
+
+
+def case_extruded_interior_facets(op2, split=None):
+    """ON_INTERIOR_FACETS (builder.py:81-129, 771-795): every map names the
+    cell below, then the cell above.  INC of 1 below and 10 above into the
+    cell field gives cell l 1 if a cell lies above it plus 10 if one lies
+    below; through the dof map the upper cell's dof ids exceed the lower's
+    by one layer each (6 per facet); reading the cell field c = id^2 the
+    jumps c[above] - c[below] telescope to c[top] - c[bottom] per column; a
+    direct INC counts the XWEDGES - 1 facets of a column, and the layer
+    argument runs 0 .. XWEDGES - 2."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    fac = op2.ON_INTERIOR_FACETS
+    f = op2.Dat(elem_set ** 1, np.zeros(XN * XWEDGES), np.float64)
+    k = op2.Kernel("static void jmp(double c[2]) { c[0] += 1.0; c[1] += 10.0; }", "jmp")
+    op2.par_loop(k, elements, f(op2.INC, fmap), iteration_region=fac)
+    want = [(1.0 if lyr < XWEDGES - 1 else 0.0) + (10.0 if lyr > 0 else 0.0)
+            for _ in range(XN) for lyr in range(XWEDGES)]
+    assert list(np.asarray(f.data)) == want
+    ids = op2.Dat(node_set ** 1, np.arange(XNODES * XLAYERS, dtype=np.float64), np.float64)
+    g = op2.Global(1, 0.0, np.float64, "g")
+    k2 = op2.Kernel("static void up(double *g, const double x[12])"
+                    " { for (int i = 0; i < 6; ++i) g[0] += x[6 + i] - x[i]; }", "up")
+    op2.par_loop(k2, elements, g(op2.INC), ids(op2.READ, cmap), iteration_region=fac)
+    assert float(np.asarray(g.data).reshape(-1)[0]) == 6.0 * XN * (XWEDGES - 1)
+    sq = op2.Dat(elem_set ** 1, np.arange(XN * XWEDGES, dtype=np.float64) ** 2, np.float64)
+    h = op2.Global(1, 0.0, np.float64, "h")
+    k3 = op2.Kernel("static void dif(double *h, const double c[2]) { h[0] += c[1] - c[0]; }", "dif")
+    op2.par_loop(k3, elements, h(op2.INC), sq(op2.READ, fmap), iteration_region=fac)
+    tele = sum(float((i * XWEDGES + XWEDGES - 1) ** 2 - (i * XWEDGES) ** 2) for i in range(XN))
+    assert float(np.asarray(h.data).reshape(-1)[0]) == tele
+    col = op2.Dat(base ** 1, np.zeros(XN), np.float64)
+    lsum = op2.Global(1, 0.0, np.float64, "lsum")
+    k4 = op2.Kernel("static void cnt(double *x, double *s, int layer) { *x += 1.0; s[0] += layer; }", "cnt")
+    op2.par_loop(k4, elements, col(op2.INC), lsum(op2.INC), iteration_region=fac, pass_layer_arg=True)
+    assert np.array_equal(np.asarray(col.data), np.full(XN, XWEDGES - 1.0))
+    assert float(np.asarray(lsum.data).reshape(-1)[0]) == XN * sum(range(XWEDGES - 1))
+
+
+def case_extruded_facet_matrix(op2, split=None):
+    """A cell-to-cell Mat over interior facets (sparsity.pyx:285-393): the
+    jump matrix [[1, -1], [-1, 1]] per facet gives, per column, the path
+    Laplacian of its XWEDGES cells (diagonal 1 at the ends, 2 inside, -1
+    between neighbours); a second pair over the bottom and top cells adds 5
+    to those cells' diagonal.  The pattern is the diagonal plus the two
+    neighbours."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    sp = op2.Sparsity((elem_set ** 1, elem_set ** 1),
+                      [(fmap, fmap, (op2.ON_INTERIOR_FACETS,)), (fmap, fmap, (op2.ON_BOTTOM, op2.ON_TOP))])
+    A = op2.Mat(sp, np.float64)
+    jump = op2.Kernel("static void jump(double a[2][2])"
+                      " { a[0][0] += 1.0; a[0][1] -= 1.0; a[1][0] -= 1.0; a[1][1] += 1.0; }", "jump")
+    op2.par_loop(jump, elements, A(op2.INC, (fmap, fmap)), iteration_region=op2.ON_INTERIOR_FACETS)
+    five = op2.Kernel("static void five(double a[1][1]) { a[0][0] += 5.0; }", "five")
+    for region in (op2.ON_BOTTOM, op2.ON_TOP):
+        op2.par_loop(five, elements, A(op2.INC, (fmap, fmap)), iteration_region=region)
+    A.assemble()
+    n = XN * XWEDGES
+    want = np.zeros((n, n))
+    for i in range(XN):
+        for lyr in range(XWEDGES):
+            c = i * XWEDGES + lyr
+            want[c, c] = (lyr > 0) + (lyr < XWEDGES - 1) + 5.0 * (lyr == 0) + 5.0 * (lyr == XWEDGES - 1)
+            if lyr < XWEDGES - 1:
+                want[c, c + 1] = want[c + 1, c] = -1.0
+    assert np.array_equal(np.asarray(A.values), want)
+    assert int(np.sum(sp.nnz)) == int(np.count_nonzero(want))
+
+
+def case_extruded_variable_regions(op2, split=None):
+    """Regions over variable layers [[0, 3], [1, 5], [2, 3], [0, 6]] (2, 3, 0
+    and 5 cells; builder.py:771-795): per column ON_BOTTOM runs one layer
+    (also on the column with no cell: its bottom layer, as the reference's
+    bounds give), ON_TOP one (none on the empty column), ON_INTERIOR_FACETS
+    cells - 1, ALL cells; through a map stepping by 1 from 10 * column,
+    interior facets add 1 to the cell below and 2 to the cell above."""
+    base = op2.Set(4)
+    lay = np.array([[0, 3], [1, 5], [2, 3], [0, 6]], dtype=np.int32)
+    ext = op2.ExtrudedSet(base, lay)
+    one = op2.Kernel("static void one(double *x) { *x += 1.0; }", "one")
+    for region, want in ((op2.ON_BOTTOM, [1, 1, 1, 1]), (op2.ON_TOP, [1, 1, 0, 1]),
+                         (op2.ON_INTERIOR_FACETS, [1, 2, 0, 4]), (op2.ALL, [2, 3, 0, 5])):
+        cnt = op2.Dat(base ** 1, np.zeros(4), np.float64)
+        op2.par_loop(one, ext, cnt(op2.INC), iteration_region=region)
+        assert list(np.asarray(cnt.data)) == want, region
+    tgt = op2.Set(40)
+    m = op2.Map(ext, tgt, 1, np.array([0, 10, 20, 30], dtype=np.int32), offset=np.ones(1, dtype=np.int32))
+    d = op2.Dat(tgt ** 1, np.zeros(40), np.float64)
+    k = op2.Kernel("static void jmp2(double c[2]) { c[0] += 1.0; c[1] += 2.0; }", "jmp2")
+    op2.par_loop(k, ext, d(op2.INC, m), iteration_region=op2.ON_INTERIOR_FACETS)
+    want = np.zeros(40)
+    for c, ncell in enumerate([2, 3, 0, 5]):
+        for s in range(ncell - 1):
+            want[10 * c + s] += 1.0
+            want[10 * c + s + 1] += 2.0
+    assert np.array_equal(np.asarray(d.data), want)
+
+
 EXTRUDED_CASES = [
     ("extruded_volume", case_extruded_volume),
     ("extruded_direct_inc", case_extruded_direct_inc),
@@ -1051,4 +1176,8 @@ EXTRUDED_CASES = [
     ("extruded_rw", case_extruded_rw),
     ("extruded_variable_layers", case_extruded_variable_layers),
     ("extruded_matrix", case_extruded_matrix),
+    ("extruded_bottom_top", case_extruded_bottom_top),
+    ("extruded_interior_facets", case_extruded_interior_facets),
+    ("extruded_facet_matrix", case_extruded_facet_matrix),
+    ("extruded_variable_regions", case_extruded_variable_regions),
 ]

@@ -7,6 +7,7 @@ import numpy as np
 from oracle import seqwrap, sparsity as osp
 
 READ, WRITE, RW, INC, MIN, MAX = "READ", "WRITE", "RW", "INC", "MIN", "MAX"
+ON_BOTTOM, ON_TOP, ON_INTERIOR_FACETS, ALL = "bottom", "top", "interior", "all"
 
 
 class MapTypeError(TypeError):
@@ -95,6 +96,22 @@ class ExtrudedSet(Set):
         a, b = self.layers_array[0 if len(self.layers_array) == 1 else n]
         return range(b - a - 1)
 
+    def layer_range(self, n, region=ALL):
+        """Layers column ``n`` runs in ``region`` (builder.py:771-795,
+        sparsity.pyx:345-356); a column with no cells runs no top cell."""
+        bottom, top = (int(v) for v in self.layers_array[0 if len(self.layers_array) == 1 else n])
+        cend = top - 1
+        if region == ON_BOTTOM:
+            return range(bottom, bottom + 1)
+        if region == ON_TOP:
+            return range(max(cend - 1, bottom), cend)
+        if region == ON_INTERIOR_FACETS:
+            return range(bottom, cend - 1)
+        return range(bottom, cend)
+
+    def bottom(self, n):
+        return int(self.layers_array[0 if len(self.layers_array) == 1 else n][0])
+
 
 class Map:
     """``table`` is what a loop gathers through: entry ``k`` of element ``n``
@@ -107,14 +124,19 @@ class Map:
         self.offset = None if offset is None else np.broadcast_to(np.asarray(offset, dtype=np.int32),
                                                                   (arity,)).copy()
 
-    def cell_rows(self):
-        """One row of entries per cell of an extruded iteration (owned columns)."""
+    def cell_rows(self, region=ALL):
+        """One row of entries per iteration of an extruded loop over
+        ``region`` (owned columns); interior facets list the cell below,
+        then the cell above (sparsity.pyx:357-377, with the layer counted
+        from the column's bottom)."""
+        nf = 2 if region == ON_INTERIOR_FACETS else 1
         rows = []
         for n in range(self.iterset.size):
-            for lyr in self.iterset.cells(n):
-                rows.append([int(v) + int(self.offset[k]) * lyr if v >= 0 else -1
-                             for k, v in enumerate(self.table[n])])
-        return np.array(rows, dtype=np.int32).reshape(-1, self.arity)
+            b = self.iterset.bottom(n)
+            for layer in self.iterset.layer_range(n, region):
+                rows.append([int(v) + int(self.offset[k]) * (layer - b + f) if v >= 0 else -1
+                             for f in range(nf) for k, v in enumerate(self.table[n])])
+        return np.array(rows, dtype=np.int32).reshape(-1, nf * self.arity)
 
     @property
     def table(self):
@@ -258,12 +280,14 @@ class Sparsity:
         rset, cset = dsets
         self.rdim, self.cdim = rset.cdim, cset.cdim
         self.pairs = [(p[0], p[1]) for p in pairs]
+        regions = [tuple(p[2]) if len(p) > 2 and p[2] is not None else (ALL,) for p in pairs]
         diag = rset is cset or (rset.set is cset.set and rset.dim == cset.dim)
         tabs = []
-        for r, c in self.pairs:
+        for (r, c), regs in zip(self.pairs, regions):
             if isinstance(r.iterset, ExtrudedSet):
-                rt, ct = r.cell_rows(), c.cell_rows()
-                tabs.append((rt, ct, len(rt)))
+                for region in regs:
+                    rt, ct = r.cell_rows(region), c.cell_rows(region)
+                    tabs.append((rt, ct, len(rt)))
             else:
                 tabs.append((r.table, c.table, r.iterset.size))
         self.rowptr, self.colidx = osp.pattern(rset.set.total_size, cset.set.total_size, tabs, set_diag=diag)
@@ -356,7 +380,7 @@ class ParLoop:
         par_loop(self.kernel, self.iterset, *self.args)
 
 
-def par_loop(kernel, iterset, *args, pass_layer_arg=False):
+def par_loop(kernel, iterset, *args, pass_layer_arg=False, iteration_region=None):
     sargs = []
     temps = []
     for kind, obj, access, path in args:
@@ -395,7 +419,8 @@ def par_loop(kernel, iterset, *args, pass_layer_arg=False):
             for m in (path if isinstance(path, tuple) else (path,)):
                 if isinstance(m, Map) and m.offset is not None:
                     offs.append((m.table, m.offset))
-        ext = dict(layers=iterset.layers_array, offsets=offs, pass_layer=pass_layer_arg)
+        ext = dict(layers=iterset.layers_array, offsets=offs, pass_layer=pass_layer_arg,
+                   region=iteration_region or ALL)
     err = seqwrap.run(kernel.code, kernel.name, sargs, parts, subset=subset, **ext)
     assert err == 0, "insertion outside the sparsity pattern"
     for g, tmp in temps:

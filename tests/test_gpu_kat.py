@@ -116,6 +116,66 @@ def test_mixed_random_against_oracle(device, strategy):
 
 
 @pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+def test_extruded_regions_random_against_oracle(device, strategy):
+    """Prism columns over a 6x6 triangle mesh with random variable layers
+    (1 to 6 cells a column): a vertex Mat and Dat over all cells, a cell Mat
+    and Dat over interior facets (jumps of a random cell field), a vertex
+    Dat over the bottom and top cells with the layer argument; device
+    against the oracle's sequential wrapper on the same inputs (1e-12), and
+    the same patterns."""
+    import numpy as np
+    import oracle_op2
+    from pyop2_amd.meshes import unit_square_tri, workload_rng
+    configuration["strategy"] = strategy
+    m = unit_square_tri(6, rng=workload_rng(2))
+    rng = np.random.default_rng(12)
+    nl = 8                                   # vertex layers reserved a column
+    bottom = rng.integers(0, 2, m.ncells).astype(np.int32)
+    top = (bottom + rng.integers(2, 8, m.ncells)).astype(np.int32)
+    lay = np.stack([bottom, np.minimum(top, nl)], axis=1).astype(np.int32)
+    # a vertex's layers start at 0 in every column, a cell's at its column's bottom
+    vbase = np.array([[v * nl + b + d for v in tri for d in (0, 1)] for tri, b in zip(m.cell_vertex, bottom)],
+                     dtype=np.int32)
+    cbase = (np.arange(m.ncells) * nl + bottom).astype(np.int32)
+    u, q = rng.random(m.nverts * nl), rng.random(m.ncells * nl)
+    k_all = ("static void pr(double A[6][6], double b[6], const double x[6]) {"
+             " for (int i = 0; i < 6; ++i) { b[i] += x[i] * (i + 1);"
+             " for (int j = 0; j < 6; ++j) A[i][j] += x[i] * x[j] + 0.5 * i - 0.25 * j; } }")
+    k_fac = ("static void fc(double A[2][2], double c[2], const double q[2]) {"
+             " c[0] += q[1] - 0.5 * q[0]; c[1] -= 0.25 * q[0] + q[1];"
+             " A[0][0] += q[0]; A[0][1] -= q[1]; A[1][0] -= q[0] * q[1]; A[1][1] += 2.0 * q[1]; }")
+    k_end = ("static void ed(double b[6], const double x[6], int layer) {"
+             " for (int i = 0; i < 6; ++i) b[i] += x[i] * (layer + 1) + i; }")
+    out = []
+    for ns in (op2, oracle_op2):
+        base = ns.Set(m.ncells)
+        ext = ns.ExtrudedSet(base, lay)
+        verts, cells = ns.Set(m.nverts * nl), ns.Set(m.ncells * nl)
+        vmap = ns.Map(ext, verts, 6, vbase, offset=np.ones(6, dtype=np.int32))
+        cmap = ns.Map(ext, cells, 1, cbase, offset=np.ones(1, dtype=np.int32))
+        ud = ns.Dat(verts ** 1, u.copy(), np.float64)
+        qd = ns.Dat(cells ** 1, q.copy(), np.float64)
+        b, be = ns.Dat(verts ** 1, np.zeros(m.nverts * nl), np.float64), ns.Dat(verts ** 1, np.zeros(m.nverts * nl), np.float64)
+        c = ns.Dat(cells ** 1, np.zeros(m.ncells * nl), np.float64)
+        Av = ns.Mat(ns.Sparsity((verts ** 1, verts ** 1), [(vmap, vmap, None)]))
+        Ac = ns.Mat(ns.Sparsity((cells ** 1, cells ** 1), [(cmap, cmap, (ns.ALL, ns.ON_INTERIOR_FACETS))]))
+        ns.par_loop(ns.Kernel(k_all, "pr"), ext, Av(ns.INC, (vmap, vmap)), b(ns.INC, vmap), ud(ns.READ, vmap))
+        ns.par_loop(ns.Kernel(k_fac, "fc"), ext, Ac(ns.INC, (cmap, cmap)), c(ns.INC, cmap), qd(ns.READ, cmap),
+                    iteration_region=ns.ON_INTERIOR_FACETS)
+        for region in (ns.ON_BOTTOM, ns.ON_TOP):
+            ns.par_loop(ns.Kernel(k_end, "ed"), ext, be(ns.INC, vmap), ud(ns.READ, vmap),
+                        iteration_region=region, pass_layer_arg=True)
+        Av.assemble()
+        Ac.assemble()
+        out.append([np.asarray(Av.values), np.asarray(Ac.values), np.asarray(b.data_ro),
+                    np.asarray(c.data_ro), np.asarray(be.data_ro)])
+    for x, y in zip(*out):
+        assert np.count_nonzero(y) > 0
+        assert np.linalg.norm(x - y) <= 1e-12 * np.linalg.norm(y)
+    assert np.array_equal(out[0][1] != 0, out[1][1] != 0)
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
 @pytest.mark.parametrize("name,case", kat_cases.EXTRUDED_CASES, ids=[n for n, _ in kat_cases.EXTRUDED_CASES])
 def test_extruded_cases(device, name, case, strategy):
     configuration["strategy"] = strategy

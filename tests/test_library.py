@@ -287,6 +287,64 @@ def test_generated_wrappers_compile_for_gfx950():
         assert size.value > 1000
 
 
+def test_generated_extruded_region_wrappers_compile_for_gfx950():
+    """Generated wrappers of extruded loops over each iteration region
+    (doubled packs for interior facets, Dat, Mat and Global arguments, the
+    layer argument) compile with hiprtc."""
+    from pyop2_amd import op2
+    from pyop2_amd.codegen import GeneratedKernel, compile_options
+    from pyop2_amd.parloop import LegacyParloop
+    base = op2.Set(4)
+    ext = op2.ExtrudedSet(base, np.array([[0, 3], [1, 5], [2, 3], [0, 6]], dtype=np.int32))
+    cells = op2.Set(40)
+    m = op2.Map(ext, cells, 2, np.array([[0, 1], [10, 11], [20, 21], [30, 31]], dtype=np.int32),
+                offset=np.full(2, 2, dtype=np.int32))
+    x = op2.Dat(cells ** 1, dtype=np.float64)
+    g = op2.Global(1, 0.0, np.float64)
+    sp = op2.Sparsity((cells ** 1, cells ** 1), [(m, m, (op2.ON_INTERIOR_FACETS, op2.ON_TOP))])
+    A = op2.Mat(sp)
+    cases = [
+        ("static void f1(double x[4], double *g) { x[0] += x[3]; g[0] += 1.0; }", "f1",
+         (x(op2.INC, m), g(op2.INC)), op2.ON_INTERIOR_FACETS, False),
+        ("static void f2(double A[4][4], const double *x) { A[0][3] = x[2]; }", "f2",
+         (A(op2.INC, (m, m)), x(op2.READ, m)), op2.ON_INTERIOR_FACETS, False),
+        ("static void f3(double x[2], int layer) { x[1] = layer; }", "f3",
+         (x(op2.WRITE, m),), op2.ON_TOP, True),
+        ("static void f4(double x[2]) { x[0] += 1.0; }", "f4",
+         (x(op2.INC, m),), op2.ON_BOTTOM, False),
+    ]
+    L = dev.lib()
+    opts = compile_options()
+    copts = (ctypes.c_char_p * len(opts))(*[o.encode() for o in opts])
+    for code, name, args, region, layer in cases:
+        pl = LegacyParloop(op2.Kernel(code, name), ext, *args, iteration_region=region, pass_layer_arg=layer)
+        src = GeneratedKernel(pl.global_kernel).source(pl)
+        size = ctypes.c_size_t(0)
+        log = ctypes.create_string_buffer(16384)
+        rc = L.op2hip_jit_code_object(src.encode(), copts, len(opts), None, ctypes.byref(size), log, len(log))
+        assert rc == 0, log.value.decode()
+        assert size.value > 1000
+
+
+def test_extruded_map_range_check():
+    """An extruded map whose offsets step past its target set in the layers
+    a region runs is refused before any launch; the same map passes over
+    the regions that stay inside."""
+    from pyop2_amd import op2
+    base = op2.Set(2)
+    ext = op2.ExtrudedSet(base, 4)          # 3 cells a column
+    tgt = op2.Set(7)
+    m = op2.Map(ext, tgt, 1, np.array([0, 5], dtype=np.int32), offset=np.ones(1, dtype=np.int32))
+    m.check_extruded(ext, op2.ON_BOTTOM)    # entries 0 and 5
+    for region in (op2.ALL, op2.ON_TOP, op2.ON_INTERIOR_FACETS):
+        with pytest.raises(op2.MapValueError):
+            m.check_extruded(ext, region)   # column 1 reaches 5 + 2 = 7
+    d = op2.Dat(base ** 1, np.zeros(2), np.float64)
+    with pytest.raises(ValueError):
+        op2.par_loop(op2.Kernel("static void k(double *x) { *x += 1.0; }", "k"), base, d(op2.INC),
+                     iteration_region=op2.ON_INTERIOR_FACETS)
+
+
 def test_integration_stub_struct_matches_the_abi():
     """The ctypes op2hip_exec in the integration guide's binding stub lists
     the fields of the C struct in order (the same list the package binds)."""

@@ -3,6 +3,8 @@
 Runs every case of ``kat_cases`` through ``oracle_op2`` -- the oracle's
 sequential wrapper (``oracle/seqwrap.py``) behind the op2 call forms -- over
 the reference's core/owned split parametrisation."""
+import re
+
 import numpy as np
 import pytest
 
@@ -109,6 +111,42 @@ def test_planted_extrusion_mistake_is_caught():
             kat_cases.case_extruded_variable_layers(oracle_op2)
     finally:
         seqwrap._generate = orig
+
+
+_REGION_PLANTS = [
+    # the cell above named as the cell below
+    ("above_is_below", lambda src: re.sub(r"\(layer - bottom \+ \(\w+\) / \d+\)", "(layer - bottom)", src),
+     "case_extruded_interior_facets"),
+    # the top region starting one layer low (two top cells)
+    ("two_top_cells", lambda src: src.replace("cend - 1 > bottom ? cend - 1", "cend - 2 > bottom ? cend - 2"),
+     "case_extruded_bottom_top"),
+    # interior facets running one layer past the last facet
+    ("facet_past_top", lambda src: src.replace("layer < cend - 1;", "layer < cend;"),
+     "case_extruded_variable_regions"),
+]
+
+
+@pytest.mark.parametrize("plant,case", [(p, c) for _, p, c in _REGION_PLANTS],
+                         ids=[n for n, _, _ in _REGION_PLANTS])
+def test_planted_region_mistake_is_caught(plant, case):
+    """Each plausible iteration-region mistake, planted in the oracle's
+    wrapper, fails one known-answer case."""
+    from oracle import seqwrap
+    orig = seqwrap._generate
+    changed = []
+
+    def broken(*a, **k):
+        src = orig(*a, **k)
+        out = plant(src)
+        changed.append(out != src)
+        return out
+    seqwrap._generate = broken
+    try:
+        with pytest.raises(AssertionError):
+            getattr(kat_cases, case)(oracle_op2)
+    finally:
+        seqwrap._generate = orig
+    assert any(changed)
 
 
 def test_planted_mistake_is_caught():
