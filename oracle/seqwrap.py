@@ -154,11 +154,25 @@ def _ctype(dtype):
 def _generate(code, name, args, nmaps_ids, subset, headers=(), extruded=None):
     """C source of the sequential wrapper (see module docstring).
 
-    ``extruded``: None, or ``(constant_layers, pass_layer, offsets, region)``
-    where ``offsets[m]`` is unique map m's per-layer offset (or None) and
-    ``region`` one of "all", "bottom", "top", "interior"."""
+    ``extruded``: None, or ``(constant_layers, pass_layer, offsets, region,
+    periodic, quotients)`` where ``offsets[m]`` is unique map m's per-layer
+    offset (or None), ``region`` one of "all", "bottom", "top", "interior",
+    and ``quotients[m]`` a periodic map's offset quotients (or None)."""
     offsets = extruded[2] if extruded else {}
     region = extruded[3] if extruded else "all"
+    periodic = bool(extruded[4]) if extruded else False
+    quots = extruded[5] if extruded else {}
+
+    def step(m, i, f):
+        # cells above the column's bottom, wrapped on a periodic column by
+        # the generated loop's remainder a < n ? a : a - n (builder.py:27-30,
+        # 103-121)
+        st = "(layer - bottom)" if f is None else f"(layer - bottom + {f})"
+        if not periodic:
+            return st
+        if quots.get(m) is None:
+            return f"op2o_rem({st}, ncell)"
+        return f"(op2o_rem({st} + quo{m}[{i}], ncell) - op2o_rem(quo{m}[{i}], ncell))"
     # interior facets (builder.py:81-129, 825, 904): every map entry is taken
     # for the cell below (f = 0) and the cell above (f = 1), f outermost
     nf = 2 if region == "interior" else 1
@@ -167,16 +181,18 @@ def _generate(code, name, args, nmaps_ids, subset, headers=(), extruded=None):
         if nf == 1:
             base = f"map{m}[(int64_t)n * {ar} + {i}]"
             if offsets.get(m) is not None:
-                return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * (layer - bottom) : -1)"
+                return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * {step(m, i, None)} : -1)"
             return base
         assert offsets.get(m) is not None, "interior facets need maps with offsets"
         base = f"map{m}[(int64_t)n * {ar} + ({i}) % {ar}]"
-        return f"(({base}) >= 0 ? ({base}) + off{m}[({i}) % {ar}] * (layer - bottom + ({i}) / {ar}) : -1)"
+        return f"(({base}) >= 0 ? ({base}) + off{m}[({i}) % {ar}] * {step(m, f'({i}) % {ar}', f'({i}) / {ar}')} : -1)"
     lines = ["#include <math.h>", "#include <stdint.h>", "#include <string.h>",
              "#include <stdlib.h>"]
     lines += [h for h in headers]
     lines.append(code)
     lines.append(r"""
+static int32_t op2o_rem(int32_t a, int32_t n) { return a < n ? a : a - n; }
+
 static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                          int32_t row, int32_t col)
 {
@@ -220,13 +236,18 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
         for m, o in sorted(offsets.items()):
             if o is not None:
                 body.append(f"    const int32_t off{m}[{len(o)}] = {{{', '.join(str(int(v)) for v in o)}}};")
+        for m, o in sorted(quots.items()):
+            if o is not None:
+                body.append(f"    const int32_t quo{m}[{len(o)}] = {{{', '.join(str(int(v)) for v in o)}}};")
         # builder.py:771-795: the region's layers; a column with no cells
         # runs no top cell
         row = "0" if extruded[0] else "n"
         body.append(f"    const int32_t bottom = layers[2 * {row}], cend = layers[2 * {row} + 1] - 1;")
+        body.append("    const int32_t ncell = cend - bottom; (void)ncell;")
+        # a periodic column has as many interior facets as cells
         lo, hi = {"all": ("bottom", "cend"), "bottom": ("bottom", "bottom + 1"),
                   "top": ("(cend - 1 > bottom ? cend - 1 : bottom)", "cend"),
-                  "interior": ("bottom", "cend - 1")}[region]
+                  "interior": ("bottom", "cend" if periodic else "cend - 1")}[region]
         body.append(f"    for (int32_t layer = {lo}; layer < {hi}; ++layer) {{")
     call = []
     unpack = []
@@ -431,7 +452,7 @@ def _map_ids(args):
 
 
 def run(code, name, args: Sequence, parts, subset=None, flavour="strict", headers=(), layers=None,
-        offsets=None, pass_layer=False, region="all"):
+        offsets=None, pass_layer=False, region="all", periodic=False, quotients=None):
     """Execute the sequential wrapper over each ``(start, end)`` in ``parts``.
 
     Arrays are modified in place (C-contiguous numpy arrays required).
@@ -439,7 +460,9 @@ def run(code, name, args: Sequence, parts, subset=None, flavour="strict", header
     Extruded loops: ``layers`` the (columns, 2) or (1, 2) int32 [bottom, top)
     layers, ``offsets`` (map array, per-layer offsets) pairs (maps matched by
     identity), ``pass_layer`` to pass the layer as the kernel's last argument,
-    ``region`` the iteration region ("all", "bottom", "top", "interior").
+    ``region`` the iteration region ("all", "bottom", "top", "interior"),
+    ``periodic`` for a periodic extrusion with ``quotients`` (map array,
+    offset quotients) pairs.
     """
     ids, maps = _map_ids(args)
     ext = None
@@ -450,7 +473,12 @@ def run(code, name, args: Sequence, parts, subset=None, flavour="strict", header
             for key, o in (offsets or ()):
                 if key is m:
                     offs[q] = o
-        ext = (layers.shape[0] == 1, pass_layer, offs, region)
+        quo = {}
+        for q, m in enumerate(maps):
+            for key, o in (quotients or ()):
+                if key is m:
+                    quo[q] = o
+        ext = (layers.shape[0] == 1, pass_layer, offs, region, periodic, quo)
     src = _generate(code, name, args, ids, subset is not None, headers, ext)
     fn = compile_wrapper(src, flavour)
     cargs = []

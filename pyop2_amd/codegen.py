@@ -126,6 +126,9 @@ template <typename T> __device__ inline T op2_wave_max(T v) {
   return v;
 }
 
+/* Remainder of a periodic column's layer offset, a in [0, 2n). */
+__device__ inline int32_t op2_rem(int32_t a, int32_t n) { return a < n ? a : a - n; }
+
 __device__ inline int32_t op2_find_slot(const int32_t *__restrict__ rowptr,
                                          const int32_t *__restrict__ colidx,
                                          int32_t row, int32_t col) {
@@ -181,11 +184,14 @@ def _dat_store(acc, tgt, val):
     return f"{tgt} = {val};"
 
 
-def _region_bounds(region):
+def _region_bounds(region, periodic=False):
     """C expressions of the first and one-past-last layer a column runs in
     ``region``, from its ``bottom_`` and cell end ``cend_``
-    (``builder.py:771-795``; a column with no cells runs no top cell)."""
+    (``builder.py:771-795``; a column with no cells runs no top cell; a
+    periodic column has as many interior facets as cells)."""
     region = reg.normalise(region)
+    if region == reg.ON_INTERIOR_FACETS and periodic:
+        return "bottom_", "cend_"
     if region == reg.ON_BOTTOM:
         return "bottom_", "bottom_ + 1"
     if region == reg.ON_TOP:
@@ -208,12 +214,27 @@ def generate_source(gk, map_index, block_flags=None):
     region = getattr(gk, "_iteration_region", None) if extruded else None
     # interior facets: every map names the cell below, then the cell above
     nf = reg.facet_factor(region)
+    periodic = extruded and bool(getattr(gk, "_extruded_periodic", False))
     # map offsets per unique map (extruded maps step by offset[i] per layer)
-    offsets = {}
+    # and, on a periodic set, their offset quotients
+    offsets, quots = {}, {}
     for k, garg in enumerate(gk.arguments):
         for j, mk in enumerate(_kernel_maps(garg)):
             if mk.offset is not None and extruded:
                 offsets[map_index[(k, j)]] = mk.offset
+                if periodic and mk.offset_quotient is not None:
+                    quots[map_index[(k, j)]] = mk.offset_quotient
+
+    def _step(m, ar, i, f):
+        """Layer offset of entry ``i`` of map ``m``: the cells above the
+        column's bottom (``f`` adds the cell above), wrapped on a periodic
+        set (``builder.py:103-121``)."""
+        st = "lyr_" if f is None else f"(lyr_ + {f})"
+        if not periodic:
+            return st
+        if m not in quots:
+            return f"op2_rem({st}, nl_)"
+        return f"(op2_rem({st} + quo{m}[{i}], nl_) - op2_rem(quo{m}[{i}], nl_))"
 
     def _mv(m, ar, i):
         """Entry ``i`` of unique map ``m`` (arity ``ar``) for element ``n``
@@ -222,12 +243,13 @@ def generate_source(gk, map_index, block_flags=None):
         if nf == 1:
             base = f"map{m}[(int64_t)n * {ar} + {i}]"
             if m in offsets:
-                return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * lyr_ : -1)"
+                return f"(({base}) >= 0 ? ({base}) + off{m}[{i}] * {_step(m, ar, i, None)} : -1)"
             return base
         if m not in offsets:
             raise ValueError("ON_INTERIOR_FACETS needs extruded maps (maps with an offset)")
-        base = f"map{m}[(int64_t)n * {ar} + ({i}) % {ar}]"
-        return f"(({base}) >= 0 ? ({base}) + off{m}[({i}) % {ar}] * (lyr_ + ({i}) / {ar}) : -1)"
+        i_ = f"({i}) % {ar}"
+        base = f"map{m}[(int64_t)n * {ar} + {i_}]"
+        return f"(({base}) >= 0 ? ({base}) + off{m}[{i_}] * {_step(m, ar, i_, f'({i}) / {ar}')} : -1)"
     headers = "\n".join(_strip_headers(h) for h in lk.headers)
     code = _strip_headers(lk.code)
     params = ["int32_t lo", "int32_t hi", "const int32_t *__restrict__ list", "int32_t atomic"]
@@ -386,6 +408,8 @@ def generate_source(gk, map_index, block_flags=None):
         params.append("const int32_t *__restrict__ layers")
     offdecl = [f"    const int32_t off{m}[{len(o)}] = {{{', '.join(str(int(v)) for v in o)}}};"
                for m, o in sorted(offsets.items())]
+    offdecl += [f"    const int32_t quo{m}[{len(o)}] = {{{', '.join(str(int(v)) for v in o)}}};"
+                for m, o in sorted(quots.items())]
     decls = "\n".join(s for kind, s in red if kind == "decl")
     reds = "\n".join(s for kind, s in red if kind == "post")
     src = [_PRELUDE,
@@ -406,7 +430,9 @@ def generate_source(gk, map_index, block_flags=None):
                *offdecl,
                "    const int32_t lay_ = %s;" % ("0" if gk._constant_layers else "n"),
                "    const int32_t bottom_ = layers[2 * lay_], cend_ = layers[2 * lay_ + 1] - 1;",
-               "    const int32_t lo_ = %s, hi_ = %s;" % _region_bounds(region),
+               "    const int32_t lo_ = %s, hi_ = %s;" % _region_bounds(region, periodic),
+               "    const int32_t nl_ = cend_ - bottom_;",
+               "    (void)nl_;",
                "    for (int32_t layer_ = lo_; layer_ < hi_; ++layer_) {",
                "    const int32_t lyr_ = layer_ - bottom_;",
                "    (void)lyr_;"]),

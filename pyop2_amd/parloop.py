@@ -518,7 +518,8 @@ def LegacyParloop(local_knl, iterset, *args, **kwargs):
     global_knl_args = tuple(a.global_kernel_arg for a in args)
     subset = isinstance(iterset, Subset)
     extruded = getattr(iterset, "_extruded", False)
-    kwargs = dict(kwargs, extruded=extruded, constant_layers=extruded and iterset.constant_layers)
+    kwargs = dict(kwargs, extruded=extruded, constant_layers=extruded and iterset.constant_layers,
+                  extruded_periodic=bool(getattr(iterset, "_extruded_periodic", False)))
     global_knl = _cached_global_kernel(local_knl, global_knl_args, subset, kwargs)
     parloop_args = tuple(a.parloop_arg for a in args)
     return Parloop(global_knl, iterset, parloop_args)

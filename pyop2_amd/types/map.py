@@ -34,9 +34,12 @@ class Map:
             self._offset = np.broadcast_to(off, (self._arity,)).copy() if off.size == 1 else off
             if self._offset.shape != (self._arity,):
                 raise ex.DataValueError("Map offset must have one entry per map entry")
-        if offset_quotient is not None and len(offset_quotient) > 0:
-            raise NotImplementedError("periodic extruded maps (offset_quotient) are not on the device path")
-        self._offset_quotient = None
+        # periodic extrusion: entry i of the layer l cells above the bottom
+        # is offset by offset[i] * (rem(l + q[i]) - rem(q[i])) (map.py:49-52)
+        if offset_quotient is None or len(offset_quotient) == 0:
+            self._offset_quotient = None
+        else:
+            self._offset_quotient = utils.verify_reshape(offset_quotient, IntType, (self._arity,))
         self._cache = {}
         self._dev = None
 
@@ -44,30 +47,59 @@ class Map:
     def _global_kernel_arg(self):
         from pyop2_amd.global_kernel import MapKernelArg
         off = tuple(int(x) for x in self._offset) if self._offset is not None else None
-        return MapKernelArg(self.arity, off, None)
+        quo = tuple(int(x) for x in self._offset_quotient) if self._offset_quotient is not None else None
+        return MapKernelArg(self.arity, off, quo)
 
-    def column_table(self, iterset, columns, region=None):
-        """Entries of ``columns`` (host ids) of an extruded loop over the
-        layers ``region`` runs (all cells by default): ``(len(columns),
-        nsteps * nf * arity)``; entry (c, s, f, i) = ``table[c][i] +
-        offset[i] * (lo_c + s + f)`` for the column's steps s, -1 past them
-        (``nf`` = 2 for interior facets: the cell below, then the one above).
-        These are the targets a column touches, for colouring and patterns."""
+    def _layer_offset(self, steps, ncell, periodic):
+        """Layer offsets of the entries: ``steps`` (..., ) the cells above
+        the column's bottom (the cell above counted for interior facets),
+        ``ncell`` broadcast alike; returns (..., arity).  Periodic columns
+        wrap with the generated loop's remainder ``a < n ? a : a - n``
+        (``codegen/builder.py:27-30, 103-121``)."""
+        steps = steps[..., None]
+        if not periodic:
+            return np.broadcast_to(steps, steps.shape[:-1] + (self._arity,))
+        n = ncell[..., None]
+
+        def rem(a):
+            return np.where(a < n, a, a - n)
+        if self._offset_quotient is None:
+            return np.broadcast_to(rem(steps), steps.shape[:-1] + (self._arity,))
+        q = self._offset_quotient.astype(np.int64)
+        return rem(steps + q) - rem(np.broadcast_to(q, steps.shape[:-1] + (self._arity,)))
+
+    def _column_entries(self, iterset, cols, region):
+        """Entries and liveness of ``cols`` over ``region``: (len(cols),
+        nsteps, nf, arity) int64 entries and bools (a step the column runs
+        and a non-negative base)."""
         from pyop2_amd.types import region as reg
-        cols = np.asarray(columns, dtype=np.int64)
+        periodic = bool(getattr(iterset, "_extruded_periodic", False))
         lay = iterset.column_layers(cols)
         ncell = (lay[:, 1] - lay[:, 0] - 1) if len(cols) else np.zeros(0, dtype=np.int64)
-        lo, hi = reg.layer_steps(ncell, region)
+        lo, hi = reg.layer_steps(ncell, region, periodic)
         nsteps = hi - lo
         ns = int(nsteps.max()) if len(cols) else 0
         nf = reg.facet_factor(region)
         base = self._table[cols].astype(np.int64)
         off = self._offset if self._offset is not None else np.zeros(self._arity, dtype=IntType)
         step = lo[:, None, None] + np.arange(ns, dtype=np.int64)[None, :, None] + np.arange(nf)[None, None, :]
-        out = base[:, None, None, :] + step[..., None] * off[None, None, None, :]
+        nc = np.broadcast_to(ncell.astype(np.int64)[:, None, None], step.shape)
+        out = base[:, None, None, :] + self._layer_offset(step, nc, periodic) * off.astype(np.int64)
         live = (np.arange(ns)[None, :] < nsteps[:, None])[:, :, None, None] & (base[:, None, None, :] >= 0)
+        return out, np.broadcast_to(live, out.shape)
+
+    def column_table(self, iterset, columns, region=None):
+        """Entries of ``columns`` (host ids) of an extruded loop over the
+        layers ``region`` runs (all cells by default): ``(len(columns),
+        nsteps * nf * arity)``; entry (c, s, f, i) = ``table[c][i] +
+        offset[i] * (lo_c + s + f)`` for the column's steps s (wrapped on a
+        periodic set), -1 past them (``nf`` = 2 for interior facets: the
+        cell below, then the one above).  These are the targets a column
+        touches, for colouring and patterns."""
+        cols = np.asarray(columns, dtype=np.int64)
+        out, live = self._column_entries(iterset, cols, region)
         out = np.where(live, out, -1)
-        return np.ascontiguousarray(out.reshape(len(cols), ns * nf * self._arity), dtype=IntType)
+        return np.ascontiguousarray(out.reshape(len(cols), -1), dtype=IntType)
 
     def cell_table(self, iterset, columns, region=None):
         """Entries of every iteration of ``columns`` of an extruded loop over
@@ -81,26 +113,22 @@ class Map:
     def check_extruded(self, iterset, region=None):
         """Raise MapValueError unless every entry an extruded loop over
         ``region`` of ``iterset`` reaches through this map lies in the
-        target set (checked once per iterset and region).  Entries are
-        linear in the layer, so the column's first and last steps bound
-        them."""
+        target set (checked once per iterset and region, in column chunks)."""
         from pyop2_amd.types import region as reg
         key = ("extruded-checked", id(iterset), int(reg.normalise(region)))
         if self._offset is None or key in self._cache:
             return
-        cols = np.arange(iterset.total_size, dtype=np.int64)
-        lay = iterset.column_layers(cols)
-        lo, hi = reg.layer_steps(lay[:, 1] - lay[:, 0] - 1, region)
-        base = self._table[cols].astype(np.int64)
-        live = (hi > lo)[:, None] & (base >= 0)
-        off = self._offset.astype(np.int64)[None, :]
-        first = base + off * lo[:, None]
-        last = base + off * (hi - 1 + reg.facet_factor(region) - 1)[:, None]
         n = self._toset.total_size
-        if live.any() and (min(first[live].min(), last[live].min()) < 0
-                           or max(first[live].max(), last[live].max()) >= n):
-            raise ex.MapValueError(f"{self} with offsets reaches entries outside [0, {n}) "
-                                   f"in the layers of {iterset}")
+        ncols = iterset.total_size
+        lay = iterset.column_layers(np.arange(ncols))
+        per_col = max(1, int((lay[:, 1] - lay[:, 0]).max(initial=1)) * 2 * self._arity)
+        chunk = max(1, (1 << 22) // per_col)
+        for c0 in range(0, ncols, chunk):
+            out, live = self._column_entries(iterset, np.arange(c0, min(c0 + chunk, ncols)), region)
+            e = out[live]
+            if e.size and (e.min() < 0 or e.max() >= n):
+                raise ex.MapValueError(f"{self} with offsets reaches entries outside [0, {n}) "
+                                       f"in the layers of {iterset}")
         self._cache[key] = True
 
     @staticmethod
@@ -108,7 +136,8 @@ class Map:
         from pyop2_amd.types import region as reg
         cols = np.asarray(columns, dtype=np.int64)
         lay = iterset.column_layers(cols)
-        lo, hi = reg.layer_steps(lay[:, 1] - lay[:, 0] - 1, region)
+        lo, hi = reg.layer_steps(lay[:, 1] - lay[:, 0] - 1, region,
+                                 bool(getattr(iterset, "_extruded_periodic", False)))
         nsteps = hi - lo
         ns = int(nsteps.max()) if len(cols) else 0
         return np.arange(ns)[None, :] < nsteps[:, None]
@@ -236,7 +265,9 @@ class PermutedMap(Map):
         self._values = map_._values
         self.shape = map_.shape
         self._name = "pmap_#x%x" % id(self)
-        self._offset = map_._offset
+        # the permuted entries keep their own per-layer offsets; the offset
+        # quotients stay as the base map's (builder.py:148-172)
+        self._offset = map_._offset[perm] if map_._offset is not None else None
         self._offset_quotient = map_._offset_quotient
         self._cache = {}
         self._dev = None
@@ -275,8 +306,9 @@ class ComposedMap(Map):
         self._values = None
         self.shape = (self._iterset.total_size, self._arity)
         self._name = name or "cmap_#x%x" % id(self)
+        # the first map carries the layer stepping (builder.py:175-193)
         self._offset = maps_[0]._offset
-        self._offset_quotient = None
+        self._offset_quotient = maps_[0]._offset_quotient
         self._cache = {}
         self._dev = None
         self.maps_ = tuple(maps_)

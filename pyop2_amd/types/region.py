@@ -6,9 +6,9 @@ A loop over an ExtrudedSet runs, per column with cells ``[bottom, cend)``
 - ``ALL``: every cell, ``[bottom, cend)``;
 - ``ON_BOTTOM``: the bottom cell, ``[bottom, bottom + 1)``;
 - ``ON_TOP``: the top cell, ``[cend - 1, cend)``;
-- ``ON_INTERIOR_FACETS``: the facets between cells, ``[bottom, cend - 1)``;
-  each map then names the entries of the cell below and of the cell above
-  (twice the arity, below first).
+- ``ON_INTERIOR_FACETS``: the facets between cells, ``[bottom, cend - 1)``
+  (``[bottom, cend)`` on a periodic set); each map then names the entries
+  of the cell below and of the cell above (twice the arity, below first).
 """
 import enum
 
@@ -45,13 +45,14 @@ def facet_factor(region):
     return 2 if normalise(region) == ON_INTERIOR_FACETS else 1
 
 
-def layer_steps(ncell, region):
+def layer_steps(ncell, region, periodic=False):
     """Per column with ``ncell`` cells, the first and one-past-last offset
     above the column's bottom that ``region`` runs (int64 arrays).
 
     A column with no cells runs nothing on top (its top cell would be the
     layer below its bottom); on the bottom it runs its bottom layer, as the
-    reference does."""
+    reference does.  A periodic column has as many interior facets as cells
+    (the last joins the top cell to the bottom one)."""
     ncell = np.asarray(ncell, dtype=np.int64)
     zero = np.zeros_like(ncell)
     region = normalise(region)
@@ -61,4 +62,6 @@ def layer_steps(ncell, region):
         return zero, zero + 1
     if region == ON_TOP:
         return np.maximum(ncell - 1, 0), ncell
+    if periodic:
+        return zero, ncell
     return zero, np.maximum(ncell - 1, 0)

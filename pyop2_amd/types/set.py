@@ -182,8 +182,6 @@ class ExtrudedSet(Set):
 
     @utils.validate_type(('parent', Set, TypeError))
     def __init__(self, parent, layers, extruded_periodic=False):
-        if extruded_periodic:
-            raise NotImplementedError("periodic extrusion is not on the device path")
         if isinstance(parent, (ExtrudedSet, Subset)):
             raise NotImplementedError("only a plain Set can be extruded on the device path")
         self._parent = parent
@@ -205,7 +203,8 @@ class ExtrudedSet(Set):
             arr = np.asarray([[0, int(layers)]], dtype=IntType)
             self.constant_layers = True
         self._layers = np.ascontiguousarray(arr, dtype=IntType)
-        self._extruded_periodic = False
+        # periodic: the top of every column's last cell is its bottom
+        self._extruded_periodic = bool(extruded_periodic)
         self._cache = {}
         self._dev_layers = None
         self._partition_size = 1024

@@ -1167,6 +1167,146 @@ def case_extruded_variable_regions(op2, split=None):
     assert np.array_equal(np.asarray(d.data), want)
 
 
+PCELLS = 4                   # cells of a periodic column
+PCOLS = 3
+PNV = PCOLS * PCELLS         # vertices (and cells) of the periodic columns
+PPAD = 2 * PCELLS            # unused targets past them: a wrong wrap stays in bounds
+
+
+def _ppad(a):
+    """``a`` (vector or square matrix over PNV targets) padded with the
+    zero targets past them."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        return np.concatenate([a, np.zeros(PPAD)])
+    out = np.zeros((PNV + PPAD, PNV + PPAD))
+    out[:PNV, :PNV] = a
+    return out
+
+
+def _periodic(op2):
+    """PCOLS periodic columns of PCELLS cells (builder.py:103-121): vertex
+    v of column c is c * PCELLS + v, the top vertex of the last cell is the
+    bottom one (offset quotient 1 on the top entry); cell l of column c is
+    c * PCELLS + l."""
+    base = op2.Set(PCOLS)
+    ext = op2.ExtrudedSet(base, PCELLS + 1, extruded_periodic=True)
+    vset, cset = op2.Set(PNV + PPAD), op2.Set(PNV + PPAD)
+    c0 = np.arange(PCOLS, dtype=np.int32) * PCELLS
+    vmap = op2.Map(ext, vset, 2, np.stack([c0, c0 + 1], axis=1), "vmap", np.ones(2, dtype=np.int32),
+                   offset_quotient=np.array([0, 1], dtype=np.int32))
+    cmap = op2.Map(ext, cset, 1, c0.reshape(-1, 1), "cmap", np.ones(1, dtype=np.int32))
+    return base, ext, vset, cset, vmap, cmap
+
+
+def _cycle(n, diag, off):
+    """The n x n matrix with ``diag`` on the diagonal and ``off`` between
+    cyclic neighbours."""
+    a = np.eye(n) * diag
+    for v in range(n):
+        a[v, (v + 1) % n] += off
+        a[(v + 1) % n, v] += off
+    return a
+
+
+def case_periodic_vertices(op2, split=None):
+    """Periodic extrusion, all cells: an INC of 1 into both vertices of every
+    cell gives every vertex 2 (each has a cell below and above it, the
+    bottom one through the wrap); the 2x2 mass-like matrix [[2, 1], [1, 2]]
+    assembles, per column, the cyclic matrix with 4 on the diagonal and 1
+    between cyclic neighbours; the top cells reach vertices PCELLS - 1 and 0."""
+    base, ext, vset, cset, vmap, cmap = _periodic(op2)
+    cnt = op2.Dat(vset ** 1, np.zeros(PNV + PPAD), np.float64)
+    two = op2.Kernel("static void two(double x[2]) { x[0] += 1.0; x[1] += 1.0; }", "two")
+    op2.par_loop(two, ext, cnt(op2.INC, vmap))
+    assert np.array_equal(np.asarray(cnt.data), _ppad(np.full(PNV, 2.0)))
+    A = op2.Mat(op2.Sparsity((vset ** 1, vset ** 1), [(vmap, vmap, None)]), np.float64)
+    mk = op2.Kernel("static void mm(double a[2][2]) { a[0][0] += 2.0; a[0][1] += 1.0; a[1][0] += 1.0; a[1][1] += 2.0; }",
+                    "mm")
+    op2.par_loop(mk, ext, A(op2.INC, (vmap, vmap)))
+    A.assemble()
+    want = np.kron(np.eye(PCOLS), _cycle(PCELLS, 4.0, 1.0))
+    assert np.array_equal(np.asarray(A.values), _ppad(want))
+    top = op2.Dat(vset ** 1, np.zeros(PNV + PPAD), np.float64)
+    op2.par_loop(two, ext, top(op2.INC, vmap), iteration_region=op2.ON_TOP)
+    want = np.zeros((PCOLS, PCELLS))
+    want[:, [0, PCELLS - 1]] = 1.0
+    assert np.array_equal(np.asarray(top.data), _ppad(want.reshape(-1)))
+
+
+def case_periodic_facets(op2, split=None):
+    """Periodic interior facets: as many as cells (the last joins the top
+    cell to the bottom one, builder.py:781-786).  INC of 1 below and 10
+    above gives every cell 11; the facet jump matrix assembles the cycle
+    Laplacian (2 on the diagonal, -1 between cyclic neighbours); through the
+    vertex map the lower cell's top vertex is the upper cell's bottom one
+    on every facet, and the upper cell's top vertex ids sum, per column, to
+    PCELLS * c * PCELLS + (0 + 1 + ... + PCELLS - 1)."""
+    base, ext, vset, cset, vmap, cmap = _periodic(op2)
+    fac = op2.ON_INTERIOR_FACETS
+    c = op2.Dat(cset ** 1, np.zeros(PNV + PPAD), np.float64)
+    jmp = op2.Kernel("static void pj(double c[2]) { c[0] += 1.0; c[1] += 10.0; }", "pj")
+    op2.par_loop(jmp, ext, c(op2.INC, cmap), iteration_region=fac)
+    assert np.array_equal(np.asarray(c.data), _ppad(np.full(PNV, 11.0)))
+    A = op2.Mat(op2.Sparsity((cset ** 1, cset ** 1), [(cmap, cmap, (fac,))]), np.float64)
+    jm = op2.Kernel("static void pjm(double a[2][2]) { a[0][0] += 1.0; a[0][1] -= 1.0; a[1][0] -= 1.0; a[1][1] += 1.0; }",
+                    "pjm")
+    op2.par_loop(jm, ext, A(op2.INC, (cmap, cmap)), iteration_region=fac)
+    A.assemble()
+    assert np.array_equal(np.asarray(A.values), _ppad(np.kron(np.eye(PCOLS), _cycle(PCELLS, 2.0, -1.0))))
+    ids = op2.Dat(vset ** 1, np.arange(PNV + PPAD, dtype=np.float64), np.float64)
+    g = op2.Global(2, [0.0, 0.0], np.float64, "g")
+    k = op2.Kernel("static void pv(double *g, const double x[4])"
+                   " { g[0] += (x[1] - x[2]) * (x[1] - x[2]); g[1] += x[3]; }", "pv")
+    op2.par_loop(k, ext, g(op2.INC), ids(op2.READ, vmap), iteration_region=fac)
+    got = np.asarray(g.data).reshape(-1)
+    assert got[0] == 0.0
+    assert got[1] == sum(PCELLS * cc * PCELLS + sum(range(PCELLS)) for cc in range(PCOLS))
+    cnt = op2.Dat(base ** 1, np.zeros(PCOLS), np.float64)
+    op2.par_loop(op2.Kernel("static void one(double *x) { *x += 1.0; }", "one"), ext, cnt(op2.INC),
+                 iteration_region=fac)
+    assert np.array_equal(np.asarray(cnt.data), np.full(PCOLS, float(PCELLS)))
+
+
+def case_periodic_variable_layers(op2, split=None):
+    """Periodic columns of 2 and 3 cells ([[0, 3], [1, 5]]): vertex k of
+    column c is 10 * c + k; every vertex gets 2 from an INC of 1 into both
+    vertices of each cell, and each column has as many interior facets as
+    cells."""
+    base = op2.Set(2)
+    ext = op2.ExtrudedSet(base, np.array([[0, 3], [1, 5]], dtype=np.int32), extruded_periodic=True)
+    tgt = op2.Set(20)
+    m = op2.Map(ext, tgt, 2, np.array([[0, 1], [10, 11]], dtype=np.int32), offset=np.ones(2, dtype=np.int32),
+                offset_quotient=np.array([0, 1], dtype=np.int32))
+    d = op2.Dat(tgt ** 1, np.zeros(20), np.float64)
+    op2.par_loop(op2.Kernel("static void two(double x[2]) { x[0] += 1.0; x[1] += 1.0; }", "two"), ext, d(op2.INC, m))
+    want = np.zeros(20)
+    want[[0, 1, 10, 11, 12]] = 2.0
+    assert np.array_equal(np.asarray(d.data), want)
+    cnt = op2.Dat(base ** 1, np.zeros(2), np.float64)
+    op2.par_loop(op2.Kernel("static void one(double *x) { *x += 1.0; }", "one"), ext, cnt(op2.INC),
+                 iteration_region=op2.ON_INTERIOR_FACETS)
+    assert list(np.asarray(cnt.data)) == [2.0, 3.0]
+
+
+def case_extruded_permuted_offsets(op2, split=None):
+    """A PermutedMap of an extruded map with different offsets per entry
+    keeps each entry's own offset (builder.py:148-172): entries (0 step 1,
+    100 step 2) permuted to (100, 0); INC of 1 into the first and 10 into
+    the second over 4 cells reaches 100, 102, 104, 106 and 0, 1, 2, 3."""
+    base = op2.Set(1)
+    ext = op2.ExtrudedSet(base, 5)
+    tgt = op2.Set(110)
+    m = op2.Map(ext, tgt, 2, np.array([[0, 100]], dtype=np.int32), offset=np.array([1, 2], dtype=np.int32))
+    pm = op2.PermutedMap(m, [1, 0])
+    d = op2.Dat(tgt ** 1, np.zeros(110), np.float64)
+    op2.par_loop(op2.Kernel("static void ab(double x[2]) { x[0] += 1.0; x[1] += 10.0; }", "ab"), ext, d(op2.INC, pm))
+    want = np.zeros(110)
+    want[[100, 102, 104, 106]] = 1.0
+    want[[0, 1, 2, 3]] = 10.0
+    assert np.array_equal(np.asarray(d.data), want)
+
+
 EXTRUDED_CASES = [
     ("extruded_volume", case_extruded_volume),
     ("extruded_direct_inc", case_extruded_direct_inc),
@@ -1180,4 +1320,8 @@ EXTRUDED_CASES = [
     ("extruded_interior_facets", case_extruded_interior_facets),
     ("extruded_facet_matrix", case_extruded_facet_matrix),
     ("extruded_variable_regions", case_extruded_variable_regions),
+    ("extruded_permuted_offsets", case_extruded_permuted_offsets),
+    ("periodic_vertices", case_periodic_vertices),
+    ("periodic_facets", case_periodic_facets),
+    ("periodic_variable_layers", case_periodic_variable_layers),
 ]

@@ -86,9 +86,10 @@ class ExtrudedSet(Set):
     """Columns of cells over ``parent`` (constant ``layers`` or a
     (parent.total_size, 2) array of [bottom, top) layers)."""
 
-    def __init__(self, parent, layers):
+    def __init__(self, parent, layers, extruded_periodic=False):
         super().__init__((parent.core_size, parent.size, parent.total_size))
         self.parent = parent
+        self.periodic = bool(extruded_periodic)
         arr = np.asarray(layers, dtype=np.int32)
         self.layers_array = arr if arr.ndim == 2 else np.array([[0, int(layers)]], dtype=np.int32)
 
@@ -106,7 +107,7 @@ class ExtrudedSet(Set):
         if region == ON_TOP:
             return range(max(cend - 1, bottom), cend)
         if region == ON_INTERIOR_FACETS:
-            return range(bottom, cend - 1)
+            return range(bottom, cend if self.periodic else cend - 1)
         return range(bottom, cend)
 
     def bottom(self, n):
@@ -118,8 +119,9 @@ class Map:
     names the target ``table[n, k]``; an extruded map's entry k of layer l
     above the column's bottom is ``table[n, k] + offset[k] * l``."""
 
-    def __init__(self, iterset, toset, arity, values=None, name=None, offset=None):
+    def __init__(self, iterset, toset, arity, values=None, name=None, offset=None, offset_quotient=None):
         self.iterset, self.toset, self.arity = iterset, toset, arity
+        self.offset_quotient = None if offset_quotient is None else np.asarray(offset_quotient, dtype=np.int32)
         self.values = np.ascontiguousarray(np.asarray(values, dtype=np.int32).reshape(iterset.total_size, arity))
         self.offset = None if offset is None else np.broadcast_to(np.asarray(offset, dtype=np.int32),
                                                                   (arity,)).copy()
@@ -133,9 +135,18 @@ class Map:
         rows = []
         for n in range(self.iterset.size):
             b = self.iterset.bottom(n)
+            ncell = len(self.iterset.cells(n))
             for layer in self.iterset.layer_range(n, region):
-                rows.append([int(v) + int(self.offset[k]) * (layer - b + f) if v >= 0 else -1
-                             for f in range(nf) for k, v in enumerate(self.table[n])])
+                row = []
+                for f in range(nf):
+                    for k, v in enumerate(self.table[n]):
+                        step = layer - b + f
+                        if self.iterset.periodic and ncell > 0:
+                            # sparsity.pyx:369-377 (layers from the bottom)
+                            q = 0 if self.offset_quotient is None else int(self.offset_quotient[k])
+                            step = (step + q) % ncell - q % ncell
+                        row.append(int(v) + int(self.offset[k]) * step if v >= 0 else -1)
+                rows.append(row)
         return np.array(rows, dtype=np.int32).reshape(-1, nf * self.arity)
 
     @property
@@ -151,12 +162,18 @@ class PermutedMap(Map):
         self.map_, self.permutation = map_, np.asarray(permutation, dtype=np.int32)
         self.iterset, self.toset, self.arity = map_.iterset, map_.toset, map_.arity
         self.values = map_.values
+        # PMap (builder.py:148-172): offsets permuted, quotients not
+        self.offset = None if map_.offset is None else map_.offset[self.permutation]
+        self.offset_quotient = map_.offset_quotient
+        self._table = None
 
     @property
     def table(self):
-        base = self.map_.table
-        return np.array([[base[n][self.permutation[k]] for k in range(self.arity)]
-                         for n in range(len(base))], dtype=np.int32).reshape(-1, self.arity)
+        if self._table is None:
+            base = self.map_.table
+            self._table = np.array([[base[n][self.permutation[k]] for k in range(self.arity)]
+                                    for n in range(len(base))], dtype=np.int32).reshape(-1, self.arity)
+        return self._table
 
 
 class ComposedMap(Map):
@@ -172,6 +189,9 @@ class ComposedMap(Map):
                 raise MapTypeError("frommap.arity must be 1")
         self.maps_ = maps_
         self.iterset, self.toset, self.arity = maps_[-1].iterset, maps_[0].toset, maps_[0].arity
+        # CMap (builder.py:175-193): the first map steps with the layers
+        self.offset, self.offset_quotient = maps_[0].offset, maps_[0].offset_quotient
+        self._table = None
 
     @property
     def values(self):
@@ -179,14 +199,18 @@ class ComposedMap(Map):
 
     @property
     def table(self):
+        if self._table is not None:
+            return self._table
         tables = [m.table for m in self.maps_]
         out = []
         for n in range(self.iterset.total_size):
             e = n
             for t in reversed(tables[1:]):
-                e = int(t[e][0])
-            out.append([int(v) for v in tables[0][e]])
-        return np.array(out, dtype=np.int32).reshape(-1, self.arity)
+                e = int(t[e][0]) if e >= 0 else -1
+            # a negative entry along the chain names no target (sparsity.pyx:329-340)
+            out.append([int(v) for v in tables[0][e]] if e >= 0 else [-1] * self.arity)
+        self._table = np.array(out, dtype=np.int32).reshape(-1, self.arity)
+        return self._table
 
 
 class MixedMap:
@@ -414,13 +438,15 @@ def par_loop(kernel, iterset, *args, pass_layer_arg=False, iteration_region=None
     subset = iterset.indices if isinstance(iterset, Subset) else None
     ext = {}
     if isinstance(iterset, ExtrudedSet):
-        offs = []
+        offs, quos = [], []
         for kind, obj, access, path in args:
             for m in (path if isinstance(path, tuple) else (path,)):
                 if isinstance(m, Map) and m.offset is not None:
                     offs.append((m.table, m.offset))
+                    if m.offset_quotient is not None:
+                        quos.append((m.table, m.offset_quotient))
         ext = dict(layers=iterset.layers_array, offsets=offs, pass_layer=pass_layer_arg,
-                   region=iteration_region or ALL)
+                   region=iteration_region or ALL, periodic=iterset.periodic, quotients=quos)
     err = seqwrap.run(kernel.code, kernel.name, sargs, parts, subset=subset, **ext)
     assert err == 0, "insertion outside the sparsity pattern"
     for g, tmp in temps:

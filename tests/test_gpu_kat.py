@@ -115,14 +115,16 @@ def test_mixed_random_against_oracle(device, strategy):
         assert np.linalg.norm(x - y) <= 1e-12 * np.linalg.norm(y)
 
 
+@pytest.mark.parametrize("periodic", [False, True], ids=["plain", "periodic"])
 @pytest.mark.parametrize("strategy", ["atomic", "coloured"])
-def test_extruded_regions_random_against_oracle(device, strategy):
+def test_extruded_regions_random_against_oracle(device, strategy, periodic):
     """Prism columns over a 6x6 triangle mesh with random variable layers
-    (1 to 6 cells a column): a vertex Mat and Dat over all cells, a cell Mat
-    and Dat over interior facets (jumps of a random cell field), a vertex
-    Dat over the bottom and top cells with the layer argument; device
-    against the oracle's sequential wrapper on the same inputs (1e-12), and
-    the same patterns."""
+    (1 to 6 cells a column), plain or periodic (the top vertices of a
+    column's last cell wrapping to its bottom ones): a vertex Mat and Dat
+    over all cells, a cell Mat and Dat over interior facets (jumps of a
+    random cell field), a vertex Dat over the bottom and top cells with the
+    layer argument; device against the oracle's sequential wrapper on the
+    same inputs (1e-12), and the same patterns."""
     import numpy as np
     import oracle_op2
     from pyop2_amd.meshes import unit_square_tri, workload_rng
@@ -149,9 +151,10 @@ def test_extruded_regions_random_against_oracle(device, strategy):
     out = []
     for ns in (op2, oracle_op2):
         base = ns.Set(m.ncells)
-        ext = ns.ExtrudedSet(base, lay)
+        ext = ns.ExtrudedSet(base, lay, extruded_periodic=periodic)
         verts, cells = ns.Set(m.nverts * nl), ns.Set(m.ncells * nl)
-        vmap = ns.Map(ext, verts, 6, vbase, offset=np.ones(6, dtype=np.int32))
+        vmap = ns.Map(ext, verts, 6, vbase, offset=np.ones(6, dtype=np.int32),
+                      offset_quotient=np.tile([0, 1], 3).astype(np.int32) if periodic else None)
         cmap = ns.Map(ext, cells, 1, cbase, offset=np.ones(1, dtype=np.int32))
         ud = ns.Dat(verts ** 1, u.copy(), np.float64)
         qd = ns.Dat(cells ** 1, q.copy(), np.float64)

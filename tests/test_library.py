@@ -290,7 +290,7 @@ def test_generated_wrappers_compile_for_gfx950():
 def test_generated_extruded_region_wrappers_compile_for_gfx950():
     """Generated wrappers of extruded loops over each iteration region
     (doubled packs for interior facets, Dat, Mat and Global arguments, the
-    layer argument) compile with hiprtc."""
+    layer argument), plain and periodic, compile with hiprtc."""
     from pyop2_amd import op2
     from pyop2_amd.codegen import GeneratedKernel, compile_options
     from pyop2_amd.parloop import LegacyParloop
@@ -313,11 +313,22 @@ def test_generated_extruded_region_wrappers_compile_for_gfx950():
         ("static void f4(double x[2]) { x[0] += 1.0; }", "f4",
          (x(op2.INC, m),), op2.ON_BOTTOM, False),
     ]
+    pext = op2.ExtrudedSet(base, 5, extruded_periodic=True)
+    pm = op2.Map(pext, cells, 2, np.array([[0, 1], [4, 5], [8, 9], [12, 13]], dtype=np.int32),
+                 offset=np.ones(2, dtype=np.int32), offset_quotient=np.array([0, 1], dtype=np.int32))
+    psp = op2.Sparsity((cells ** 1, cells ** 1), [(pm, pm, (op2.ALL, op2.ON_INTERIOR_FACETS))])
+    pA = op2.Mat(psp)
+    cases_p = [
+        ("static void p1(double A[4][4], double x[4]) { A[0][3] = x[2]; x[1] += 1.0; }", "p1",
+         (pA(op2.INC, (pm, pm)), x(op2.INC, pm)), op2.ON_INTERIOR_FACETS, False),
+        ("static void p2(double x[2], int layer) { x[1] = layer; }", "p2",
+         (x(op2.WRITE, pm),), None, True),
+    ]
     L = dev.lib()
     opts = compile_options()
     copts = (ctypes.c_char_p * len(opts))(*[o.encode() for o in opts])
-    for code, name, args, region, layer in cases:
-        pl = LegacyParloop(op2.Kernel(code, name), ext, *args, iteration_region=region, pass_layer_arg=layer)
+    for it, (code, name, args, region, layer) in [(ext, c) for c in cases] + [(pext, c) for c in cases_p]:
+        pl = LegacyParloop(op2.Kernel(code, name), it, *args, iteration_region=region, pass_layer_arg=layer)
         src = GeneratedKernel(pl.global_kernel).source(pl)
         size = ctypes.c_size_t(0)
         log = ctypes.create_string_buffer(16384)

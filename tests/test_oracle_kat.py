@@ -123,6 +123,16 @@ _REGION_PLANTS = [
     # interior facets running one layer past the last facet
     ("facet_past_top", lambda src: src.replace("layer < cend - 1;", "layer < cend;"),
      "case_extruded_variable_regions"),
+    # a periodic column missing the facet that joins its top cell to its bottom
+    ("periodic_facet_missing", lambda src: src.replace("layer < cend; ++layer", "layer < cend - 1; ++layer")
+     if "op2o_rem((layer - bottom + " in src else src, "case_periodic_facets"),
+    # the offset quotient dropped (top vertex not wrapped)
+    ("quotient_dropped", lambda src: re.sub(r"\(op2o_rem\(([^;]*?) \+ quo\d+\[[^\]]*\], ncell\) - "
+                                           r"op2o_rem\(quo\d+\[[^\]]*\], ncell\)\)",
+                                           r"op2o_rem(\1, ncell)", src),
+     "case_periodic_vertices"),
+    # no wrap at all on a periodic column
+    ("no_wrap", lambda src: re.sub(r"op2o_rem\(([^,]*?), ncell\)", r"\1", src), "case_periodic_facets"),
 ]
 
 
