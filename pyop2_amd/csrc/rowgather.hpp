@@ -333,6 +333,21 @@ struct BatchDesc {
   int32_t r0, nr, z0, nz, i0, ni, pad0, pad1;
 };
 
+// A slot entry read once per launch: non-temporal 8-byte loads, so the
+// streamed table does not evict the gathered coordinates and maps.
+template <int kCar>
+__device__ __forceinline__ SlotEntry<kCar> load_slot_entry_nt(const SlotEntry<kCar> *p) {
+  static_assert(sizeof(SlotEntry<kCar>) % 8 == 0, "slot entries are whole 8-byte words");
+  constexpr int kW = sizeof(SlotEntry<kCar>) / 8;
+  const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+  uint64_t w[kW];
+#pragma unroll
+  for (int k = 0; k < kW; ++k) w[k] = __builtin_nontemporal_load(q + k);
+  SlotEntry<kCar> out;
+  __builtin_memcpy(&out, w, sizeof(out));
+  return out;
+}
+
 template <int kCar>
 __global__ void build_slot_table(int32_t nbatches, const int32_t *__restrict__ batch_rows,
                                  const int32_t *__restrict__ rowptr,
@@ -528,11 +543,11 @@ struct RoundOf<Elem, std::void_t<decltype(Elem::kRound)>> {
 };
 
 // One round of a split-gather batch: kR row incidences per thread.  load()
-// issues the incidence entries, slot entries and vertex ids of all of them
-// before any element is evaluated, so the dependent loads of one incidence do
-// not wait behind the arithmetic of the previous one; eval() gathers each
-// incidence's coordinates, evaluates its block row and adds it into the
-// batch's LDS accumulators.
+// issues the incidence entries, slot entries (both streamed, non-temporal)
+// and vertex ids of all of them before any element is evaluated, so the
+// dependent loads of one incidence do not wait behind the arithmetic of the
+// previous one; eval() gathers each incidence's coordinates, evaluates its
+// block row and adds it into the batch's LDS accumulators.
 template <class Elem>
 struct SplitRound {
   using T = typename Elem::T;
@@ -552,8 +567,8 @@ struct SplitRound {
       const int32_t t0 = base + u * bd;
       ok[u] = t0 < ni;
       const int32_t t = SpreadOf<Elem>::value ? spread_incidence(ok[u] ? t0 : 0, ni) : t0;
-      ent[u] = ok[u] ? ents[t] : 0;
-      if (ok[u]) se[u] = table[t];
+      ent[u] = ok[u] ? __builtin_nontemporal_load(&ents[t]) : 0;
+      if (ok[u]) se[u] = load_slot_entry_nt(&table[t]);
     }
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
