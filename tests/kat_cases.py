@@ -323,6 +323,32 @@ def case_sparsity_nnz(op2, split=None):
 
 LOOP_CASES = [case_write, case_rw, case_inc_unitset, case_max, case_min, case_global_read,
               case_global_inc, case_2d_dat]
+def case_empty_iterset(op2):
+    """test_api.py:1593-1601: a loop over a zero-sized set whose map holds no
+    values runs nothing.  Over the empty set an INC Dat keeps its values,
+    INC and MIN Globals keep theirs (the INC adds a zero temporary,
+    parloop.py:514-530), and a Mat built from a pair of empty maps holds its
+    zero diagonal only."""
+    s1, s2 = op2.Set(0), op2.Set(10)
+    m = op2.Map(s1, s2, 3)
+    d = op2.Dat(s2 ** 1, np.arange(10, dtype=np.int32), np.int32)
+    op2.par_loop(op2.Kernel("static void k(int *x) {}", "k"), s1, d(op2.READ, m))
+    op2.par_loop(op2.Kernel("static void ki(int *x) { x[0] += 1; }", "ki"), s1, d(op2.INC, m))
+    assert list(np.asarray(d.data)) == list(range(10))
+    g, h = op2.Global(1, 5.0, np.float64, "g"), op2.Global(1, 5.0, np.float64, "h")
+    op2.par_loop(op2.Kernel("static void kg(double *g, double *h) { g[0] += 1.0; h[0] = h[0] < 0.0 ? h[0] : 0.0; }",
+                            "kg"), s1, g(op2.INC), h(op2.MIN))
+    assert float(np.asarray(g.data).reshape(-1)[0]) == 5.0
+    assert float(np.asarray(h.data).reshape(-1)[0]) == 5.0
+    e = op2.Map(s1, s2, 3, np.zeros((0, 3), dtype=np.int32))
+    sp = op2.Sparsity((s2 ** 1, s2 ** 1), [(e, e, None)])
+    A = op2.Mat(sp, np.float64)
+    op2.par_loop(op2.Kernel("static void km(double a[3][3]) { a[0][0] += 1.0; }", "km"), s1, A(op2.INC, (e, e)))
+    A.assemble()
+    assert int(np.sum(sp.nnz)) == 10
+    assert not np.asarray(A.values).any()
+
+
 OTHER_CASES = [
     ("edge_sum_write", lambda op2: case_edge_sum(op2, mode="WRITE")),
     ("edge_sum_inc", lambda op2: case_edge_sum(op2, mode="INC")),
@@ -337,6 +363,7 @@ OTHER_CASES = [
     ("rhs", case_rhs),
     ("write_vs_inc_matrix", case_write_vs_inc_matrix),
     ("sparsity_nnz", case_sparsity_nnz),
+    ("empty_iterset", case_empty_iterset),
 ]
 
 

@@ -122,6 +122,8 @@ class Map:
     def __init__(self, iterset, toset, arity, values=None, name=None, offset=None, offset_quotient=None):
         self.iterset, self.toset, self.arity = iterset, toset, arity
         self.offset_quotient = None if offset_quotient is None else np.asarray(offset_quotient, dtype=np.int32)
+        if values is None:
+            values = np.zeros((iterset.total_size, arity), dtype=np.int32)
         self.values = np.ascontiguousarray(np.asarray(values, dtype=np.int32).reshape(iterset.total_size, arity))
         self.offset = None if offset is None else np.broadcast_to(np.asarray(offset, dtype=np.int32),
                                                                   (arity,)).copy()

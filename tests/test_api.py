@@ -229,3 +229,14 @@ def test_parloop_without_device_fails_loudly():
     m = op2.Map(it, ind, 1, [0, 1, 2, 3])
     with pytest.raises(ex.DeviceError):
         op2.par_loop(op2.Kernel("static void w(double *x) { *x = 1.0; }", "w"), it, x(op2.WRITE, m))
+
+
+def test_empty_map_and_iterset():
+    """test_api.py:1593-1601: over a zero-sized iteration set a map without
+    values is not an error, and nothing is launched."""
+    s1 = op2.Set(0)
+    s2 = op2.Set(10)
+    m = op2.Map(s1, s2, 3)
+    d = op2.Dat(s2 ** 1, [0] * 10, dtype=int)
+    k = op2.Kernel("static void k(int *x) {}", "k")
+    op2.par_loop(k, s1, d(op2.READ, m))
