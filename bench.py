@@ -301,8 +301,10 @@ def secondary(args):
         ms = time_loop(pl, reps, name)
         res[key] = dict(cells=m1.ncells, kernel_us=ms * 1e3, cells_per_s=m1.ncells / (ms * 1e-3),
                         gbs=alg / (ms * 1e-3) / 1e9, alg_bytes=alg,
-                        strategy=pl.global_kernel.compile().strategy_used)
-        log(f"C1 {key}: {ms * 1e3:.2f} us, {res[key]['gbs']:.0f} GB/s ({res[key]['strategy']})")
+                        strategy=pl.global_kernel.compile().strategy_used,
+                        relerr_vs_cpu=c1_relerr(m1, pl, name))
+        log(f"C1 {key}: {ms * 1e3:.2f} us, {res[key]['gbs']:.0f} GB/s ({res[key]['strategy']}), "
+            f"rel err vs CPU {res[key]['relerr_vs_cpu']:.1e}")
 
     # C3: P2 elasticity, 64^3 (16^3 with --quick)
     n3 = 16 if args.quick else 64
@@ -342,8 +344,12 @@ def secondary(args):
                           cells_per_s=m3.ncells / (ms * 1e-3), gbs=alg / (ms * 1e-3) / 1e9,
                           frac=alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
                           strategy=pl.global_kernel.compile().strategy_used)
+            if key == "fused":
+                res[k].update(c3_rows_relerr(m3, pl, A, R))
             log(f"C3 {k}: {ms * 1e3:.1f} us, {res[k]['gbs']:.0f} GB/s = {res[k]['frac']:.1%} "
-                f"({res[k]['strategy']})")
+                f"({res[k]['strategy']})" + (f", first {res[k]['parity_rows']} rows vs CPU: Mat rel err "
+                                              f"{res[k]['relerr_mat_vs_cpu']:.1e}, residual "
+                                              f"{res[k]['relerr_residual_vs_cpu']:.1e}" if key == "fused" else ""))
         if tag == "f64":
             res["c4_adjoint"] = adjoint_ratio(m3, cells, c2v, c2n, X, u, reps)
             if not args.no_cpu:
@@ -452,6 +458,59 @@ def batched(args):
         f"batched {t_batch * 1e3:.3f} ms/pass (kernel {kms * 1e3:.1f} us, grid {batch.grid}, "
         f"{batch.nitems} items): {t_plain / t_batch:.1f}x; batched vs plain rel err {worst:.1e}")
     return res
+
+
+def c1_relerr(m1, pl, name):
+    """One pass of the timed C1 loop from zero against the oracle's
+    sequential wrapper (float64, strict IEEE) on the whole mesh."""
+    from oracle import seqwrap
+    from pyop2_amd import elements
+    out = pl.arguments[0].data
+    out.zero()
+    pl()
+    got = np.asarray(out.data_ro, dtype=np.float64).reshape(-1)
+    ref = np.zeros(m1.nverts)
+    args = [seqwrap.DatArg(ref, "INC", m1.cell_vertex), seqwrap.DatArg(m1.coords, "READ", m1.cell_vertex)]
+    if name == "p1_laplace_residual":
+        args.append(seqwrap.DatArg(m1.u, "READ", m1.cell_vertex))
+    seqwrap.run(elements.kernel_source(name), name, args, [(0, m1.ncells)])
+    return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+
+def c3_rows_relerr(m3, pl, A, R, rows=4096):
+    """One pass of the timed fused loop (residual + BSR(3) assembly) from
+    zero against the oracle's sequential wrapper (float64, strict IEEE) on
+    the first ``rows`` P2 nodes: the oracle runs every cell incident to
+    them, so their block rows and residual entries are complete; the
+    pattern of those rows must match bit for bit."""
+    from oracle import seqwrap, sparsity as osp
+    from pyop2_amd import device as dev, elements
+    A.zero()
+    R.zero()
+    pl()
+    A.assemble()
+    sel = np.nonzero((m3.cell_node < rows).any(axis=1))[0]
+    cn = np.ascontiguousarray(m3.cell_node[sel], dtype=np.int32)
+    cv = np.ascontiguousarray(m3.cell_vertex[sel], dtype=np.int32)
+    rp, ci = osp.pattern(m3.nnodes, m3.nnodes, [(cn, cn, len(sel))])
+    Ao = seqwrap.CSRMat(rp, ci, np.zeros((len(ci), 3, 3)), 3, 3, block=True)
+    Ro = np.zeros((m3.nnodes, 3))
+    rcode = elements.kernel_source("p2_elasticity_residual", lam=1.0, mu=0.5)
+    mcode = elements.kernel_source("p2_elasticity_matrix", lam=1.0, mu=0.5)
+    seqwrap.run(rcode, "p2_elasticity_residual", [seqwrap.DatArg(Ro, "INC", cn), seqwrap.DatArg(m3.coords, "READ", cv),
+                                                  seqwrap.DatArg(m3.u, "READ", cn)], [(0, len(sel))])
+    seqwrap.run(mcode, "p2_elasticity_matrix", [seqwrap.MatArg(Ao, "INC", cn, cn, 3, 3),
+                                                seqwrap.DatArg(m3.coords, "READ", cv)], [(0, len(sel))])
+    grp = dev.to_host(A.rowptr[:rows + 1]).astype(np.int64)
+    nz = int(grp[-1])
+    gci = dev.to_host(A.colidx[:nz])
+    gv = dev.to_host(A.device_values[:nz * 9]).astype(np.float64).reshape(nz, 3, 3)
+    pattern = bool(np.array_equal(grp, rp[:rows + 1]) and np.array_equal(gci, ci[:nz]))
+    ov = Ao.values[:nz]
+    gr = dev.to_host(R.device_data_ro[:rows]).astype(np.float64)
+    return dict(parity_rows=rows, parity_cells=int(len(sel)), pattern_rows_bit_exact=pattern,
+                relerr_mat_vs_cpu=float(np.linalg.norm(gv - ov) / np.linalg.norm(ov)),
+                relerr_residual_vs_cpu=float(np.linalg.norm(gr - Ro[:rows]) / np.linalg.norm(Ro[:rows])))
 
 
 def c3_cpu_baseline(m3, res, sample=24576):
