@@ -240,3 +240,25 @@ def test_empty_map_and_iterset():
     d = op2.Dat(s2 ** 1, [0] * 10, dtype=int)
     k = op2.Kernel("static void k(int *x) {}", "k")
     op2.par_loop(k, s1, d(op2.READ, m))
+
+
+def test_iteration_region_and_periodic_api():
+    """Iteration regions are the reference's IntEnum (types/__init__.py:14-31);
+    a Sparsity stores each pair's regions sorted (ALL when None is given,
+    mat.py:139-156); an ExtrudedSet may be periodic and a Map carries its
+    offset quotients into the kernel argument."""
+    assert [int(r) for r in (op2.ON_BOTTOM, op2.ON_TOP, op2.ON_INTERIOR_FACETS, op2.ALL)] == [1, 2, 3, 4]
+    base = op2.Set(3)
+    ext = op2.ExtrudedSet(base, 4, extruded_periodic=True)
+    assert ext._extruded_periodic and ext.constant_layers and ext.layers == 4
+    t = op2.Set(20)
+    m = op2.Map(ext, t, 2, [[0, 1], [5, 6], [10, 11]], offset=[1, 1], offset_quotient=[0, 1])
+    assert m._global_kernel_arg.offset == (1, 1) and m._global_kernel_arg.offset_quotient == (0, 1)
+    sp = op2.Sparsity((t ** 1, t ** 1), [(m, m, (op2.ON_TOP, op2.ON_BOTTOM))])
+    assert sp.iteration_regions[(0, 0)] == [(op2.ON_BOTTOM, op2.ON_TOP)]
+    sp_all = op2.Sparsity((t ** 1, t ** 1), [(m, m, None)])
+    assert sp_all.iteration_regions[(0, 0)] == [(op2.ALL,)]
+    pm = op2.PermutedMap(op2.Map(ext, t, 2, [[0, 1], [5, 6], [10, 11]], offset=[1, 2]), [1, 0])
+    assert list(pm.offset) == [2, 1]
+    with pytest.raises(ValueError):
+        op2.Sparsity((t ** 1, t ** 1), [(m, m, (7,))])
