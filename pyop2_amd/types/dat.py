@@ -210,8 +210,11 @@ class AbstractDat(DataCarrier, DeviceStorage):
         return other
 
     def _iop(self, other, op):
+        # the result keeps this Dat's type, as the reference's generated
+        # ``self[i] = self[i] op other[i]`` does (dat.py:466-491): computed in
+        # the promoted type, then converted (integers truncate toward zero)
         a = self._device_array()[:self.dataset.size]
-        getattr(a, op)(self._operand(other))
+        a.copy_(getattr(a, op.rstrip("_"))(self._operand(other)))
         self._device_modified()
         self.increment_dat_version()
         return self
@@ -254,6 +257,16 @@ class AbstractDat(DataCarrier, DeviceStorage):
 
     def __neg__(self):
         return self._op(-1, "mul")
+
+    def __pos__(self):
+        """A copy (``dat.py:495-497``)."""
+        return Dat(self)
+
+    def __rsub__(self, other):
+        """``other - self`` (``dat.py:538-544``)."""
+        ret = -self
+        ret += other
+        return ret
 
     def inner(self, other):
         """Inner product of the owned values (a float)."""
