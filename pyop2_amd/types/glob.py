@@ -139,6 +139,22 @@ class SetFreeDataCarrier(DataCarrier, DeviceStorage):
     def __truediv__(self, other):
         return type(self)(self.dim, data=self.data_ro / self._scalar(other), dtype=self.dtype, name=self.name)
 
+    def __radd__(self, other):
+        return self + other
+
+    def __rsub__(self, other):
+        """``other - self`` (``glob.py:164-170``)."""
+        ret = -self
+        ret += other
+        return ret
+
+    def __rmul__(self, other):
+        return self * other
+
+    def __pos__(self):
+        """A copy (``glob.py:147-148``)."""
+        return self.duplicate()
+
     def __iadd__(self, other):
         self.data = self.data_ro + self._scalar(other)
         return self

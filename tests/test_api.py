@@ -262,3 +262,19 @@ def test_iteration_region_and_periodic_api():
     assert list(pm.offset) == [2, 1]
     with pytest.raises(ValueError):
         op2.Sparsity((t ** 1, t ** 1), [(m, m, (7,))])
+
+
+def test_global_arithmetic():
+    """Globals combine with scalars and Globals from either side
+    (glob.py:147-190): the result is a new Global; unary plus copies."""
+    g = op2.Global(2, [2.0, 4.0], np.float64, "g")
+    h = op2.Global(2, [1.0, 1.0], np.float64, "h")
+    assert list((g + h).data_ro) == [3.0, 5.0]
+    assert list((1.0 + g).data_ro) == [3.0, 5.0]
+    assert list((1.0 - g).data_ro) == [-1.0, -3.0]
+    assert list((3.0 * g).data_ro) == [6.0, 12.0]
+    assert list((g / 2.0).data_ro) == [1.0, 2.0]
+    p = +g
+    assert p is not g and list(p.data_ro) == [2.0, 4.0]
+    g -= h
+    assert list(g.data_ro) == [1.0, 3.0]
