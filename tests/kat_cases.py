@@ -349,6 +349,19 @@ def case_empty_iterset(op2):
     assert not np.asarray(A.values).any()
 
 
+def case_region_on_plain_set(op2):
+    """A region other than interior facets on a set that is not extruded
+    changes nothing (builder.py:771-795 only shape the layer loop): every
+    element runs once."""
+    it, tgt = op2.Set(6), op2.Set(3)
+    m = op2.Map(it, tgt, 1, np.array([0, 1, 2, 0, 1, 2], dtype=np.int32))
+    for region in (op2.ON_BOTTOM, op2.ON_TOP, op2.ALL):
+        d = op2.Dat(tgt ** 1, np.zeros(3), np.float64)
+        op2.par_loop(op2.Kernel("static void one(double *x) { *x += 1.0; }", "one"), it, d(op2.INC, m),
+                     iteration_region=region)
+        assert list(np.asarray(d.data)) == [2.0, 2.0, 2.0]
+
+
 OTHER_CASES = [
     ("edge_sum_write", lambda op2: case_edge_sum(op2, mode="WRITE")),
     ("edge_sum_inc", lambda op2: case_edge_sum(op2, mode="INC")),
@@ -364,6 +377,7 @@ OTHER_CASES = [
     ("write_vs_inc_matrix", case_write_vs_inc_matrix),
     ("sparsity_nnz", case_sparsity_nnz),
     ("empty_iterset", case_empty_iterset),
+    ("region_on_plain_set", case_region_on_plain_set),
 ]
 
 
