@@ -16,12 +16,16 @@ The descriptor classes keep the reference's fields and cache keys
 (``global_kernel.py:23-247``).
 """
 import collections
+import ctypes
 import itertools
 from dataclasses import dataclass
 from typing import Optional, Tuple
 
+import numpy as np
+
 from pyop2_amd.configuration import configuration
 from pyop2_amd.types.access import READ
+from pyop2_amd.types.region import IterationRegion
 from pyop2_amd.utils import cached_property
 
 
@@ -216,5 +220,35 @@ class GlobalKernel:
         from pyop2_amd.codegen import GeneratedKernel
         return not isinstance(self.compile(), GeneratedKernel)
 
+    @property
+    def _wrapper_name(self):
+        import warnings
+        warnings.warn("GlobalKernel._wrapper_name is a deprecated alias for GlobalKernel.name",
+                      DeprecationWarning)
+        return self.name
+
+    @cached_property
+    def argtypes(self):
+        """ctypes types of ``__call__``'s ``(start, end, *ptrs)``
+        (``global_kernel.py:410-417``): two int32, then one pointer for an
+        extruded set's layers, one for a Subset's indices, one per data
+        pointer of each argument (a Mat is one descriptor, a mixed argument
+        one per component or block), and one per distinct map."""
+        n = int(self._extruded) + int(bool(self._subset))
+        for a in self.arguments:
+            n += len(a.arguments) if isinstance(a, (MixedDatKernelArg, MixedMatKernelArg)) else 1
+        n += len({id(m) for a in self.arguments for m in a.maps})
+        return (ctypes.c_int32, ctypes.c_int32) + (ctypes.c_void_p,) * n
+
     def num_flops(self, iterset):
-        return self.local_kernel.num_flops
+        """Flops of one execution over ``iterset``: the local kernel's, times
+        the layers an extruded loop runs per column (``global_kernel.py:419-429``)."""
+        size = 1
+        if getattr(iterset, "_extruded", False):
+            region = self._iteration_region
+            layers = np.mean(iterset.layers_array[:, 1] - iterset.layers_array[:, 0])
+            if region == IterationRegion.INTERIOR_FACETS:
+                size = layers - 2
+            elif region not in {IterationRegion.TOP, IterationRegion.BOTTOM}:
+                size = layers - 1
+        return size * self.local_kernel.num_flops

@@ -186,15 +186,15 @@ class Parloop:
 
     @property
     def arglist(self):
-        """Pointers in the reference's order: data of each argument, then
-        the unique maps in first-use order (``parloop.py:203-212``)."""
-        arglist = ()
+        """Pointers in the reference's order (``parloop.py:203-212``): the
+        iteration set's (an extruded set's layers, a Subset's indices), the
+        data of each argument, then the maps' pointers without repeats, in
+        first-use order."""
+        arglist = self.iterset._kernel_args_
         for d in self.arguments:
             arglist += d._kernel_args_
-        maps = {m: None for d in self.arguments for m in d.maps}
-        for m in maps:
-            arglist += m._kernel_args_
-        return arglist
+        maps = {p: None for d in self.arguments for p in d.map_kernel_args}
+        return arglist + tuple(maps)
 
     @property
     def zipped_arguments(self):

@@ -226,6 +226,13 @@ class ExtrudedSet(Set):
         return self._layers
 
     @property
+    def _kernel_args_(self):
+        """The layers array's device pointer, the wrapper's first pointer
+        argument (``set.py:349-351``)."""
+        from pyop2_amd import device as dev
+        return (dev.ptr(self.device_layers),)
+
+    @property
     def device_layers(self):
         """Device copy of ``layers_array``."""
         if self._dev_layers is None:
@@ -325,6 +332,13 @@ class Subset(Set):
     @utils.cached_property
     def owned_indices(self):
         return self.indices[self.indices < self.superset.size]
+
+    @property
+    def _kernel_args_(self):
+        """The superset's pointers, then the indices' device pointer
+        (``set.py:432-434``)."""
+        from pyop2_amd import device as dev
+        return self._superset._kernel_args_ + (dev.ptr(self.device_indices),)
 
     @property
     def device_indices(self):

@@ -183,3 +183,33 @@ def test_extruded_regions_random_against_oracle(device, strategy, periodic):
 def test_extruded_cases(device, name, case, strategy):
     configuration["strategy"] = strategy
     case(op2)
+
+
+def test_arglist_matches_argtypes(device):
+    """Parloop.arglist (iteration set pointers, data pointers, distinct map
+    pointers; parloop.py:203-212) has the length GlobalKernel.argtypes
+    declares after start and end (global_kernel.py:410-417): plain, Subset,
+    extruded, mixed and Mat arguments."""
+    import numpy as np
+    from pyop2_amd.parloop import LegacyParloop
+    s, t = op2.Set(4), op2.Set(10)
+    m = op2.Map(s, t, 2, np.arange(8, dtype=np.int32).reshape(4, 2))
+    d = op2.Dat(t ** 1, dtype=np.float64)
+    g = op2.Global(1, 0.0, np.float64)
+    A = op2.Mat(op2.Sparsity((t ** 1, t ** 1), [(m, m, None)]), np.float64)
+    ext = op2.ExtrudedSet(s, 3)
+    me = op2.Map(ext, t, 2, np.array([[0, 1], [2, 3], [4, 5], [6, 7]], dtype=np.int32), offset=[1, 1])
+    ms = op2.MixedSet((t, t))
+    mm = op2.MixedMap((m, m))
+    md = op2.MixedDat(ms)
+    loops = [
+        LegacyParloop(op2.Kernel("static void a1(double *x, double *g) {}", "a1"), s, d(op2.INC, m), g(op2.INC)),
+        LegacyParloop(op2.Kernel("static void a2(double *x) {}", "a2"), s([1, 3]), d(op2.INC, m)),
+        LegacyParloop(op2.Kernel("static void a3(double *x, double *y) {}", "a3"), ext, d(op2.INC, me),
+                      d(op2.READ, me), iteration_region=op2.ON_INTERIOR_FACETS),
+        LegacyParloop(op2.Kernel("static void a4(double *x) {}", "a4"), s, md(op2.INC, mm)),
+        LegacyParloop(op2.Kernel("static void a5(double *a, double *x) {}", "a5"), s, A(op2.INC, (m, m)),
+                      d(op2.READ, m)),
+    ]
+    for pl in loops:
+        assert len(pl.global_kernel.argtypes) == 2 + len(pl.arglist)
