@@ -158,6 +158,18 @@ class _SubLoop:
         self.arguments = arguments
 
 
+def fits_32bit_indexing(parloop):
+    """True if every element and node index product of the loop (element x
+    map arity, node x 3 components) stays below 2**31, the range of the
+    hand-written loops' int32 arithmetic."""
+    limit = 2 ** 31 - 1
+    for a in parloop.arguments:
+        for m in getattr(a, "maps", ()):
+            if m.iterset.total_size * m.arity > limit or m.toset.total_size * 3 > limit:
+                return False
+    return True
+
+
 class HandWrittenKernel:
     """Launches a registered element loop through its C entry point."""
 
@@ -171,7 +183,10 @@ class HandWrittenKernel:
         self.strategy_used = None
 
     def _check_maps(self, parloop):
-        """Arguments that the entry point reads through one map must share it."""
+        """Arguments that the entry point reads through one map must share it,
+        and the loop must fit the hand-written loops' 32-bit indexing."""
+        if not fits_32bit_indexing(parloop):
+            return False   # the generated loops index in 64 bits
         maps = [a.maps if hasattr(a, "maps") else () for a in parloop.arguments]
         if any(m.has_negative for ms in maps for m in ms):
             return False   # the hand-written loops assume every entry has a target
@@ -227,7 +242,7 @@ class HandWrittenKernel:
         if parloop is None:
             raise RuntimeError("hand-written loops are launched by a Parloop")
         if not self._check_maps(parloop):
-            # same values through different Map objects: take the generated path
+            # a loop the entry point does not take: the generated path
             from pyop2_amd.codegen import GeneratedKernel
             return GeneratedKernel(self.gk)(start, end, ptrs, parloop)
         self.launch(start, end, parloop, self._choose(parloop))

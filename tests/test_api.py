@@ -278,3 +278,17 @@ def test_global_arithmetic():
     assert p is not g and list(p.data_ro) == [2.0, 4.0]
     g -= h
     assert list(g.data_ro) == [1.0, 3.0]
+
+
+def test_large_loops_take_the_64bit_path():
+    """Loops whose element x arity or node x 3 products pass 2**31 leave the
+    hand-written (int32-indexed) entry points for the generated loops."""
+    from types import SimpleNamespace
+    from pyop2_amd.registry import fits_32bit_indexing
+    small, big, nodes = op2.Set(1000), op2.Set(300_000_000), op2.Set(1000)
+    for it, want in ((small, True), (big, False)):
+        m = op2.Map(it, nodes, 10)
+        assert fits_32bit_indexing(SimpleNamespace(arguments=[SimpleNamespace(maps=(m,))])) is want
+    huge_nodes = op2.Set(800_000_000)
+    m = op2.Map(small, huge_nodes, 4)
+    assert not fits_32bit_indexing(SimpleNamespace(arguments=[SimpleNamespace(maps=(m,))]))
