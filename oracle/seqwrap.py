@@ -31,6 +31,17 @@ A Subset iteration set indirects ``n`` through ``subset_indices``
 (``builder.py:725-733``).  Map entries below zero are skipped, as PETSc skips
 negative indices in ``MatSetValues``.
 
+Extruded iteration sets (``builder.py:683-800``): column ``n`` runs the
+layers of its iteration region (all cells ``[bottom, cend)``, the bottom
+cell, the top cell, or the interior facets ``[bottom, cend - 1)``, with
+``cend = layers[n][1] - 1``; a column with no cells runs no top cell), and
+a map entry of the cell ``l`` layers above the bottom is ``base + offset *
+l``; over interior facets every map names the cell below, then the cell
+above (``builder.py:81-129``).  On a periodic set the layer offset wraps
+with the generated loop's remainder ``a < n ? a : a - n`` over the column's
+``n`` cells, shifted by the entry's offset quotient, and a column has as
+many interior facets as cells (``builder.py:27-30, 103-121``).
+
 Two builds are produced: ``strict`` (``-O2 -ffp-contract=off``; the parity
 oracle) and ``fast`` (the reference's own flags ``-march=native -O3
 -ffast-math``, ``pyop2/compilation.py:328``; the CPU baseline).
