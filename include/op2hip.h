@@ -121,6 +121,11 @@ typedef struct op2hip_exec {
                                       (NULL: read from the Mat's rowptr) */
   /* Subset iteration: element n = subset[i] for i in [start,end) (or NULL) */
   const int32_t *subset;           /* device */
+  /* OP2HIP_ROWGATHER_ROWLIST, optional (arity-3 row and column maps): one
+     word per batch incidence in place of batch_ent and slot_table -- bits
+     0-31 the incidence (element * 3 + i), 32-47 the row's block offset in
+     the batch, 48-62 the three column slots (5 bits each, 31 = missing) */
+  const uint64_t *batch_packed;    /* device, or NULL */
 } op2hip_exec;
 
 /* Device CSR / BSR matrix on a fixed pattern (the reference's PETSc AIJ or

@@ -1331,7 +1331,8 @@ rowgather_rowlist_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
                          const BatchDesc *__restrict__ descs, int32_t nbatches,
                          const int2 *__restrict__ row_order, const int32_t *__restrict__ bent,
                          const SlotEntry<Elem::kCar> *__restrict__ table,
-                         const int32_t *__restrict__ rbase, typename Elem::T *__restrict__ rowdat) {
+                         const int32_t *__restrict__ rbase, const uint64_t *__restrict__ packed,
+                         typename Elem::T *__restrict__ rowdat) {
   using T = typename Elem::T;
   using Acc = typename AccumOf<T>::type;
   constexpr int kBs = Elem::kRd * Elem::kCd;
@@ -1369,8 +1370,28 @@ rowgather_rowlist_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
 
   for (int32_t t0 = threadIdx.x; t0 < d.ni; t0 += blockDim.x) {
     const int32_t t = SpreadOf<Elem>::value ? spread_incidence(t0, d.ni) : t0;
-    const int32_t ent = bent[d.i0 + t];
-    const SlotEntry<Elem::kCar> se = table[d.i0 + t];
+    int32_t ent;
+    SlotEntry<Elem::kCar> se;
+    if constexpr (Elem::kCar == 3) {
+      if (packed) {
+        // one word: incidence, row offset and three 5-bit slots (31: none)
+        const uint64_t w = packed[d.i0 + t];
+        ent = (int32_t)(uint32_t)w;
+        se.row_off = (uint16_t)(w >> 32);
+        se.lr = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const uint32_t s = (uint32_t)(w >> (48 + 5 * j)) & 31u;
+          se.slot[j] = s == 31u ? uint8_t(255) : uint8_t(s);
+        }
+      } else {
+        ent = bent[d.i0 + t];
+        se = table[d.i0 + t];
+      }
+    } else {
+      ent = bent[d.i0 + t];
+      se = table[d.i0 + t];
+    }
     const int32_t e = ent / Elem::kRar;
     if (e < start || e >= end) continue;
     const int i = ent - e * Elem::kRar;
@@ -1433,10 +1454,12 @@ int launch_rowgather_rowlist(int32_t start, int32_t end, const Elem &elem, const
   const int64_t per_thread = ((int64_t)ex->batch_max_nnz * Elem::kRd * Elem::kCd + bs - 1) / bs;
   if (per_thread <= 8)
     rowgather_rowlist_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(
-        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base, rowdat);
+        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base,
+        Elem::kCar == 3 && Elem::kRar == 3 ? ex->batch_packed : nullptr, rowdat);
   else
     rowgather_rowlist_kernel<Elem, 16><<<ex->nbatches, bs, lds, s>>>(
-        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base, rowdat);
+        start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base,
+        Elem::kCar == 3 && Elem::kRar == 3 ? ex->batch_packed : nullptr, rowdat);
   OP2HIP_LAUNCHED();
   return OP2HIP_OK;
 }

@@ -71,6 +71,7 @@ class ExecPlan(ctypes.Structure):
         ("batch_ent", ctypes.c_void_p),
         ("row_base", ctypes.c_void_p),
         ("subset", ctypes.c_void_p),
+        ("batch_packed", ctypes.c_void_p),
     ]
 
 

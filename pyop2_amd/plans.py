@@ -392,6 +392,7 @@ class RowList:
     max_nnz: int
     max_rows: int
     row_base: object = None   # device, rowptr[row] per row_order entry
+    packed: object = None     # device, one word per incidence (arity-3 maps), or None
 
     def exec_fields(self, ex):
         ex.nbatches = self.nbatches
@@ -399,10 +400,31 @@ class RowList:
         ex.row_order = dev.ptr(self.row_order)
         ex.batch_ent = dev.ptr(self.ent)
         ex.row_base = dev.ptr(self.row_base) if self.row_base is not None else None
+        ex.batch_packed = dev.ptr(self.packed) if self.packed is not None else None
         ex.slot_table = dev.ptr(self.table)
         ex.batch_max_inc = self.max_inc
         ex.batch_max_nnz = self.max_nnz
         ex.batch_max_rows = self.max_rows
+
+
+def pack_row_list(ent, table, rarity, carity):
+    """The row list's per-incidence words for arity-3 maps (None otherwise):
+    bits 0-31 the incidence, 32-47 the row's block offset in its batch,
+    48-62 the three column slots, 5 bits each (31: not in the pattern).
+    ``table`` holds the slot entries as bytes: row offset (u16), batch row
+    (u16), one u8 slot per column (255: missing)."""
+    if rarity != 3 or carity != 3 or table.shape[1] < 7 or len(ent) == 0:
+        return None
+    t = table.astype(np.uint64)
+    slots = t[:, 4:7]
+    if ((slots >= 31) & (slots != 255)).any():
+        return None
+    slots = np.where(slots == 255, np.uint64(31), slots)
+    word = ent.astype(np.int64).astype(np.uint64) & np.uint64(0xFFFFFFFF)
+    word |= (t[:, 0] | (t[:, 1] << np.uint64(8))) << np.uint64(32)
+    for j in range(3):
+        word |= slots[:, j] << np.uint64(48 + 5 * j)
+    return np.ascontiguousarray(word)
 
 
 def row_list(rmap, cmap, tr, mat, max_inc, max_nnz, key):
@@ -437,8 +459,10 @@ def row_list(rmap, cmap, tr, mat, max_inc, max_nnz, key):
         from pyop2_amd.exceptions import PatternError
         raise PatternError(f"{missing.value} map entries are not in the sparsity pattern")
     base = rowptr[order[0:2 * nrows:2]] if nrows else np.zeros(1, dtype=np.int32)
+    packed = pack_row_list(ent[:ninc], table.reshape(max(ninc, 1), -1)[:ninc], rmap.arity, cmap.arity)
     plan = RowList(nb.value, dev.to_device(desc[:max(nb.value, 1)]), dev.to_device(order),
                    dev.to_device(ent), dev.to_device(table), mi.value, mn.value, mr.value,
-                   dev.to_device(np.ascontiguousarray(base, dtype=np.int32)))
+                   dev.to_device(np.ascontiguousarray(base, dtype=np.int32)),
+                   dev.to_device(packed.view(np.int64)) if packed is not None else None)
     rmap._cache[ck] = plan
     return plan

@@ -364,3 +364,30 @@ def test_integration_stub_struct_matches_the_abi():
     text = open(path).read()
     stub = text[text.index("class op2hip_exec"):text.index("_f = _lib")]
     assert re.findall(r'\("(\w+)"', stub) == [f[0] for f in dev.ExecPlan._fields_]
+
+
+def test_row_list_packing_round_trips():
+    """The packed row-list words hold each incidence, its row offset and
+    its three slots (255 -> 31 -> missing); a slot the 5 bits cannot hold,
+    or maps of another arity, keep the unpacked plan."""
+    from pyop2_amd.plans import pack_row_list
+    rng = np.random.default_rng(5)
+    n = 1000
+    ent = rng.integers(0, 2 ** 31 - 1, n).astype(np.int32)
+    table = np.zeros((n, 8), dtype=np.uint8)
+    row_off = rng.integers(0, 2 ** 16, n)
+    table[:, 0], table[:, 1] = row_off & 0xFF, row_off >> 8
+    table[:, 2:4] = rng.integers(0, 256, (n, 2))       # batch row: not packed
+    slots = rng.integers(0, 31, (n, 3))
+    slots[rng.random((n, 3)) < 0.1] = 255
+    table[:, 4:7] = slots
+    w = pack_row_list(ent, table, 3, 3)
+    assert w.dtype == np.uint64
+    assert np.array_equal((w & 0xFFFFFFFF).astype(np.int64), ent.astype(np.int64))
+    assert np.array_equal((w >> 32) & 0xFFFF, row_off.astype(np.uint64))
+    for j in range(3):
+        s = (w >> np.uint64(48 + 5 * j)) & np.uint64(31)
+        assert np.array_equal(np.where(s == 31, 255, s), slots[:, j])
+    table[0, 5] = 31
+    assert pack_row_list(ent, table, 3, 3) is None
+    assert pack_row_list(ent, table, 4, 4) is None
