@@ -591,8 +591,8 @@ def main():
                     help="skip the cache-flushed C2 variant (profiles of the warm kernels)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: pyop2_amd has no multi-GPU path "
+              "(its sharding is designed, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     result = run(args)
     print(json.dumps(result), flush=True)
