@@ -608,9 +608,10 @@ class GeneratedKernel:
         extruded = getattr(self.gk, "_extruded", False)
         if extruded:
             region = getattr(self.gk, "_iteration_region", None)
+            columns_of = iterset.superset if isinstance(iterset, Subset) else iterset
             for m in maps:
                 if getattr(m, "offset", None) is not None:
-                    m.check_extruded(iterset, region)
+                    m.check_extruded(columns_of, region)
             vals.append(ctypes.c_void_p(dev.ptr(iterset.device_layers)))
         L = dev.lib()
         s = dev.stream()
@@ -635,10 +636,11 @@ class GeneratedKernel:
                 # a column's targets over all its layers (a thread runs the
                 # layers of its column one after the other)
                 region = getattr(self.gk, "_iteration_region", None)
-                targets = [(_ColumnTargets(m, iterset, start, end, region), id(m.toset)) for m in conflict]
-                elems = np.arange(end - start, dtype=np.int32)
-                plan = plans.colour_plan(conflict[0], elems, targets, ("extruded",) + key)
-                plan = _shifted_plan(plan, start, conflict[0], ("extruded-shift",) + key)
+                cols = np.ascontiguousarray(elems, dtype=np.int32)   # column ids, in launch order
+                targets = [(_ColumnTargets(m, iterset, cols, region), id(m.toset)) for m in conflict]
+                plan = plans.colour_plan(conflict[0], np.arange(len(cols), dtype=np.int32), targets,
+                                         ("extruded",) + key)
+                plan = _column_plan(plan, cols, conflict[0], ("extruded-cols",) + key)
             else:
                 plan = plans.colour_plan(conflict[0], elems, [(m, id(m.toset)) for m in conflict], key)
             ep = dev.ptr(plan.elems)
@@ -654,20 +656,21 @@ class GeneratedKernel:
 
 
 class _ColumnTargets:
-    """Duck-typed map for colouring the columns [start, end) of an extruded
-    loop: row c lists every target column start + c touches."""
+    """Duck-typed map for colouring the columns ``cols`` of an extruded loop
+    (a range of the set, or a Subset's columns): row c lists every target
+    column cols[c] touches over the layers it runs."""
 
-    def __init__(self, m, iterset, start, end, region=None):
-        self._table = m.column_table(iterset, np.arange(start, end), region)
+    def __init__(self, m, iterset, cols, region=None):
+        self._table = m.column_table(iterset, cols, region)
         self.toset = m.toset
 
 
-def _shifted_plan(plan, start, owner, key):
-    """A colour plan over local column positions turned into column ids."""
+def _column_plan(plan, cols, owner, key):
+    """A colour plan over positions in ``cols`` turned into column ids."""
     from pyop2_amd import plans
     got = owner._cache.get(key)
     if got is None:
-        elems = dev.to_host(plan.elems).astype(np.int32) + np.int32(start)
+        elems = cols[dev.to_host(plan.elems).astype(np.int64)].astype(np.int32)
         got = plans.ColourPlan(plan.colours, dev.to_device(elems), plan.offsets, plan.ncolours)
         owner._cache[key] = got
     return got

@@ -286,7 +286,7 @@ class Subset(Set):
         if isinstance(superset, Subset):
             indices = superset.indices[indices]
             superset = superset.superset
-        assert type(superset) is Set, "Subset construction failed, should not happen"
+        assert type(superset) in (Set, ExtrudedSet), "Subset construction failed, should not happen"
         self._superset = superset
         self._indices = utils.verify_reshape(indices, IntType, (len(indices),))
         if len(self._indices) > 0 and (self._indices[0] < 0 or self._indices[-1] >= self._superset.total_size):
@@ -296,8 +296,9 @@ class Subset(Set):
         self._sizes = (int((self._indices < superset.core_size).sum()),
                        int((self._indices < superset.size).sum()),
                        len(self._indices))
-        self._extruded = False
-        self._extruded_periodic = False
+        # a Subset of an ExtrudedSet runs the chosen columns (set.py:427-428)
+        self._extruded = superset._extruded
+        self._extruded_periodic = superset._extruded_periodic
         self._device_indices = None
 
     def __getattr__(self, name):

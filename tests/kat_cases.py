@@ -1348,6 +1348,38 @@ def case_extruded_permuted_offsets(op2, split=None):
     assert np.array_equal(np.asarray(d.data), want)
 
 
+def case_extruded_subset(op2, split=None):
+    """A Subset of an ExtrudedSet runs the chosen columns only (set.py:394-428):
+    an INC of 1 through the 6-entry dof map over columns {1, 5, 6} reaches
+    each node of those columns' triangles, every layer, as often as chosen
+    elements share it; a WRITE of the layer through the cell map sets the
+    chosen columns' cells only; a direct INC counts their cells."""
+    base, elements, node_set, elem_set, cmap, fmap, coords, field = _strip(op2)
+    chosen = [1, 5, 6]
+    sub = elements(chosen)
+    cnt = op2.Dat(node_set ** 1, np.zeros(XNODES * XLAYERS), np.float64)
+    one6 = op2.Kernel("static void one6(double x[6]) { for (int i = 0; i < 6; ++i) x[i] += 1.0; }", "one6")
+    op2.par_loop(one6, sub, cnt(op2.INC, cmap))
+    want = np.zeros((XNODES, XLAYERS))
+    for i in chosen:
+        for k in (i, i + 1, i + 2):
+            want[k, 0] += 1.0
+            want[k, 1:XLAYERS - 1] += 2.0
+            want[k, XLAYERS - 1] += 1.0
+    assert np.array_equal(np.asarray(cnt.data), want.reshape(-1))
+    f = op2.Dat(elem_set ** 1, -np.ones(XN * XWEDGES), np.float64)
+    k = op2.Kernel("static void lay(double *y, int layer) { y[0] = layer; }", "lay")
+    op2.par_loop(k, sub, f(op2.WRITE, fmap), pass_layer_arg=True)
+    want = -np.ones((XN, XWEDGES))
+    want[chosen, :] = np.arange(XWEDGES)
+    assert np.array_equal(np.asarray(f.data), want.reshape(-1))
+    col = op2.Dat(base ** 1, np.zeros(XN), np.float64)
+    op2.par_loop(op2.Kernel("static void one(double *x) { *x += 1.0; }", "one"), sub, col(op2.INC))
+    want = np.zeros(XN)
+    want[chosen] = XWEDGES
+    assert np.array_equal(np.asarray(col.data), want)
+
+
 EXTRUDED_CASES = [
     ("extruded_volume", case_extruded_volume),
     ("extruded_direct_inc", case_extruded_direct_inc),
@@ -1365,4 +1397,5 @@ EXTRUDED_CASES = [
     ("periodic_vertices", case_periodic_vertices),
     ("periodic_facets", case_periodic_facets),
     ("periodic_variable_layers", case_periodic_variable_layers),
+    ("extruded_subset", case_extruded_subset),
 ]

@@ -439,7 +439,8 @@ def par_loop(kernel, iterset, *args, pass_layer_arg=False, iteration_region=None
     parts = [(0, iterset.core_size), (iterset.core_size, iterset.size)]
     subset = iterset.indices if isinstance(iterset, Subset) else None
     ext = {}
-    if isinstance(iterset, ExtrudedSet):
+    columns = iterset.superset if isinstance(iterset, Subset) else iterset
+    if isinstance(columns, ExtrudedSet):
         offs, quos = [], []
         for kind, obj, access, path in args:
             for m in (path if isinstance(path, tuple) else (path,)):
@@ -447,8 +448,8 @@ def par_loop(kernel, iterset, *args, pass_layer_arg=False, iteration_region=None
                     offs.append((m.table, m.offset))
                     if m.offset_quotient is not None:
                         quos.append((m.table, m.offset_quotient))
-        ext = dict(layers=iterset.layers_array, offsets=offs, pass_layer=pass_layer_arg,
-                   region=iteration_region or ALL, periodic=iterset.periodic, quotients=quos)
+        ext = dict(layers=columns.layers_array, offsets=offs, pass_layer=pass_layer_arg,
+                   region=iteration_region or ALL, periodic=columns.periodic, quotients=quos)
     err = seqwrap.run(kernel.code, kernel.name, sargs, parts, subset=subset, **ext)
     assert err == 0, "insertion outside the sparsity pattern"
     for g, tmp in temps:
