@@ -86,14 +86,21 @@ _FLAGS = {
 
 @dataclass
 class DatArg:
-    """A Dat argument: ``data`` is (total, *dim); ``map_`` (iter_total, arity) or None."""
+    """A Dat argument: ``data`` is (total, *dim); ``map_`` (iter_total, arity)
+    or None; ``comp`` the flat component a DatView passes (the kernel sees
+    that one value per entry, ``builder.py:303-410`` view_index), or None."""
     data: np.ndarray
     access: str
     map_: Optional[np.ndarray] = None
+    comp: Optional[int] = None
+
+    @property
+    def stride(self):
+        return int(np.prod(self.data.shape[1:], dtype=np.int64)) if self.data.ndim > 1 else 1
 
     @property
     def cdim(self):
-        return int(np.prod(self.data.shape[1:], dtype=np.int64)) if self.data.ndim > 1 else 1
+        return self.stride if self.comp is None else 1
 
 
 @dataclass
@@ -265,9 +272,9 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
     for k, a in enumerate(args):
         if isinstance(a, DatArg):
             t = _ctype(a.data.dtype)
-            dim = a.cdim
+            dim, stride, comp = a.cdim, a.stride, (a.comp or 0)
             if a.map_ is None:
-                call.append(f"(void *)(arg{k} + (int64_t)n * {dim})")
+                call.append(f"(void *)(arg{k} + (int64_t)n * {stride} + {comp})")
                 continue
             m = nmaps_ids[k]
             bar = a.map_.shape[1]
@@ -277,7 +284,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                 body.append(f"    for (int t_ = 0; t_ < {ar * dim}; ++t_) buf{k}[t_] = 0;")
             else:
                 body.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, bar, 'i')};"
-                            f" for (int j = 0; j < {dim}; ++j) buf{k}[i * {dim} + j] = m_ >= 0 ? arg{k}[(int64_t)m_ * {dim} + j] : 0; }}")
+                            f" for (int j = 0; j < {dim}; ++j) buf{k}[i * {dim} + j] = m_ >= 0 ? arg{k}[(int64_t)m_ * {stride} + {comp} + j] : 0; }}")
             call.append(f"(void *)buf{k}")
             if a.access == READ:
                 continue
@@ -288,7 +295,7 @@ static int32_t op2o_slot(const int32_t *rowptr, const int32_t *colidx,
                   MAX: "{{ const {t} v_ = buf{k}[i * {dim} + j]; if (v_ > arg{k}[o_]) arg{k}[o_] = v_; }}"}[a.access]
             op = op.format(k=k, dim=dim, t=t)
             unpack.append(f"    for (int i = 0; i < {ar}; ++i) {{ const int32_t m_ = {mv(m, bar, 'i')};"
-                          f" if (m_ < 0) continue; for (int j = 0; j < {dim}; ++j) {{ const int64_t o_ = (int64_t)m_ * {dim} + j; {op} }} }}")
+                          f" if (m_ < 0) continue; for (int j = 0; j < {dim}; ++j) {{ const int64_t o_ = (int64_t)m_ * {stride} + {comp} + j; {op} }} }}")
         elif isinstance(a, GlobalArg):
             t = _ctype(a.data.dtype)
             dim = int(a.data.size)

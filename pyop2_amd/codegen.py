@@ -260,9 +260,13 @@ def generate_source(gk, map_index, block_flags=None):
         t = as_cstr(np.dtype(larg.dtype))
         if isinstance(garg, DatKernelArg):
             dim = int(np.prod(garg.dim))
+            # a DatView passes one component: stride dim, offset comp, width 1
+            # (builder.py:303-410, view_index)
+            comp = 0 if garg.index is None else int(np.ravel_multi_index(garg.index, garg.dim))
+            stride, dim = dim, (dim if garg.index is None else 1)
             params.append(f"{t} *__restrict__ arg{k}")
             if garg.map_ is None:
-                call.append(f"op2_ptr_{{(void *)(arg{k} + (int64_t)n * {dim})}}")
+                call.append(f"op2_ptr_{{(void *)(arg{k} + (int64_t)n * {stride} + {comp})}}")
                 continue
             m = map_index[(k, 0)]
             ar = garg.map_.arity
@@ -274,11 +278,11 @@ def generate_source(gk, map_index, block_flags=None):
                 pre.append(f"    for (int q = 0; q < {ear * dim}; ++q) buf{k}[q] = ({t})0;")
             else:
                 pre.append(f"    for (int i = 0; i < {ear}; ++i) for (int j = 0; j < {dim}; ++j)"
-                           f" buf{k}[i * {dim} + j] = mi{k}[i] >= 0 ? arg{k}[(int64_t)mi{k}[i] * {dim} + j] : ({t})0;")
+                           f" buf{k}[i * {dim} + j] = mi{k}[i] >= 0 ? arg{k}[(int64_t)mi{k}[i] * {stride} + {comp} + j] : ({t})0;")
             call.append(f"op2_ptr_{{(void *)buf{k}}}")
             if acc == READ:
                 continue
-            tgt = f"arg{k}[(int64_t)mi{k}[i] * {dim} + j]"
+            tgt = f"arg{k}[(int64_t)mi{k}[i] * {stride} + {comp} + j]"
             val = f"buf{k}[i * {dim} + j]"
             post.append(f"    for (int i = 0; i < {ear}; ++i) {{ if (mi{k}[i] < 0) continue;"
                         f" for (int j = 0; j < {dim}; ++j) {{ {_dat_store(acc, tgt, val)} }} }}")

@@ -7,7 +7,7 @@ from pyop2_amd.global_kernel import (GlobalKernel, GlobalKernelArg, DatKernelArg
 from pyop2_amd.parloop import (GlobalParloopArg, DatParloopArg, MatParloopArg,  # noqa: F401
                                Parloop, ParLoop, par_loop, parloop, PassthroughArg)
 from pyop2_amd.types import (Access, READ, WRITE, RW, INC, MIN, MAX,  # noqa: F401
-                             Set, Subset, SetPartition, MixedSet, ExtrudedSet, DataSet, Map, Dat,
+                             Set, Subset, SetPartition, MixedSet, ExtrudedSet, DataSet, Map, Dat, DatView,
                              Global, Constant, Sparsity, Mat, PermutedMap, ComposedMap,
                              MixedDataSet, MixedMap, MixedDat,
                              IterationRegion, ON_BOTTOM, ON_TOP, ON_INTERIOR_FACETS, ALL)
@@ -16,7 +16,7 @@ __all__ = ['configuration', 'READ', 'WRITE', 'RW', 'INC', 'MIN', 'MAX',
            'Set', 'Subset', 'ExtrudedSet', 'DataSet', 'MixedSet', 'MixedDataSet', 'Map', 'MixedMap',
            'PermutedMap', 'ComposedMap', 'MixedDat',
            'ON_BOTTOM', 'ON_TOP', 'ON_INTERIOR_FACETS', 'ALL',
-           'Dat', 'Global', 'Constant',
+           'Dat', 'DatView', 'Global', 'Constant',
            'Kernel', 'Sparsity', 'Mat', 'par_loop', 'parloop', 'ParLoop', 'Parloop',
            'GlobalKernel', 'DatKernelArg', 'MatKernelArg', 'GlobalKernelArg', 'MapKernelArg',
            'GlobalParloopArg', 'DatParloopArg', 'MatParloopArg', 'PassthroughArg',

@@ -400,8 +400,10 @@ class DatLegacyArg(LegacyArg):
 
     @property
     def global_kernel_arg(self):
+        from pyop2_amd.types.dat import DatView
         map_arg = self.map_._global_kernel_arg if self.map_ is not None else None
-        return DatKernelArg(self.data.dataset.dim, map_arg)
+        index = self.data.index if isinstance(self.data, DatView) else None
+        return DatKernelArg(self.data.dataset.dim, map_arg, index=index)
 
     @property
     def parloop_arg(self):

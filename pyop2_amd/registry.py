@@ -137,7 +137,7 @@ def lookup(gk):
         if kind == "dat":
             if not isinstance(garg, DatKernelArg) or garg.map_ is None:
                 return None
-            if tuple(garg.dim) != dims or garg.map_.arity != arity:
+            if tuple(garg.dim) != dims or garg.map_.arity != arity or garg.index is not None:
                 return None
         else:
             if not isinstance(garg, MatKernelArg) or garg.unroll:

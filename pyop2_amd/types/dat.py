@@ -342,6 +342,118 @@ class Dat(AbstractDat):
         raise NotImplementedError("PETSc Vec views are not part of the device path")
 
 
+class DatView(AbstractDat):
+    """One component of a Dat (``pyop2/types/dat.py:668-760``).  A par_loop
+    hands the kernel that component only (one value per map entry, or a
+    pointer to the element's component for a direct argument); the data
+    accessors are views of the parent's arrays, and versions and halo
+    validity are the parent's."""
+
+    def __init__(self, dat, index):
+        index = utils.as_tuple(index, int)
+        if len(index) != len(dat.dim):
+            raise ex.IndexValueError("DatView index %s has the wrong rank for shape %s" % (index, dat.dim))
+        for i, d in zip(index, dat.dim):
+            if not (0 <= i < d):
+                raise ex.IndexValueError("Can't create DatView with index %s for Dat with shape %s"
+                                         % (index, dat.dim))
+        self.index = index
+        self._idx = (slice(None), *index) if dat.cdim > 1 else (slice(None),)
+        self._parent = dat
+        self._dataset = dat.dataset
+        self.comm = dat.comm
+        self._name = "view[%s](%s)" % (index, dat.name)
+        self._halo_frozen = False
+        self._frozen_access_mode = None
+
+    @property
+    def dtype(self):
+        return self._parent.dtype
+
+    @property
+    def cdim(self):
+        return 1
+
+    @property
+    def dim(self):
+        return (1,)
+
+    @property
+    def shape(self):
+        return (self.dataset.total_size,)
+
+    @property
+    def _kernel_args_(self):
+        return self._parent._kernel_args_
+
+    def _device_array(self):
+        return self._parent._device_array()
+
+    def _device_modified(self):
+        self._parent._device_modified()
+
+    @property
+    def halo_valid(self):
+        return self._parent.halo_valid
+
+    @halo_valid.setter
+    def halo_valid(self, value):
+        self._parent.halo_valid = value
+
+    @property
+    def dat_version(self):
+        return self._parent.dat_version
+
+    def increment_dat_version(self):
+        self._parent.increment_dat_version()
+
+    @property
+    def _data(self):
+        return self._parent._data[self._idx]
+
+    @property
+    def data(self):
+        return self._parent.data[self._idx]
+
+    @property
+    def data_ro(self):
+        return self._parent.data_ro[self._idx]
+
+    @property
+    def data_wo(self):
+        return self._parent.data_wo[self._idx]
+
+    @property
+    def data_with_halos(self):
+        return self._parent.data_with_halos[self._idx]
+
+    @property
+    def data_ro_with_halos(self):
+        return self._parent.data_ro_with_halos[self._idx]
+
+    @property
+    def data_wo_with_halos(self):
+        return self._parent.data_wo_with_halos[self._idx]
+
+    def zero(self, subset=None):
+        """Zero this component (``dat.py:576-590``): all of it, halo
+        included, which makes the halo valid; or a Subset's owned entries."""
+        self._parent.increment_dat_version()
+        if subset is None:
+            self._parent._data[self._idx] = 0
+            self._parent.halo_valid = True
+        elif subset.superset != self.dataset.set:
+            raise ex.MapValueError("The subset and dataset are incompatible")
+        else:
+            self.data[subset.owned_indices] = 0
+
+    def __str__(self):
+        return "OP2 DatView: %s" % self._name
+
+    def __repr__(self):
+        return "DatView(%r, %r)" % (self._parent, self.index)
+
+
 @contextlib.contextmanager
 def frozen_halo(dat, access_mode):
     """Temporarily disable halo exchanges of ``dat``."""

@@ -45,7 +45,10 @@ class DataCarrier:
         return self._cdim
 
     def increment_dat_version(self):
-        pass
+        """Count a modification (``data_carrier.py:42-48``): carriers with
+        storage keep a version; a Mat has none."""
+        if isinstance(self, DeviceStorage):
+            DeviceStorage.increment_dat_version(self)
 
 
 class DeviceStorage:

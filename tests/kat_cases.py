@@ -349,6 +349,28 @@ def case_empty_iterset(op2):
     assert not np.asarray(A.values).any()
 
 
+def case_dat_view(op2):
+    """A DatView hands the kernel one component (dat.py:668-760,
+    builder.py:303-410): INC of 1 into component 1 of a 2-vector through an
+    arity-2 edge map counts each node's edges there and leaves component 0;
+    READ of component 0 through the same map sums its two ends; a direct
+    WRITE of component 1 sets it per node."""
+    nodes, edges = op2.Set(5), op2.Set(4)
+    e2n = op2.Map(edges, nodes, 2, np.array([[0, 1], [1, 2], [2, 3], [3, 4]], dtype=np.int32))
+    v = op2.Dat(nodes ** 2, np.stack([np.arange(5.0), np.zeros(5)], axis=1), np.float64)
+    inc = op2.Kernel("static void inc2(double x[2]) { x[0] += 1.0; x[1] += 1.0; }", "inc2")
+    op2.par_loop(inc, edges, op2.DatView(v, 1)(op2.INC, e2n))
+    assert list(np.asarray(v.data)[:, 1]) == [1.0, 2.0, 2.0, 2.0, 1.0]
+    assert list(np.asarray(v.data)[:, 0]) == [0.0, 1.0, 2.0, 3.0, 4.0]
+    s = op2.Dat(edges ** 1, np.zeros(4), np.float64)
+    add = op2.Kernel("static void ends(double *s, const double x[2]) { s[0] = x[0] + x[1]; }", "ends")
+    op2.par_loop(add, edges, s(op2.WRITE), op2.DatView(v, 0)(op2.READ, e2n))
+    assert list(np.asarray(s.data)) == [1.0, 3.0, 5.0, 7.0]
+    put = op2.Kernel("static void put(double *x) { x[0] = 9.0; }", "put")
+    op2.par_loop(put, nodes, op2.DatView(v, 1)(op2.WRITE))
+    assert list(np.asarray(v.data)[:, 1]) == [9.0] * 5 and list(np.asarray(v.data)[:, 0]) == [0.0, 1.0, 2.0, 3.0, 4.0]
+
+
 def case_region_on_plain_set(op2):
     """A region other than interior facets on a set that is not extruded
     changes nothing (builder.py:771-795 only shape the layer loop): every
@@ -378,6 +400,7 @@ OTHER_CASES = [
     ("sparsity_nnz", case_sparsity_nnz),
     ("empty_iterset", case_empty_iterset),
     ("region_on_plain_set", case_region_on_plain_set),
+    ("dat_view", case_dat_view),
 ]
 
 

@@ -272,6 +272,22 @@ class Dat:
         return ("dat", self, access, path)
 
 
+class DatView:
+    """Component ``index`` of ``dat``: a par_loop passes only that value."""
+
+    def __init__(self, dat, index):
+        self.dat = dat
+        self.index = tuple(index) if isinstance(index, (tuple, list)) else (int(index),)
+        self.comp = int(np.ravel_multi_index(self.index, dat.dataset.dim))
+
+    @property
+    def data(self):
+        return self.dat._a.reshape(len(self.dat._a), -1)[:self.dat.dataset.set.size, self.comp]
+
+    def __call__(self, access, path=None):
+        return ("view", self, access, path)
+
+
 class Global:
     def __init__(self, dim, data=None, dtype=None, name=None, comm=None):
         self.dim = (dim,) if isinstance(dim, int) else tuple(dim)
@@ -412,6 +428,9 @@ def par_loop(kernel, iterset, *args, pass_layer_arg=False, iteration_region=None
     for kind, obj, access, path in args:
         if kind == "dat":
             sargs.append(seqwrap.DatArg(obj._a, access, path.table if path is not None else None))
+        elif kind == "view":
+            sargs.append(seqwrap.DatArg(obj.dat._a, access, path.table if path is not None else None,
+                                        comp=obj.comp))
         elif kind == "mdat":
             sargs.append(seqwrap.MixedDatArg([seqwrap.DatArg(d._a, access, m.table) for d, m in zip(obj, path)],
                                              access))

@@ -292,3 +292,83 @@ def test_large_loops_take_the_64bit_path():
     huge_nodes = op2.Set(800_000_000)
     m = op2.Map(small, huge_nodes, 4)
     assert not fits_32bit_indexing(SimpleNamespace(arguments=[SimpleNamespace(maps=(m,))]))
+
+
+@pytest.fixture
+def vdat():
+    return op2.Dat(op2.Set(5) ** 2, np.zeros((5, 2)), np.float64, "vdat")
+
+
+def test_dat_view_assign(vdat):
+    """test_dats.py:269-282: writing a view writes the parent's component
+    and invalidates the halo."""
+    vdat.data[:, 0] = 3
+    vdat.data[:, 1] = 4
+    comp = op2.DatView(vdat, 1)
+    comp.data[:] = 7
+    assert not vdat.halo_valid and not comp.halo_valid
+    assert all(comp.data == 7) and all(vdat.data[:, 0] == 3) and all(vdat.data[:, 1] == 7)
+
+
+def test_dat_view_zero(vdat):
+    """test_dats.py:284-297: zeroing a view zeroes the component and leaves
+    the halo valid."""
+    vdat.data[:, 0] = 3
+    vdat.data[:, 1] = 4
+    comp = op2.DatView(vdat, 1)
+    comp.zero()
+    assert vdat.halo_valid and comp.halo_valid
+    assert all(comp.data == 0) and all(vdat.data[:, 0] == 3)
+
+
+def test_dat_view_halo_valid(vdat):
+    """test_dats.py:299-316: reading a view keeps the version; a writable
+    view with halos bumps the parent's version and dirties its halo."""
+    comp = op2.DatView(vdat, 1)
+    assert vdat.halo_valid and comp.halo_valid
+    assert vdat.dat_version == 0 and comp.dat_version == 0
+    comp.data_ro_with_halos
+    assert vdat.halo_valid and vdat.dat_version == 0 and comp.dat_version == 0
+    comp.data_with_halos
+    assert not vdat.halo_valid and not comp.halo_valid
+    assert vdat.dat_version == 1 and comp.dat_version == 1
+
+
+def test_dat_view_index_checks(vdat):
+    with pytest.raises(op2.IndexValueError):
+        op2.DatView(vdat, 2)
+
+
+def test_dat_version():
+    """test_dats.py:143-171: data access, zero and copy each count once."""
+    s = op2.Set(5)
+    d1 = op2.Dat(s, list(range(5)), dtype=np.float64)
+    d2 = op2.Dat(s)
+    assert d1.dat_version == 0 and d2.dat_version == 0
+    d1.data
+    assert d1.dat_version == 1 and d2.dat_version == 0
+    d2.data[:] += 1
+    assert d1.dat_version == 1 and d2.dat_version == 1
+    d1.zero()
+    assert d1.dat_version == 2 and d2.dat_version == 1
+    d2.copy(d1)
+    assert d1.dat_version == 3 and d2.dat_version == 1
+
+
+def test_mixed_dat_version():
+    """test_dats.py:206-232: a MixedDat's version is the sum of its
+    components'; zeroing [d1, d1] zeroes d1 twice."""
+    s = op2.Set(5)
+    d1 = op2.Dat(s, list(range(5)), dtype=np.float64)
+    mdat = op2.MixedDat([d1, d1])
+    d2 = op2.Dat(s)
+    mdat2 = op2.MixedDat([d1, d2])
+    assert mdat.dat_version == 0 and mdat2.dat_version == 0
+    mdat2.data
+    assert d1.dat_version == 1 and d2.dat_version == 1
+    assert mdat.dat_version == 2 and mdat2.dat_version == 2
+    mdat.zero()
+    assert d1.dat_version == 3 and d2.dat_version == 1
+    assert mdat.dat_version == 6 and mdat2.dat_version == 4
+    d1.zero()
+    assert d1.dat_version == 4 and mdat.dat_version == 8 and mdat2.dat_version == 5

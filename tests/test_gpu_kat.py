@@ -213,3 +213,19 @@ def test_arglist_matches_argtypes(device):
     ]
     for pl in loops:
         assert len(pl.global_kernel.argtypes) == 2 + len(pl.arglist)
+
+
+def test_parloop_dat_versions(device):
+    """test_dats.py:195-204 and 234-258: a par_loop writing a Dat (or each
+    component of a MixedDat) counts once per written Dat."""
+    import numpy as np
+    s = op2.Set(5)
+    d3 = op2.Dat(s ** 1, data=None, dtype=np.uint32)
+    op2.par_loop(op2.Kernel("static void write(unsigned int *v) { *v = 1; }", "write"), s, d3(op2.WRITE))
+    assert d3.dat_version == 1
+    d3, d4 = op2.Dat(s ** 1, data=None, dtype=np.uint32), op2.Dat(s ** 1, data=None, dtype=np.uint32)
+    d3d4 = op2.MixedDat([d3, d4])
+    m = op2.Map(s, s, 1, values=[0, 1, 2, 3, 4])
+    op2.par_loop(op2.Kernel("static void write2(unsigned int *v) { v[0] = 1; v[1] = 2; }", "write2"), s,
+                 d3d4(op2.WRITE, op2.MixedMap([m, m])))
+    assert d3.dat_version == 1 and d4.dat_version == 1 and d3d4.dat_version == 2
