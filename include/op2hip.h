@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define OP2HIP_ABI_VERSION 1
+#define OP2HIP_ABI_VERSION 2
 
 enum op2hip_status {
   OP2HIP_OK = 0,

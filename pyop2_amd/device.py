@@ -187,7 +187,7 @@ def lib():
                     raise DeviceError(f"op2hip library not built: {_LIB_PATH} is missing "
                                       "(run `python -m pyop2_amd.build`)")
                 L = _Bound(ctypes.CDLL(_LIB_PATH))
-                if L.op2hip_abi_version() != 1:
+                if L.op2hip_abi_version() != 2:
                     raise DeviceError("op2hip ABI version mismatch")
                 _lib = L
     return _lib

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 25
     for n in names:
         assert hasattr(L, n), n
-    assert L.op2hip_abi_version() == 1
+    assert L.op2hip_abi_version() == 2
 
 
 def test_binding_covers_every_declared_symbol():
