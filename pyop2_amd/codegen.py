@@ -612,10 +612,9 @@ class GeneratedKernel:
         extruded = getattr(self.gk, "_extruded", False)
         if extruded:
             region = getattr(self.gk, "_iteration_region", None)
-            columns_of = iterset.superset if isinstance(iterset, Subset) else iterset
             for m in maps:
                 if getattr(m, "offset", None) is not None:
-                    m.check_extruded(columns_of, region)
+                    m.check_extruded(iterset, region)
             vals.append(ctypes.c_void_p(dev.ptr(iterset.device_layers)))
         L = dev.lib()
         s = dev.stream()
@@ -634,7 +633,7 @@ class GeneratedKernel:
                 elems = iterset.indices[start:end]
             else:
                 elems = np.arange(start, end, dtype=np.int32)
-            key = (id(iterset), start, end, tuple(id(m) for m in conflict),
+            key = (iterset._plan_key(), start, end, tuple(id(m) for m in conflict),
                    int(getattr(self.gk, "_iteration_region", None) or 0))
             if extruded:
                 # a column's targets over all its layers (a thread runs the

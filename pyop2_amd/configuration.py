@@ -1,80 +1,96 @@
-"""Typed configuration dictionary (mirrors ``pyop2/configuration.py``).
+"""Run-time options of the device par_loop.
 
-Keys kept from the reference: ``type_check``, ``block_sparsity``, ``matnest``,
-``compute_kernel_flops``, ``simd_width``, ``debug``, ``cache_dir``.
-Device keys:
+The option names and their environment variables are the reference's
+interface (``pyop2/configuration.py:83-119``) for the options that mean
+something on this path: ``type_check``, ``block_sparsity``, ``matnest``,
+``compute_kernel_flops``, ``simd_width``, ``debug``, ``cache_dir``.  The
+device adds:
 
-* ``strategy`` -- INC conflict resolution: ``auto`` (autotuned per loop),
-  ``atomic``, ``coloured``, ``staged`` or ``rowgather`` (Mat only).
-* ``autotune`` -- time the candidate strategies on first use (``auto`` only).
-* ``device`` -- HIP device ordinal.
+* ``strategy`` -- how indirect INC conflicts are resolved: ``auto`` (the
+  autotuner's choice per loop), ``atomic``, ``coloured``, ``staged``,
+  ``split`` or one of the Mat row gathers (``rowgather*``);
+* ``autotune`` -- whether ``auto`` times its candidates on first use;
+* ``device`` -- the HIP device ordinal;
+* ``staged_partials`` -- staged Dat INC loops send the nodes their tiles
+  share through per-tile partial sums and a second pass (False: device
+  atomic adds).
+
+Every option is read once from its environment variable at import.  A
+value assigned later must have the option's type (``ConfigurationError``
+otherwise); ``unsafe_reconfigure`` skips that check.
 """
 import os
+from collections import namedtuple
 from tempfile import gettempdir
 
 from pyop2_amd.exceptions import ConfigurationError
 
+_Option = namedtuple("_Option", "env type default")
+
+_STRATEGIES = frozenset(("auto", "atomic", "coloured", "staged", "split", "rowgather",
+                         "rowgather_thread", "rowgather_slots", "rowgather_staged",
+                         "rowgather_cells", "rowgather_rows"))
+
+
+def _from_env(opt):
+    raw = os.environ.get(opt.env)
+    if raw is None:
+        return opt.default
+    try:
+        return bool(int(raw)) if opt.type is bool else opt.type(raw)
+    except ValueError:
+        raise ValueError("environment variable %s=%r is not a valid %s"
+                         % (opt.env, raw, opt.type.__name__))
+
 
 class Configuration(dict):
-    cache_dir = os.path.join(gettempdir(), "pyop2_amd-cache-uid%s" % os.getuid())
-    DEFAULTS = {
-        "simd_width": ("PYOP2_SIMD_WIDTH", int, 4),
-        "debug": ("PYOP2_DEBUG", bool, False),
-        "compute_kernel_flops": ("PYOP2_COMPUTE_KERNEL_FLOPS", bool, False),
-        "type_check": ("PYOP2_TYPE_CHECK", bool, True),
-        "cache_dir": ("PYOP2_CACHE_DIR", str, cache_dir),
-        "matnest": ("PYOP2_MATNEST", bool, True),
-        "block_sparsity": ("PYOP2_BLOCK_SPARSITY", bool, True),
-        "strategy": ("PYOP2_AMD_STRATEGY", str, "auto"),
-        "autotune": ("PYOP2_AMD_AUTOTUNE", bool, True),
-        "device": ("PYOP2_AMD_DEVICE", int, 0),
-        # staged Dat INC loops: nodes shared between tiles through partial
-        # sums and a second pass (False: device atomic adds)
-        "staged_partials": ("PYOP2_AMD_STAGED_PARTIALS", bool, True),
+    """The options as a dict whose assignments are type-checked."""
+
+    OPTIONS = {
+        "simd_width": _Option("PYOP2_SIMD_WIDTH", int, 4),
+        "debug": _Option("PYOP2_DEBUG", bool, False),
+        "compute_kernel_flops": _Option("PYOP2_COMPUTE_KERNEL_FLOPS", bool, False),
+        "type_check": _Option("PYOP2_TYPE_CHECK", bool, True),
+        "cache_dir": _Option("PYOP2_CACHE_DIR", str,
+                             os.path.join(gettempdir(), "pyop2_amd-cache-uid%d" % os.getuid())),
+        "matnest": _Option("PYOP2_MATNEST", bool, True),
+        "block_sparsity": _Option("PYOP2_BLOCK_SPARSITY", bool, True),
+        "strategy": _Option("PYOP2_AMD_STRATEGY", str, "auto"),
+        "autotune": _Option("PYOP2_AMD_AUTOTUNE", bool, True),
+        "device": _Option("PYOP2_AMD_DEVICE", int, 0),
+        "staged_partials": _Option("PYOP2_AMD_STAGED_PARTIALS", bool, True),
     }
 
     def __init__(self):
-        def convert(env, typ, v):
-            if not isinstance(typ, type):
-                typ = typ[0]
-            try:
-                if typ is bool:
-                    return bool(int(os.environ.get(env, v)))
-                return typ(os.environ.get(env, v))
-            except ValueError:
-                raise ValueError("Cannot convert value of environment variable %s to %r" % (env, typ))
-        defaults = dict((k, convert(env, typ, v)) for k, (env, typ, v) in Configuration.DEFAULTS.items())
-        super().__init__(**defaults)
-        self._set = set()
-        self._defaults = defaults
+        self._initial = {name: _from_env(opt) for name, opt in self.OPTIONS.items()}
+        super().__init__(self._initial)
 
-    def reset(self):
-        """Reset the configuration parameters to the default values."""
-        self.update(self._defaults)
-        self._set = set()
-
-    def reconfigure(self, **kwargs):
-        """Update the configuration parameters with new values."""
-        for k, v in kwargs.items():
-            self[k] = v
-
-    def unsafe_reconfigure(self, **kwargs):
-        """Update the configuration without type checks."""
-        self.update(kwargs)
+    @staticmethod
+    def _validated(key, value):
+        opt = Configuration.OPTIONS.get(key)
+        if opt is not None and not isinstance(value, opt.type):
+            raise ConfigurationError("option %r takes %s values, not %r"
+                                     % (key, opt.type.__name__, type(value).__name__))
+        if key == "strategy" and value not in _STRATEGIES:
+            raise ConfigurationError("unknown strategy %r (one of %s)" % (value, sorted(_STRATEGIES)))
+        return value
 
     def __setitem__(self, key, value):
-        if key in Configuration.DEFAULTS:
-            valid_type = Configuration.DEFAULTS[key][1]
-            if not isinstance(value, valid_type):
-                raise ConfigurationError("Values for configuration key %s must be of type %r, not %r"
-                                         % (key, valid_type, type(value)))
-        if key == "strategy" and value not in ("auto", "atomic", "coloured", "staged", "rowgather",
-                                               "rowgather_thread", "rowgather_slots",
-                                               "rowgather_staged", "rowgather_cells",
-                                               "rowgather_rows", "split"):
-            raise ConfigurationError(f"unknown strategy {value!r}")
-        self._set.add(key)
-        super().__setitem__(key, value)
+        super().__setitem__(key, self._validated(key, value))
+
+    def reconfigure(self, **kwargs):
+        """Assign several options, each checked."""
+        for key, value in kwargs.items():
+            self[key] = value
+
+    def unsafe_reconfigure(self, **kwargs):
+        """Assign several options without checks (restoring saved values)."""
+        dict.update(self, kwargs)
+
+    def reset(self):
+        """Back to the values read at import."""
+        dict.clear(self)
+        dict.update(self, self._initial)
 
 
 configuration = Configuration()

@@ -1,11 +1,19 @@
-"""Local (per-element) kernels (``pyop2/local_kernel.py``).
+"""Element kernels (the reference's ``pyop2/local_kernel.py`` interface).
 
-``Kernel(code, name)`` with a C string.  Loopy kernels are not supported
-(loopy is absent): they raise ``TypeError`` as any other unknown code type.
+An element kernel is a C function, given as a string, that the loop calls
+once per iteration element on packed buffers.  ``Kernel(code, name)`` builds
+one.  Loopy kernels (the reference's other form) need loopy, which is absent
+here, so any code that is not a string raises ``TypeError``.
+
+The kernel's compile options (``cpp``, ``flop_count``, ``headers``, ...) are
+kept together in one frozen ``KernelOptions`` record, and the kernel's
+identity is a digest of its text, name and options.  The argument accesses
+and dtypes are the loop's business: a legacy ``par_loop`` fills them in from
+its arguments before the loop is built, so two loops that call one kernel
+with different argument types get different cached loops.
 """
-import abc
 import hashlib
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Union
 
 import numpy as np
@@ -13,77 +21,96 @@ import numpy as np
 from pyop2_amd import __version__
 from pyop2_amd.exceptions import NameTypeError
 from pyop2_amd.types.access import Access
-from pyop2_amd.utils import cached_property, validate_type
+from pyop2_amd.utils import validate_type
+
+__all__ = ["Kernel", "LocalKernel", "CStringLocalKernel", "LocalKernelArg", "KernelOptions"]
 
 
 @dataclass(frozen=True)
 class LocalKernelArg:
+    """One argument of an element kernel: how it is accessed, and its type."""
     access: Access
     dtype: Union[np.dtype, str]
 
 
+@dataclass(frozen=True)
+class KernelOptions:
+    """Compile options of an element kernel (the reference's keywords)."""
+    cpp: bool = False
+    flop_count: int = None
+    headers: tuple = ()
+    include_dirs: tuple = ()
+    ldargs: tuple = ()
+    opts: dict = field(default_factory=dict, hash=False, compare=False)
+    requires_zeroed_output_arguments: bool = False
+    user_code: str = ""
+    events: tuple = ()
+
+    def digest_items(self):
+        return (self.cpp, self.flop_count, tuple(self.headers), tuple(self.include_dirs),
+                tuple(self.ldargs), tuple(sorted(self.opts.items())),
+                self.requires_zeroed_output_arguments, self.user_code, tuple(self.events))
+
+
 @validate_type(("name", str, NameTypeError))
 def Kernel(code, name, **kwargs):
-    """Construct a local kernel from a C string."""
-    if isinstance(code, str):
-        return CStringLocalKernel(code, name, **kwargs)
-    raise TypeError("code argument is the wrong type")
+    """An element kernel from a C string (see ``LocalKernel`` for keywords)."""
+    if not isinstance(code, str):
+        raise TypeError("code argument is the wrong type")
+    return CStringLocalKernel(code, name, **kwargs)
 
 
-class LocalKernel(abc.ABC):
-    """The kernel executed per member of the iteration set.
+class LocalKernel:
+    """An element kernel: ``code`` defines a C function called ``name``.
 
-    Keyword arguments as in the reference (``cpp``, ``flop_count``,
-    ``headers``, ``include_dirs``, ``ldargs``, ``opts``,
-    ``requires_zeroed_output_arguments``, ``user_code``, ``events``).
+    Keywords are the reference's: ``cpp``, ``flop_count``, ``headers``,
+    ``include_dirs``, ``ldargs``, ``opts``,
+    ``requires_zeroed_output_arguments``, ``user_code``, ``events``.
     """
 
     @validate_type(("name", str, NameTypeError))
-    def __init__(self, code, name, accesses=None, *, cpp=False, flop_count=None, headers=(),
-                 include_dirs=(), ldargs=(), opts=None, requires_zeroed_output_arguments=False,
-                 user_code="", events=()):
+    def __init__(self, code, name, accesses=None, **options):
         self.code = code
         self.name = name
         self.accesses = accesses
-        self.cpp = cpp
-        self.flop_count = flop_count
-        self.headers = headers
-        self.include_dirs = include_dirs
-        self.ldargs = ldargs
-        self.opts = opts or {}
-        self.requires_zeroed_output_arguments = requires_zeroed_output_arguments
-        self.user_code = user_code
-        self.events = events
+        self.options = KernelOptions(**{k: v for k, v in options.items() if v is not None or k == "flop_count"})
+        self._digest = None
+
+    def __getattr__(self, attr):
+        # the compile options read as attributes (kernel.cpp, kernel.headers, ...)
+        if attr in KernelOptions.__dataclass_fields__:
+            return getattr(self.options, attr)
+        raise AttributeError(attr)
 
     @property
-    @abc.abstractmethod
     def dtypes(self):
-        """The dtypes of the kernel arguments."""
+        raise NotImplementedError
+
+    def _text_digest(self):
+        """Digest of everything that fixes the generated code but the
+        argument signature."""
+        if self._digest is None:
+            items = (self.code, self.name, self.options.digest_items(), __version__)
+            self._digest = hashlib.sha256(repr(items).encode()).hexdigest()
+        return self._digest
 
     @property
     def cache_key(self):
-        return self._immutable_cache_key, self.accesses, self.dtypes
-
-    @cached_property
-    def _immutable_cache_key(self):
-        key = (self.code, self.name, self.cpp, self.flop_count, self.headers, self.include_dirs,
-               self.ldargs, sorted(self.opts.items()), self.requires_zeroed_output_arguments,
-               self.user_code, __version__)
-        return hashlib.md5(str(key).encode()).hexdigest()
+        return self._text_digest(), self.accesses, self.dtypes
 
     @property
     def arguments(self):
-        assert len(self.accesses) == len(self.dtypes)
-        return tuple(LocalKernelArg(acc, dtype) for acc, dtype in zip(self.accesses, self.dtypes))
+        accesses, dtypes = self.accesses, self.dtypes
+        if accesses is None or dtypes is None or len(accesses) != len(dtypes):
+            raise RuntimeError(f"kernel {self.name} has no complete argument signature yet")
+        return tuple(map(LocalKernelArg, accesses, dtypes))
 
-    @cached_property
+    @property
     def num_flops(self):
-        return self.flop_count if self.flop_count is not None else 0
+        return self.options.flop_count or 0
 
     def __eq__(self, other):
-        if not isinstance(other, LocalKernel):
-            return NotImplemented
-        return self.cache_key == other.cache_key
+        return isinstance(other, LocalKernel) and self.cache_key == other.cache_key
 
     def __hash__(self):
         return hash(self.cache_key)
@@ -96,11 +123,12 @@ class LocalKernel(abc.ABC):
 
 
 class CStringLocalKernel(LocalKernel):
-    """A :class:`LocalKernel` whose ``code`` is a string of C code."""
+    """An element kernel whose code is C source text; its argument dtypes
+    are given, or filled in by the par_loop that calls it."""
 
     @validate_type(("code", str, TypeError))
-    def __init__(self, code, name, accesses=None, dtypes=None, **kwargs):
-        super().__init__(code, name, accesses, **kwargs)
+    def __init__(self, code, name, accesses=None, dtypes=None, **options):
+        super().__init__(code, name, accesses, **options)
         self._dtypes = dtypes
 
     @property
@@ -108,5 +136,5 @@ class CStringLocalKernel(LocalKernel):
         return self._dtypes
 
     @dtypes.setter
-    def dtypes(self, dtypes):
-        self._dtypes = dtypes
+    def dtypes(self, value):
+        self._dtypes = value

@@ -1,16 +1,30 @@
-"""Parallel loops (``pyop2/parloop.py``) on one device rank.
+"""Parallel loops on one device rank (the reference's ``pyop2/parloop.py``
+interface).
 
-Same construction, checks and call sequence as the reference
-(``Parloop.__call__``, ``parloop.py:243-260``): version bump, zeroed
-temporaries for INC Globals (``:514-530``), halo hooks, ``_compute`` over
-the core part and then the owned part, Global finalisation and Mat state
-updates.  ``_compute`` hands ``(start, end, *ptrs)`` to the GlobalKernel,
-whose executor launches HIP kernels on the current torch stream.
+A loop is built from a ``GlobalKernel`` (the compiled form of an element
+kernel over one argument signature), an iteration set and one bound
+argument per kernel argument.  Running it follows the reference's order
+(``Parloop.__call__``, ``parloop.py:243-260``):
+
+1. written data gets a new version, INC Globals are swapped for zeroed
+   temporaries and Mats are put in the insertion state of their access;
+2. the core band of the iteration set is computed while the halos of the
+   indirectly read Dats would be exchanged, then the owned band;
+3. written Dats are marked dirty on the device, the temporaries are added to
+   their Globals, and the reverse halo exchange of INC/MIN/MAX Dats would
+   finish (on one rank both exchanges have nothing to move).
+
+Each band is one call ``global_kernel(comm, start, end, *pointers)``, the
+pointers being the iteration set's, each argument's data, then each map
+once, in first-use order (``parloop.py:203-212``); the GlobalKernel's
+executor launches HIP kernels on the current stream.
+
+Bound arguments are one class, ``_Bound``, configured per kind (Dat,
+MixedDat, Mat, MixedMat, Global, passthrough) by the reference's names
+(``DatParloopArg`` ...).  The legacy form -- ``dat(op2.INC, map)`` -- is one
+class, ``_Legacy``, that knows its kernel-argument descriptor and its bound
+argument, again under the reference's names (``DatLegacyArg`` ...).
 """
-import abc
-from dataclasses import dataclass
-from typing import Any, Optional, Tuple
-
 import numpy as np
 
 from pyop2_amd.configuration import configuration
@@ -19,72 +33,46 @@ from pyop2_amd.global_kernel import (DatKernelArg, GlobalKernel, GlobalKernelArg
                                      MixedDatKernelArg, MixedMatKernelArg, PassthroughKernelArg)
 from pyop2_amd.local_kernel import CStringLocalKernel, LocalKernel
 from pyop2_amd.types.access import Access
-from pyop2_amd.types.dat import AbstractDat, Dat
-from pyop2_amd.types.data_carrier import DataCarrier
-from pyop2_amd.types.glob import Constant, Global
-from pyop2_amd.types.map import Map
+from pyop2_amd.types.dat import AbstractDat
+from pyop2_amd.types.glob import Global
 from pyop2_amd.types.mat import Mat
 from pyop2_amd.types.set import ExtrudedSet, MixedSet, Set, Subset
-from pyop2_amd.utils import cached_property
+
+__all__ = ["Parloop", "LegacyParloop", "ParLoop", "par_loop", "parloop",
+           "DatParloopArg", "GlobalParloopArg", "MatParloopArg", "MixedDatParloopArg",
+           "MixedMatParloopArg", "PassthroughParloopArg",
+           "DatLegacyArg", "GlobalLegacyArg", "MatLegacyArg", "MixedDatLegacyArg",
+           "MixedMatLegacyArg", "PassthroughArg"]
+
+_INSERT_STATE = {Access.INC: Mat.ADD_VALUES, Access.WRITE: Mat.INSERT_VALUES}
+_EXCHANGED_BEFORE = frozenset((Access.READ, Access.RW, Access.INC, Access.MIN, Access.MAX))
+_EXCHANGED_AFTER = frozenset((Access.INC, Access.MIN, Access.MAX))
 
 
-class ParloopArg(abc.ABC):
+# ---------------------------------------------------------------------------
+# Bound arguments
 
-    @staticmethod
-    def check_map(m):
-        if configuration["type_check"]:
-            if m.iterset.total_size > 0 and len(m._table) == 0:
-                raise MapValueError(f"{m} is not initialized")
-
-
-@dataclass
-class GlobalParloopArg(ParloopArg):
-    data: Global
-
-    @property
-    def _kernel_args_(self):
-        return self.data._kernel_args_
-
-    @property
-    def map_kernel_args(self):
-        return ()
-
-    @property
-    def maps(self):
-        return ()
+def _require_values(m):
+    """A map without values cannot be read by a loop (``MapValueError``)."""
+    if configuration["type_check"] and m.iterset.total_size > 0 and len(m._table) == 0:
+        raise MapValueError(f"{m} is not initialized")
 
 
-@dataclass
-class DatParloopArg(ParloopArg):
-    data: Dat
-    map_: Optional[Map] = None
+class _Bound:
+    """A kernel argument's data bound to the maps it is read through.
 
-    def __post_init__(self):
-        if self.map_ is not None:
-            self.check_map(self.map_)
+    ``data`` is the object; ``maps`` the plain maps the loop reads, in the
+    order their pointers are passed (a MixedMap contributes its components,
+    a Mat its row then column maps).
+    """
 
-    @property
-    def _kernel_args_(self):
-        return self.data._kernel_args_
+    kind = None
 
-    @property
-    def map_kernel_args(self):
-        return self.map_._kernel_args_ if self.map_ else ()
-
-    @property
-    def maps(self):
-        return (self.map_,) if self.map_ is not None else ()
-
-
-@dataclass
-class MatParloopArg(ParloopArg):
-    data: Mat
-    maps: Tuple[Map, Map]
-    lgmaps: Optional[Any] = None
-
-    def __post_init__(self):
+    def __init__(self, data, maps=()):
+        self.data = data
+        self.maps = tuple(maps)
         for m in self.maps:
-            self.check_map(m)
+            _require_values(m)
 
     @property
     def _kernel_args_(self):
@@ -92,89 +80,129 @@ class MatParloopArg(ParloopArg):
 
     @property
     def map_kernel_args(self):
-        rmap, cmap = self.maps
-        return tuple(rmap._kernel_args_) + tuple(cmap._kernel_args_)
+        return tuple(p for m in self.maps for p in m._kernel_args_)
+
+    def __repr__(self):
+        return "%s(%r, maps=%r)" % (type(self).__name__, self.data, self.maps)
 
 
-@dataclass
-class MixedDatParloopArg(ParloopArg):
-    """A MixedDat through a MixedMap: component data pointers, then the
-    component maps (``parloop.py:90-117``)."""
-    data: Any
-    map_: Any
+class DatParloopArg(_Bound):
+    kind = "dat"
 
-    def __post_init__(self):
-        for m in self.map_.split:
-            self.check_map(m)
-
-    @property
-    def _kernel_args_(self):
-        return self.data._kernel_args_
-
-    @property
-    def map_kernel_args(self):
-        return self.map_._kernel_args_
-
-    @property
-    def maps(self):
-        return tuple(self.map_.split)
+    def __init__(self, data, map_=None):
+        self.map_ = map_
+        super().__init__(data, () if map_ is None else (map_,))
 
 
-@dataclass
-class MixedMatParloopArg(ParloopArg):
-    """A MixedMat through a pair of MixedMaps; ``maps`` lists the row
-    component maps, then the column component maps."""
-    data: Any
-    mixed_maps: Tuple[Any, Any]
+class MixedDatParloopArg(_Bound):
+    kind = "mixed_dat"
 
-    def __post_init__(self):
-        for m in self.maps:
-            self.check_map(m)
+    def __init__(self, data, map_):
+        self.map_ = map_
+        super().__init__(data, map_.split)
 
-    @property
-    def _kernel_args_(self):
-        return self.data._kernel_args_
 
-    @property
-    def map_kernel_args(self):
-        return sum((m._kernel_args_ for m in self.maps), ())
+class MatParloopArg(_Bound):
+    kind = "mat"
 
-    @property
-    def maps(self):
+    def __init__(self, data, maps, lgmaps=None):
+        self.lgmaps = lgmaps
+        super().__init__(data, maps)
+
+
+class MixedMatParloopArg(_Bound):
+    kind = "mixed_mat"
+
+    def __init__(self, data, mixed_maps):
+        self.mixed_maps = tuple(mixed_maps)
         rmap, cmap = self.mixed_maps
-        return tuple(rmap.split) + tuple(cmap.split)
+        super().__init__(data, tuple(rmap.split) + tuple(cmap.split))
 
 
-@dataclass
-class PassthroughParloopArg(ParloopArg):
-    data: int
+class GlobalParloopArg(_Bound):
+    kind = "global"
+
+    def __init__(self, data):
+        super().__init__(data)
+
+
+class PassthroughParloopArg(_Bound):
+    kind = "passthrough"
+
+    def __init__(self, data):
+        super().__init__(data)
 
     @property
     def _kernel_args_(self):
         return (self.data,)
 
-    @property
-    def map_kernel_args(self):
-        return ()
 
-    @property
-    def maps(self):
-        return ()
+# ---------------------------------------------------------------------------
+# Checks made when a loop is built
 
+def _direct_set(iterset):
+    """The set a direct Dat of a loop over ``iterset`` must live on."""
+    base = iterset.superset if isinstance(iterset, Subset) else iterset
+    return base.parent if isinstance(base, ExtrudedSet) else base
+
+
+def _check_iteration(iterset, global_knl, arguments):
+    """The iteration set is a plain set, and every direct Dat and every map
+    of the arguments lives on it (``MapValueError`` otherwise)."""
+    if not configuration["type_check"]:
+        return
+    if not isinstance(iterset, Set):
+        raise SetTypeError("Iteration set is of the wrong type")
+    if isinstance(iterset, MixedSet):
+        raise SetTypeError("Cannot iterate over mixed sets")
+    base = iterset.superset if isinstance(iterset, Subset) else iterset
+    for i, (garg, arg) in enumerate(zip(global_knl.arguments, arguments)):
+        if isinstance(garg, DatKernelArg) and garg.is_direct and arg.data.dataset.set != _direct_set(iterset):
+            raise MapValueError(f"argument {i} is a direct Dat on another set than the loop's")
+        for j, m in enumerate(arg.maps):
+            if m.iterset != base and m.iterset not in base:
+                raise MapValueError(f"map {j} of argument {i} does not start from the loop's set")
+
+
+def _check_frozen_halos(local_knl, arguments):
+    """A Dat whose halo is frozen keeps the access it was frozen with."""
+    for larg, arg in zip(local_knl.arguments, arguments):
+        if isinstance(arg.data, AbstractDat):
+            for d in arg.data:
+                if d._halo_frozen and d._frozen_access_mode != larg.access:
+                    raise RuntimeError("Dats with frozen halos must always be accessed with the same access mode")
+
+
+# ---------------------------------------------------------------------------
+# The loop
 
 class Parloop:
-    """A parallel loop invocation of ``global_knl`` over ``iterset``."""
+    """``global_knl`` over ``iterset`` with bound ``arguments``."""
 
     def __init__(self, global_knl, iterset, arguments):
         if len(global_knl.arguments) != len(arguments):
-            raise ValueError("You are trying to pass in a different number of "
-                             "arguments than the kernel is expecting")
-        self.check_iterset(iterset, global_knl, arguments)
-        self._check_frozen_access_modes(global_knl.local_kernel, arguments)
+            raise ValueError(f"the kernel takes {len(global_knl.arguments)} arguments, "
+                             f"{len(arguments)} were given")
+        _check_iteration(iterset, global_knl, arguments)
+        _check_frozen_halos(global_knl.local_kernel, arguments)
         self.global_kernel = global_knl
         self.iterset = iterset
         self.comm = iterset.comm
-        self.arguments, self.reduced_globals = self.prepare_reduced_globals(arguments, global_knl)
+        self.arguments = list(arguments)
+        # INC Globals accumulate into a zeroed temporary of the kernel's dtype
+        # per call; the temporary is then added to the Global
+        self.reduced_globals = {}
+        for k, (larg, garg, arg) in enumerate(self._triples()):
+            if isinstance(garg, GlobalKernelArg) and larg.access == Access.INC:
+                tmp = Global(garg.dim, data=np.zeros(arg.data.dim, dtype=larg.dtype), dtype=larg.dtype,
+                             comm=self.comm)
+                self.reduced_globals[tmp] = arg
+                self.arguments[k] = GlobalParloopArg(tmp)
+
+    def _triples(self):
+        """(local kernel argument, kernel argument descriptor, bound argument)."""
+        return zip(self.global_kernel.local_kernel.arguments, self.global_kernel.arguments,
+                   self.arguments)
 
     @property
     def local_kernel(self):
@@ -185,200 +213,126 @@ class Parloop:
         return self.local_kernel.accesses
 
     @property
-    def arglist(self):
-        """Pointers in the reference's order (``parloop.py:203-212``): the
-        iteration set's (an extruded set's layers, a Subset's indices), the
-        data of each argument, then the maps' pointers without repeats, in
-        first-use order."""
-        arglist = self.iterset._kernel_args_
-        for d in self.arguments:
-            arglist += d._kernel_args_
-        maps = {p: None for d in self.arguments for p in d.map_kernel_args}
-        return arglist + tuple(maps)
+    def zipped_arguments(self):
+        return tuple(self._triples())
 
     @property
-    def zipped_arguments(self):
-        return self.zip_arguments(self.global_kernel, self.arguments)
+    def arglist(self):
+        """The executor's pointers: the iteration set's (an extruded set's
+        layers, a Subset's indices), each argument's data, then each map's
+        pointer once, in first-use order."""
+        data = [p for arg in self.arguments for p in arg._kernel_args_]
+        maps = dict.fromkeys(p for arg in self.arguments for p in arg.map_kernel_args)
+        return tuple(self.iterset._kernel_args_) + tuple(data) + tuple(maps)
+
+    @property
+    def num_flops(self):
+        return self.global_kernel.num_flops(self.iterset)
 
     def replace_data(self, index, new_argument):
         self.arguments[index].data = new_argument
 
+    # -- running --------------------------------------------------------------------
     def _compute(self, part):
-        """Execute the kernel over ``[part.offset, part.offset + part.size)``."""
-        if part.size == 0:
-            return
-        self.global_kernel(self.comm, part.offset, part.offset + part.size, *self.arglist,
-                           parloop=self)
-
-    @cached_property
-    def num_flops(self):
-        return self.global_kernel.num_flops(self.iterset)
+        """One band ``[part.offset, part.offset + part.size)``."""
+        if part.size:
+            self.global_kernel(self.comm, part.offset, part.offset + part.size, *self.arglist,
+                               parloop=self)
 
     def compute(self):
         self()
 
     def __call__(self):
-        """Execute the kernel over all members of the iteration space."""
-        self.increment_dat_version()
-        self.zero_global_increments()
-        self.replace_lgmaps()
-        self.global_to_local_begin()
+        self._prepare_outputs()
+        self._exchange("global_to_local_begin", _EXCHANGED_BEFORE)
         self._compute(self.iterset.core_part)
-        self.global_to_local_end()
+        self._exchange("global_to_local_end", _EXCHANGED_BEFORE)
         self._compute(self.iterset.owned_part)
-        self.local_to_global_begin()
-        self.update_arg_data_state()
-        self.finalize_global_increments()
-        self.local_to_global_end()
+        self._exchange("local_to_global_begin", _EXCHANGED_AFTER)
+        self._mark_outputs()
+        self._add_reductions()
+        self._exchange("local_to_global_end", _EXCHANGED_AFTER)
 
-    def increment_dat_version(self):
-        for lk_arg, gk_arg, pl_arg in self.zipped_arguments:
-            if isinstance(pl_arg, PassthroughParloopArg):
+    def _written(self):
+        return [(garg, arg, larg.access) for larg, garg, arg in self._triples()
+                if larg.access != Access.READ and arg.kind != "passthrough"]
+
+    def _prepare_outputs(self):
+        """New versions for written data (a temporary's Global is the one
+        written), zeroed reduction temporaries, Mat insertion states."""
+        for garg, arg, access in self._written():
+            target = self.reduced_globals.get(arg.data, arg).data
+            target.increment_dat_version()
+        for tmp in self.reduced_globals:
+            tmp._device_array().zero_()
+            tmp._device_modified()
+        for garg, arg, access in self._written():
+            if arg.kind in ("mat", "mixed_mat"):
+                arg.data.change_assembly_state(_INSERT_STATE[access])
+
+    def _exchange(self, step, modes):
+        """Calls ``step`` (a halo exchange half) once per distinct indirect Dat
+        argument accessed with one of ``modes``."""
+        done = []
+        for larg, garg, arg in self._triples():
+            if not isinstance(garg, (DatKernelArg, MixedDatKernelArg)) or not garg.is_indirect:
                 continue
-            if lk_arg.access is not Access.READ:
-                if pl_arg.data in self.reduced_globals:
-                    self.reduced_globals[pl_arg.data].data.increment_dat_version()
-                else:
-                    pl_arg.data.increment_dat_version()
+            if larg.access in modes and not any(arg.data is d for d in done):
+                done.append(arg.data)
+                getattr(arg.data, step)(larg.access)
 
-    def zero_global_increments(self):
-        for g in self.reduced_globals.keys():
-            g._device_array().zero_()
-            g._device_modified()
+    def _mark_outputs(self):
+        for garg, arg, access in self._written():
+            if arg.kind in ("dat", "mixed_dat"):
+                arg.data.halo_valid = False
+                arg.data._device_modified()
+            elif arg.kind == "global":
+                arg.data._device_modified()
+            elif arg.kind == "mat":
+                arg.data.assembly_state = _INSERT_STATE[access]
+            elif arg.kind == "mixed_mat":
+                arg.data.change_assembly_state(_INSERT_STATE[access])
 
-    def replace_lgmaps(self):
-        for lk_arg, gk_arg, pl_arg in self.zipped_arguments:
-            if isinstance(gk_arg, (MatKernelArg, MixedMatKernelArg)):
-                new_state = {Access.INC: Mat.ADD_VALUES, Access.WRITE: Mat.INSERT_VALUES}[lk_arg.access]
-                pl_arg.data.change_assembly_state(new_state)
-
-    def _indirect_dat_idxs(self, modes):
-        seen = set()
-        out = []
-        for i, (lk, gk, pl) in enumerate(self.zipped_arguments):
-            if isinstance(gk, (DatKernelArg, MixedDatKernelArg)) and gk.is_indirect and pl.data not in seen \
-                    and lk.access in modes:
-                out.append(i)
-                seen.add(pl.data)
-        return out
-
-    def global_to_local_begin(self):
-        for i in self._indirect_dat_idxs({Access.READ, Access.RW, Access.INC, Access.MIN, Access.MAX}):
-            self.arguments[i].data.global_to_local_begin(self.accesses[i])
-
-    def global_to_local_end(self):
-        for i in self._indirect_dat_idxs({Access.READ, Access.RW, Access.INC, Access.MIN, Access.MAX}):
-            self.arguments[i].data.global_to_local_end(self.accesses[i])
-
-    def local_to_global_begin(self):
-        for i in self._indirect_dat_idxs({Access.INC, Access.MIN, Access.MAX}):
-            self.arguments[i].data.local_to_global_begin(self.accesses[i])
-
-    def local_to_global_end(self):
-        for i in self._indirect_dat_idxs({Access.INC, Access.MIN, Access.MAX}):
-            self.arguments[i].data.local_to_global_end(self.accesses[i])
-
-    def finalize_global_increments(self):
-        """Add the loop's increments into the user's Globals (single rank:
-        the reference's allreduce of ``parloop.py:409-452`` is the identity)."""
-        torch = None
-        for tmp, glob in self.reduced_globals.items():
-            g = glob.data
-            dst, src = g._device_array(), tmp._device_array()
-            if dst.dtype != src.dtype:
-                src = src.to(dst.dtype)
-            if not dst.dtype.is_floating_point and not dst.dtype.is_signed:
-                # torch has no unsigned add: two's-complement add of the
-                # same-width signed view gives the same (wrapping) bits
-                from pyop2_amd import device as dev
-                torch = torch or dev.torch()
-                signed = {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[dst.element_size()]
-                dst.view(signed).add_(src.view(signed))
-            else:
+    def _add_reductions(self):
+        """Each temporary added into its Global (one rank: the reference's
+        allreduce, ``parloop.py:409-452``, is the identity)."""
+        for tmp, arg in self.reduced_globals.items():
+            glob = arg.data
+            dst, src = glob._device_array(), tmp._device_array()
+            src = src.to(dst.dtype) if src.dtype != dst.dtype else src
+            if dst.dtype.is_floating_point or dst.dtype.is_signed:
                 dst.add_(src)
-            g._device_modified()
-
-    def update_arg_data_state(self):
-        for i, (wrapper_arg, d) in enumerate(zip(self.global_kernel.arguments, self.arguments)):
-            access = self.accesses[i]
-            if access is Access.READ:
-                continue
-            if isinstance(wrapper_arg, (DatKernelArg, MixedDatKernelArg)):
-                d.data.halo_valid = False
-                d.data._device_modified()
-            elif isinstance(wrapper_arg, GlobalKernelArg):
-                d.data._device_modified()
-            elif isinstance(wrapper_arg, MatKernelArg):
-                d.data.assembly_state = {Access.WRITE: Mat.INSERT_VALUES, Access.INC: Mat.ADD_VALUES}[access]
-            elif isinstance(wrapper_arg, MixedMatKernelArg):
-                d.data.change_assembly_state({Access.WRITE: Mat.INSERT_VALUES, Access.INC: Mat.ADD_VALUES}[access])
-
-    @classmethod
-    def check_iterset(cls, iterset, global_knl, arguments):
-        if not configuration["type_check"]:
-            return
-        if not isinstance(iterset, Set):
-            raise SetTypeError("Iteration set is of the wrong type")
-        if isinstance(iterset, MixedSet):
-            raise SetTypeError("Cannot iterate over mixed sets")
-        if isinstance(iterset, Subset):
-            iterset = iterset.superset
-        for i, (lk_arg, gk_arg, pl_arg) in enumerate(cls.zip_arguments(global_knl, arguments)):
-            if isinstance(gk_arg, DatKernelArg) and gk_arg.is_direct:
-                _iterset = iterset.parent if isinstance(iterset, ExtrudedSet) else iterset
-                if pl_arg.data.dataset.set != _iterset:
-                    raise MapValueError(f"Iterset of direct arg {i} does not match parloop iterset")
-            for j, m in enumerate(pl_arg.maps):
-                if m.iterset != iterset and m.iterset not in iterset:
-                    raise MapValueError(f"Iterset of arg {i} map {j} does not match parloop iterset")
-
-    @classmethod
-    def _check_frozen_access_modes(cls, local_knl, arguments):
-        for lknl_arg, pl_arg in zip(local_knl.arguments, arguments):
-            if isinstance(pl_arg.data, AbstractDat):
-                if any(d._halo_frozen and d._frozen_access_mode != lknl_arg.access for d in pl_arg.data):
-                    raise RuntimeError("Dats with frozen halos must always be accessed with the same access mode")
-
-    def prepare_reduced_globals(self, arguments, global_knl):
-        """Swap INC Globals for zeroed temporaries (``parloop.py:514-530``)."""
-        arguments = list(arguments)
-        reduced_globals = {}
-        for i, (lk_arg, gk_arg, pl_arg) in enumerate(self.zip_arguments(global_knl, arguments)):
-            if isinstance(gk_arg, GlobalKernelArg) and lk_arg.access == Access.INC:
-                tmp = Global(gk_arg.dim, data=np.zeros(pl_arg.data.dim, dtype=lk_arg.dtype),
-                             dtype=lk_arg.dtype, comm=self.comm)
-                reduced_globals[tmp] = pl_arg
-                arguments[i] = GlobalParloopArg(tmp)
-        return arguments, reduced_globals
-
-    @staticmethod
-    def zip_arguments(global_knl, arguments):
-        return tuple(zip(global_knl.local_kernel.arguments, global_knl.arguments, arguments))
+            else:
+                # torch cannot add unsigned integers: the signed view of the
+                # same width adds to the same wrapped bits
+                dst.view(_signed_of(dst)).add_(src.view(_signed_of(src)))
+            glob._device_modified()
 
 
-class LegacyArg(abc.ABC):
-    """Old-style par_loop input carrying data and access together."""
-
-    @property
-    @abc.abstractmethod
-    def global_kernel_arg(self):
-        """The corresponding kernel argument descriptor."""
-
-    @property
-    @abc.abstractmethod
-    def parloop_arg(self):
-        """The corresponding ParloopArg."""
+def _signed_of(t):
+    from pyop2_amd import device as dev
+    torch = dev.torch()
+    return {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[t.element_size()]
 
 
-@dataclass
-class GlobalLegacyArg(LegacyArg):
-    data: Global
-    access: Access
+# ---------------------------------------------------------------------------
+# The legacy form: data(access, maps)
+
+class _Legacy:
+    """One ``data(access, ...)`` argument of a legacy par_loop: it gives the
+    kernel argument's descriptor (``global_kernel_arg``) and its bound
+    argument (``parloop_arg``)."""
+
+    def __init__(self, data, access):
+        self.data = data
+        self.access = access
 
     @property
     def dtype(self):
         return self.data.dtype
 
+
+class GlobalLegacyArg(_Legacy):
     @property
     def global_kernel_arg(self):
         return GlobalKernelArg(self.data.dim)
@@ -388,39 +342,29 @@ class GlobalLegacyArg(LegacyArg):
         return GlobalParloopArg(self.data)
 
 
-@dataclass
-class DatLegacyArg(LegacyArg):
-    data: Dat
-    map_: Optional[Map]
-    access: Access
-
-    @property
-    def dtype(self):
-        return self.data.dtype
+class DatLegacyArg(_Legacy):
+    def __init__(self, data, map_, access):
+        super().__init__(data, access)
+        self.map_ = map_
 
     @property
     def global_kernel_arg(self):
         from pyop2_amd.types.dat import DatView
-        map_arg = self.map_._global_kernel_arg if self.map_ is not None else None
-        index = self.data.index if isinstance(self.data, DatView) else None
-        return DatKernelArg(self.data.dataset.dim, map_arg, index=index)
+        return DatKernelArg(self.data.dataset.dim,
+                            None if self.map_ is None else self.map_._global_kernel_arg,
+                            index=self.data.index if isinstance(self.data, DatView) else None)
 
     @property
     def parloop_arg(self):
         return DatParloopArg(self.data, self.map_)
 
 
-@dataclass
-class MatLegacyArg(LegacyArg):
-    data: Mat
-    maps: Tuple[Map, Map]
-    access: Access
-    lgmaps: Optional[Tuple[Any, Any]] = None
-    needs_unrolling: Optional[bool] = False
-
-    @property
-    def dtype(self):
-        return self.data.dtype
+class MatLegacyArg(_Legacy):
+    def __init__(self, data, maps, access, lgmaps=None, needs_unrolling=False):
+        super().__init__(data, access)
+        self.maps = tuple(maps)
+        self.lgmaps = lgmaps
+        self.needs_unrolling = needs_unrolling
 
     @property
     def global_kernel_arg(self):
@@ -432,15 +376,10 @@ class MatLegacyArg(LegacyArg):
         return MatParloopArg(self.data, self.maps, self.lgmaps)
 
 
-@dataclass
-class MixedDatLegacyArg(LegacyArg):
-    data: Any
-    map_: Any
-    access: Access
-
-    @property
-    def dtype(self):
-        return self.data.dtype
+class MixedDatLegacyArg(_Legacy):
+    def __init__(self, data, map_, access):
+        super().__init__(data, access)
+        self.map_ = map_
 
     @property
     def global_kernel_arg(self):
@@ -452,34 +391,33 @@ class MixedDatLegacyArg(LegacyArg):
         return MixedDatParloopArg(self.data, self.map_)
 
 
-@dataclass
-class MixedMatLegacyArg(LegacyArg):
-    data: Any
-    maps: Tuple[Any, Any]
-    access: Access
-
-    @property
-    def dtype(self):
-        return self.data.dtype
+class MixedMatLegacyArg(_Legacy):
+    def __init__(self, data, maps, access):
+        super().__init__(data, access)
+        self.maps = tuple(maps)
 
     @property
     def global_kernel_arg(self):
-        rmap, cmap = self.maps
-        nr, nc = len(rmap), len(cmap)
-        blocks = tuple(MatKernelArg(self.data[i, j].dims, (rmap[i]._global_kernel_arg, cmap[j]._global_kernel_arg))
-                       for i in range(nr) for j in range(nc))
-        return MixedMatKernelArg(blocks, (nr, nc))
+        rows, cols = self.maps
+        blocks = tuple(MatKernelArg(self.data[i, j].dims, (r._global_kernel_arg, c._global_kernel_arg))
+                       for i, r in enumerate(rows) for j, c in enumerate(cols))
+        return MixedMatKernelArg(blocks, (len(rows), len(cols)))
 
     @property
     def parloop_arg(self):
         return MixedMatParloopArg(self.data, self.maps)
 
 
-@dataclass
-class PassthroughArg(LegacyArg):
-    access = Access.RW
-    dtype: Any
-    data: int
+class PassthroughArg(_Legacy):
+    """A raw value handed to the kernel unchanged (accessed RW)."""
+
+    def __init__(self, dtype, data):
+        super().__init__(data, Access.RW)
+        self._dtype = dtype
+
+    @property
+    def dtype(self):
+        return self._dtype
 
     @property
     def global_kernel_arg(self):
@@ -493,49 +431,48 @@ class PassthroughArg(LegacyArg):
 _GLOBAL_KERNELS = {}
 
 
-def _cached_global_kernel(local_knl, global_knl_args, subset, kwargs):
-    """GlobalKernels (and their compiled executors) are reused for equal
-    cache keys, as the reference reuses compiled wrappers
-    (``pyop2/compilation.py:427-489``, memory cache keyed by ``cache_key``)."""
-    gk = GlobalKernel(local_knl, global_knl_args, subset=subset, **kwargs)
+def _shared_global_kernel(local_knl, kernel_args, **kwargs):
+    """One GlobalKernel (and so one compiled executor) per cache key, as the
+    reference reuses its compiled wrappers (``pyop2/compilation.py:427-489``)."""
+    gk = GlobalKernel(local_knl, kernel_args, **kwargs)
     try:
         return _GLOBAL_KERNELS.setdefault(gk.cache_key, gk)
-    except TypeError:   # unhashable key component: no caching
+    except TypeError:   # a key part that cannot be hashed: not shared
         return gk
 
 
-def ParLoop(*args, **kwargs):
-    return LegacyParloop(*args, **kwargs)
-
-
 def LegacyParloop(local_knl, iterset, *args, **kwargs):
-    """Create a :class:`Parloop` from :class:`LegacyArg` inputs."""
-    if not all(isinstance(a, LegacyArg) for a in args):
+    """A Parloop from legacy ``data(access, ...)`` arguments: the element
+    kernel's accesses and dtypes are taken from them."""
+    if not all(isinstance(a, _Legacy) for a in args):
         raise ValueError("LegacyParloop only expects LegacyArg arguments")
     if not isinstance(iterset, Set):
         raise SetTypeError("Iteration set is of the wrong type")
     local_knl.accesses = tuple(a.access for a in args)
     if isinstance(local_knl, CStringLocalKernel):
         local_knl.dtypes = tuple(a.dtype for a in args)
-    global_knl_args = tuple(a.global_kernel_arg for a in args)
-    subset = isinstance(iterset, Subset)
-    extruded = getattr(iterset, "_extruded", False)
-    kwargs = dict(kwargs, extruded=extruded, constant_layers=extruded and iterset.constant_layers,
+    extruded = bool(getattr(iterset, "_extruded", False))
+    kwargs.update(subset=isinstance(iterset, Subset), extruded=extruded,
+                  constant_layers=extruded and bool(iterset.constant_layers),
                   extruded_periodic=bool(getattr(iterset, "_extruded_periodic", False)))
-    global_knl = _cached_global_kernel(local_knl, global_knl_args, subset, kwargs)
-    parloop_args = tuple(a.parloop_arg for a in args)
-    return Parloop(global_knl, iterset, parloop_args)
+    global_knl = _shared_global_kernel(local_knl, tuple(a.global_kernel_arg for a in args), **kwargs)
+    return Parloop(global_knl, iterset, [a.parloop_arg for a in args])
+
+
+ParLoop = LegacyParloop
+
+
+def parloop(knl, *args, **kwargs):
+    """Build and run a loop: a GlobalKernel with bound arguments, or an
+    element kernel with legacy arguments."""
+    if isinstance(knl, GlobalKernel):
+        loop = Parloop(knl, *args, **kwargs)
+    elif isinstance(knl, LocalKernel):
+        loop = LegacyParloop(knl, *args, **kwargs)
+    else:
+        raise KernelTypeError
+    loop()
 
 
 def par_loop(*args, **kwargs):
     parloop(*args, **kwargs)
-
-
-def parloop(knl, *args, **kwargs):
-    """Construct and execute a :class:`Parloop`."""
-    if isinstance(knl, GlobalKernel):
-        Parloop(knl, *args, **kwargs)()
-    elif isinstance(knl, LocalKernel):
-        LegacyParloop(knl, *args, **kwargs)()
-    else:
-        raise KernelTypeError

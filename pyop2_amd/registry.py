@@ -299,7 +299,7 @@ class HandWrittenKernel:
             else:
                 elems = np.arange(start, end, dtype=np.int32)
             owner = self._inc_maps(parloop)[0]
-            key = (id(iterset), start, end, tuple(id(m) for m in self._inc_maps(parloop)))
+            key = (iterset._plan_key(), start, end, tuple(id(m) for m in self._inc_maps(parloop)))
             plan = plans.colour_plan(owner, elems,
                                      [(m, id(m.toset)) for m in self._inc_maps(parloop)], key)
             plan.exec_fields(ex)

@@ -1,4 +1,19 @@
-"""Access descriptors (``pyop2/types/access.py:4-37``)."""
+"""Access descriptors of par_loop arguments.
+
+The six modes and their integer values are the reference's interface
+(``pyop2/types/access.py:4-37``), kept so that user code and the argument
+signatures of cached loops are unchanged.  What each mode means for the
+device loops:
+
+* ``READ``  -- the kernel only gathers the values;
+* ``WRITE`` -- the kernel's packed values replace the target entries (an
+  indirect WRITE whose targets collide is coloured, so the last element in
+  loop order wins as in a sequential loop);
+* ``RW``    -- gathered, then written back (coloured when indirect);
+* ``INC``   -- the kernel's packed values start at zero and are added to the
+  target (Dat, Global or Mat); the device resolves the conflicts;
+* ``MIN`` / ``MAX`` -- reductions into a Global over all elements.
+"""
 import enum
 
 
@@ -12,27 +27,8 @@ class Access(enum.IntEnum):
 
 
 READ = Access.READ
-"""The :class:`Global`, :class:`Dat`, or :class:`Mat` is accessed read-only."""
-
 WRITE = Access.WRITE
-"""The :class:`Global`, :class:`Dat`, or :class:`Mat` is accessed write-only,
-and OP2 is not required to handle write conflicts."""
-
 RW = Access.RW
-"""The :class:`Global` or :class:`Dat` is accessed for reading and writing,
-and OP2 is not required to handle write conflicts."""
-
 INC = Access.INC
-"""The kernel computes increments to be summed onto a :class:`Global`,
-:class:`Dat`, or :class:`Mat`. OP2 is responsible for managing the write
-conflicts caused."""
-
 MIN = Access.MIN
-"""The kernel contributes to a reduction into a :class:`Global` using a ``min``
-operation. OP2 is responsible for reducing over the different kernel
-invocations."""
-
 MAX = Access.MAX
-"""The kernel contributes to a reduction into a :class:`Global` using a ``max``
-operation. OP2 is responsible for reducing over the different kernel
-invocations."""

@@ -17,6 +17,10 @@ from pyop2_amd.types.dataset import DataSet
 from pyop2_amd.types.set import Set
 
 
+_READING = frozenset((Access.READ, Access.RW))
+_REDUCING = frozenset((Access.INC, Access.MIN, Access.MAX))
+
+
 class AbstractDat(DataCarrier, DeviceStorage):
 
     _modes = [Access.READ, Access.WRITE, Access.RW, Access.INC, Access.MIN, Access.MAX]
@@ -132,56 +136,63 @@ class AbstractDat(DataCarrier, DeviceStorage):
         return v
 
     def save(self, filename):
-        """Write the data array to file ``filename`` in NumPy format."""
-        np.save(filename, self.data_ro)
+        """The owned values as a ``.npy`` file (numpy's format, no pickles)."""
+        np.save(filename, np.asarray(self.data_ro), allow_pickle=False)
 
     def load(self, filename):
-        """Read values written by :meth:`save`."""
-        if filename[-4:] != ".npy":
-            filename = filename + ".npy"
-        self.data[:] = np.load(filename)
+        """Owned values from a ``.npy`` file written by ``save`` (the suffix
+        may be left out)."""
+        path = filename if filename.endswith(".npy") else filename + ".npy"
+        self.data[:] = np.load(path, allow_pickle=False)
 
     @property
     def nbytes(self):
         return self.dtype.itemsize * self.dataset.total_size * self.dataset.cdim
 
-    def zero(self, subset=None):
-        """Zero the data (on the device when that copy is current)."""
-        self.increment_dat_version()
-        if subset is None:
-            if self._where in (2, 3):   # device copy current: zero it there
-                self._device_array().zero_()
-                self._device_modified()
-            else:
-                self._host_array()[...] = 0
-                self._host_modified()
-            self.halo_valid = True
-        elif subset.superset != self.dataset.set:
+    def _owned_rows_of(self, subset):
+        """The owned rows a Subset of this Dat's set selects."""
+        if subset.superset != self.dataset.set:
             raise ex.MapValueError("The subset and dataset are incompatible")
+        return subset.owned_indices
+
+    def _device_current(self):
+        return self._where in (2, 3)
+
+    def zero(self, subset=None):
+        """Zero every value, halo included (the halo is then valid), on the
+        side that holds the current copy; or the owned rows of a Subset."""
+        self.increment_dat_version()
+        if subset is not None:
+            self.data[self._owned_rows_of(subset)] = 0
+            return
+        if self._device_current():
+            self._device_array().zero_()
+            self._device_modified()
         else:
-            self.data[subset.owned_indices] = 0
+            self._host_array().fill(0)
+            self._host_modified()
+        self.halo_valid = True
 
     def copy(self, other, subset=None):
-        """Copy the data in this Dat into another."""
+        """Copies the values into the Dat ``other``: all of them (halo
+        included when this Dat's halo is valid), or the owned rows of a
+        Subset."""
         if other is self:
             return
-        if subset is None:
-            if self._where in (2, 3) and self.halo_valid:
-                src = self._device_array()
-                dst = other._device_array()
-                dst.copy_(src)
-                other._device_modified()
-                other.halo_valid = True
-            elif self.halo_valid:
-                other._host_array()[...] = self._host_array()
-                other._host_modified()
-                other.halo_valid = True
-            else:
-                other.data[:] = self.data_ro
-        elif subset.superset != self.dataset.set:
-            raise ex.MapValueError("The subset and dataset are incompatible")
+        if subset is not None:
+            rows = self._owned_rows_of(subset)
+            other.data[rows] = self.data_ro[rows]
+            return
+        if not self.halo_valid:
+            other.data[:] = self.data_ro
+            return
+        if self._device_current():
+            other._device_array().copy_(self._device_array())
+            other._device_modified()
         else:
-            other.data[subset.owned_indices] = self.data_ro[subset.owned_indices]
+            np.copyto(other._host_array(), self._host_array())
+            other._host_modified()
+        other.halo_valid = True
 
     def __iter__(self):
         yield self
@@ -280,43 +291,54 @@ class AbstractDat(DataCarrier, DeviceStorage):
         """Euclidean norm of the owned values."""
         return self.inner(self) ** 0.5
 
-    # -- halo exchange hooks (single rank: only the INC halo reset) ---------------
-    def global_to_local_begin(self, access_mode):
+    # -- halo exchanges -------------------------------------------------------------
+    # The DataSet's halo object (the reference's Halo interface, with the
+    # forward and reverse exchange halves) moves the values; on one rank
+    # there is none and every hook returns at once.  Before a read of stale
+    # halo values the forward exchange refreshes them.  Before an INC, MIN or
+    # MAX loop the halo entries are reset to the operation's identity, so
+    # that after the loop they hold only what it contributed, which the
+    # reverse exchange sends to the owners.
+    def _live_halo(self):
         halo = self.dataset.halo
-        if halo is None or self._halo_frozen:
+        return None if (halo is None or self._halo_frozen) else halo
+
+    def _reset_halo_entries(self, access_mode):
+        from pyop2_amd.datatypes import dtype_limits
+        lowest, highest = dtype_limits(self.dtype)
+        identity = {Access.INC: 0, Access.MIN: highest, Access.MAX: lowest}[access_mode]
+        self._device_array()[self.dataset.size:] = identity
+        self._device_modified()
+
+    def global_to_local_begin(self, access_mode):
+        halo = self._live_halo()
+        if halo is None:
             return
-        if not self.halo_valid and access_mode in {Access.READ, Access.RW}:
+        if access_mode in _REDUCING:
+            self._reset_halo_entries(access_mode)
+        elif access_mode in _READING and not self.halo_valid:
             halo.global_to_local_begin(self, Access.WRITE)
-        elif access_mode in {Access.INC, Access.MIN, Access.MAX}:
-            from pyop2_amd.datatypes import dtype_limits
-            min_, max_ = dtype_limits(self.dtype)
-            val = {Access.MAX: min_, Access.MIN: max_, Access.INC: 0}[access_mode]
-            t = self._device_array()
-            t[self.dataset.size:] = val
-            self._device_modified()
 
     def global_to_local_end(self, access_mode):
-        halo = self.dataset.halo
-        if halo is None or self._halo_frozen:
+        halo = self._live_halo()
+        if halo is None:
             return
-        if not self.halo_valid and access_mode in {Access.READ, Access.RW}:
+        if access_mode in _REDUCING:
+            self.halo_valid = False
+        elif access_mode in _READING and not self.halo_valid:
             halo.global_to_local_end(self, Access.WRITE)
             self.halo_valid = True
-        elif access_mode in {Access.INC, Access.MIN, Access.MAX}:
-            self.halo_valid = False
 
     def local_to_global_begin(self, insert_mode):
-        halo = self.dataset.halo
-        if halo is None or self._halo_frozen:
-            return
-        halo.local_to_global_begin(self, insert_mode)
+        halo = self._live_halo()
+        if halo is not None:
+            halo.local_to_global_begin(self, insert_mode)
 
     def local_to_global_end(self, insert_mode):
-        halo = self.dataset.halo
-        if halo is None or self._halo_frozen:
-            return
-        halo.local_to_global_end(self, insert_mode)
-        self.halo_valid = False
+        halo = self._live_halo()
+        if halo is not None:
+            halo.local_to_global_end(self, insert_mode)
+            self.halo_valid = False
 
     def frozen_halo(self, access_mode):
         return frozen_halo(self, access_mode)

@@ -113,18 +113,24 @@ class Map:
     def check_extruded(self, iterset, region=None):
         """Raise MapValueError unless every entry an extruded loop over
         ``region`` of ``iterset`` reaches through this map lies in the
-        target set (checked once per iterset and region, in column chunks)."""
+        target set.  A Subset's loop reaches only its chosen columns, so only
+        those are checked; the result is kept per iteration set contents and
+        region, and columns are checked in chunks."""
         from pyop2_amd.types import region as reg
-        key = ("extruded-checked", id(iterset), int(reg.normalise(region)))
+        from pyop2_amd.types.set import Subset
+        key = ("extruded-checked", iterset._plan_key(), int(reg.normalise(region)))
         if self._offset is None or key in self._cache:
             return
         n = self._toset.total_size
-        ncols = iterset.total_size
-        lay = iterset.column_layers(np.arange(ncols))
+        if isinstance(iterset, Subset):
+            columns_of, cols = iterset.superset, np.asarray(iterset.indices, dtype=np.int64)
+        else:
+            columns_of, cols = iterset, np.arange(iterset.total_size, dtype=np.int64)
+        lay = columns_of.column_layers(cols)
         per_col = max(1, int((lay[:, 1] - lay[:, 0]).max(initial=1)) * 2 * self._arity)
         chunk = max(1, (1 << 22) // per_col)
-        for c0 in range(0, ncols, chunk):
-            out, live = self._column_entries(iterset, np.arange(c0, min(c0 + chunk, ncols)), region)
+        for c0 in range(0, len(cols), chunk):
+            out, live = self._column_entries(columns_of, cols[c0:c0 + chunk], region)
             e = out[live]
             if e.size and (e.min() < 0 or e.max() >= n):
                 raise ex.MapValueError(f"{self} with offsets reaches entries outside [0, {n}) "

@@ -205,7 +205,7 @@ def test_kernel_cache_keys():
     k1 = op2.Kernel("void a(double *x) {}", "a")
     k2 = op2.Kernel("void a(double *x) {}", "a")
     k3 = op2.Kernel("void a(double *x) { }", "a")
-    assert k1._immutable_cache_key == k2._immutable_cache_key != k3._immutable_cache_key
+    assert k1.cache_key == k2.cache_key != k3.cache_key
     with pytest.raises(ex.NameTypeError):
         op2.Kernel("", 3)
 
