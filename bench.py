@@ -1,17 +1,22 @@
 """Benchmark of the indirect INC par_loop and CSR assembly on one MI355X.
 
-One step = one ``op2.par_loop`` INC of the 3x3 P1 Laplace element matrices
-into the CSR Mat of the 1024x1024 unit square (2,097,152 cells, 1,050,625
-vertices, randomly permuted vertex numbering), inputs resident in HBM.
-``value`` is cells/s over the timed steps (no per-launch events; the kernel
-duration of the roofline comes from as many further steps with HIP event
-pairs).  The same run also measures, with HIP events around each launch:
-the P1 lumped mass / Laplace residual on the 256^2 square, the P2
-tetrahedral elasticity residual, BSR(3) assembly and fused residual +
-assembly on the 64^3 cube (float64 and float32), and the adjoint/forward
-time ratio of the P2 residual loop.  ``cpu_baseline`` times
-the oracle's restatement of the reference's sequential wrapper, built with
-the reference's flags, on one host core.
+Headline: one step = one ``op2.par_loop`` of the fused P2 tetrahedral
+elasticity kernel over the 64^3 unit cube (1,572,864 cells, 2,146,689 P2
+nodes): INC of the 30x30 element matrices into the float64 BSR(3) Mat
+(60,859,905 blocks) and INC of the element residuals into a vector Dat,
+inputs resident in HBM.  ``value`` is cells/s over the timed steps (no
+per-launch events); the kernel duration behind ``roofline`` comes from as
+many further steps with HIP event pairs on the launch stream.  The timed
+loop's result from zero is compared with the oracle on the whole Mat and
+the whole residual, and ``cpu_baseline`` times the oracle's restatement of
+the reference's sequential wrapper (reference compiler flags, one core) on
+the whole mesh.
+
+The same run measures, as ``secondary``: the P1 Laplacian CSR assembly on
+the permuted 1024^2 square (plus a cache-flushed variant), the P1 lumped
+mass / Laplace residual on the 256^2 square, the P2 residual and assembly
+alone and the float32 variants, the adjoint/forward time ratio, the batched
+512-mesh launch against plain launches, and the streaming bandwidth.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--quick]
 """
@@ -41,7 +46,7 @@ def kernel_ms(timer, name):
 
 
 def time_loop(pl, reps, name):
-    """Average kernel duration (HIP events) of ``reps`` executions of ``pl``."""
+    """Average duration (HIP events around each launch) of ``reps`` runs."""
     from pyop2_amd import device as dev
     pl()
     dev.torch().cuda.synchronize()
@@ -50,6 +55,23 @@ def time_loop(pl, reps, name):
             pl()
     dev.torch().cuda.synchronize()
     return kernel_ms(t, name)
+
+
+def time_many(pl, n):
+    """Average duration of ``n`` back-to-back runs between one event pair
+    (for loops so short that an event pair per launch would dominate)."""
+    from pyop2_amd import device as dev
+    torch = dev.torch()
+    pl()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream(dev.device())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(n):
+        pl()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / n
 
 
 def cpu_cores():
@@ -68,27 +90,6 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor()
-
-
-def c2_bytes(m, nnz):
-    return (m.ncells * 3 * 4 + m.nverts * 2 * 8 + nnz * 8 * 2 + nnz * 4 + (m.nverts + 1) * 4)
-
-
-KERNEL_OF = {
-    "rowgather_rows": "rowgather_rowlist_kernel<{row}, 8>",
-    "rowgather_staged": "rowgather_staged_kernel<{row}, 2>",
-    "rowgather_cells": "rowgather_cells_kernel<{cells}>",
-    "rowgather_slots": "rowgather_slots_kernel<{row}, 8>",
-    "rowgather_thread": "rowgather_kernel<{row}, 8>",
-    "rowgather": "rowgather_lanes_kernel<{lanes}>",
-    "atomic": "{cell}<true>",
-    "coloured": "{cell}<false>",
-}
-
-
-def kernel_name(strategy, row, lanes, cell):
-    return KERNEL_OF.get(strategy, "{cell}").format(row=row, lanes=lanes, cell=cell,
-                                                    cells=row.replace("Row", "Cells"))
 
 
 def measured_traffic(key):
@@ -144,18 +145,196 @@ def cold_kernel_ms(pl, reps, name):
     return float(np.median(ts))
 
 
-def run(args):
-    from pyop2_amd import device as dev, elements, op2
-    from pyop2_amd.configuration import configuration
-    from pyop2_amd.meshes import unit_cube_tet_p2, unit_square_tri, workload_rng
-    torch = dev.torch()
-    d = dev.device()
-    gpu_name = torch.cuda.get_device_name(d)
-    log(f"device {d} {gpu_name}")
-    configuration["strategy"] = args.strategy
-    out = {}
+# ---------------------------------------------------------------------------
+# P2 workload: loops and algorithmic bytes
 
-    # ---- primary: P1 Laplacian CSR assembly (1024^2, permuted) ----------------------
+def p2_loops(m3, dt):
+    """op2 objects of the P2 problem in dtype ``dt`` and its three loops
+    (residual, assembly, fused) with their algorithmic bytes per launch."""
+    from pyop2_amd import elements, op2
+    isz = np.dtype(dt).itemsize
+    cells = op2.Set(m3.ncells, "cells")
+    vset = op2.Set(m3.nverts, "verts")
+    nodes = op2.Set(m3.nnodes, "nodes")
+    c2v = op2.Map(cells, vset, 4, m3.cell_vertex, "c2v")
+    c2n = op2.Map(cells, nodes, 10, m3.cell_node, "c2n")
+    X = op2.Dat(vset ** 3, m3.coords.astype(dt), dt, name="X")
+    u = op2.Dat(nodes ** 3, m3.u.astype(dt), dt, name="u")
+    R = op2.Dat(nodes ** 3, dtype=dt, name="R")
+    sp = op2.Sparsity((nodes ** 3, nodes ** 3), [(c2n, c2n, None)])
+    nnzb = sp.block_nnz
+    A = op2.Mat(sp, dt)
+    maps_b = m3.ncells * (40 + 16)
+    x_b = m3.nverts * 3 * isz
+    u_b = m3.nnodes * 3 * isz
+    r_b = 2 * u_b
+    asm_b = nnzb * 9 * isz * 2 + nnzb * 4 + (m3.nnodes + 1) * 4 + maps_b + x_b
+    loops = {
+        "residual": (op2.ParLoop(elements.p2_elasticity_residual(), cells, R(op2.INC, c2n), X(op2.READ, c2v),
+                                 u(op2.READ, c2n)), "p2_elasticity_residual", maps_b + x_b + u_b + r_b),
+        "assembly": (op2.ParLoop(elements.p2_elasticity_matrix(), cells, A(op2.INC, (c2n, c2n)),
+                                 X(op2.READ, c2v)), "p2_elasticity_matrix", asm_b),
+        "fused": (op2.ParLoop(elements.p2_elasticity_fused(), cells, A(op2.INC, (c2n, c2n)), R(op2.INC, c2n),
+                              X(op2.READ, c2v), u(op2.READ, c2n)), "p2_elasticity_fused", asm_b + u_b + r_b),
+    }
+    objs = dict(cells=cells, c2v=c2v, c2n=c2n, X=X, u=u, R=R, A=A, sp=sp, nnzb=nnzb)
+    return objs, loops
+
+
+def c3_oracle(m3, cells_sel=None, pattern=None, flavour="strict"):
+    """The oracle's residual and BSR(3) Mat of the P2 loop over the cells
+    ``cells_sel`` (all by default) on ``pattern`` (host rowptr, colidx;
+    default: the oracle's own closed-form pattern of those cells).  Returns
+    (R, Mat, seconds of the residual pass, seconds of the assembly pass,
+    Mat insertions that missed the pattern)."""
+    from oracle import seqwrap, sparsity as osp
+    from pyop2_amd import elements
+    if cells_sel is None:
+        cn, cv, n = m3.cell_node, m3.cell_vertex, m3.ncells
+    else:
+        cn = np.ascontiguousarray(m3.cell_node[cells_sel], dtype=np.int32)
+        cv = np.ascontiguousarray(m3.cell_vertex[cells_sel], dtype=np.int32)
+        n = len(cells_sel)
+    if pattern is None:
+        rp, ci = osp.pattern(m3.nnodes, m3.nnodes, [(cn, cn, n)])
+    else:
+        rp, ci = pattern
+    Ao = seqwrap.CSRMat(rp, ci, np.zeros((len(ci), 3, 3)), 3, 3, block=True)
+    Ro = np.zeros((m3.nnodes, 3))
+    rcode = elements.kernel_source("p2_elasticity_residual", lam=1.0, mu=0.5)
+    mcode = elements.kernel_source("p2_elasticity_matrix", lam=1.0, mu=0.5)
+    t0 = time.perf_counter()
+    seqwrap.run(rcode, "p2_elasticity_residual", [seqwrap.DatArg(Ro, "INC", cn), seqwrap.DatArg(m3.coords, "READ", cv),
+                                                  seqwrap.DatArg(m3.u, "READ", cn)], [(0, n)], flavour=flavour)
+    t1 = time.perf_counter()
+    miss = seqwrap.run(mcode, "p2_elasticity_matrix", [seqwrap.MatArg(Ao, "INC", cn, cn, 3, 3),
+                                                       seqwrap.DatArg(m3.coords, "READ", cv)], [(0, n)],
+                       flavour=flavour)
+    t2 = time.perf_counter()
+    return Ro, Ao, t1 - t0, t2 - t1, miss
+
+
+def normwise(a, b):
+    d = np.linalg.norm((np.asarray(a, dtype=np.float64) - b).ravel())
+    return float(d / np.linalg.norm(np.asarray(b).ravel()))
+
+
+def c3_parity(m3, pl, A, R, sample_rows=8192):
+    """One pass of the timed fused loop from zero against the oracle:
+
+    * pattern: the device pattern must hold every element block the oracle
+      inserts (no missed insertion), have sorted unique rows and exactly the
+      closed-form count (60,859,905 at 64^3) -- together the same pattern --
+      and ``sample_rows`` rows spread evenly over the mesh (so over every
+      part of the row-batch sequence) must equal the oracle's own pattern;
+    * values: the whole Mat and the whole residual, normwise."""
+    from oracle import sparsity as osp
+    from pyop2_amd import device as dev
+    A.zero()
+    R.zero()
+    pl()
+    A.assemble()
+    rp = dev.to_host(A.rowptr).astype(np.int32)
+    ci = dev.to_host(A.colidx)
+    steps = np.diff(ci.astype(np.int64))
+    inside = np.ones(len(steps), dtype=bool)
+    starts = rp[1:-1].astype(np.int64)
+    inside[starts[(starts > 0) & (starts < len(ci))] - 1] = False
+    rows_sorted = bool(np.all(steps[inside] > 0))
+    Ro, Ao, tr, ta, miss = c3_oracle(m3, pattern=(rp, ci))
+    sel = np.linspace(0, m3.nnodes - 1, min(sample_rows, m3.nnodes)).astype(np.int64)
+    cells = np.nonzero(np.isin(m3.cell_node, sel).any(axis=1))[0]
+    cn = np.ascontiguousarray(m3.cell_node[cells], dtype=np.int32)
+    srp, sci = osp.pattern(m3.nnodes, m3.nnodes, [(cn, cn, len(cells))])
+    sample_ok = all(np.array_equal(ci[rp[r]:rp[r + 1]], sci[srp[r]:srp[r + 1]]) for r in sel)
+    gv = dev.to_host(A.device_values).astype(np.float64).reshape(-1, 3, 3)[:len(ci)]
+    mat_err = normwise(gv, Ao.values)
+    del gv
+    res_err = normwise(dev.to_host(R.device_data_ro).reshape(-1, 3), Ro)
+    return dict(pattern_missed_insertions=int(miss), pattern_rows_sorted_unique=rows_sorted,
+                block_nnz=int(len(ci)), pattern_sample_rows=int(len(sel)), pattern_sample_bit_exact=bool(sample_ok),
+                relerr_mat_vs_cpu=mat_err, relerr_residual_vs_cpu=res_err,
+                oracle_strict_s=tr + ta), (rp, ci)
+
+
+def c3_headline(args):
+    """The fused P2 residual + BSR(3) assembly, float64, 64^3."""
+    from pyop2_amd import device as dev
+    torch = dev.torch()
+    from pyop2_amd.meshes import unit_cube_tet_p2, workload_rng
+    n3 = 16 if args.quick else 64
+    m3 = unit_cube_tet_p2(n3, rng=workload_rng(3))
+    objs, loops = p2_loops(m3, np.float64)
+    pl, name, alg = loops["fused"]
+    log(f"C3 mesh {n3}^3: {m3.ncells} cells, {m3.nnodes} P2 nodes, {objs['nnzb']} blocks")
+    for _ in range(args.warmup):
+        pl()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pl()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    with dev.launch_timer() as timer:
+        for _ in range(args.steps):
+            pl()
+        torch.cuda.synchronize()
+    kms = kernel_ms(timer, name)
+    parts = {k: kernel_ms(timer, k) * 1e3 for k in ("p2_elasticity_matrix", "p2_elasticity_residual")
+             if timer.times(k)}
+    strategy = pl.global_kernel.compile().strategy_used
+    ms_per_step = wall * 1e3 / args.steps
+    achieved = alg / (kms * 1e-3) / 1e9
+    log(f"C3 fused f64: {ms_per_step:.3f} ms/step, kernel {kms * 1e3:.1f} us ({strategy}; parts {parts}), "
+        f"{achieved:.0f} GB/s algorithmic = {achieved / HBM_PEAK_GBS:.1%} of 8 TB/s")
+    parity, pattern = c3_parity(m3, pl, objs["A"], objs["R"])
+    log(f"C3 parity (one pass from zero, whole Mat and residual): {parity}")
+    cpu = None
+    if not args.no_cpu:
+        ts = []
+        budget = time.time() + 30.0
+        while len(ts) < 3 and (time.time() < budget or not ts):
+            _, _, tr, ta, _ = c3_oracle(m3, pattern=pattern, flavour="fast")
+            ts.append((tr, ta))
+        tr, ta = (float(np.median([t[i] for t in ts])) for i in (0, 1))
+        cpu_s = tr + ta
+        cpu = {"value": m3.ncells / cpu_s, "unit": "cells/s", "cores": 1, "kind": "port",
+               "sample": f"the whole {n3}^3 workload ({m3.ncells} cells, residual {tr:.2f} s + BSR(3) "
+                         f"assembly {ta:.2f} s per pass), median of {len(ts)} passes of the oracle's "
+                         f"sequential wrapper (gcc -march=native -O3 -ffast-math) on one core of {cpu_model()} "
+                         f"({cpu_cores()} cores visible)"}
+        log(f"C3 CPU baseline: {cpu_s:.2f} s per pass = {cpu['value']:.3e} cells/s; GPU kernel "
+            f"{cpu_s / (kms * 1e-3):.0f}x faster")
+    tkey = f"p2_elasticity_fused:{strategy}:{n3}:f64"
+    tr_meas = measured_traffic(tkey)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": tr_meas[0] if tr_meas else None,
+            "traffic_source": tr_meas[1] if tr_meas else None, "traffic_key": tkey,
+            "kernel": f"p2_elasticity_fused ({strategy}): " + ", ".join(f"{k} {v:.1f} us" for k, v in parts.items()),
+            "kernel_us": kms * 1e3, "alg_bytes_per_launch": alg}
+    detail = dict(cells=m3.ncells, block_nnz=objs["nnzb"], kernel_us=kms * 1e3, kernel_parts_us=parts,
+                  ms_per_step=ms_per_step, strategy=strategy, alg_bytes=alg, gbs=achieved,
+                  frac=achieved / HBM_PEAK_GBS, **parity)
+    if cpu:
+        detail["speedup_vs_cpu"] = (m3.ncells / cpu["value"]) / (kms * 1e-3)
+    del objs, loops, pl
+    torch.cuda.empty_cache()
+    return m3, dict(value=m3.ncells / (wall / args.steps), ms_per_step=ms_per_step, roofline=roof,
+                    cpu_baseline=cpu, detail=detail, workload=n3)
+
+
+# ---------------------------------------------------------------------------
+# C2: P1 Laplacian CSR assembly on the permuted 1024^2 square
+
+def c2_bytes(m, nnz):
+    return (m.ncells * 3 * 4 + m.nverts * 2 * 8 + nnz * 8 * 2 + nnz * 4 + (m.nverts + 1) * 4)
+
+
+def c2(args):
+    from pyop2_amd import device as dev, elements, op2
+    from pyop2_amd.meshes import unit_square_tri, workload_rng
+    from oracle import seqwrap, sparsity as osp
+    torch = dev.torch()
     n2 = 256 if args.quick else 1024
     m2 = unit_square_tri(n2, rng=workload_rng(2), permute=True)
     cells = op2.Set(m2.ncells, "cells")
@@ -168,41 +347,28 @@ def run(args):
     torch.cuda.synchronize()
     t_sp = time.time() - t0
     A = op2.Mat(sp, np.float64)
-    kern = elements.p1_laplace_matrix()
-    step = op2.ParLoop(kern, cells, A(op2.INC, (c2v, c2v)), X(op2.READ, c2v))
-    log(f"C2 mesh {n2}^2: {m2.ncells} cells, nnz {nnz}, sparsity build {t_sp * 1e3:.1f} ms (host clock)")
+    step = op2.ParLoop(elements.p1_laplace_matrix(), cells, A(op2.INC, (c2v, c2v)), X(op2.READ, c2v))
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    # throughput: K steps as a user runs them (no per-launch events)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # kernel duration: K more steps with a HIP event pair around each launch
     with dev.launch_timer() as timer:
         for _ in range(args.steps):
             step()
         torch.cuda.synchronize()
     kms = kernel_ms(timer, "p1_laplace_matrix")
-    strategy_used = step.global_kernel.compile().strategy_used
-    ms_per_step = wall * 1e3 / args.steps
-    cells_per_s = m2.ncells / (wall / args.steps)
+    strategy = step.global_kernel.compile().strategy_used
     alg = c2_bytes(m2, nnz)
     achieved = alg / (kms * 1e-3) / 1e9
-    log(f"C2 assembly: {ms_per_step:.3f} ms/step, kernel {kms * 1e3:.1f} us ({strategy_used}), "
-        f"{achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.1%} of 8 TB/s)")
-    if args.no_cold:
-        kms_cold, nrun_extra = float("nan"), 0
-    else:
-        kms_cold = cold_kernel_ms(step, max(3, min(args.steps, 10)), "p1_laplace_matrix")
-        log(f"C2 assembly, last-level cache flushed before each launch: kernel {kms_cold * 1e3:.1f} us, "
-            f"{alg / (kms_cold * 1e-3) / 1e9:.0f} GB/s algorithmic")
-        nrun_extra = 1 + max(3, min(args.steps, 10))
-
-    # parity of the timed Mat against the oracle (values scale with the step count)
-    from oracle import seqwrap, sparsity as osp
+    kms_cold = float("nan") if args.no_cold else cold_kernel_ms(step, max(3, min(args.steps, 10)),
+                                                                "p1_laplace_matrix")
+    # one pass from zero against the oracle
+    A.zero()
+    step()
     A.assemble()
     rp, ci = osp.pattern(m2.nverts, m2.nverts, [(m2.cell_vertex, m2.cell_vertex, m2.ncells)])
     ref = seqwrap.CSRMat(rp, ci, np.zeros(len(ci)))
@@ -211,87 +377,43 @@ def run(args):
              seqwrap.DatArg(m2.coords, "READ", m2.cell_vertex)]
     seqwrap.run(code, "p1_laplace_matrix", cargs, [(0, m2.ncells)])
     grp, gci, gv = A.csr()
-    nrun = 2 * args.steps + args.warmup + nrun_extra
     pattern_exact = bool(np.array_equal(grp, rp) and np.array_equal(gci, ci))
-    c2_err = float(np.linalg.norm(gv / nrun - ref.values) / np.linalg.norm(ref.values))
-    log(f"C2 parity: pattern bit-exact {pattern_exact}, values normwise rel err {c2_err:.2e}")
-
-    # CPU baseline: the oracle's sequential wrapper with the reference's flags
-    cpu = None
+    err = normwise(gv, ref.values)
+    tr = measured_traffic(f"p1_laplace_matrix:{strategy}:{n2}")
+    out = dict(workload=f"P1 Laplacian CSR assembly, {n2}^2 permuted", cells=m2.ncells, nnz=nnz,
+               kernel_us=kms * 1e3, kernel_us_cold=kms_cold * 1e3, ms_per_step=wall * 1e3 / args.steps,
+               cells_per_s=m2.ncells / (wall / args.steps), strategy=strategy, alg_bytes=alg, gbs=achieved,
+               frac=achieved / HBM_PEAK_GBS, traffic=tr[0] if tr else None,
+               pattern_bit_exact=pattern_exact, relerr_vs_cpu=err, sparsity_build_ms_host=t_sp * 1e3)
     if not args.no_cpu:
         ref.values[:] = 0
         seqwrap.run(code, "p1_laplace_matrix", cargs, [(0, m2.ncells)], flavour="fast")
         ts = []
-        budget = time.time() + 20.0
+        budget = time.time() + 10.0
         while len(ts) < 5 and (time.time() < budget or len(ts) < 2):
-            t0 = time.perf_counter()
+            t1 = time.perf_counter()
             seqwrap.run(code, "p1_laplace_matrix", cargs, [(0, m2.ncells)], flavour="fast")
-            ts.append(time.perf_counter() - t0)
-        tcpu = float(np.median(ts))
-        cpu = {"value": m2.ncells / tcpu, "unit": "cells/s", "cores": 1, "kind": "port",
-               "sample": f"full {n2}^2 workload ({m2.ncells} cells), median of {len(ts)} runs of the "
-                         f"oracle's sequential wrapper (gcc -march=native -O3 -ffast-math) on one core of "
-                         f"{cpu_model()} ({cpu_cores()} cores visible)"}
-        log(f"C2 CPU baseline: {tcpu * 1e3:.1f} ms per pass = {cpu['value']:.3e} cells/s "
-            f"(GPU step {m2.ncells / (kms * 1e-3) / cpu['value']:.0f}x faster by kernel time)")
-    out["c2"] = dict(cells=m2.ncells, nnz=nnz, kernel_us=kms * 1e3, kernel_us_cold=kms_cold * 1e3,
-                     ms_per_step=ms_per_step,
-                     strategy=strategy_used, alg_bytes=alg, gbs=achieved, frac=achieved / HBM_PEAK_GBS,
-                     pattern_bit_exact=pattern_exact, relerr_vs_cpu=c2_err,
-                     sparsity_build_ms_host=t_sp * 1e3)
-    del A, sp
-
-    if not args.primary_only:
-        out.update(secondary(args))
-        out["c5_batched"] = batched(args)
-    copy_gbs, copy_ms = copy_bandwidth()
-    out["stream_copy"] = dict(bytes_each_way=1 << 29, ms=copy_ms, gbs=copy_gbs)
-    log(f"streaming y = -x over 512 MiB: {copy_ms * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)")
-
-    kname = kernel_name(strategy_used, "op2hip::P1MatrixRow", "op2hip::P1MatrixLanes", "p1_matrix_kernel")
-    tkey = f"p1_laplace_matrix:{strategy_used}:{n2}"
-    tr = measured_traffic(tkey)
-    if tr is not None:
-        log(f"C2 measured HBM traffic per launch {tr[0]:.4g} B ({tr[1]}), "
-            f"{tr[0] / alg:.2f}x the algorithmic bytes")
-
-    result = {
-        "metric": METRIC,
-        "value": cells_per_s,
-        "unit": "cells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"P1 Laplacian CSR assembly: par_loop INC of 3x3 element matrices into a "
-                               f"float64 CSR Mat, {n2}x{n2} unit square ({m2.ncells} cells, {m2.nverts} "
-                               f"vertices, {nnz} nnz), randomly permuted vertex numbering",
-                   "strategy": strategy_used, "device": gpu_name},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": tr[0] if tr else None,
-                     "traffic_source": tr[1] if tr else None, "traffic_key": tkey,
-                     "kernel": kname, "kernel_us": kms * 1e3, "alg_bytes_per_launch": alg,
-                     "frac_of_measured_copy": achieved / copy_gbs},
-        "cpu_baseline": cpu,
-        "secondary": out,
-    }
-    return result
+            ts.append(time.perf_counter() - t1)
+        out["cpu_cells_per_s_1core"] = m2.ncells / float(np.median(ts))
+    log(f"C2: kernel {kms * 1e3:.1f} us ({strategy}), cold {kms_cold * 1e3:.1f} us, "
+        f"{achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.1%}; pattern bit-exact {pattern_exact}, "
+        f"rel err {err:.1e}")
+    return out
 
 
-def secondary(args):
-    """Configs 1, 3 and 4 (kernel-time measurements)."""
+# ---------------------------------------------------------------------------
+# secondary measurements
+
+def secondary(args, m3=None):
+    """C1, the P2 loops alone and in float32, and C4."""
     from pyop2_amd import device as dev, elements, op2
     from pyop2_amd.meshes import unit_cube_tet_p2, unit_square_tri, workload_rng
     torch = dev.torch()
     reps = args.reps
     res = {}
 
-    # C1: P1 lumped mass and Laplace residual, 256^2
+    # C1: P1 lumped mass and Laplace residual, 256^2 (launch-bound: timed
+    # over 200 back-to-back launches per event pair)
     m1 = unit_square_tri(256, rng=workload_rng(1))
     cells = op2.Set(m1.ncells, "cells")
     verts = op2.Set(m1.nverts, "verts")
@@ -305,80 +427,114 @@ def secondary(args):
     base = m1.ncells * 12 + m1.nverts * 16
     for key, pl, name, alg in (("c1a_mass", pm, "p1_lumped_mass", base + m1.nverts * 16),
                                ("c1b_residual", pr, "p1_laplace_residual", base + m1.nverts * 24)):
-        ms = time_loop(pl, reps, name)
+        ms = time_many(pl, 200)
         res[key] = dict(cells=m1.ncells, kernel_us=ms * 1e3, cells_per_s=m1.ncells / (ms * 1e-3),
-                        gbs=alg / (ms * 1e-3) / 1e9, alg_bytes=alg,
+                        gbs=alg / (ms * 1e-3) / 1e9, alg_bytes=alg, timing="200 launches per event pair",
                         strategy=pl.global_kernel.compile().strategy_used,
                         relerr_vs_cpu=c1_relerr(m1, pl, name))
-        log(f"C1 {key}: {ms * 1e3:.2f} us, {res[key]['gbs']:.0f} GB/s ({res[key]['strategy']}), "
+        log(f"C1 {key}: {ms * 1e3:.2f} us per launch, {res[key]['gbs']:.0f} GB/s ({res[key]['strategy']}), "
             f"rel err vs CPU {res[key]['relerr_vs_cpu']:.1e}")
 
-    # C3: P2 elasticity, 64^3 (16^3 with --quick)
+    # C3: P2 residual and assembly alone (f64), and all three in f32
     n3 = 16 if args.quick else 64
-    m3 = unit_cube_tet_p2(n3, rng=workload_rng(3))
+    if m3 is None:
+        m3 = unit_cube_tet_p2(n3, rng=workload_rng(3))
     for dt, tag in ((np.float64, "f64"), (np.float32, "f32")):
-        isz = np.dtype(dt).itemsize
-        cells = op2.Set(m3.ncells, "cells")
-        vset = op2.Set(m3.nverts, "verts")
-        nodes = op2.Set(m3.nnodes, "nodes")
-        c2v = op2.Map(cells, vset, 4, m3.cell_vertex, "c2v")
-        c2n = op2.Map(cells, nodes, 10, m3.cell_node, "c2n")
-        X = op2.Dat(vset ** 3, m3.coords.astype(dt), dt, name="X")
-        u = op2.Dat(nodes ** 3, m3.u.astype(dt), dt, name="u")
-        R = op2.Dat(nodes ** 3, dtype=dt, name="R")
-        sp = op2.Sparsity((nodes ** 3, nodes ** 3), [(c2n, c2n, None)])
-        nnzb = sp.block_nnz
-        A = op2.Mat(sp, dt)
-        maps_b = m3.ncells * (40 + 16)
-        x_b = m3.nverts * 3 * isz
-        u_b = m3.nnodes * 3 * isz
-        r_b = 2 * u_b
-        asm_b = nnzb * 9 * isz * 2 + nnzb * 4 + (m3.nnodes + 1) * 4 + maps_b + x_b
-        loops = [
-            ("residual", op2.ParLoop(elements.p2_elasticity_residual(), cells, R(op2.INC, c2n),
-                                     X(op2.READ, c2v), u(op2.READ, c2n)),
-             "p2_elasticity_residual", maps_b + x_b + u_b + r_b),
-            ("assembly", op2.ParLoop(elements.p2_elasticity_matrix(), cells, A(op2.INC, (c2n, c2n)),
-                                     X(op2.READ, c2v)), "p2_elasticity_matrix", asm_b),
-            ("fused", op2.ParLoop(elements.p2_elasticity_fused(), cells, A(op2.INC, (c2n, c2n)),
-                                  R(op2.INC, c2n), X(op2.READ, c2v), u(op2.READ, c2n)),
-             "p2_elasticity_fused", asm_b + u_b + r_b),
-        ]
-        for key, pl, name, alg in loops:
+        objs, loops = p2_loops(m3, dt)
+        for key in (("residual", "assembly") if tag == "f64" else ("residual", "assembly", "fused")):
+            pl, name, alg = loops[key]
             ms = time_loop(pl, reps, name)
             k = f"c3_{key}_{tag}"
-            res[k] = dict(cells=m3.ncells, block_nnz=nnzb, kernel_us=ms * 1e3,
+            res[k] = dict(cells=m3.ncells, block_nnz=objs["nnzb"], kernel_us=ms * 1e3,
                           cells_per_s=m3.ncells / (ms * 1e-3), gbs=alg / (ms * 1e-3) / 1e9,
                           frac=alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
                           strategy=pl.global_kernel.compile().strategy_used)
-            if key == "fused":
-                res[k].update(c3_rows_relerr(m3, pl, A, R))
-            log(f"C3 {k}: {ms * 1e3:.1f} us, {res[k]['gbs']:.0f} GB/s = {res[k]['frac']:.1%} "
-                f"({res[k]['strategy']})" + (f", first {res[k]['parity_rows']} rows vs CPU: Mat rel err "
-                                              f"{res[k]['relerr_mat_vs_cpu']:.1e}, residual "
-                                              f"{res[k]['relerr_residual_vs_cpu']:.1e}" if key == "fused" else ""))
+            log(f"C3 {k}: {ms * 1e3:.1f} us, {res[k]['gbs']:.0f} GB/s = {res[k]['frac']:.1%} ({res[k]['strategy']})")
+        if tag == "f32":
+            pl, _, _ = loops["fused"]
+            res["c3_fused_f32"].update(c3_f32_relerr(m3, pl, objs["A"], objs["R"]))
+            log(f"C3 fused f32 vs the float64 oracle: Mat {res['c3_fused_f32']['relerr_mat_vs_cpu']:.1e}, "
+                f"residual {res['c3_fused_f32']['relerr_residual_vs_cpu']:.1e}")
         if tag == "f64":
-            res["c4_adjoint"] = adjoint_ratio(m3, cells, c2v, c2n, X, u, reps)
-            if not args.no_cpu:
-                res["c3_cpu"] = c3_cpu_baseline(m3, res)
-        del A, sp
+            res["c4_adjoint"] = adjoint_ratio(objs, reps)
+        del objs, loops
         torch.cuda.empty_cache()
     return res
+
+
+def c3_f32_relerr(m3, pl, A, R):
+    """One pass of the float32 fused loop from zero against the float64
+    oracle on the float32-rounded inputs (whole Mat and residual)."""
+    from pyop2_amd import device as dev
+    import dataclasses
+    A.zero()
+    R.zero()
+    pl()
+    A.assemble()
+    rp = dev.to_host(A.rowptr).astype(np.int32)
+    ci = dev.to_host(A.colidx)
+    m32 = dataclasses.replace(m3, coords=m3.coords.astype(np.float32).astype(np.float64),
+                              u=m3.u.astype(np.float32).astype(np.float64))
+    Ro, Ao, _, _, miss = c3_oracle(m32, pattern=(rp, ci))
+    gv = dev.to_host(A.device_values).astype(np.float64).reshape(-1, 3, 3)[:len(ci)]
+    return dict(pattern_missed_insertions=int(miss), relerr_mat_vs_cpu=normwise(gv, Ao.values),
+                relerr_residual_vs_cpu=normwise(dev.to_host(R.device_data_ro).reshape(-1, 3), Ro))
+
+
+def c1_relerr(m1, pl, name):
+    """One pass of the timed C1 loop from zero against the oracle's
+    sequential wrapper (float64, strict IEEE) on the whole mesh."""
+    from oracle import seqwrap
+    from pyop2_amd import elements
+    out = pl.arguments[0].data
+    out.zero()
+    pl()
+    got = np.asarray(out.data_ro, dtype=np.float64).reshape(-1)
+    ref = np.zeros(m1.nverts)
+    args = [seqwrap.DatArg(ref, "INC", m1.cell_vertex), seqwrap.DatArg(m1.coords, "READ", m1.cell_vertex)]
+    if name == "p1_laplace_residual":
+        args.append(seqwrap.DatArg(m1.u, "READ", m1.cell_vertex))
+    seqwrap.run(elements.kernel_source(name), name, args, [(0, m1.ncells)])
+    return normwise(got, ref)
+
+
+def adjoint_ratio(objs, reps):
+    """t(adjoint loop) / t(forward residual loop), P2, float64."""
+    from pyop2_amd import adjoint, device as dev
+    torch = dev.torch()
+    prob = adjoint.P2Elasticity(objs["cells"], objs["c2v"], objs["c2n"], objs["X"], objs["u"])
+    uu = objs["u"].device_data_ro.clone().requires_grad_(True)
+    xx = objs["X"].device_data_ro.clone().requires_grad_(True)
+    for _ in range(2):
+        J = prob.objective(uu, xx)
+        J.backward()
+    torch.cuda.synchronize()
+    with dev.launch_timer() as t:
+        for _ in range(reps):
+            J = prob.objective(uu, xx)
+            J.backward()
+    torch.cuda.synchronize()
+    fwd = kernel_ms(t, "p2_elasticity_residual")
+    adj = kernel_ms(t, "p2_elasticity_adjoint")
+    log(f"C4 adjoint: forward {fwd * 1e3:.1f} us, adjoint {adj * 1e3:.1f} us, ratio {adj / fwd:.2f}")
+    return dict(forward_us=fwd * 1e3, adjoint_us=adj * 1e3, ratio=adj / fwd)
 
 
 def batched(args):
     """Config 5: Laplace CSR assembly + residual INC of 512 unit squares (n ~
     U{16..128}) as one persistent launch in a HIP graph, against the same
-    loops as 512 x 2 plain par_loop launches (wall clock per full pass)."""
+    loops as 512 x 2 plain par_loop launches (wall clock per full pass); the
+    batched results of sampled meshes against the oracle."""
     from pyop2_amd import device as dev, elements, op2
     from pyop2_amd.batch import BatchedLaplace
     from pyop2_amd.configuration import configuration
     from pyop2_amd.meshes import batched_tri_meshes
+    from oracle import seqwrap, sparsity as osp
     torch = dev.torch()
     count = 32 if args.quick else 512
     t0 = time.time()
     meshes = batched_tri_meshes(count=count)
-    keep, plain, pairs = [], [], []
+    plain, pairs = [], []
     for m in meshes:
         cells = op2.Set(m.ncells)
         verts = op2.Set(m.nverts)
@@ -395,19 +551,12 @@ def batched(args):
                                    u(op2.READ, c2v)), A, R))
         plain.append(ls[0])
         pairs.append(ls[1])
-        keep.append((cells, verts, c2v, X, u, sp))
     ncells = sum(m.ncells for m in meshes)
     nnz = sum(p[2].sparsity.block_nnz for p in plain)
-    # plain: each mesh's two loops, with the single-mesh strategies the
-    # autotuner picks for these loops (fixed here so 1024 loops are not tuned)
+    alg = sum(c2_bytes(m, p[2].sparsity.block_nnz) + m.ncells * 12 + m.nverts * 40
+              for m, p in zip(meshes, plain))
     saved = configuration["strategy"]
-    configuration["strategy"] = "rowgather_slots"
-    for pm, _, _, _ in plain:
-        pm()
-    configuration["strategy"] = "staged"
-    for _, pr, _, _ in plain:
-        pr()
-    torch.cuda.synchronize()
+    saved_partials = configuration["staged_partials"]
 
     def plain_pass():
         configuration["strategy"] = "rowgather_slots"
@@ -417,10 +566,7 @@ def batched(args):
         for _, pr, _, _ in plain:
             pr()
     reps = max(2, min(args.reps, 5))
-    # the faster of the two shared-node treatments of the staged residual
-    # (partial sums add a launch per mesh; atomics do not)
     t_plain, plain_partials = None, None
-    saved_partials = configuration["staged_partials"]
     for partials in (True, False):
         configuration["staged_partials"] = partials
         plain_pass()
@@ -446,135 +592,79 @@ def batched(args):
         torch.cuda.synchronize()
         t_batch = (time.perf_counter() - t1) / (reps * 4)
     kms = kernel_ms(tm, "batched_p1_laplace")
-    # parity: batched and plain results of every mesh (batched ran 1 + 4 reps
-    # times, plain 1 + 2 x (1 + reps) times)
-    nb, npl = 1 + 4 * reps, 3 + 2 * reps
+    nb = 1 + 4 * reps   # batched passes from zero: the warm-up and the timed ones (tuning restores)
+    # sampled meshes against the oracle (first, last and six in between)
     worst = 0.0
-    for (_, _, A0, R0), (_, _, A1, R1) in zip(plain, pairs):
-        v0, v1 = A0.device_values / npl, A1.device_values / nb
-        worst = max(worst, float((v1 - v0).norm() / v0.norm()))
-        r0, r1 = R0._device_array() / npl, R1._device_array() / nb
-        worst = max(worst, float((r1 - r0).norm() / max(float(r0.norm()), 1e-300)))
+    for k in sorted(set(np.linspace(0, count - 1, 8).astype(int))):
+        m, (_, _, A1, R1) = meshes[k], pairs[k]
+        rp, ci = osp.pattern(m.nverts, m.nverts, [(m.cell_vertex, m.cell_vertex, m.ncells)])
+        ref = seqwrap.CSRMat(rp, ci, np.zeros(len(ci)))
+        seqwrap.run(elements.kernel_source("p1_laplace_matrix"), "p1_laplace_matrix",
+                    [seqwrap.MatArg(ref, "INC", m.cell_vertex, m.cell_vertex),
+                     seqwrap.DatArg(m.coords, "READ", m.cell_vertex)], [(0, m.ncells)])
+        rr = np.zeros(m.nverts)
+        seqwrap.run(elements.kernel_source("p1_laplace_residual"), "p1_laplace_residual",
+                    [seqwrap.DatArg(rr, "INC", m.cell_vertex), seqwrap.DatArg(m.coords, "READ", m.cell_vertex),
+                     seqwrap.DatArg(m.u, "READ", m.cell_vertex)], [(0, m.ncells)])
+        worst = max(worst, normwise(dev.to_host(A1.device_values)[:len(ci)] / nb, ref.values),
+                    normwise(dev.to_host(R1._device_array()) / nb, rr))
+    achieved = alg / (kms * 1e-3) / 1e9
+    trm = measured_traffic(f"batched_p1_laplace:{count}")
     res = dict(meshes=count, cells=ncells, nnz=nnz, plain_ms_per_pass=t_plain * 1e3,
                plain_staged_partials=plain_partials,
                batched_ms_per_pass=t_batch * 1e3, batched_kernel_us=kms * 1e3,
                speedup=t_plain / t_batch, cells_per_s=ncells / t_batch, grid=batch.grid,
-               items=batch.nitems, relerr_batched_vs_plain=worst, setup_s=time.time() - t0,
-               batch_plan_s=t_setup)
+               items=batch.nitems, relerr_sampled_meshes_vs_cpu=worst, setup_s=time.time() - t0,
+               batch_plan_s=t_setup,
+               roofline={"bound": "hbm", "alg_bytes_per_launch": alg, "achieved": achieved,
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": trm[0] if trm else None})
     log(f"C5 batched {count} meshes ({ncells} cells, {nnz} nnz): plain {t_plain * 1e3:.2f} ms/pass, "
-        f"batched {t_batch * 1e3:.3f} ms/pass (kernel {kms * 1e3:.1f} us, grid {batch.grid}, "
-        f"{batch.nitems} items): {t_plain / t_batch:.1f}x; batched vs plain rel err {worst:.1e}")
+        f"batched {t_batch * 1e3:.3f} ms/pass (kernel {kms * 1e3:.1f} us = {achieved:.0f} GB/s, "
+        f"{achieved / HBM_PEAK_GBS:.1%}; grid {batch.grid}, {batch.nitems} items): {t_plain / t_batch:.1f}x; "
+        f"8 sampled meshes vs oracle, worst rel err {worst:.1e}")
     return res
 
 
-def c1_relerr(m1, pl, name):
-    """One pass of the timed C1 loop from zero against the oracle's
-    sequential wrapper (float64, strict IEEE) on the whole mesh."""
-    from oracle import seqwrap
-    from pyop2_amd import elements
-    out = pl.arguments[0].data
-    out.zero()
-    pl()
-    got = np.asarray(out.data_ro, dtype=np.float64).reshape(-1)
-    ref = np.zeros(m1.nverts)
-    args = [seqwrap.DatArg(ref, "INC", m1.cell_vertex), seqwrap.DatArg(m1.coords, "READ", m1.cell_vertex)]
-    if name == "p1_laplace_residual":
-        args.append(seqwrap.DatArg(m1.u, "READ", m1.cell_vertex))
-    seqwrap.run(elements.kernel_source(name), name, args, [(0, m1.ncells)])
-    return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-
-
-def c3_rows_relerr(m3, pl, A, R, rows=4096):
-    """One pass of the timed fused loop (residual + BSR(3) assembly) from
-    zero against the oracle's sequential wrapper (float64, strict IEEE) on
-    the first ``rows`` P2 nodes: the oracle runs every cell incident to
-    them, so their block rows and residual entries are complete; the
-    pattern of those rows must match bit for bit."""
-    from oracle import seqwrap, sparsity as osp
-    from pyop2_amd import device as dev, elements
-    A.zero()
-    R.zero()
-    pl()
-    A.assemble()
-    sel = np.nonzero((m3.cell_node < rows).any(axis=1))[0]
-    cn = np.ascontiguousarray(m3.cell_node[sel], dtype=np.int32)
-    cv = np.ascontiguousarray(m3.cell_vertex[sel], dtype=np.int32)
-    rp, ci = osp.pattern(m3.nnodes, m3.nnodes, [(cn, cn, len(sel))])
-    Ao = seqwrap.CSRMat(rp, ci, np.zeros((len(ci), 3, 3)), 3, 3, block=True)
-    Ro = np.zeros((m3.nnodes, 3))
-    rcode = elements.kernel_source("p2_elasticity_residual", lam=1.0, mu=0.5)
-    mcode = elements.kernel_source("p2_elasticity_matrix", lam=1.0, mu=0.5)
-    seqwrap.run(rcode, "p2_elasticity_residual", [seqwrap.DatArg(Ro, "INC", cn), seqwrap.DatArg(m3.coords, "READ", cv),
-                                                  seqwrap.DatArg(m3.u, "READ", cn)], [(0, len(sel))])
-    seqwrap.run(mcode, "p2_elasticity_matrix", [seqwrap.MatArg(Ao, "INC", cn, cn, 3, 3),
-                                                seqwrap.DatArg(m3.coords, "READ", cv)], [(0, len(sel))])
-    grp = dev.to_host(A.rowptr[:rows + 1]).astype(np.int64)
-    nz = int(grp[-1])
-    gci = dev.to_host(A.colidx[:nz])
-    gv = dev.to_host(A.device_values[:nz * 9]).astype(np.float64).reshape(nz, 3, 3)
-    pattern = bool(np.array_equal(grp, rp[:rows + 1]) and np.array_equal(gci, ci[:nz]))
-    ov = Ao.values[:nz]
-    gr = dev.to_host(R.device_data_ro[:rows]).astype(np.float64)
-    return dict(parity_rows=rows, parity_cells=int(len(sel)), pattern_rows_bit_exact=pattern,
-                relerr_mat_vs_cpu=float(np.linalg.norm(gv - ov) / np.linalg.norm(ov)),
-                relerr_residual_vs_cpu=float(np.linalg.norm(gr - Ro[:rows]) / np.linalg.norm(Ro[:rows])))
-
-
-def c3_cpu_baseline(m3, res, sample=24576):
-    """The oracle's sequential wrapper (reference flags, one core) on the
-    first ``sample`` cells of the P2 mesh: residual INC + BSR(3) assembly,
-    scaled to the whole mesh; against the GPU's residual + assembly."""
-    from oracle import seqwrap, sparsity as osp
-    from pyop2_amd import elements
-    n = min(sample, m3.ncells)
-    cn = np.ascontiguousarray(m3.cell_node[:n], dtype=np.int32)
-    cv = np.ascontiguousarray(m3.cell_vertex[:n], dtype=np.int32)
-    R = np.zeros((m3.nnodes, 3))
-    rp, ci = osp.pattern(m3.nnodes, m3.nnodes, [(cn, cn, n)])
-    A = seqwrap.CSRMat(rp, ci, np.zeros((len(ci), 3, 3)), 3, 3, block=True)
-    rcode = elements.kernel_source("p2_elasticity_residual", lam=1.0, mu=0.5)
-    mcode = elements.kernel_source("p2_elasticity_matrix", lam=1.0, mu=0.5)
-    rargs = [seqwrap.DatArg(R, "INC", cn), seqwrap.DatArg(m3.coords, "READ", cv), seqwrap.DatArg(m3.u, "READ", cn)]
-    margs = [seqwrap.MatArg(A, "INC", cn, cn, 3, 3), seqwrap.DatArg(m3.coords, "READ", cv)]
-    seqwrap.run(rcode, "p2_elasticity_residual", rargs, [(0, n)], flavour="fast")
-    seqwrap.run(mcode, "p2_elasticity_matrix", margs, [(0, n)], flavour="fast")
-    t0 = time.perf_counter()
-    seqwrap.run(rcode, "p2_elasticity_residual", rargs, [(0, n)], flavour="fast")
-    t1 = time.perf_counter()
-    seqwrap.run(mcode, "p2_elasticity_matrix", margs, [(0, n)], flavour="fast")
-    t2 = time.perf_counter()
-    scale = m3.ncells / n
-    cpu_s = ((t1 - t0) + (t2 - t1)) * scale
-    gpu_s = (res["c3_residual_f64"]["kernel_us"] + res["c3_assembly_f64"]["kernel_us"]) * 1e-6
-    out = dict(sample_cells=n, residual_s=(t1 - t0) * scale, assembly_s=(t2 - t1) * scale, cpu_s=cpu_s,
-               gpu_s=gpu_s, speedup=cpu_s / gpu_s, cores=1,
-               sample=f"first {n} of {m3.ncells} cells, oracle wrapper with gcc -march=native -O3 -ffast-math "
-                      f"on one core of {cpu_model()}, scaled by {scale:.0f}")
-    log(f"C3 CPU (1 core, {n} cells scaled x{scale:.0f}): residual {out['residual_s']:.2f} s + assembly "
-        f"{out['assembly_s']:.2f} s; GPU residual + assembly {gpu_s * 1e3:.2f} ms: {out['speedup']:.0f}x")
-    return out
-
-
-def adjoint_ratio(m3, cells, c2v, c2n, X, u, reps):
-    """t(adjoint loop) / t(forward residual loop), P2 64^3, float64."""
-    from pyop2_amd import adjoint, device as dev
+def run(args):
+    from pyop2_amd import device as dev
+    from pyop2_amd.configuration import configuration
     torch = dev.torch()
-    prob = adjoint.P2Elasticity(cells, c2v, c2n, X, u)
-    uu = u.device_data_ro.clone().requires_grad_(True)
-    xx = X.device_data_ro.clone().requires_grad_(True)
-    for _ in range(2):
-        J = prob.objective(uu, xx)
-        J.backward()
-    torch.cuda.synchronize()
-    with dev.launch_timer() as t:
-        for _ in range(reps):
-            J = prob.objective(uu, xx)
-            J.backward()
-    torch.cuda.synchronize()
-    fwd = kernel_ms(t, "p2_elasticity_residual")
-    adj = kernel_ms(t, "p2_elasticity_adjoint")
-    log(f"C4 adjoint: forward {fwd * 1e3:.1f} us, adjoint {adj * 1e3:.1f} us, ratio {adj / fwd:.2f}")
-    return dict(forward_us=fwd * 1e3, adjoint_us=adj * 1e3, ratio=adj / fwd)
+    gpu_name = torch.cuda.get_device_name(dev.device())
+    log(f"device {dev.device()} {gpu_name}")
+    configuration["strategy"] = args.strategy
+    m3, head = c3_headline(args)
+    out = {"c3_fused_f64": head["detail"], "c2": c2(args)}
+    if not args.primary_only:
+        out.update(secondary(args, m3))
+        out["c5_batched"] = batched(args)
+    copy_gbs, copy_ms = copy_bandwidth()
+    out["stream_copy"] = dict(bytes_each_way=1 << 29, ms=copy_ms, gbs=copy_gbs)
+    log(f"streaming y = -x over 512 MiB: {copy_ms * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)")
+    head["roofline"]["frac_of_measured_copy"] = head["roofline"]["achieved"] / copy_gbs
+    n3 = head["workload"]
+    d = head["detail"]
+    return {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"P2 tetrahedral linear elasticity, residual + BSR(3) assembly in one par_loop "
+                               f"(INC of 30x30 element matrices into a float64 Mat and of element residuals "
+                               f"into a vector Dat), {n3}^3 unit cube x 6 tets ({d['cells']} cells, "
+                               f"{d['block_nnz']} 3x3 blocks), lambda=1, mu=0.5",
+                   "strategy": d["strategy"], "device": gpu_name},
+        "roofline": head["roofline"],
+        "cpu_baseline": head["cpu_baseline"],
+        "secondary": out,
+    }
 
 
 def main():
@@ -585,7 +675,7 @@ def main():
     ap.add_argument("--reps", type=int, default=10, help="repetitions of each secondary loop")
     ap.add_argument("--strategy", default="auto")
     ap.add_argument("--quick", action="store_true", help="small meshes (smoke of the harness)")
-    ap.add_argument("--primary-only", action="store_true")
+    ap.add_argument("--primary-only", action="store_true", help="the headline and C2 only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cold", action="store_true",
                     help="skip the cache-flushed C2 variant (profiles of the warm kernels)")

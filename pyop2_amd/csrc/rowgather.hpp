@@ -1372,7 +1372,8 @@ rowgather_rowlist_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
     const int32_t t = SpreadOf<Elem>::value ? spread_incidence(t0, d.ni) : t0;
     int32_t ent;
     SlotEntry<Elem::kCar> se;
-    if constexpr (Elem::kCar == 3) {
+    if constexpr (Elem::kCar == 3 && !Elem::kHasRowDat) {
+      // the packed words carry no row index (se.lr), which only a row Dat reads
       if (packed) {
         // one word: incidence, row offset and three 5-bit slots (31: none)
         const uint64_t w = packed[d.i0 + t];
@@ -1455,11 +1456,11 @@ int launch_rowgather_rowlist(int32_t start, int32_t end, const Elem &elem, const
   if (per_thread <= 8)
     rowgather_rowlist_kernel<Elem, 8><<<ex->nbatches, bs, lds, s>>>(
         start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base,
-        Elem::kCar == 3 && Elem::kRar == 3 ? ex->batch_packed : nullptr, rowdat);
+        Elem::kCar == 3 && Elem::kRar == 3 && !Elem::kHasRowDat ? ex->batch_packed : nullptr, rowdat);
   else
     rowgather_rowlist_kernel<Elem, 16><<<ex->nbatches, bs, lds, s>>>(
         start, end, elem, *A, descs, ex->nbatches, ro, ex->batch_ent, table, ex->row_base,
-        Elem::kCar == 3 && Elem::kRar == 3 ? ex->batch_packed : nullptr, rowdat);
+        Elem::kCar == 3 && Elem::kRar == 3 && !Elem::kHasRowDat ? ex->batch_packed : nullptr, rowdat);
   OP2HIP_LAUNCHED();
   return OP2HIP_OK;
 }

@@ -484,73 +484,110 @@ class Mat(DataCarrier):
         self._init()
         self._values.zero_()
 
+    # -- entries set from the host (the reference's MatSetValues calls) ----------
+    # A few entries change on the device; only their positions are found on
+    # the host, from a host copy of the pattern kept after the first use.
+    def _host_pattern(self):
+        if getattr(self, "_hpattern", None) is None:
+            from pyop2_amd import device as dev
+            self._init()
+            self._hpattern = (dev.to_host(self._rowptr).astype(np.int64),
+                              dev.to_host(self._colidx).astype(np.int64))
+        return self._hpattern
+
+    def _positions(self, rows, cols):
+        """Flat positions in the values array of the scalar (dof) entries
+        ``(rows[k], cols[k])``; PatternError for an entry off the pattern."""
+        rp, ci = self._host_pattern()
+        rd, cd = self.block_shape
+        rows = np.asarray(rows, dtype=np.int64).ravel()
+        cols = np.asarray(cols, dtype=np.int64).ravel()
+        br, ri = np.divmod(rows, rd)
+        bc, cj = np.divmod(cols, cd)
+        out = np.empty(len(rows), dtype=np.int64)
+        for k in range(len(rows)):
+            lo, hi = rp[br[k]], rp[br[k] + 1]
+            s = lo + int(np.searchsorted(ci[lo:hi], bc[k]))
+            if s >= hi or ci[s] != bc[k]:
+                raise ex.PatternError(f"entry ({rows[k]}, {cols[k]}) is not in the sparsity pattern")
+            out[k] = (s * rd + ri[k]) * cd + cj[k]
+        return out
+
+    def _write_entries(self, rows, cols, vals, add):
+        """values[(r, c)] += v (add) or = v; among repeated entries an insert
+        keeps the last, as successive MatSetValues calls do."""
+        from pyop2_amd import device as dev
+        vals = np.asarray(vals, dtype=self.dtype).ravel()
+        if not len(vals):
+            return
+        pos = self._positions(rows, cols)
+        if not add:
+            last = len(pos) - 1 - np.unique(pos[::-1], return_index=True)[1]
+            pos, vals = pos[last], vals[last]
+        idx = dev.to_device(pos)
+        self._values.index_put_((idx,), dev.to_device(vals), accumulate=add)
+
     def zero_rows(self, rows, diag_val=1.0):
-        """Zero rows (dof rows) except the diagonal, set to ``diag_val``."""
+        """Zero the dof rows ``rows`` (an iterable or a Subset), then set
+        their diagonal entry to ``diag_val``."""
+        from pyop2_amd import device as dev
         self.assemble()
         rows = rows.indices if isinstance(rows, Subset) else rows
-        rows = np.asarray(rows, dtype=np.int64).ravel()
-        rp, ci, vals = self._host_csr()
+        rows = np.unique(np.asarray(rows, dtype=np.int64).ravel())
+        if not len(rows):
+            return
+        rp, ci = self._host_pattern()
         rd, cd = self.block_shape
-        for r in rows:
-            br, ri = divmod(int(r), rd)
-            for s in range(rp[br], rp[br + 1]):
-                blk = vals[s]
-                blk[ri, :] = 0.0
-                if ci[s] * cd <= r < (ci[s] + 1) * cd:
-                    blk[ri, r - ci[s] * cd] = diag_val
-        self._values.copy_(self._to_dev_values(vals))
+        br, ri = np.divmod(rows, rd)
+        spans = [np.arange(rp[b], rp[b + 1]) for b in br]
+        pos = np.concatenate([((s * rd + r)[:, None] * cd + np.arange(cd)[None, :]).ravel()
+                              for s, r in zip(spans, ri)])
+        self._values.index_fill_(0, dev.to_device(pos), 0)
+        on_pattern = [r for r in rows if r < self.ncols and self._has_entry(r, r)]
+        self._write_entries(on_pattern, on_pattern, np.full(len(on_pattern), diag_val), add=False)
+
+    def _has_entry(self, r, c):
+        rp, ci = self._host_pattern()
+        rd, cd = self.block_shape
+        b, bc = r // rd, c // cd
+        lo, hi = rp[b], rp[b + 1]
+        s = lo + int(np.searchsorted(ci[lo:hi], bc))
+        return s < hi and ci[s] == bc
 
     def set_local_diagonal_entries(self, rows, diag_val=1.0, idx=None):
-        rows = np.asarray(rows, dtype=np.int64)
+        """Insert ``diag_val`` on the diagonal of the block rows ``rows``
+        (every component, or component ``idx``)."""
+        rows = np.asarray(rows, dtype=np.int64).ravel()
         rbs, _ = self.dims[0][0]
         if rbs > 1:
-            rows = rbs * rows + idx if idx is not None else np.dstack([rbs * rows + i for i in range(rbs)]).flatten()
-        if len(rows):
-            self._set_entries(rows, rows, np.full(len(rows), diag_val), add=False)
+            comps = [idx] if idx is not None else range(rbs)
+            rows = (rbs * rows[:, None] + np.asarray(list(comps))[None, :]).ravel()
+        self.change_assembly_state(Mat.INSERT_VALUES)
+        self._write_entries(rows, rows, np.full(len(rows), diag_val), add=False)
 
-    def addto_values(self, rows, cols, values):
-        """Add a dense block of values at (block) ``rows`` x ``cols``."""
-        self._blocked_entries(rows, cols, values, add=True)
-
-    def set_values(self, rows, cols, values):
-        """Set a dense block of values at (block) ``rows`` x ``cols``."""
-        self._blocked_entries(rows, cols, values, add=False)
-
-    def _blocked_entries(self, rows, cols, values, add):
+    def _dense_block_entries(self, rows, cols, values):
+        """The dof entries of the dense block ``values`` at block ``rows`` x
+        ``cols`` (MatSetValuesBlockedLocal's layout)."""
         rd, cd = self.dims[0][0]
         rows = np.atleast_1d(np.asarray(rows, dtype=np.int64))
         cols = np.atleast_1d(np.asarray(cols, dtype=np.int64))
         vals = np.asarray(values, dtype=self.dtype).reshape(len(rows) * rd, len(cols) * cd)
-        rr, cc, vv = [], [], []
-        for i, r in enumerate(rows):
-            for ri in range(rd):
-                for j, c in enumerate(cols):
-                    for cj in range(cd):
-                        rr.append(r * rd + ri)
-                        cc.append(c * cd + cj)
-                        vv.append(vals[i * rd + ri, j * cd + cj])
-        if vv:
-            self._set_entries(np.array(rr), np.array(cc), np.array(vv), add)
+        drows = (rows[:, None] * rd + np.arange(rd)[None, :]).ravel()
+        dcols = (cols[:, None] * cd + np.arange(cd)[None, :]).ravel()
+        rr, cc = np.meshgrid(drows, dcols, indexing="ij")
+        return rr.ravel(), cc.ravel(), vals.ravel()
 
-    def _set_entries(self, rows, cols, vals, add):
-        rp, ci, v = self._host_csr()
-        rd, cd = self.block_shape
-        for r, c, x in zip(rows, cols, vals):
-            br, ri = divmod(int(r), rd)
-            bc, cj = divmod(int(c), cd)
-            lo, hi = rp[br], rp[br + 1]
-            k = lo + np.searchsorted(ci[lo:hi], bc)
-            if k >= hi or ci[k] != bc:
-                raise ex.PatternError(f"entry ({r}, {c}) is not in the sparsity pattern")
-            if add:
-                v[k][ri, cj] += x
-            else:
-                v[k][ri, cj] = x
-        self._values.copy_(self._to_dev_values(v))
+    def addto_values(self, rows, cols, values):
+        """Add the dense block ``values`` at block ``rows`` x ``cols``."""
+        self.change_assembly_state(Mat.ADD_VALUES)
+        if len(np.atleast_1d(values)):
+            self._write_entries(*self._dense_block_entries(rows, cols, values), add=True)
 
-    def _to_dev_values(self, v):
-        from pyop2_amd import device as dev
-        return dev.to_device(np.ascontiguousarray(v).reshape(-1))[:self._values.numel()]
+    def set_values(self, rows, cols, values):
+        """Insert the dense block ``values`` at block ``rows`` x ``cols``."""
+        self.change_assembly_state(Mat.INSERT_VALUES)
+        if len(np.atleast_1d(values)):
+            self._write_entries(*self._dense_block_entries(rows, cols, values), add=False)
 
     def _host_csr(self):
         """Host ``(rowptr, colidx, values[nnz, rd, cd])`` copies."""

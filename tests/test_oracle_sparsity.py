@@ -108,3 +108,52 @@ def test_greedy_colouring_passes_of_32():
     maps = [np.zeros((40, 1), dtype=np.int32)]
     colours, nc = osp.greedy_colour(40, maps, [0], [1])
     assert nc == 40 and sorted(colours) == list(range(40))
+
+
+def _random_block_pattern(rng, diag):
+    nrows, ncols = (int(v) for v in rng.integers(1, 40, size=2))
+    ne = int(rng.integers(0, 30))
+    ra, ca = (int(v) for v in rng.integers(1, 4, size=2))
+    rmap = rng.integers(-1, nrows, size=(ne, ra)).astype(np.int32)
+    cmap = rng.integers(-1, ncols, size=(ne, ca)).astype(np.int32)
+    rp, ci = osp.pattern(nrows, ncols, [(rmap, cmap, ne)], set_diag=diag)
+    return nrows, ncols, rp, ci
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("rdim,cdim", [(1, 2), (2, 1), (2, 3), (3, 3), (1, 1)])
+def test_expand_block_pattern_is_the_kronecker_pattern(seed, rdim, cdim):
+    """The dof pattern of rdim x cdim blocks (sparsity.pyx:168-173: every
+    block entry of every block) is the pattern of kron(B, ones(rdim, cdim))
+    computed by scipy, on random block patterns with empty rows; its row
+    lengths are dof_nnz's unwinding."""
+    rng = np.random.default_rng([2026, 98, seed])
+    nrows, ncols, rp, ci = _random_block_pattern(rng, diag=bool(seed % 2))
+    B = ss.csr_matrix((np.ones(len(ci)), ci, rp), shape=(nrows, ncols))
+    D = ss.kron(B, np.ones((rdim, cdim))).tocsr()
+    D.sort_indices()
+    erp, eci = osp.expand_block_pattern(rp, ci, rdim, cdim)
+    assert np.array_equal(erp, D.indptr) and np.array_equal(eci, D.indices)
+    assert np.array_equal(np.diff(erp), osp.dof_nnz(rp, rdim, cdim, block_sparse=False))
+
+
+def test_expand_block_pattern_catches_transposed_block_shape():
+    """Planted mistake: the block shape given as (cdim, rdim) yields another
+    pattern than the Kronecker one."""
+    rng = np.random.default_rng([2026, 97])
+    nrows, ncols, rp, ci = _random_block_pattern(rng, diag=True)
+    B = ss.csr_matrix((np.ones(len(ci)), ci, rp), shape=(nrows, ncols))
+    D = ss.kron(B, np.ones((1, 2))).tocsr()
+    D.sort_indices()
+    erp, eci = osp.expand_block_pattern(rp, ci, 2, 1)
+    assert not (np.array_equal(erp, D.indptr) and np.array_equal(eci, D.indices))
+
+
+def test_expand_block_pattern_reference_counts():
+    """test_matrices.py:555-566: nnz [1, 2, 1, 2] at block level for the
+    diagonal-augmented pattern, and its dof unwinding for 2x2 blocks."""
+    m = np.array([[1, 3]], dtype=np.int32)
+    rp, ci = osp.pattern(4, 4, [(m, m, 1)], set_diag=True)
+    assert list(np.diff(rp)) == [1, 2, 1, 2]
+    erp, _ = osp.expand_block_pattern(rp, ci, 2, 2)
+    assert list(np.diff(erp)) == [2, 2, 4, 4, 2, 2, 4, 4]
