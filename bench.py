@@ -282,7 +282,9 @@ def c3_headline(args):
     kms = kernel_ms(timer, name)
     parts = {k: kernel_ms(timer, k) * 1e3 for k in ("p2_elasticity_matrix", "p2_elasticity_residual")
              if timer.times(k)}
-    strategy = pl.global_kernel.compile().strategy_used
+    executor = pl.global_kernel.compile()
+    strategy = executor.strategy_used
+    launch_config = getattr(executor, "last_config", None)
     ms_per_step = wall * 1e3 / args.steps
     achieved = alg / (kms * 1e-3) / 1e9
     log(f"C3 fused f64: {ms_per_step:.3f} ms/step, kernel {kms * 1e3:.1f} us ({strategy}; parts {parts}), "
@@ -312,7 +314,7 @@ def c3_headline(args):
             "traffic_source": tr_meas[1] if tr_meas else None, "traffic_key": tkey,
             "kernel": f"p2_elasticity_fused ({strategy}): " + ", ".join(f"{k} {v:.1f} us" for k, v in parts.items()),
             "kernel_us": kms * 1e3, "alg_bytes_per_launch": alg}
-    detail = dict(cells=m3.ncells, block_nnz=objs["nnzb"], kernel_us=kms * 1e3, kernel_parts_us=parts,
+    detail = dict(cells=m3.ncells, kernel_us=kms * 1e3, kernel_parts_us=parts, launch_config=launch_config,
                   ms_per_step=ms_per_step, strategy=strategy, alg_bytes=alg, gbs=achieved,
                   frac=achieved / HBM_PEAK_GBS, **parity)
     if cpu:
@@ -446,6 +448,7 @@ def secondary(args, m3=None):
             ms = time_loop(pl, reps, name)
             k = f"c3_{key}_{tag}"
             res[k] = dict(cells=m3.ncells, block_nnz=objs["nnzb"], kernel_us=ms * 1e3,
+                          launch_config=getattr(pl.global_kernel.compile(), "last_config", None),
                           cells_per_s=m3.ncells / (ms * 1e-3), gbs=alg / (ms * 1e-3) / 1e9,
                           frac=alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, alg_bytes=alg,
                           strategy=pl.global_kernel.compile().strategy_used)
@@ -660,7 +663,7 @@ def run(args):
                                f"(INC of 30x30 element matrices into a float64 Mat and of element residuals "
                                f"into a vector Dat), {n3}^3 unit cube x 6 tets ({d['cells']} cells, "
                                f"{d['block_nnz']} 3x3 blocks), lambda=1, mu=0.5",
-                   "strategy": d["strategy"], "device": gpu_name},
+                   "strategy": d["strategy"], "launch_config": d.get("launch_config"), "device": gpu_name},
         "roofline": head["roofline"],
         "cpu_baseline": head["cpu_baseline"],
         "secondary": out,

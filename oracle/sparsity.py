@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-__all__ = ["pattern", "dof_nnz", "expand_block_pattern", "greedy_colour"]
+__all__ = ["pattern", "dof_nnz", "expand_block_pattern", "monolithic_pattern", "greedy_colour"]
 
 
 def pattern(nrows, ncols, pairs, set_diag=True):
@@ -72,6 +72,38 @@ def expand_block_pattern(rowptr, colidx, rdim, cdim):
     np.cumsum(np.array(rows, dtype=np.int64), out=rp[1:])
     ci = np.concatenate(cols) if cols else np.zeros(0, dtype=np.int64)
     return rp.astype(np.int32), ci.astype(np.int32)
+
+
+def monolithic_pattern(blocks, rdims, cdims, rsizes, csizes):
+    """Scalar pattern of a monolithic mixed matrix (``sparsity.pyx:141-160``:
+    one dof-level preallocator, each block (i, j) filled through the local
+    submatrix of rows of component i and columns of component j with
+    rdims[i] x cdims[j] blocks).  ``blocks[i][j]`` is the block-level
+    ``(rowptr, colidx)`` of block (i, j) (diagonal included where the
+    reference sets it), or None for an empty block; component i's dofs start
+    at the sum of rsizes[q] * rdims[q] over q < i (``dataset.py:166-180``),
+    likewise for the columns.  Row R of the result is, column component by
+    column component, the unwound row of each block shifted by that
+    component's offset."""
+    nr, nc = len(rdims), len(cdims)
+    coff = np.concatenate([[0], np.cumsum([csizes[j] * cdims[j] for j in range(nc)])]).astype(np.int64)
+    rows = []
+    for i in range(nr):
+        expanded = []
+        for j in range(nc):
+            if blocks[i][j] is None:
+                rp = np.zeros(rsizes[i] + 1, dtype=np.int32)
+                ci = np.zeros(0, dtype=np.int32)
+            else:
+                rp, ci = blocks[i][j]
+            erp, eci = expand_block_pattern(rp, ci, rdims[i], cdims[j])
+            expanded.append((erp.astype(np.int64), eci.astype(np.int64) + coff[j]))
+        for r in range(rsizes[i] * rdims[i]):
+            rows.append(np.concatenate([eci[erp[r]:erp[r + 1]] for erp, eci in expanded]))
+    rowptr = np.zeros(len(rows) + 1, dtype=np.int64)
+    np.cumsum([len(r) for r in rows], out=rowptr[1:])
+    colidx = np.concatenate(rows) if rows else np.zeros(0, dtype=np.int64)
+    return rowptr.astype(np.int32), colidx.astype(np.int32)
 
 
 _LIB = None

@@ -1,16 +1,34 @@
-"""Autotuner of the conflict-resolution strategy of hand-written loops.
+"""Autotuner of the hand-written loops' launch configuration.
 
-On the first launch of a (kernel family, precision, iteration set, maps)
-combination with ``configuration["strategy"] == "auto"``, every applicable
-strategy is timed with HIP events on scratch copies of the written
-arguments (so the user's data is untouched), and the fastest is kept in
-memory and in ``<cache_dir>/autotune.json``.  With ``autotune`` disabled the
-first candidate is used: ``rowgather_slots`` for Mat assembly, ``staged``
-for Dat INC over a Set.
+On the first launch of a loop with ``configuration["strategy"] == "auto"``
+the tuner times candidate configurations with HIP events on scratch copies
+of the written arguments (so the user's data is untouched) and keeps the
+fastest, in memory and in ``<cache_dir>/autotune.json``.  A configuration
+is the conflict-resolution strategy plus its launch parameters:
+
+* staged tiles: elements per tile (``tile``), the LDS budget a tile's
+  staged values and accumulators must fit (``lds``, which caps the nodes of
+  a tile and so sets how many tiles share a CU) and the workgroup size
+  (``block``) -- the reference's plan sizes its mini-partitions to the
+  shared memory the same way (``doc/sphinx/source/plan.rst:13-19``);
+* row gathers: the batch budget (``batch`` = most row incidences, most
+  blocks of a batch, which sets its LDS) and the workgroup size;
+* split loops: each part's configuration, tuned as a loop of its own.
+
+Tuning runs in two stages: every applicable strategy with its default
+parameters, then the parameter candidates of the fastest strategy.  The key
+of a decision is the loop's kernel family and precision, the iteration
+set's sizes and a digest of the contents of every map the loop reads, so
+two meshes share a decision only when their connectivity is the same.
+With ``autotune`` off the first candidate strategy is used with its default
+parameters.
 """
+import hashlib
 import json
 import os
 import threading
+
+import numpy as np
 
 from pyop2_amd import device as dev
 from pyop2_amd.configuration import configuration
@@ -21,6 +39,27 @@ _memo = {}
 _loaded = False
 
 REPEATS = 3
+
+# parameter candidates per strategy (the first is the strategy's default)
+STAGED_PARAMS = [
+    {"tile": 256, "lds": 65536, "block": 256},
+    {"tile": 256, "lds": 40960, "block": 256},
+    {"tile": 384, "lds": 65536, "block": 256},
+    {"tile": 384, "lds": 40960, "block": 192},
+    {"tile": 512, "lds": 54272, "block": 256},
+    {"tile": 512, "lds": 65536, "block": 256},
+]
+ROWGATHER_PARAMS = {
+    # P2 BSR(3) float64 / float32: (incidences, blocks) of a batch
+    "p2": [{"batch": None, "block": 256},
+           {"batch": [384, 480], "block": 192},
+           {"batch": [512, 640], "block": 192},
+           {"batch": [768, 880], "block": 256}],
+    # P1 CSR
+    "p1": [{"batch": None, "block": 256},
+           {"batch": [512, 1024], "block": 128},
+           {"batch": [2048, 3584], "block": 256}],
+}
 
 
 def _path():
@@ -51,6 +90,7 @@ def _save():
 
 
 def candidates(executor, parloop):
+    """Strategies the loop can run, the default first."""
     has_mat = executor.spec["args"][0][0] == "mat"
     if has_mat:
         if isinstance(parloop.iterset, Subset):
@@ -63,33 +103,62 @@ def candidates(executor, parloop):
     return ["atomic", "coloured"]
 
 
+def param_candidates(executor, strategy):
+    """Parameter sets of ``strategy`` for this family (the default first)."""
+    if strategy == "staged":
+        return STAGED_PARAMS
+    if strategy in ("rowgather_slots", "rowgather_rows"):
+        return ROWGATHER_PARAMS["p2" if executor.family.startswith("p2_") else "p1"]
+    return [{}]
+
+
 def default(executor, parloop):
-    return candidates(executor, parloop)[0]
+    return {"strategy": candidates(executor, parloop)[0]}
+
+
+def map_digest(m):
+    """Digest of a map's contents (cached on the map: maps are immutable
+    once built)."""
+    d = m._cache.get("content-digest")
+    if d is None:
+        t = np.ascontiguousarray(m._table)
+        h = hashlib.blake2b(digest_size=16)
+        h.update(np.asarray(t.shape, dtype=np.int64).tobytes())
+        h.update(t.tobytes())
+        d = m._cache["content-digest"] = h.hexdigest()
+    return d
 
 
 def _key(executor, parloop):
     it = parloop.iterset
-    maps = []
-    for a in parloop.arguments:
-        for m in a.maps:
-            maps.append((m.name, m.arity, m.toset.total_size))
-    return json.dumps([executor.family, executor.variant, type(it).__name__, it.size, it.core_size, maps])
+    maps = [map_digest(m) for a in parloop.arguments for m in a.maps]
+    sub = it._plan_key()[-1] if isinstance(it, Subset) else None
+    return json.dumps([executor.family, executor.variant, type(it).__name__, it.core_size, it.size,
+                       it.total_size, sub, maps])
 
 
 def choice(executor, parloop):
+    """The loop's tuned configuration (tuned now on a first encounter)."""
     if not configuration["autotune"]:
         return default(executor, parloop)
     key = _key(executor, parloop)
     with _lock:
         _load()
         hit = _memo.get(key)
-    if hit is not None:
+    if isinstance(hit, dict):
         return hit
     best = tune(executor, parloop)
     with _lock:
         _memo[key] = best
         _save()
     return best
+
+
+def decisions():
+    """Every decision held (key -> configuration), for reports."""
+    with _lock:
+        _load()
+        return dict(_memo)
 
 
 def cached(key):
@@ -105,8 +174,7 @@ def store(key, value):
         _save()
 
 
-def tune(executor, parloop, repeats=REPEATS, report=None):
-    """Time each candidate on scratch outputs; return the fastest."""
+def _scratch_outputs(executor, parloop):
     torch = dev.torch()
     outputs = {}
     for k, ((kind, access, _, _), arg) in enumerate(zip(executor.spec["args"], parloop.arguments)):
@@ -115,19 +183,79 @@ def tune(executor, parloop, repeats=REPEATS, report=None):
                 outputs["mat"] = torch.zeros_like(arg.data.device_values)
             else:
                 outputs[k] = torch.zeros_like(arg.data._device_array())
+    return outputs
+
+
+def _time(executor, parloop, cfg, outputs, repeats):
+    """Mean time (ms) of ``repeats`` launches of ``cfg`` (after a warm-up
+    that builds its plans); None if the configuration cannot run."""
+    from pyop2_amd.exceptions import DeviceError
+    torch = dev.torch()
     start, end = 0, parloop.iterset.size
-    times = {}
     s = torch.cuda.current_stream()
-    for cand in candidates(executor, parloop):
-        executor.launch(start, end, parloop, cand, outputs)   # warm-up, builds plans
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(s)
-        for _ in range(repeats):
-            executor.launch(start, end, parloop, cand, outputs)
-        e1.record(s)
-        e1.synchronize()
-        times[cand] = e0.elapsed_time(e1) / repeats
+    try:
+        executor.launch(start, end, parloop, cfg, outputs)
+    except DeviceError:
+        return None
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(repeats):
+        executor.launch(start, end, parloop, cfg, outputs)
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / repeats
+
+
+def _tune_split(executor, parloop, outputs, repeats, report):
+    """Each part of a split loop tuned as a loop of its own (strategy fixed
+    to the part's kind, parameters tuned), on the split loop's scratch
+    outputs."""
+    from pyop2_amd.registry import _SubLoop
+    parts = []
+    for sub, idx in executor._split_parts():
+        part = _SubLoop(parloop.iterset, [parloop.arguments[k] for k in idx])
+        sub_out = {}
+        for n, k in enumerate(idx):
+            if k in outputs:
+                sub_out[n] = outputs[k]
+        if "mat" in outputs:
+            sub_out["mat"] = outputs["mat"]
+        strat = "rowgather_slots" if sub.spec["args"][0][0] == "mat" else "staged"
+        strat = sub.applicable(part, strat)
+        best, best_t = {"strategy": strat}, None
+        for params in param_candidates(sub, strat):
+            cfg = dict(params, strategy=strat)
+            t = _time(sub, part, cfg, sub_out, repeats)
+            if report is not None:
+                report[f"{sub.family}:{json.dumps(cfg, sort_keys=True)}"] = t
+            if t is not None and (best_t is None or t < best_t):
+                best, best_t = cfg, t
+        parts.append(best)
+    return {"strategy": "split", "parts": parts}
+
+
+def tune(executor, parloop, repeats=REPEATS, report=None):
+    """Time the candidates on scratch outputs; return the fastest
+    configuration."""
+    outputs = _scratch_outputs(executor, parloop)
+    times = {}
+    for strat in candidates(executor, parloop):
+        t = _time(executor, parloop, {"strategy": strat}, outputs, repeats)
+        if t is not None:
+            times[strat] = t
     if report is not None:
         report.update(times)
-    return min(times, key=times.get)
+    strat = min(times, key=times.get)
+    if strat == "split":
+        best = _tune_split(executor, parloop, outputs, repeats, report)
+        return best
+    best, best_t = {"strategy": strat}, times[strat]
+    for params in param_candidates(executor, strat)[1:]:
+        cfg = dict(params, strategy=strat)
+        t = _time(executor, parloop, cfg, outputs, repeats)
+        if report is not None:
+            report[json.dumps(cfg, sort_keys=True)] = t
+        if t is not None and t < best_t:
+            best, best_t = cfg, t
+    return best

@@ -358,7 +358,11 @@ def generate_source(gk, map_index, block_flags=None):
                     params += [f"const int32_t *__restrict__ rowptr{k}_{b}",
                                f"const int32_t *__restrict__ colidx{k}_{b}", f"{t} *__restrict__ arg{k}_{b}"]
                     val = f"buf{k}[({roff[i]} + ii * {rd} + ri) * {NC} + {coff[j]} + jj * {cd} + cj]"
-                    if flags[b]:
+                    # flags[b]: True = BSR block Mat; False = scalar CSR;
+                    # ("offset", r0, c0) = block of a monolithic Mat, rows and
+                    # columns shifted by its offsets in the parent's CSR
+                    shift = flags[b] if isinstance(flags[b], tuple) else ("offset", 0, 0)
+                    if flags[b] is True:
                         tgt = f"arg{k}_{b}[(int64_t)s_ * {rd * cd} + ri * {cd} + cj]"
                         post.append(
                             f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar // nf, 'ii')}; if (r_ < 0) continue;\n"
@@ -370,7 +374,7 @@ def generate_source(gk, map_index, block_flags=None):
                         post.append(
                             f"    for (int ii = 0; ii < {rar}; ++ii) {{ const int32_t r_ = {_mv(rm, rar // nf, 'ii')}; if (r_ < 0) continue;\n"
                             f"      for (int ri = 0; ri < {rd}; ++ri) for (int jj = 0; jj < {car}; ++jj) {{ const int32_t c_ = {_mv(cm, car // nf, 'jj')}; if (c_ < 0) continue;\n"
-                            f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, r_ * {rd} + ri, c_ * {cd} + cj);"
+                            f"        for (int cj = 0; cj < {cd}; ++cj) {{ const int32_t s_ = op2_find_slot(rowptr{k}_{b}, colidx{k}_{b}, {shift[1]} + r_ * {rd} + ri, {shift[2]} + c_ * {cd} + cj);"
                             f" if (s_ < 0) {{ atomicAdd(status, 1); continue; }} {store.format(tgt=tgt, val=val)} }} }} }}")
         elif isinstance(garg, MatKernelArg):
             rd, cd = garg.dims[0][0]
@@ -532,7 +536,7 @@ class GeneratedKernel:
             if isinstance(garg, MatKernelArg):
                 out.append((k, a.data._block))
             elif isinstance(garg, MixedMatKernelArg):
-                out.append((k, tuple(b._block for b in a.data)))
+                out.append((k, tuple(getattr(b, "_gen_flag", b._block) for b in a.data)))
         return tuple(out)
 
     def source(self, parloop):

@@ -163,21 +163,130 @@ __device__ __forceinline__ void residual_q(const T (&uu)[10][3], const Geom<T> &
   test_gradients<T, Q>(s, g, re);
 }
 
+
+// Vertex-stress form of the element residual.  grad u is linear on the
+// element, so the stress is linear too: s(L) = sum_n L_n s_n with s_n the
+// stress of grad u at vertex n,
+//   grad u(vertex n) = 4 (u_n (x) gl_n + sum_{b != n} u_nb (x) gl_b) - P,
+//   P = sum_j u_j (x) gl_j  (vertices j).
+// The 4-point rule integrates the degree-2 integrand exactly, and its
+// moments sum_q L_qn L_qm = 0.2 (1 + [n == m]) turn the sum over the points
+// into sums over vertices (S = sum_n s_n):
+//   vertex i:     R_i  = 0.2 wdet (4 s_i - S) gl_i,
+//   edge (a, b):  R_ab = 0.8 wdet ((S + s_b) gl_a + (S + s_a) gl_b).
+// About 1000 flops against 1600 of the point-by-point form, and no array of
+// the element's 30 values lives across the whole evaluation: u(j, c) reads
+// node j's component c on demand, out(i, v) takes node i's three values.
+// Symmetric stresses are kept as (xx, yy, zz, xy, xz, yz).
+template <typename T>
+__device__ __forceinline__ void sym_stress(const T (&G)[3][3], T lam, T mu, T (&s)[6]) {
+  const T lt = lam * (G[0][0] + G[1][1] + G[2][2]);
+  const T m2 = mu + mu;
+  s[0] = m2 * G[0][0] + lt;
+  s[1] = m2 * G[1][1] + lt;
+  s[2] = m2 * G[2][2] + lt;
+  s[3] = mu * (G[0][1] + G[1][0]);
+  s[4] = mu * (G[0][2] + G[2][0]);
+  s[5] = mu * (G[1][2] + G[2][1]);
+}
+
+template <typename T>
+__device__ __forceinline__ void sym_apply(const T (&s)[6], const T (&v)[3], T (&o)[3]) {
+  o[0] = s[0] * v[0] + s[3] * v[1] + s[4] * v[2];
+  o[1] = s[3] * v[0] + s[1] * v[1] + s[5] * v[2];
+  o[2] = s[4] * v[0] + s[5] * v[1] + s[2] * v[2];
+}
+
+// the local node of the edge joining vertices n and b (n != b)
+constexpr int kEdgeOf[4][4] = {{-1, 4, 5, 6}, {4, -1, 7, 8}, {5, 7, -1, 9}, {6, 8, 9, -1}};
+
+template <typename T, class U, class Out>
+__device__ __forceinline__ void residual_vertex_form(const Geom<T> &g, T lam, T mu, U &&u, Out &&out) {
+  // pass 1: the stress of P = sum_j u_j (x) gl_j
+  T sP[6];
+  {
+    T P[3][3] = {};
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const T v0 = u(n, 0), v1 = u(n, 1), v2 = u(n, 2);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        P[0][k] += v0 * g.gl[n][k];
+        P[1][k] += v1 * g.gl[n][k];
+        P[2][k] += v2 * g.gl[n][k];
+      }
+    }
+    sym_stress(P, lam, mu, sP);
+  }
+  // pass 2: the vertex stresses s_n = 4 s(u_n (x) gl_n + sum_b u_nb (x) gl_b) - sP
+  T sv[4][6];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    T G[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const T un = u(n, a);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) G[a][k] = un * g.gl[n][k];
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (b == n) continue;
+      const int e = kEdgeOf[n][b];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const T ue = u(e, a);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[a][k] += ue * g.gl[b][k];
+      }
+    }
+    sym_stress(G, T(4) * lam, T(4) * mu, sv[n]);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sv[n][c] -= sP[c];
+    // memory clobber: each vertex's reads stay in its own iteration (not
+    // hoisted and merged across iterations, which keeps them all live)
+    }
+  // pass 3: the ten increments, S = sum_n s_n
+  T S[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) S[c] = sv[0][c] + sv[1][c] + sv[2][c] + sv[3][c];
+  const T cv = T(0.2) * g.wdet, ce = T(0.8) * g.wdet;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    T M[6], r[3];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) M[c] = cv * (T(4) * sv[i][c] - S[c]);
+    sym_apply(M, g.gl[i], r);
+    out(i, r);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a = kEa[k], b = kEb[k];
+    T Ma[6], Mb[6], ra[3], rb[3], r[3];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      Ma[c] = ce * (S[c] + sv[b][c]);
+      Mb[c] = ce * (S[c] + sv[a][c]);
+    }
+    sym_apply(Ma, g.gl[a], ra);
+    sym_apply(Mb, g.gl[b], rb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = ra[c] + rb[c];
+    out(4 + k, r);
+  }
+}
+
+// The element residual in the vertex-stress form (about 1000 flops; the
+// point-by-point form residual_q gives the same values to rounding).
 template <typename T>
 __device__ __forceinline__ void element_residual(const T (&uu)[10][3], const Geom<T> &g, T lam,
                                                  T mu, T (&re)[10][3]) {
+  residual_vertex_form(
+      g, lam, mu, [&](int j, int c) { return uu[j][c]; },
+      [&](int i, const T (&r)[3]) {
 #pragma unroll
-  for (int i = 0; i < 10; ++i)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) re[i][a] = T(0);
-  residual_q<T, 0>(uu, g, lam, mu, re);
-  residual_q<T, 1>(uu, g, lam, mu, re);
-  residual_q<T, 2>(uu, g, lam, mu, re);
-  residual_q<T, 3>(uu, g, lam, mu, re);
-#pragma unroll
-  for (int i = 0; i < 10; ++i)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) re[i][a] *= g.wdet;
+        for (int c = 0; c < 3; ++c) re[i][c] = r[c];
+      });
 }
 
 template <typename T, bool kAtomic>

@@ -166,6 +166,19 @@ staged_reduce_kernel(int32_t nshared, const int32_t *__restrict__ nodes,
   for (int a = 0; a < kOut; ++a) o[a] = T(double(o[a]) + sum[a]);
 }
 
+// Elements that read their inputs from the tile's LDS and add their
+// increments into it themselves (Elem::kLdsElement) provide
+//   void element_lds(int32_t e, const T *in0, const uint16_t (&l0)[A0],
+//                    const T *in1, const uint16_t (&l1)[A1], double *acc0, double *acc1) const;
+template <class Elem, class = void>
+struct LdsElementOf {
+  static constexpr bool value = false;
+};
+template <class Elem>
+struct LdsElementOf<Elem, std::void_t<decltype(Elem::kLdsElement)>> {
+  static constexpr bool value = Elem::kLdsElement;
+};
+
 // Tile b of a staged launch (the workgroup's share, or one work item of a
 // batched launch).  The local node ids of a thread's first element are
 // loaded before the staging barrier.
@@ -204,6 +217,10 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
 
   for (int32_t t = threadIdx.x; t < ne; t += blockDim.x) {
     if (t != (int32_t)threadIdx.x) load_locals(p0 + t);
+    if constexpr (LdsElementOf<Elem>::value) {
+      elem.element_lds(elem_id, in0, l0, in1, l1, acc0, acc1);
+      continue;
+    }
     T x0[A0][I0], x1[A1][I1], y0[A0][O0], y1[A1][O1];
 #pragma unroll
     for (int i = 0; i < A0; ++i)
@@ -234,8 +251,19 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
   if constexpr (Elem::kMaps > 1) flush_map<Elem, 1>(elem, P, b, acc1, ids1);
 }
 
+// Waves per SIMD an element asks the compiler to fit (Elem::kWavesPerEU:
+// the register budget follows, 512 / waves), default 1 (no constraint).
+template <class Elem, class = void>
+struct WavesPerEUOf {
+  static constexpr int value = 1;
+};
 template <class Elem>
-__global__ void __launch_bounds__(256)
+struct WavesPerEUOf<Elem, std::void_t<decltype(Elem::kWavesPerEU)>> {
+  static constexpr int value = Elem::kWavesPerEU;
+};
+
+template <class Elem>
+__global__ void __launch_bounds__(256, WavesPerEUOf<Elem>::value)
 staged_kernel(Elem elem, TilePlan P) {
   staged_tile(elem, P, xcd_contiguous(blockIdx.x, P.ntiles));
 }

@@ -196,39 +196,49 @@ class HandWrittenKernel:
                 return False
         return True
 
-    def _choose(self, parloop):
+    def applicable(self, parloop, strategy):
+        """``strategy`` if this loop can run it, else the fallback it takes."""
         has_mat = any(s[0] == "mat" for s in self.spec["args"])
         subset = isinstance(parloop.iterset, Subset)
+        if strategy.startswith("rowgather") and (not has_mat or subset):
+            return "atomic"
+        if strategy == "staged" and (subset or "staged" not in self.spec):
+            return "atomic"
+        if strategy == "split" and "split" not in self.spec:
+            return "atomic"
+        if has_mat and strategy.startswith("rowgather"):
+            mat = parloop.arguments[0].data
+            if mat.block_shape != self.spec["args"][0][2]:
+                return "atomic"
+        return strategy
+
+    def _choose(self, parloop):
+        """The launch configuration: ``{"strategy": ..., parameters}`` (the
+        autotuner's choice under ``auto``)."""
         want = configuration["strategy"]
         if want == "auto":
             from pyop2_amd import autotune
-            want = autotune.choice(self, parloop)
-        if want.startswith("rowgather") and (not has_mat or subset):
-            want = "atomic"
-        if want == "staged" and (subset or "staged" not in self.spec):
-            want = "atomic"
-        if want == "split" and "split" not in self.spec:
-            want = "atomic"
-        if has_mat and want.startswith("rowgather"):
-            mat = parloop.arguments[0].data
-            if mat.block_shape != self.spec["args"][0][2]:
-                want = "atomic"
-        return want
+            cfg = dict(autotune.choice(self, parloop))
+        else:
+            cfg = {"strategy": want}
+        cfg["strategy"] = self.applicable(parloop, cfg["strategy"])
+        return cfg
 
-    def _tile_plan(self, args, start, end):
-        """Tile plan whose LDS fits a workgroup: the node cap of map 0 shrinks
-        until it does (None if even small tiles do not fit)."""
+    def _tile_plan(self, args, start, end, tile=_TILE, lds_budget=_LDS):
+        """Tile plan of ``tile`` elements whose LDS fits ``lds_budget``: the
+        node cap of map 0 shrinks until it does (None if even small tiles do
+        not fit)."""
         from pyop2_amd import plans
         maps = [args[k].maps[0] for k in self.spec["staged"]]
         isz = 8 if self.variant == "f64" else 4
         nin, nout = self.spec["staged_vals"][0]
         cap = 0
-        for _ in range(6):
-            tp = plans.tile_plan(maps, start, end, _TILE, cap)
-            if _tile_lds(self.spec, isz, tp.max_nodes) <= _LDS:
+        for _ in range(8):
+            tp = plans.tile_plan(maps, start, end, tile, cap)
+            if _tile_lds(self.spec, isz, tp.max_nodes) <= lds_budget:
                 return tp
             per0 = nin * isz + nout * 8
-            cap = max(maps[0].arity, int(0.85 * min(tp.max_nodes[0], _LDS // per0)))
+            cap = max(maps[0].arity, int(0.9 * min(tp.max_nodes[0], lds_budget // per0)))
         return None
 
     def _inc_maps(self, parloop):
@@ -247,6 +257,11 @@ class HandWrittenKernel:
             return GeneratedKernel(self.gk)(start, end, ptrs, parloop)
         self.launch(start, end, parloop, self._choose(parloop))
 
+    @property
+    def last_config(self):
+        """The configuration of the last launch (strategy and parameters)."""
+        return getattr(self, "_last_config", None)
+
     def _split_parts(self):
         parts = getattr(self, "_parts", None)
         if parts is None:
@@ -254,11 +269,13 @@ class HandWrittenKernel:
                                    for fam, idx in self.spec["split"]]
         return parts
 
-    def _launch_split(self, start, end, parloop, outputs):
+    def _launch_split(self, start, end, parloop, outputs, cfg=None):
         """The loop as its split parts: the assembly with its best row gather,
         the residual with LDS-staged tiles (one element pass each, no
-        redundant gathers of the solution per row incidence)."""
-        for sub, idx in self._split_parts():
+        redundant gathers of the solution per row incidence); ``cfg["parts"]``
+        holds each part's launch configuration."""
+        parts_cfg = (cfg or {}).get("parts") or [None] * len(self.spec["split"])
+        for (sub, idx), pcfg in zip(self._split_parts(), parts_cfg):
             part = _SubLoop(parloop.iterset, [parloop.arguments[k] for k in idx])
             sub_out = None
             if outputs:
@@ -268,24 +285,29 @@ class HandWrittenKernel:
                         sub_out[n] = outputs[k]
                 if "mat" in outputs:
                     sub_out["mat"] = outputs["mat"]
-            if sub.spec["args"][0][0] == "mat":
-                same = part.arguments[0].data.block_shape == sub.spec["args"][0][2]
-                strat = "rowgather_slots" if same else "atomic"
-            else:
-                strat = "staged"
-            if isinstance(parloop.iterset, Subset):
-                strat = "atomic"
-            sub.launch(start, end, part, strat, sub_out)
+            if pcfg is None:
+                pcfg = {"strategy": "rowgather_slots" if sub.spec["args"][0][0] == "mat" else "staged"}
+            pcfg = dict(pcfg, strategy=sub.applicable(part, pcfg["strategy"]))
+            sub.launch(start, end, part, pcfg, sub_out)
         self.strategy_used = "split"
 
-    def launch(self, start, end, parloop, strategy, outputs=None):
-        """Launch with ``strategy``; ``outputs`` optionally replaces the
-        written arguments' device pointers (autotuning on scratch copies)."""
+    def launch(self, start, end, parloop, cfg, outputs=None):
+        """Launch with the configuration ``cfg`` (a strategy name, or a dict
+        of the strategy and its parameters: ``tile`` / ``lds`` / ``block``
+        for staged tiles, ``batch`` (max incidences, max blocks) / ``block``
+        for row gathers, ``parts`` for a split loop); ``outputs`` optionally
+        replaces the written arguments' device pointers (autotuning on scratch
+        copies)."""
+        if isinstance(cfg, str):
+            cfg = {"strategy": cfg}
+        strategy = cfg["strategy"]
+        self._last_config = cfg
         if strategy == "split":
             with dev.timed_launch(self.family):
-                return self._launch_split(start, end, parloop, outputs)
+                return self._launch_split(start, end, parloop, outputs, cfg)
         ex = dev.ExecPlan()
         ex.strategy = dev.STRATEGIES[strategy]
+        ex.block_size = int(cfg.get("block") or 0)
         keep = []
         iterset = parloop.iterset
         if isinstance(iterset, Subset):
@@ -305,7 +327,7 @@ class HandWrittenKernel:
             plan.exec_fields(ex)
             keep.append(plan)
         if strategy == "staged":
-            tp = self._tile_plan(args, start, end)
+            tp = self._tile_plan(args, start, end, int(cfg.get("tile") or _TILE), int(cfg.get("lds") or _LDS))
             if tp is None:
                 strategy = "atomic"
                 ex.strategy = dev.STRATEGIES[strategy]
@@ -319,7 +341,7 @@ class HandWrittenKernel:
             if strategy == "rowgather_rows":
                 from pyop2_amd import plans
                 tr = plans.transpose(rmap, mat.sparsity.nrows, rmap.iterset.size)
-                budget = _BATCH[(self.family, self.variant)]
+                budget = tuple(cfg.get("batch") or _BATCH[(self.family, self.variant)])
                 rl = plans.row_list(rmap, args[0].maps[1], tr, mat, budget[0], budget[1],
                                     (id(mat.sparsity), mat.block_shape, self.variant))
                 rl.exec_fields(ex)
@@ -338,7 +360,7 @@ class HandWrittenKernel:
                         # a row with more incidences than one batch can stage
                         rb, strategy = None, "rowgather_slots"
                 if rb is None:
-                    budget = _BATCH[(self.family, self.variant)]
+                    budget = tuple(cfg.get("batch") or _BATCH[(self.family, self.variant)])
                     rb = plans.row_batches(rmap, tr, mat.rowptr, budget[0], budget[1], key)
                 rb.exec_fields(ex)
                 keep.append(rb)

@@ -27,7 +27,7 @@ from pyop2_amd.types import region as reg
 from pyop2_amd.types.map import Map
 from pyop2_amd.types.region import ALL
 from pyop2_amd.types.mixed import MixedDataSet, MixedMap
-from pyop2_amd.types.set import MixedSet, Subset
+from pyop2_amd.types.set import MixedSet, Set, Subset
 
 
 def _cells_map(m, region=None):
@@ -46,6 +46,22 @@ def _cells_map(m, region=None):
         f = Map(Set(len(tab)), m.toset, reg.facet_factor(region) * m.arity, tab)
         m._cache[key] = f
     return f
+
+
+def _dof_space(n):
+    """A plain Set of ``n`` dofs (the target of a dof-level map)."""
+    from pyop2_amd.types.set import Set
+    return Set(int(n))
+
+
+def _dof_map(m, dim, offset):
+    """``m`` unwound to dofs: entry (e, k * dim + a) = offset + m[e][k] * dim
+    + a, or -1 where m[e][k] is negative (no target)."""
+    t = m._table.astype(np.int64)
+    d = offset + t[:, :, None] * dim + np.arange(dim)[None, None, :]
+    d = np.where(t[:, :, None] >= 0, d, -1).reshape(t.shape[0], -1)
+    return Map(m.iterset, _dof_space(offset + m.toset.total_size * dim), m.arity * dim,
+               np.ascontiguousarray(d, dtype=np.int32))
 
 
 class Sparsity(caching.ObjectCached):
@@ -70,14 +86,21 @@ class Sparsity(caching.ObjectCached):
         self._pattern = None
         self._dof_pattern = None
         if isinstance(rset, MixedDataSet):
-            # one block Sparsity per (row, column) component (mat.py:86-114)
-            self._nested = True
-            self._blocks = [[Sparsity((rd, cd), sorted(maps_and_regions.get((i, j), ()),
-                                                       key=lambda t: (id(t[0]), id(t[1]))),
-                                      name="%s_%d_%d" % (self._name, i, j), block_sparse=block_sparse,
-                                      diagonal_block=diagonal_block and i == j)
-                             for j, cd in enumerate(cset.split)] for i, rd in enumerate(rset.split)]
+            # one block Sparsity per (row, column) component (mat.py:86-114);
+            # a monolithic Sparsity (nest=False) keeps them for their block
+            # patterns and builds one dof-level pattern over the whole space
+            blocks = [[Sparsity((rd, cd), sorted(maps_and_regions.get((i, j), ()),
+                                                 key=lambda t: (id(t[0]), id(t[1]))),
+                                name="%s_%d_%d" % (self._name, i, j), block_sparse=block_sparse,
+                                diagonal_block=diagonal_block and i == j)
+                       for j, cd in enumerate(cset.split)] for i, rd in enumerate(rset.split)]
             self._dims = tuple(tuple((rd.cdim, cd.cdim) for cd in cset.split) for rd in rset.split)
+            self._nested = bool(nest)
+            if self._nested:
+                self._blocks = blocks
+            else:
+                self._blocks = [[self]]
+                self._component_blocks = blocks
         else:
             self._nested = False
             self._blocks = [[self]]
@@ -170,7 +193,28 @@ class Sparsity(caching.ObjectCached):
 
     @utils.cached_property
     def shape(self):
+        if self.monolithic:
+            return (len(self._component_blocks), len(self._component_blocks[0]))
         return (len(self._blocks), len(self._blocks[0]))
+
+    @property
+    def monolithic(self):
+        """A Sparsity of mixed DataSets stored as one matrix (nest=False)."""
+        return hasattr(self, "_component_blocks")
+
+    def component(self, i, j):
+        """The Sparsity of block (i, j) of a mixed Sparsity, nested or not."""
+        return self._component_blocks[i][j] if self.monolithic else self._blocks[i][j]
+
+    @utils.cached_property
+    def offsets(self):
+        """Monolithic: the first dof row of each row component and the first
+        dof column of each column component (the reference's local_ises,
+        ``dataset.py:166-180``), and the totals."""
+        rset, cset = self._dsets
+        roff = np.concatenate([[0], np.cumsum([d.total_size * d.cdim for d in rset])]).astype(np.int64)
+        coff = np.concatenate([[0], np.cumsum([d.total_size * d.cdim for d in cset])]).astype(np.int64)
+        return roff, coff
 
     @property
     def nested(self):
@@ -198,12 +242,50 @@ class Sparsity(caching.ObjectCached):
     # -- the pattern -----------------------------------------------------------
     @property
     def nrows(self):
-        """Block rows (the row Set's elements)."""
+        """Block rows (the row Set's elements); dof rows when monolithic."""
+        if self.monolithic:
+            return int(self.offsets[0][-1])
         return self._dsets[0].set.total_size
 
     @property
     def ncols(self):
+        if self.monolithic:
+            return int(self.offsets[1][-1])
         return self._dsets[1].set.total_size
+
+    def _pair_maps(self, key):
+        """The plain (rmap, cmap) pairs of block ``key``: an extruded pair
+        becomes its per-cell maps over each of its regions."""
+        pairs = []
+        for rmap, cmap, regions in sorted(self._maps_and_regions.get(key, ()),
+                                          key=lambda t: (id(t[0]), id(t[1]))):
+            if getattr(rmap.iterset, "_extruded", False):
+                pairs += [(_cells_map(rmap, r), _cells_map(cmap, r)) for r in regions]
+            else:
+                pairs.append((rmap, cmap))
+        return pairs
+
+    def _monolithic_pairs(self):
+        """Dof-level map pairs of a monolithic Sparsity: every map entry of
+        block (i, j) unwound to its ``dim`` dofs and shifted to its
+        component's place, plus the diagonal node blocks of the diagonal
+        blocks (``sparsity.pyx:141-160, 212-219``)."""
+        rset, cset = self._dsets
+        roff, coff = self.offsets
+        pairs = []
+        for (i, j) in sorted(self._maps_and_regions):
+            for rmap, cmap in self._pair_maps((i, j)):
+                pairs.append((_dof_map(rmap, rset[i].cdim, roff[i]), _dof_map(cmap, cset[j].cdim, coff[j])))
+        if self._has_diagonal:
+            for i in range(min(len(rset), len(cset))):
+                n = min(rset[i].total_size, cset[i].total_size)
+                nodes = np.arange(n, dtype=np.int64)[:, None]
+                it = Set(n)
+                rows = roff[i] + nodes * rset[i].cdim + np.arange(rset[i].cdim)[None, :]
+                cols = coff[i] + nodes * cset[i].cdim + np.arange(cset[i].cdim)[None, :]
+                pairs.append((Map(it, _dof_space(roff[-1]), rset[i].cdim, rows.astype(np.int32)),
+                              Map(it, _dof_space(coff[-1]), cset[i].cdim, cols.astype(np.int32))))
+        return pairs
 
     def _build(self):
         if self._pattern is not None:
@@ -211,13 +293,12 @@ class Sparsity(caching.ObjectCached):
         from pyop2_amd import device as dev, plans
         L = dev.lib()
         s = dev.stream()
-        pairs = []
-        for rmap, cmap, regions in sorted(self._maps_and_regions[(0, 0)],
-                                          key=lambda t: (id(t[0]), id(t[1]))):
-            if getattr(rmap.iterset, "_extruded", False):
-                pairs += [(_cells_map(rmap, r), _cells_map(cmap, r)) for r in regions]
-            else:
-                pairs.append((rmap, cmap))
+        if self.monolithic:
+            pairs = self._monolithic_pairs()
+            set_diag = False
+        else:
+            pairs = self._pair_maps((0, 0))
+            set_diag = self._has_diagonal
         nrows, ncols = self.nrows, self.ncols
         arr = (dev.MapPair * max(len(pairs), 1))()
         incp, ince = [], []
@@ -236,12 +317,12 @@ class Sparsity(caching.ObjectCached):
         work = dev.workspace(need.value + 256 + 4 * (nrows + 64))
         nnz = ctypes.c_int64()
         dev.check(L.op2hip_sparsity_count(nrows, ncols, arr, len(pairs), ip, ie,
-                                          int(self._has_diagonal), dev.ptr(rowptr),
+                                          int(set_diag), dev.ptr(rowptr),
                                           ctypes.byref(nnz), dev.ptr(work), work.numel(), s),
                   "sparsity_count")
         colidx = dev.empty((max(nnz.value, 1),), np.int32)
         dev.check(L.op2hip_sparsity_fill(nrows, ncols, arr, len(pairs), ip, ie,
-                                         int(self._has_diagonal), dev.ptr(rowptr),
+                                         int(set_diag), dev.ptr(rowptr),
                                          dev.ptr(colidx), s), "sparsity_fill")
         self._pattern = (rowptr, colidx[:nnz.value], int(nnz.value))
         return self._pattern
@@ -259,7 +340,7 @@ class Sparsity(caching.ObjectCached):
     def dof_pattern(self):
         """Device ``(rowptr, colidx)`` of the dof (scalar) pattern."""
         rdim, cdim = self._dims[0][0]
-        if rdim == 1 and cdim == 1:
+        if (rdim == 1 and cdim == 1) or self.monolithic:
             return self.block_pattern
         if self._dof_pattern is None:
             from pyop2_amd import device as dev
@@ -275,12 +356,13 @@ class Sparsity(caching.ObjectCached):
     @utils.cached_property
     def nnz(self):
         """Non-zeros per row as the reference reports them (block rows when
-        block-sparse with square blocks, else dof rows)."""
+        block-sparse with square blocks, else dof rows; a monolithic
+        Sparsity's pattern is at dof level already)."""
         from pyop2_amd import device as dev
         rowptr, _, _ = self._build()
         nnz = np.diff(dev.to_host(rowptr)).astype(np.int32)
         rdim, cdim = self._dims[0][0]
-        if not (self._block_sparse and rdim == cdim):
+        if not self.monolithic and not (self._block_sparse and rdim == cdim):
             nnz = np.repeat(nnz * cdim, rdim)
         return nnz
 
@@ -309,6 +391,8 @@ class Mat(DataCarrier):
     def __new__(cls, sparsity=None, *args, **kwargs):
         if cls is Mat and getattr(sparsity, "nested", False):
             cls = MixedMat
+        elif cls is Mat and getattr(sparsity, "monolithic", False):
+            cls = MonolithicMat
         return super().__new__(cls)
 
     @utils.validate_type(('sparsity', Sparsity, ex.SparsityTypeError),
@@ -689,7 +773,7 @@ class MixedMat(Mat):
         if conf.configuration["type_check"]:
             for i in range(nr):
                 for j in range(nc):
-                    if (rmap[i], cmap[j]) not in self._sparsity[i, j]:
+                    if (rmap[i], cmap[j]) not in self._sparsity.component(i, j):
                         raise ex.MapValueError("Path maps not in sparsity maps")
         return MixedMatLegacyArg(self, (rmap, cmap), access)
 
@@ -758,3 +842,241 @@ class MixedMat(Mat):
 
     def __repr__(self):
         return "MixedMat(%r, %r, %r)" % (self._sparsity, self._datatype, self._name)
+
+
+class MonolithicMat(Mat):
+    """A Matrix on a monolithic mixed Sparsity (``nest=False``): one scalar
+    CSR over the whole mixed dof space, component i's rows and columns
+    contiguous at their offsets (``mat.py:613-654``).  ``mat[i, j]`` is a
+    :class:`MatBlock` view through which a par_loop adds the element blocks
+    of row component i and column component j."""
+
+    def __init__(self, sparsity, dtype=None, name=None, mat_type=None):
+        self._sparsity = sparsity
+        self.comm = sparsity.comm
+        self._datatype = np.dtype(dtype or np.float64)
+        if self._datatype not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise RuntimeError("Can only create a matrix of type float64 (or float32), %s is not supported"
+                               % self._datatype)
+        self._name = name or "mat_#x%x" % id(self)
+        self.mat_type = mat_type
+        self._block = False
+        self._values = None
+        self._status = None
+        self._desc = None
+        self.assembly_state = Mat.ASSEMBLED
+        nr, nc = sparsity.shape
+        self._blocks = [[MatBlock(self, i, j) for j in range(nc)] for i in range(nr)]
+
+    def _init(self):
+        if self._values is not None:
+            return
+        from pyop2_amd import device as dev
+        rowptr, colidx = self._sparsity.dof_pattern()
+        nnz = colidx.numel()
+        self._rowptr, self._colidx = rowptr, colidx
+        self._values = dev.zeros((max(nnz, 1),), self._datatype)
+        self._status = dev.zeros((1,), np.int32)
+        self._nnz, self._bs = nnz, 1
+        d = dev.MatDesc()
+        d.rowptr, d.colidx, d.values = dev.ptr(rowptr), dev.ptr(colidx), dev.ptr(self._values)
+        d.nrows, d.ncols, d.rdim, d.cdim = self._sparsity.nrows, self._sparsity.ncols, 1, 1
+        d.status = dev.ptr(self._status)
+        d.inc_ptr = d.inc_ent = None
+        self._desc = d
+
+    @property
+    def block_shape(self):
+        return (1, 1)
+
+    @property
+    def nrows(self):
+        return self._sparsity.nrows
+
+    @property
+    def ncols(self):
+        return self._sparsity.ncols
+
+    @property
+    def nblock_rows(self):
+        return self._sparsity.nrows
+
+    @property
+    def nblock_cols(self):
+        return self._sparsity.ncols
+
+    @property
+    def nblocks(self):
+        nr, nc = self._sparsity.shape
+        return nr * nc
+
+    @property
+    def is_mixed(self):
+        return True
+
+    @property
+    def nbytes(self):
+        return self._sparsity.nz * self.dtype.itemsize
+
+    def set_local_diagonal_entries(self, rows, diag_val=1.0, idx=None):
+        """Insert ``diag_val`` on the diagonal of the dof rows ``rows``."""
+        rows = np.asarray(rows, dtype=np.int64).ravel()
+        self.change_assembly_state(Mat.INSERT_VALUES)
+        self._write_entries(rows, rows, np.full(len(rows), diag_val), add=False)
+
+    @utils.validate_in(('access', Mat._modes, ex.ModeValueError))
+    def __call__(self, access, path, lgmaps=None, unroll_map=False):
+        return MixedMat.__call__(self, access, path, lgmaps, unroll_map)
+
+    def __iter__(self):
+        for row in self._blocks:
+            for b in row:
+                yield b
+
+    def __getitem__(self, idx):
+        try:
+            i, j = idx
+            return self._blocks[i][j]
+        except TypeError:
+            return self._blocks[idx]
+
+    @property
+    def blocks(self):
+        return self._blocks
+
+    def __str__(self):
+        return "OP2 monolithic Mat %s on %s" % (self._name, self._sparsity)
+
+    def __repr__(self):
+        return "Mat(%r, %r, %r)" % (self._sparsity, self._datatype, self._name)
+
+
+class MatBlock:
+    """Block (i, j) of a monolithic Mat (``mat.py:868-960``): the rows of
+    row component i and the columns of column component j of its parent's
+    storage.  Its assembly state is the parent's; assembling a block alone
+    is an error, as in the reference."""
+
+    _block = False
+
+    def __init__(self, parent, i, j):
+        self._parent = parent
+        self._i, self._j = i, j
+        roff, coff = parent.sparsity.offsets
+        self.row_offset, self.col_offset = int(roff[i]), int(coff[j])
+        self._sparsity = parent.sparsity.component(i, j)
+        self.comm = parent.comm
+
+    # storage and launch descriptors are the parent's
+    @property
+    def rowptr(self):
+        return self._parent.rowptr
+
+    @property
+    def colidx(self):
+        return self._parent.colidx
+
+    @property
+    def device_values(self):
+        return self._parent.device_values
+
+    @property
+    def _status(self):
+        self._parent._init()
+        return self._parent._status
+
+    @property
+    def _gen_flag(self):
+        """How a generated loop scatters into this block: the parent's scalar
+        CSR, rows and columns shifted by the block's offsets."""
+        return ("offset", self.row_offset, self.col_offset)
+
+    @property
+    def sparsity(self):
+        return self._sparsity
+
+    @property
+    def dims(self):
+        return self._sparsity.dims
+
+    @property
+    def dtype(self):
+        return self._parent.dtype
+
+    @property
+    def block_shape(self):
+        return (1, 1)
+
+    @property
+    def nrows(self):
+        rd, _ = self.dims[0][0]
+        return self._sparsity.nrows * rd
+
+    @property
+    def ncols(self):
+        _, cd = self.dims[0][0]
+        return self._sparsity.ncols * cd
+
+    @property
+    def assembly_state(self):
+        return self._parent.assembly_state
+
+    @assembly_state.setter
+    def assembly_state(self, value):
+        self._parent.assembly_state = value
+
+    def change_assembly_state(self, new_state):
+        self._parent.change_assembly_state(new_state)
+
+    def _flush_assembly(self):
+        self._parent._flush_assembly()
+
+    def assemble(self):
+        raise RuntimeError("Should never call assemble on MatBlock")
+
+    def __iter__(self):
+        yield self
+
+    def __getitem__(self, idx):
+        return self
+
+    def _shifted(self, rows, cols):
+        return (np.asarray(rows, dtype=np.int64).ravel() + self.row_offset,
+                np.asarray(cols, dtype=np.int64).ravel() + self.col_offset)
+
+    def _dense_block_entries(self, rows, cols, values):
+        rr, cc, vv = Mat._dense_block_entries(self, rows, cols, values)
+        return (*self._shifted(rr, cc), vv)
+
+    def addto_values(self, rows, cols, values):
+        self.change_assembly_state(Mat.ADD_VALUES)
+        if len(np.atleast_1d(values)):
+            self._parent._write_entries(*self._dense_block_entries(rows, cols, values), add=True)
+
+    def set_values(self, rows, cols, values):
+        self.change_assembly_state(Mat.INSERT_VALUES)
+        if len(np.atleast_1d(values)):
+            self._parent._write_entries(*self._dense_block_entries(rows, cols, values), add=False)
+
+    def set_local_diagonal_entries(self, rows, diag_val=1.0, idx=None):
+        rows = np.asarray(rows, dtype=np.int64).ravel()
+        rbs, _ = self.dims[0][0]
+        if rbs > 1:
+            comps = [idx] if idx is not None else range(rbs)
+            rows = (rbs * rows[:, None] + np.asarray(list(comps))[None, :]).ravel()
+        r, c = self._shifted(rows, rows)
+        self.change_assembly_state(Mat.INSERT_VALUES)
+        self._parent._write_entries(r, c, np.full(len(rows), diag_val), add=False)
+
+    @property
+    def values(self):
+        """Dense copy of the block (assembles the parent)."""
+        full = self._parent.values
+        return full[self.row_offset:self.row_offset + self.nrows,
+                    self.col_offset:self.col_offset + self.ncols]
+
+    def __str__(self):
+        return "OP2 MatBlock (%d, %d) of %s" % (self._i, self._j, self._parent.name)
+
+    def __repr__(self):
+        return "MatBlock(%r, %d, %d)" % (self._parent, self._i, self._j)

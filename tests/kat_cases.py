@@ -944,6 +944,66 @@ def case_mixed_blockwise_sparsity(op2, split=None):
     assert list(s._blocks[1][1].nnz) == [1, 1, 1]
 
 
+def case_mixed_mat_monolithic(op2, split=None):
+    """test_matrices.py:878-916 on a monolithic Sparsity (nest=False): one
+    matrix over the mixed dof space, whose blocks hold the same closed forms
+    (the reference's _init_monolithic, mat.py:613-654)."""
+    mset, mmap, mdat = _mixed(op2)
+    sp = op2.Sparsity((mset ** 1, mset ** 1), {(i, j): [(rm, cm, None)] for i, rm in enumerate(mmap)
+                                               for j, cm in enumerate(mmap)}, nest=False)
+    assert not sp.nested and sp.shape == (2, 2)
+    mat = op2.Mat(sp)
+    assert (mat.nrows, mat.ncols) == (7, 7)
+    k = op2.Kernel("static void outer3(double v[9], const double d[3])"
+                   " { for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) v[i * 3 + j] += d[i] * d[j]; }",
+                   "outer3")
+    op2.par_loop(k, mmap.iterset, mat(op2.INC, (mmap, mmap)), mdat(op2.READ, mmap))
+    mat.assemble()
+    want = np.block([[np.diag([1.0, 4.0, 9.0]), MIXED_OD], [MIXED_OD.T, MIXED_LL]])
+    assert np.allclose(mat.values, want, rtol=1e-12, atol=0)
+    assert np.allclose(mat[0, 1].values, MIXED_OD, rtol=1e-12, atol=0)
+    assert np.allclose(mat[1, 1].values, MIXED_LL, rtol=1e-12, atol=0)
+
+
+def case_monolithic_states(op2, split=None):
+    """test_matrices.py:798-838 for the non-nested fixture: a block's
+    insertion sets the whole matrix's state, assembling a block alone is an
+    error, and INSERT / ADD mix block by block."""
+    mset, mmap, _ = _mixed(op2)
+    sp = op2.Sparsity((mset ** 1, mset ** 1), {(i, j): [(rm, cm, None)] for i, rm in enumerate(mmap)
+                                               for j, cm in enumerate(mmap)}, nest=False)
+    mat = op2.Mat(sp)
+    assert mat.assembly_state is op2.Mat.ASSEMBLED
+    mat[0, 0].set_local_diagonal_entries([0])
+    assert mat[0, 0].assembly_state is op2.Mat.INSERT_VALUES
+    assert mat.assembly_state is op2.Mat.INSERT_VALUES
+    try:
+        mat[0, 0].assemble()
+    except RuntimeError:
+        pass
+    else:
+        raise AssertionError("assembling a MatBlock must raise")
+    mat.assemble()
+    mat.zero()
+    mat[0, 0].addto_values(0, 0, [1])
+    assert mat.assembly_state is op2.Mat.ADD_VALUES
+    mat[1, 1].addto_values(1, 1, [3])
+    mat[1, 1].set_values(0, 0, [2])
+    mat[0, 0].set_values(1, 1, [4])
+    mat[1, 1].addto_values(0, 0, [3])
+    mat.assemble()
+    assert np.allclose(mat[0, 0].values, np.diag([1, 4, 0]))
+    assert np.allclose(mat[1, 1].values, np.diag([5, 3, 0, 0]))
+    assert np.allclose(mat[0, 1].values, 0)
+    assert np.allclose(mat[1, 0].values, 0)
+
+
+MONOLITHIC_CASES = [
+    ("mixed_mat_monolithic", case_mixed_mat_monolithic),
+    ("monolithic_states", case_monolithic_states),
+]
+
+
 MIXED_CASES = [
     ("mixed_mat", case_mixed_mat),
     ("mixed_rhs", case_mixed_rhs),

@@ -81,6 +81,59 @@ def test_mixed_cases(device, name, case, strategy):
 
 
 @pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+@pytest.mark.parametrize("name,case", kat_cases.MONOLITHIC_CASES, ids=[n for n, _ in kat_cases.MONOLITHIC_CASES])
+def test_monolithic_cases(device, name, case, strategy):
+    configuration["strategy"] = strategy
+    case(op2)
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
+def test_monolithic_random_against_oracle(device, strategy):
+    """A monolithic MixedMat (nest=False) on a (cell x2, vertex) mixed space
+    of a 12x12 triangle mesh: the dof pattern equals the oracle's
+    monolithic_pattern of the block patterns, and the values the oracle's
+    nested blocks placed at their offsets (1e-12)."""
+    import numpy as np
+    import oracle_op2
+    from oracle import sparsity as osp
+    from pyop2_amd.meshes import unit_square_tri, workload_rng
+    configuration["strategy"] = strategy
+    m = unit_square_tri(12, rng=workload_rng(1), permute=True)
+    rng = np.random.default_rng(12)
+    cval, vval = rng.random((m.ncells, 2)), rng.random(m.nverts)
+    code = ("static void mk(double A[5][5], const double d[5]) {"
+            " for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j)"
+            " A[i][j] += d[i] * d[j] + 0.5 * i - 0.25 * j; }")
+    out = []
+    for ns, nest in ((op2, False), (oracle_op2, True)):
+        cells, verts = ns.Set(m.ncells), ns.Set(m.nverts)
+        ms = ns.MixedSet((cells, verts))
+        c2c = ns.Map(cells, cells, 1, np.arange(m.ncells, dtype=np.int32))
+        c2v = ns.Map(cells, verts, 3, m.cell_vertex.astype(np.int32))
+        mmap = ns.MixedMap((c2c, c2v))
+        d = ns.MixedDat([ns.Dat(cells ** 2, cval.copy(), np.float64), ns.Dat(verts ** 1, vval.copy(), np.float64)])
+        mds = ns.MixedDataSet((cells ** 2, verts ** 1))
+        maps = {(i, j): [(rm, cm, None)] for i, rm in enumerate(mmap) for j, cm in enumerate(mmap)}
+        sp = ns.Sparsity((mds, mds), maps, nest=nest) if ns is op2 else ns.Sparsity((mds, mds), maps)
+        A = ns.Mat(sp)
+        ns.par_loop(ns.Kernel(code, "mk"), cells, A(ns.INC, (mmap, mmap)), d(ns.READ, mmap))
+        A.assemble()
+        out.append(A)
+    Ad, Ao = out
+    dense = np.block([[np.asarray(Ao[i, j].values) for j in range(2)] for i in range(2)])
+    assert np.linalg.norm(Ad.values - dense) <= 1e-12 * np.linalg.norm(dense)
+    tri = m.cell_vertex.astype(np.int32)
+    ident = np.arange(m.ncells, dtype=np.int32)[:, None]
+    blocks = [[osp.pattern(m.ncells, m.ncells, [(ident, ident, m.ncells)]),
+               osp.pattern(m.ncells, m.nverts, [(ident, tri, m.ncells)], set_diag=False)],
+              [osp.pattern(m.nverts, m.ncells, [(tri, ident, m.ncells)], set_diag=False),
+               osp.pattern(m.nverts, m.nverts, [(tri, tri, m.ncells)])]]
+    rp, ci = osp.monolithic_pattern(blocks, [2, 1], [2, 1], [m.ncells, m.nverts], [m.ncells, m.nverts])
+    grp, gci, _ = Ad.csr()
+    assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured"])
 def test_mixed_random_against_oracle(device, strategy):
     """A mixed (cell, vertex) space on a 16x16 triangle mesh: MixedMat and
     MixedDat INC of a non-symmetric element kernel, device against the

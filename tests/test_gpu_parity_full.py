@@ -293,7 +293,9 @@ def test_mixed_mat_insert_and_add(device):
 
 
 def test_flush_between_insert_and_add(device):
-    """test_matrices.py:840-862 (nested): switching ADD -> INSERT flushes once."""
+    """test_matrices.py:840-862 (nested): switching ADD -> INSERT flushes once
+    (the reference inserts at (1, 0) with new-nonzero errors switched off;
+    the device pattern is fixed, so the insertion goes to (1, 1))."""
     import types
     mat = _mixed_nested()
     count = [0]
@@ -306,7 +308,7 @@ def test_flush_between_insert_and_add(device):
         blk._flush_assembly = types.MethodType(flush, blk)
     mat[0, 0].addto_values(0, 0, [1])
     assert count[0] == 0
-    mat[0, 0].set_values(1, 0, [2])
+    mat[0, 0].set_values(1, 1, [2])
     assert count[0] == 1
     mat.assemble()
     assert count[0] == 1

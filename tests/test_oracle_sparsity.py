@@ -157,3 +157,37 @@ def test_expand_block_pattern_reference_counts():
     assert list(np.diff(rp)) == [1, 2, 1, 2]
     erp, _ = osp.expand_block_pattern(rp, ci, 2, 2)
     assert list(np.diff(erp)) == [2, 2, 4, 4, 2, 2, 4, 4]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_monolithic_pattern_is_the_block_matrix_pattern(seed):
+    """The monolithic dof pattern equals scipy's pattern of bmat of the
+    Kronecker-expanded blocks (empty blocks included)."""
+    rng = np.random.default_rng([2026, 96, seed])
+    nr, nc = int(rng.integers(1, 4)), int(rng.integers(1, 4))
+    rsz = [int(v) for v in rng.integers(1, 15, size=nr)]
+    csz = [int(v) for v in rng.integers(1, 15, size=nc)]
+    rd = [int(v) for v in rng.integers(1, 4, size=nr)]
+    cd = [int(v) for v in rng.integers(1, 4, size=nc)]
+    ne = 20
+    blocks, dense = [], []
+    for i in range(nr):
+        brow, drow = [], []
+        for j in range(nc):
+            if rng.random() < 0.25:
+                brow.append(None)
+                drow.append(ss.csr_matrix((rsz[i] * rd[i], csz[j] * cd[j])))
+                continue
+            rmap = rng.integers(-1, rsz[i], size=(ne, 2)).astype(np.int32)
+            cmap = rng.integers(-1, csz[j], size=(ne, 3)).astype(np.int32)
+            rp, ci = osp.pattern(rsz[i], csz[j], [(rmap, cmap, ne)], set_diag=(i == j))
+            brow.append((rp, ci))
+            B = ss.csr_matrix((np.ones(len(ci)), ci, rp), shape=(rsz[i], csz[j]))
+            drow.append(ss.kron(B, np.ones((rd[i], cd[j]))))
+        blocks.append(brow)
+        dense.append(drow)
+    M = ss.bmat(dense).tocsr()
+    M.eliminate_zeros()
+    M.sort_indices()
+    rp, ci = osp.monolithic_pattern(blocks, rd, cd, rsz, csz)
+    assert np.array_equal(rp, M.indptr) and np.array_equal(ci, M.indices)
