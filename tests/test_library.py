@@ -391,3 +391,20 @@ def test_row_list_packing_round_trips():
     table[0, 5] = 31
     assert pack_row_list(ent, table, 3, 3) is None
     assert pack_row_list(ent, table, 4, 4) is None
+
+
+def test_integration_p2_stub_argtypes_match_the_abi():
+    """The guide's P2 fused stub declares the entry point's C signature (the
+    one the package binds): start, end, the Mat descriptor and five data /
+    map pointers, lam and mu, then the plan and the stream."""
+    import ctypes
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "INTEGRATION.md")
+    text = open(path).read()
+    stub = text[text.index("_g.argtypes"):text.index("def compile_p2_fused")]
+    names = re.findall(r"ctypes\.(c_\w+|POINTER)", stub)
+    kinds = {"c_int32": ctypes.c_int32, "c_void_p": ctypes.c_void_p, "c_double": ctypes.c_double,
+             "POINTER": ctypes.c_void_p}
+    want = [getattr(t, "_type_", None) for t in dev._SIGNATURES["op2hip_wrap_p2_elasticity_fused"][1]]
+    got = [kinds[n]._type_ for n in names]
+    assert got == want
