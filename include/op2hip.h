@@ -380,6 +380,27 @@ int op2hip_wrap_p2_elasticity_fused_f32(int32_t start, int32_t end,
                                         const int32_t *c2n, const int32_t *c2v,
                                         double lam, double mu,
                                         const op2hip_exec *ex, void *stream);
+/* The P2 residual loop without allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as op2hip_wrap_p2_elasticity_residual, plus
+ * partial_slots, the partial-slot count of the tile plan's map 0 (the last
+ * entry of tile_shared_off[0]).  Under OP2HIP_STAGED with a partial-sum plan
+ * the partial sums go to a buffer inside the library (a device array of
+ * OP2HIP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends) and
+ * ex->partials are ignored; a plan needing more than that many bytes
+ * (partial_slots * 3 doubles) is rejected with OP2HIP_ERR_ARG.  Other
+ * strategies use no scratch.  Allocates nothing, synchronises nothing.
+ * Every call on one GPU shares that GPU's buffer: calls that overlap on one
+ * GPU (concurrent streams) race; nothing in it needs clearing between calls
+ * (each slot is written by its tile before the second pass reads it). */
+#define OP2HIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int op2hip_wrap_p2_elasticity_residual_noalloc(int32_t start, int32_t end, double *R,
+                                               const double *X, const double *u,
+                                               const int32_t *c2n, const int32_t *c2v,
+                                               double lam, double mu,
+                                               const op2hip_exec *ex, int64_t partial_slots,
+                                               void *stream);
 /* Adjoint of the P2 residual loop: given Rbar (nnodes, 3), adds
  * ubar += K(X)^T Rbar and Xbar += d(Rbar . R(u, X))/dX (transposed
  * scatter/gather of the residual loop; no reference counterpart). */

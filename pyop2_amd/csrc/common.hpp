@@ -15,6 +15,10 @@ namespace op2hip {
 
 void set_error(const char *fmt, ...);
 
+// Device address of the no-allocation entry point's scratch array on
+// `device` (the current device); defined beside the array (p2.hip).
+int noalloc_scratch_address(int device, void **out);
+
 #define OP2HIP_CHECK(expr)                                                     \
   do {                                                                         \
     hipError_t e_ = (expr);                                                    \

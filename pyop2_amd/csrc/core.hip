@@ -384,7 +384,9 @@ int op2hip_init(int32_t device) {
               prop.gcnArchName);
     return OP2HIP_ERR_ARG;
   }
-  return OP2HIP_OK;
+  // the no-allocation entry point's scratch address on this device
+  void *scratch = nullptr;
+  return noalloc_scratch_address(device, &scratch);
 }
 
 int op2hip_scan_workspace(int64_t n, size_t *bytes) {

@@ -13,6 +13,8 @@
 // and  R_i += wdet sum_q sum_m D[q][i][m] sigma_q gl_m.
 // Element matrix block (i, j): with M = sum_q grad phi_i (x) grad phi_j,
 // B = wdet (lam M + mu M^T + mu tr(M) I).
+#include <mutex>
+
 #include "rowgather.hpp"
 #include "staged.hpp"
 
@@ -924,9 +926,69 @@ int matrix(int32_t start, int32_t end, const op2hip_mat *A, T *R, const T *X, co
 }  // namespace p2
 }  // namespace op2hip
 
+// ---- the residual without allocation ---------------------------------------
+// Partial sums of the nodes that staged tiles share, for
+// op2hip_wrap_p2_elasticity_residual_noalloc: a file-scope device array of
+// OP2HIP_NOALLOC_SCRATCH_BYTES, which every GPU the library runs on holds
+// until the process ends.  Host code never passes the array's name to a
+// kernel (in host code it is a stand-in, not a device address): it uses the
+// device address looked up once per device.
+static __device__ __align__(256) double scratch[OP2HIP_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace op2hip {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+// Device address of `scratch` on `device` (the current device), looked up
+// with hipGetSymbolAddress on the first call per device and kept.
+int noalloc_scratch_address(int device, void **out) {
+  OP2HIP_REQUIRE(device >= 0 && device < kMaxDevices, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+      return OP2HIP_ERR_HIP;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return OP2HIP_OK;
+}
+}  // namespace op2hip
+
 using namespace op2hip;
 
 extern "C" {
+
+int op2hip_wrap_p2_elasticity_residual_noalloc(int32_t start, int32_t end, double *R,
+                                               const double *X, const double *u,
+                                               const int32_t *c2n, const int32_t *c2v,
+                                               double lam, double mu, const op2hip_exec *ex,
+                                               int64_t partial_slots, void *stream) {
+  OP2HIP_REQUIRE(ex, "residual_noalloc: no execution plan");
+  op2hip_exec local = *ex;
+  if (local.strategy == OP2HIP_STAGED && local.tile_shared_off[0]) {
+    const int64_t cap = (int64_t)(OP2HIP_NOALLOC_SCRATCH_BYTES / (3 * sizeof(double)));
+    OP2HIP_REQUIRE(partial_slots >= 0 && partial_slots <= cap,
+                   "residual_noalloc: %lld partial slots of shared tile nodes exceed the "
+                   "library's scratch of %lld slots (%u bytes); use "
+                   "op2hip_wrap_p2_elasticity_residual with a workspace",
+                   (long long)partial_slots, (long long)cap, (unsigned)OP2HIP_NOALLOC_SCRATCH_BYTES);
+    int device = 0;
+    OP2HIP_CHECK(hipGetDevice(&device));
+    void *p = nullptr;
+    const int rc = noalloc_scratch_address(device, &p);
+    if (rc != OP2HIP_OK) return rc;
+    local.partials[0] = static_cast<double *>(p);
+    local.partials[1] = nullptr;
+  }
+  return p2::residual<double>(start, end, R, X, u, c2n, c2v, lam, mu, &local, stream_of(stream));
+}
 
 int op2hip_wrap_p2_elasticity_residual(int32_t start, int32_t end, double *R, const double *X,
                                        const double *u, const int32_t *c2n, const int32_t *c2v,

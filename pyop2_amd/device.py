@@ -137,6 +137,9 @@ _SIGNATURES = {
     "op2hip_wrap_p1_laplace_matrix": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V]),
     "op2hip_wrap_p2_elasticity_residual": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V,
                                                           ctypes.c_double, ctypes.c_double, _V, _V]),
+    "op2hip_wrap_p2_elasticity_residual_noalloc": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V,
+                                                                  ctypes.c_double, ctypes.c_double, _V,
+                                                                  ctypes.c_int64, _V]),
     "op2hip_wrap_p2_elasticity_residual_f32": (ctypes.c_int, [_I, _I, _V, _V, _V, _V, _V,
                                                               ctypes.c_double, ctypes.c_double, _V, _V]),
     "op2hip_wrap_p2_elasticity_matrix": (ctypes.c_int, [_I, _I, _V, _V, _V, _V,
