@@ -60,8 +60,10 @@ struct TileLayout {
   }
 };
 
-// Node slots per thread whose ids stay in registers from staging to flush.
-constexpr int kStageSlots = 2;
+// kStageSlots: node slots per thread and staging round; the ids of a
+// thread's first round stay in registers from staging to flush.  When a
+// tile has at most kStageSlots * blockDim nodes the tile stages in one
+// round: every node id of the thread, then every value, one dependent wait.
 
 // Stage map M of tile b: inputs into LDS, and the accumulators start from the
 // old output values of the nodes only this tile has (no other tile of the
@@ -69,7 +71,7 @@ constexpr int kStageSlots = 2;
 // changes nothing) and from zero for shared nodes.  All loads of a thread's
 // node slots are issued together.  ids[] keeps the node ids of the thread's
 // first kStageSlots slots for the flush.
-template <class Elem, int M>
+template <class Elem, int M, int kStageSlots>
 __device__ __forceinline__ void stage_map(const Elem &elem, const TilePlan &P, int32_t b,
                                           typename Elem::T *in, double *acc,
                                           int32_t (&ids)[kStageSlots]) {
@@ -108,7 +110,7 @@ __device__ __forceinline__ void stage_map(const Elem &elem, const TilePlan &P, i
 // started from the old value); shared nodes write it to their partial slot
 // (contiguous per tile) when the plan has partials, else add it with a
 // device atomic.  No load waits.
-template <class Elem, int M>
+template <class Elem, int M, int kStageSlots>
 __device__ __forceinline__ void flush_map(const Elem &elem, const TilePlan &P, int32_t b,
                                           const double *acc, const int32_t (&ids)[kStageSlots]) {
   using T = typename Elem::T;
@@ -182,7 +184,7 @@ struct LdsElementOf<Elem, std::void_t<decltype(Elem::kLdsElement)>> {
 // Tile b of a staged launch (the workgroup's share, or one work item of a
 // batched launch).  The local node ids of a thread's first element are
 // loaded before the staging barrier.
-template <class Elem>
+template <class Elem, int kStageSlots = 2>
 __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P, int32_t b) {
   using T = typename Elem::T;
   using L = TileLayout<Elem>;
@@ -211,8 +213,8 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
   if ((int32_t)threadIdx.x < ne) load_locals(p0 + threadIdx.x);
 
   int32_t ids0[kStageSlots], ids1[kStageSlots];
-  stage_map<Elem, 0>(elem, P, b, in0, acc0, ids0);
-  if constexpr (Elem::kMaps > 1) stage_map<Elem, 1>(elem, P, b, in1, acc1, ids1);
+  stage_map<Elem, 0, kStageSlots>(elem, P, b, in0, acc0, ids0);
+  if constexpr (Elem::kMaps > 1) stage_map<Elem, 1, kStageSlots>(elem, P, b, in1, acc1, ids1);
   __syncthreads();
 
   for (int32_t t = threadIdx.x; t < ne; t += blockDim.x) {
@@ -247,8 +249,8 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
   }
   __syncthreads();
 
-  flush_map<Elem, 0>(elem, P, b, acc0, ids0);
-  if constexpr (Elem::kMaps > 1) flush_map<Elem, 1>(elem, P, b, acc1, ids1);
+  flush_map<Elem, 0, kStageSlots>(elem, P, b, acc0, ids0);
+  if constexpr (Elem::kMaps > 1) flush_map<Elem, 1, kStageSlots>(elem, P, b, acc1, ids1);
 }
 
 // Waves per SIMD an element asks the compiler to fit (Elem::kWavesPerEU:
@@ -262,10 +264,10 @@ struct WavesPerEUOf<Elem, std::void_t<decltype(Elem::kWavesPerEU)>> {
   static constexpr int value = Elem::kWavesPerEU;
 };
 
-template <class Elem>
+template <class Elem, int kStageSlots>
 __global__ void __launch_bounds__(256, WavesPerEUOf<Elem>::value)
 staged_kernel(Elem elem, TilePlan P) {
-  staged_tile(elem, P, xcd_contiguous(blockIdx.x, P.ntiles));
+  staged_tile<Elem, kStageSlots>(elem, P, xcd_contiguous(blockIdx.x, P.ntiles));
 }
 
 template <class Elem>
@@ -297,7 +299,12 @@ int launch_staged(int32_t start, int32_t end, const Elem &elem, const op2hip_exe
   OP2HIP_REQUIRE(lds <= 64 * 1024, "tile needs %zu bytes of LDS (max 65536)", lds);
   const int bs = block_size_of(ex, 256);
   OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % 64 == 0, "staged block size %d", bs);
-  staged_kernel<Elem><<<ex->ntiles, bs, lds, s>>>(elem, P);
+  // tiles whose nodes take at most four per thread stage in one round
+  const int most = std::max(P.max_nodes[0], Elem::kMaps > 1 ? P.max_nodes[1] : 0);
+  if (most <= 4 * bs)
+    staged_kernel<Elem, 4><<<ex->ntiles, bs, lds, s>>>(elem, P);
+  else
+    staged_kernel<Elem, 2><<<ex->ntiles, bs, lds, s>>>(elem, P);
   OP2HIP_LAUNCHED();
   if constexpr (Elem::kOut0 > 0) {
     if (P.shared_off[0] && ex->nshared[0] > 0)
