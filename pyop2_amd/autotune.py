@@ -8,12 +8,16 @@ is the conflict-resolution strategy plus its launch parameters:
 
 * staged tiles: elements per tile (``tile``), the LDS budget a tile's
   staged values and accumulators must fit (``lds``, which caps the nodes of
-  a tile and so sets how many tiles share a CU) and the workgroup size
-  (``block``) -- the reference's plan sizes its mini-partitions to the
-  shared memory the same way (``doc/sphinx/source/plan.rst:13-19``);
+  a tile and so sets how many tiles share a CU), the workgroup size
+  (``block``) and how the nodes tiles share are reduced (``partials``:
+  partial sums added by a second pass, or device atomics) -- the
+  reference's plan sizes its mini-partitions to the shared memory the same
+  way (``doc/sphinx/source/plan.rst:13-19``);
 * row gathers: the batch budget (``batch`` = most row incidences, most
   blocks of a batch, which sets its LDS) and the workgroup size;
-* split loops: each part's configuration, tuned as a loop of its own.
+* split loops: each part's configuration, tuned as a loop of its own, then
+  whether the parts that write no matrix overlap the assembly on a side
+  stream (``overlap``).
 
 Tuning runs in two stages: every applicable strategy with its default
 parameters, then the parameter candidates of the fastest strategy.  The key
@@ -49,7 +53,13 @@ STAGED_PARAMS = [
     {"tile": 384, "lds": 40960, "block": 128},
     {"tile": 512, "lds": 54272, "block": 256},
     {"tile": 512, "lds": 65536, "block": 256},
+    {"tile": 384, "lds": 40960, "block": 192, "partials": False},
+    {"tile": 256, "lds": 65536, "block": 256, "partials": False},
 ]
+# split loops: the parts in turn on the calling stream, or the parts that
+# write no matrix overlapped on a side stream, issued before / after the
+# assembly
+SPLIT_OVERLAP = [None, "before", "after"]
 ROWGATHER_PARAMS = {
     # P2 BSR(3) float64 / float32: (incidences, blocks) of a batch
     "p2": [{"batch": None, "block": 256},
@@ -233,7 +243,17 @@ def _tune_split(executor, parloop, outputs, repeats, report):
             if t is not None and (best_t is None or t < best_t):
                 best, best_t = cfg, t
         parts.append(best)
-    return {"strategy": "split", "parts": parts}
+    best, best_t = None, None
+    for overlap in SPLIT_OVERLAP:
+        cfg = {"strategy": "split", "parts": parts}
+        if overlap:
+            cfg["overlap"] = overlap
+        t = _time(executor, parloop, cfg, outputs, repeats)
+        if report is not None:
+            report[f"split:overlap={overlap}"] = t
+        if t is not None and (best_t is None or t < best_t):
+            best, best_t = cfg, t
+    return best
 
 
 def tune(executor, parloop, repeats=REPEATS, report=None):

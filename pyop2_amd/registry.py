@@ -81,6 +81,16 @@ FAMILIES = {
 _TILE = 256
 _LDS = 64 * 1024
 
+_side_streams = {}
+
+
+def _side_stream(device):
+    """The side stream (default priority) split loops overlap parts on."""
+    s = _side_streams.get(device)
+    if s is None:
+        s = _side_streams[device] = dev.torch().cuda.Stream(device=device)
+    return s
+
 
 def _tile_lds(spec, itemsize, max_nodes):
     b = 0
@@ -273,8 +283,17 @@ class HandWrittenKernel:
         """The loop as its split parts: the assembly with its best row gather,
         the residual with LDS-staged tiles (one element pass each, no
         redundant gathers of the solution per row incidence); ``cfg["parts"]``
-        holds each part's launch configuration."""
-        parts_cfg = (cfg or {}).get("parts") or [None] * len(self.spec["split"])
+        holds each part's launch configuration.
+
+        The parts write disjoint outputs and only read what they share, so
+        ``cfg["overlap"]`` may run the parts that write no matrix on a side
+        stream (default priority), forked from and joined back into the
+        calling stream: ``"before"`` issues them ahead of the assembly,
+        ``"after"`` behind it.  The latency-bound residual then fills the CU
+        slots the bandwidth-bound assembly leaves idle."""
+        cfg = cfg or {}
+        parts_cfg = cfg.get("parts") or [None] * len(self.spec["split"])
+        jobs = []
         for (sub, idx), pcfg in zip(self._split_parts(), parts_cfg):
             part = _SubLoop(parloop.iterset, [parloop.arguments[k] for k in idx])
             sub_out = None
@@ -288,8 +307,32 @@ class HandWrittenKernel:
             if pcfg is None:
                 pcfg = {"strategy": "rowgather_slots" if sub.spec["args"][0][0] == "mat" else "staged"}
             pcfg = dict(pcfg, strategy=sub.applicable(part, pcfg["strategy"]))
-            sub.launch(start, end, part, pcfg, sub_out)
+            jobs.append((sub, part, pcfg, sub_out))
         self.strategy_used = "split"
+        overlap = cfg.get("overlap")
+        if overlap not in ("before", "after"):
+            for sub, part, pcfg, sub_out in jobs:
+                sub.launch(start, end, part, pcfg, sub_out)
+            return
+        torch = dev.torch()
+        main = torch.cuda.current_stream(dev.device())
+        side = _side_stream(dev.device())
+        side.wait_stream(main)
+        on_side = [j for j in jobs if j[0].spec["args"][0][0] != "mat"]
+        on_main = [j for j in jobs if j[0].spec["args"][0][0] == "mat"]
+
+        def issue_side():
+            with torch.cuda.stream(side):
+                for sub, part, pcfg, sub_out in on_side:
+                    sub.launch(start, end, part, pcfg, sub_out)
+
+        if overlap == "before":
+            issue_side()
+        for sub, part, pcfg, sub_out in on_main:
+            sub.launch(start, end, part, pcfg, sub_out)
+        if overlap == "after":
+            issue_side()
+        main.wait_stream(side)
 
     def launch(self, start, end, parloop, cfg, outputs=None):
         """Launch with the configuration ``cfg`` (a strategy name, or a dict
@@ -333,7 +376,8 @@ class HandWrittenKernel:
                 ex.strategy = dev.STRATEGIES[strategy]
             else:
                 nouts = [nout for _, nout in self.spec["staged_vals"]]
-                tp.exec_fields(ex, nouts if configuration["staged_partials"] else None)
+                partials = cfg.get("partials", configuration["staged_partials"])
+                tp.exec_fields(ex, nouts if partials else None)
                 keep.append(tp)
         if self.spec["args"][0][0] == "mat":
             mat = args[0].data

@@ -5,7 +5,8 @@
   pattern bit-exact and values within 1e-12;
 * the fused P2 residual + BSR(3) assembly on the 32^3 cube (196,608 cells),
   the whole Mat and the whole residual, float64 within 1e-12 and float32
-  within 1e-5 of the float64 oracle;
+  within 1e-5 of the float64 oracle, and the split loop with its residual
+  overlapped on a side stream, two passes in a row;
 * two successive temporary Subsets of one set with indirect INC and WRITE
   arguments (plans must follow the Subset's contents);
 * the arity-3 row-list assembly on a mesh with a node of valence > 31 (the
@@ -134,6 +135,36 @@ def test_p2_fused_32cube_f64(device, tet32, tet32_oracle, strategy):
     assert len(ci) == 7678721
     assert relerr(v, Aref.values) < 1e-12
     assert relerr(R.data_ro, Rref) < 1e-12
+
+
+SPLIT_PARTS = {
+    "defaults": None,
+    "tiles384_atomics": [{"strategy": "rowgather_slots", "batch": [384, 480], "block": 192},
+                         {"strategy": "staged", "tile": 384, "lds": 40960, "block": 192, "partials": False}],
+}
+
+
+@pytest.mark.parametrize("overlap,parts", [("before", "defaults"), ("after", "defaults"),
+                                           ("after", "tiles384_atomics")])
+def test_p2_fused_32cube_split_overlap(device, tet32, tet32_oracle, monkeypatch, overlap, parts):
+    """The split loop with its residual overlapped on a side stream: two
+    passes in a row (the second adds onto the first) equal twice the
+    oracle, and the calling stream sees both parts' results."""
+    from pyop2_amd import autotune
+    configuration["strategy"] = "auto"
+    cfg = {"strategy": "split", "overlap": overlap, "parts": SPLIT_PARTS[parts]}
+    monkeypatch.setattr(autotune, "choice", lambda ex, pl: cfg)
+    Rref, Aref = tet32_oracle
+    p = TetProblem(tet32)
+    A = op2.Mat(p.sparsity(), np.float64)
+    R = op2.Dat(p.nodes ** 3, name="R")
+    for _ in range(2):
+        op2.par_loop(elements.p2_elasticity_fused(), p.cells, A(op2.INC, (p.c2n, p.c2n)), R(op2.INC, p.c2n),
+                     p.X(op2.READ, p.c2v), p.u(op2.READ, p.c2n))
+    A.assemble()
+    _, _, v = A._host_csr()
+    assert relerr(v, 2 * Aref.values) < 1e-12
+    assert relerr(R.data_ro, 2 * Rref) < 1e-12
 
 
 def test_p2_fused_32cube_f32(device, tet32):
