@@ -63,8 +63,10 @@ SPLIT_OVERLAP = [None, "before", "after"]
 ROWGATHER_PARAMS = {
     # P2 BSR(3) float64 / float32: (incidences, blocks) of a batch
     "p2": [{"batch": None, "block": 256},
+           {"batch": [384, 480], "block": 256},
+           {"batch": [320, 400], "block": 256},
+           {"batch": [512, 640], "block": 256},
            {"batch": [384, 480], "block": 192},
-           {"batch": [512, 640], "block": 192},
            {"batch": [768, 880], "block": 256}],
     # P1 CSR
     "p1": [{"batch": None, "block": 256},

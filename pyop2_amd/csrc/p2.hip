@@ -496,6 +496,11 @@ struct MatrixRow {
   // (three waves per SIMD, the occupancy their LDS allows)
   static constexpr int kMaxPrefetch = sizeof(TT) == 8 ? (kFused ? 8 : 16) : 24;
   static constexpr bool kSpread = true;
+  // split-gather rounds of one incidence per thread for float64: a batch's
+  // block budget (at most 640 blocks, about 160 incidences) leaves threads
+  // to spare, and the second slot's registers would cost a wave per SIMD
+  // (121 registers: four waves per SIMD)
+  static constexpr int kRound = sizeof(TT) == 8 ? 1 : 2;
   const T *X;
   const T *u;
   const int32_t *c2n;

@@ -99,11 +99,12 @@ def _tile_lds(spec, itemsize, max_nodes):
     return b
 
 # Row-batch budgets (max row incidences, max blocks) of the row-gather
-# assembly (LDS <= 64 KiB per workgroup).
+# assembly (LDS <= 64 KiB per workgroup; float64 P2 batches of at most 560
+# blocks take 40,320 bytes, so four workgroups share a CU).
 _BATCH = {("p1_laplace_matrix", "f64"): (1024, 2048),
-          ("p2_elasticity_matrix", "f64"): (512, 640),
+          ("p2_elasticity_matrix", "f64"): (448, 560),
           ("p2_elasticity_matrix", "f32"): (512, 640),
-          ("p2_elasticity_fused", "f64"): (512, 640),
+          ("p2_elasticity_fused", "f64"): (448, 560),
           ("p2_elasticity_fused", "f32"): (512, 640)}
 # The staged variant holds one batch's element block rows in LDS: at most
 # 256 / (column groups of the element) incidences per batch.
