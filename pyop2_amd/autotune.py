@@ -42,7 +42,7 @@ _lock = threading.Lock()
 _memo = {}
 _loaded = False
 
-REPEATS = 3
+REPEATS = 5
 
 # parameter candidates per strategy (the first is the strategy's default)
 STAGED_PARAMS = [
