@@ -290,8 +290,8 @@ class HandWrittenKernel:
         ``cfg["overlap"]`` may run the parts that write no matrix on a side
         stream (default priority), forked from and joined back into the
         calling stream: ``"before"`` issues them ahead of the assembly,
-        ``"after"`` behind it.  The latency-bound residual then fills the CU
-        slots the bandwidth-bound assembly leaves idle."""
+        ``"after"`` behind it, and the residual's kernels run beside the
+        assembly's instead of after them."""
         cfg = cfg or {}
         parts_cfg = cfg.get("parts") or [None] * len(self.spec["split"])
         jobs = []
@@ -327,13 +327,16 @@ class HandWrittenKernel:
                 for sub, part, pcfg, sub_out in on_side:
                     sub.launch(start, end, part, pcfg, sub_out)
 
-        if overlap == "before":
-            issue_side()
-        for sub, part, pcfg, sub_out in on_main:
-            sub.launch(start, end, part, pcfg, sub_out)
-        if overlap == "after":
-            issue_side()
-        main.wait_stream(side)
+        try:
+            if overlap == "before":
+                issue_side()
+            for sub, part, pcfg, sub_out in on_main:
+                sub.launch(start, end, part, pcfg, sub_out)
+            if overlap == "after":
+                issue_side()
+        finally:
+            # the calling stream always rejoins what the side stream has
+            main.wait_stream(side)
 
     def launch(self, start, end, parloop, cfg, outputs=None):
         """Launch with the configuration ``cfg`` (a strategy name, or a dict
