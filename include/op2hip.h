@@ -68,8 +68,14 @@ enum op2hip_strategy {
 typedef struct op2hip_exec {
   int32_t strategy;            /* enum op2hip_strategy */
   int32_t block_size;          /* threads per workgroup (0: kernel default) */
-  /* OP2HIP_COLOURED: elements of [start,end) sorted by colour */
-  const int32_t *colour_elems;     /* device, end-start entries */
+  /* OP2HIP_COLOURED: elements of [start,end) sorted by colour.
+     OP2HIP_STAGED with ncolours > 0 (coloured tiles, the plan's two-level
+     colouring of doc/sphinx/source/plan.rst:46-51): colour_elems lists the
+     tile ids 0..ntiles-1 sorted by colour, tiles of one colour sharing no
+     node of a map with increments; each colour runs as one launch and every
+     node of a tile is written with a plain read-modify-write (no partials,
+     no atomics, no second pass; tile_shared_off is ignored). */
+  const int32_t *colour_elems;     /* device, end-start entries (tiles: ntiles) */
   const int32_t *colour_offsets;   /* host, ncolours+1 offsets into colour_elems */
   int32_t ncolours;
   /* OP2HIP_STAGED: tiles of the elements of [start,end) (op2hip_plan_tiles);

@@ -13,6 +13,9 @@ is the conflict-resolution strategy plus its launch parameters:
   partial sums added by a second pass, or device atomics) -- the
   reference's plan sizes its mini-partitions to the shared memory the same
   way (``doc/sphinx/source/plan.rst:13-19``);
+* coloured staged tiles (``staged_coloured``, the two-level colouring of
+  ``doc/sphinx/source/plan.rst:46-51``): tile size, LDS budget and workgroup
+  size;
 * row gathers: the batch budget (``batch`` = most row incidences, most
   blocks of a batch, which sets its LDS) and the workgroup size;
 * split loops: each part's configuration, tuned as a loop of its own, then
@@ -55,6 +58,12 @@ STAGED_PARAMS = [
     {"tile": 512, "lds": 65536, "block": 256},
     {"tile": 384, "lds": 40960, "block": 192, "partials": False},
     {"tile": 256, "lds": 65536, "block": 256, "partials": False},
+]
+# coloured tiles: small tiles give each colour's launch more workgroups
+STAGED_COLOURED_PARAMS = [
+    {"tile": 256, "lds": 65536, "block": 256},
+    {"tile": 128, "lds": 32768, "block": 128},
+    {"tile": 384, "lds": 40960, "block": 192},
 ]
 # split loops: the parts in turn on the calling stream, or the parts that
 # write no matrix overlapped on a side stream, issued before / after the
@@ -112,7 +121,7 @@ def candidates(executor, parloop):
         return split + ["rowgather_slots", "rowgather_rows", "rowgather_cells", "rowgather_staged",
                         "rowgather_thread", "rowgather", "atomic"]
     if "staged" in executor.spec and not isinstance(parloop.iterset, Subset):
-        return ["staged", "atomic", "coloured"]
+        return ["staged", "staged_coloured", "atomic", "coloured"]
     return ["atomic", "coloured"]
 
 
@@ -120,6 +129,8 @@ def param_candidates(executor, strategy):
     """Parameter sets of ``strategy`` for this family (the default first)."""
     if strategy == "staged":
         return STAGED_PARAMS
+    if strategy == "staged_coloured":
+        return STAGED_COLOURED_PARAMS
     if strategy in ("rowgather_slots", "rowgather_rows"):
         return ROWGATHER_PARAMS["p2" if executor.family.startswith("p2_") else "p1"]
     return [{}]

@@ -27,7 +27,7 @@ from pyop2_amd.exceptions import ConfigurationError
 
 _Option = namedtuple("_Option", "env type default")
 
-_STRATEGIES = frozenset(("auto", "atomic", "coloured", "staged", "split", "rowgather",
+_STRATEGIES = frozenset(("auto", "atomic", "coloured", "staged", "staged_coloured", "split", "rowgather",
                          "rowgather_thread", "rowgather_slots", "rowgather_staged",
                          "rowgather_cells", "rowgather_rows"))
 

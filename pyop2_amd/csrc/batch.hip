@@ -26,7 +26,8 @@ batched_p1_kernel(const op2hip_batch_mesh *__restrict__ meshes, const int2 *__re
       const TilePlan P{M.ntiles, M.tile_ptr, M.tile_elems,
                        {M.tile_node_ptr, nullptr}, {M.tile_nodes, nullptr},
                        {M.tile_nexcl, nullptr}, {M.elem_local, nullptr},
-                       {M.max_tile_nodes, 0}, {nullptr, nullptr}, {nullptr, nullptr}};
+                       {M.max_tile_nodes, 0}, {nullptr, nullptr}, {nullptr, nullptr},
+                       nullptr, false};
       staged_tile(P1ResidualStaged{M.R, M.X, M.u}, P, it.y & ~OP2HIP_BATCH_TILE);
     }
     __syncthreads();  // the next item reuses the LDS

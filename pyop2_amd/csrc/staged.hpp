@@ -39,6 +39,11 @@ struct TilePlan {
   // shared-node partials (null shared_off[m]: device atomics)
   const int32_t *shared_off[2];
   double *partials[2];
+  // coloured tiles: the launch runs the tiles tile_list[0..gridDim.x), which
+  // share no node, so every node of a tile is written like an exclusive one
+  // (null: tile blockIdx.x of all ntiles, shared nodes as above)
+  const int32_t *tile_list;
+  bool all_exclusive;
 };
 
 constexpr int at_least_1(int v) { return v > 0 ? v : 1; }
@@ -79,7 +84,7 @@ __device__ __forceinline__ void stage_map(const Elem &elem, const TilePlan &P, i
   constexpr int kIn = M == 0 ? Elem::kIn0 : Elem::kIn1;
   constexpr int kOut = M == 0 ? Elem::kOut0 : Elem::kOut1;
   const int32_t n0 = P.node_ptr[M][b], nn = P.node_ptr[M][b + 1] - n0;
-  const int32_t ne = kOut > 0 ? P.nexcl[M][b] : 0;
+  const int32_t ne = kOut > 0 ? (P.all_exclusive ? nn : P.nexcl[M][b]) : 0;
   const int32_t bd = (int32_t)blockDim.x;
   const T *out = nullptr;
   if constexpr (kOut > 0) out = elem.template out<M>();
@@ -117,11 +122,12 @@ __device__ __forceinline__ void flush_map(const Elem &elem, const TilePlan &P, i
   constexpr int kOut = M == 0 ? Elem::kOut0 : Elem::kOut1;
   if constexpr (kOut > 0) {
     const int32_t n0 = P.node_ptr[M][b], nn = P.node_ptr[M][b + 1] - n0;
-    const int32_t ne = P.nexcl[M][b];
+    const int32_t ne = P.all_exclusive ? nn : P.nexcl[M][b];
     const int32_t bd = (int32_t)blockDim.x;
     T *out = elem.template out<M>();
     double *part = nullptr;
-    if (P.shared_off[M]) part = P.partials[M] + ((size_t)P.shared_off[M][b] - ne) * kOut;
+    if (P.shared_off[M] && !P.all_exclusive)
+      part = P.partials[M] + ((size_t)P.shared_off[M][b] - ne) * kOut;
     for (int32_t k0 = threadIdx.x; k0 < nn; k0 += kStageSlots * bd) {
 #pragma unroll
       for (int u = 0; u < kStageSlots; ++u) {
@@ -267,7 +273,8 @@ struct WavesPerEUOf<Elem, std::void_t<decltype(Elem::kWavesPerEU)>> {
 template <class Elem, int kStageSlots>
 __global__ void __launch_bounds__(256, WavesPerEUOf<Elem>::value)
 staged_kernel(Elem elem, TilePlan P) {
-  staged_tile<Elem, kStageSlots>(elem, P, xcd_contiguous(blockIdx.x, P.ntiles));
+  staged_tile<Elem, kStageSlots>(
+      elem, P, P.tile_list ? P.tile_list[blockIdx.x] : xcd_contiguous(blockIdx.x, P.ntiles));
 }
 
 template <class Elem>
@@ -294,17 +301,41 @@ int launch_staged(int32_t start, int32_t end, const Elem &elem, const op2hip_exe
              {ex->max_tile_nodes[0], Elem::kMaps > 1 ? ex->max_tile_nodes[1] : 0},
              {Elem::kOut0 > 0 ? ex->tile_shared_off[0] : nullptr,
               Elem::kMaps > 1 && Elem::kOut1 > 0 ? ex->tile_shared_off[1] : nullptr},
-             {ex->partials[0], Elem::kMaps > 1 ? ex->partials[1] : nullptr}};
+             {ex->partials[0], Elem::kMaps > 1 ? ex->partials[1] : nullptr},
+             nullptr, false};
   const size_t lds = TileLayout<Elem>::bytes(P.max_nodes);
   OP2HIP_REQUIRE(lds <= 64 * 1024, "tile needs %zu bytes of LDS (max 65536)", lds);
   const int bs = block_size_of(ex, 256);
   OP2HIP_REQUIRE(bs > 0 && bs <= 256 && bs % 64 == 0, "staged block size %d", bs);
   // tiles whose nodes take at most four per thread stage in one round
   const int most = std::max(P.max_nodes[0], Elem::kMaps > 1 ? P.max_nodes[1] : 0);
-  if (most <= 4 * bs)
-    staged_kernel<Elem, 4><<<ex->ntiles, bs, lds, s>>>(elem, P);
-  else
-    staged_kernel<Elem, 2><<<ex->ntiles, bs, lds, s>>>(elem, P);
+  auto run = [&](int32_t grid) {
+    if (most <= 4 * bs)
+      staged_kernel<Elem, 4><<<grid, bs, lds, s>>>(elem, P);
+    else
+      staged_kernel<Elem, 2><<<grid, bs, lds, s>>>(elem, P);
+  };
+  if (ex->ncolours > 0) {
+    // coloured tiles: one launch per colour; the tiles of a colour share no
+    // node, so no partials, atomics or second pass
+    OP2HIP_REQUIRE(ex->colour_elems && ex->colour_offsets,
+                   "coloured staged launch without its tile colouring");
+    OP2HIP_REQUIRE(ex->colour_offsets[0] == 0 && ex->colour_offsets[ex->ncolours] == ex->ntiles,
+                   "tile colouring covers %d of %d tiles", ex->colour_offsets[ex->ncolours],
+                   ex->ntiles);
+    P.all_exclusive = true;
+    P.shared_off[0] = P.shared_off[1] = nullptr;
+    for (int32_t c = 0; c < ex->ncolours; ++c) {
+      const int32_t lo = ex->colour_offsets[c], hi = ex->colour_offsets[c + 1];
+      OP2HIP_REQUIRE(hi >= lo, "tile colouring offsets decrease at colour %d", c);
+      if (hi == lo) continue;
+      P.tile_list = ex->colour_elems + lo;
+      run(hi - lo);
+    }
+    OP2HIP_LAUNCHED();
+    return OP2HIP_OK;
+  }
+  run(ex->ntiles);
   OP2HIP_LAUNCHED();
   if constexpr (Elem::kOut0 > 0) {
     if (P.shared_off[0] && ex->nshared[0] > 0)

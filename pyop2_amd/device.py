@@ -21,7 +21,8 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "li
 OK, ERR_HIP, ERR_ARG, ERR_WORKSPACE, ERR_OVERFLOW, ERR_JIT, ERR_PATTERN = range(7)
 (ATOMIC, COLOURED, STAGED, ROWGATHER, ROWGATHER_THREAD, ROWGATHER_SLOTS, ROWGATHER_STAGED,
  ROWGATHER_CELLS, ROWGATHER_ROWLIST) = range(9)
-STRATEGIES = {"atomic": ATOMIC, "coloured": COLOURED, "staged": STAGED, "rowgather": ROWGATHER,
+STRATEGIES = {"atomic": ATOMIC, "coloured": COLOURED, "staged": STAGED, "staged_coloured": STAGED,
+              "rowgather": ROWGATHER,
               "rowgather_thread": ROWGATHER_THREAD, "rowgather_slots": ROWGATHER_SLOTS,
               "rowgather_staged": ROWGATHER_STAGED, "rowgather_cells": ROWGATHER_CELLS,
               "rowgather_rows": ROWGATHER_ROWLIST}

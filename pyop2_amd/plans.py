@@ -285,6 +285,23 @@ class TilePlan:
     shared_slots: list = None
     nslot_total: list = None
     workspaces: dict = field(default_factory=dict)
+    host_node_ptr: list = None     # per map, host copies for the tile colouring
+    host_nodes: list = None
+    colourings: dict = field(default_factory=dict)
+
+    def colouring(self, maps, sizes):
+        """Greedy colouring of the tiles (the plan's partition colouring,
+        ``doc/sphinx/source/plan.rst:46-51, 78-80``): tiles of one colour share
+        no node of the staged maps ``maps`` (indices; ``sizes`` their target
+        set sizes), so they can write every node without atomics.  Built once
+        per map selection."""
+        key = tuple(maps)
+        hit = self.colourings.get(key)
+        if hit is None:
+            hit = self.colourings[key] = colour_tiles(
+                self.ntiles, [self.host_node_ptr[m] for m in maps], [self.host_nodes[m] for m in maps],
+                list(sizes))
+        return hit
 
     def partials(self, m, nout):
         """Device workspace of map m's partial slots (nout float64 each),
@@ -316,6 +333,60 @@ class TilePlan:
                 ex.shared_ptr[m] = dev.ptr(self.shared_ptr[m])
                 ex.shared_slots[m] = dev.ptr(self.shared_slots[m])
                 ex.partials[m] = dev.ptr(self.partials(m, nouts[m]))
+
+
+@dataclass
+class TileColouring:
+    """Tiles sorted by colour (device) and the colour boundaries (host)."""
+    order: object
+    offsets: np.ndarray
+    ncolours: int
+
+    def exec_fields(self, ex):
+        ex.colour_elems = dev.ptr(self.order)
+        ex.colour_offsets = self.offsets.ctypes.data
+        ex.ncolours = self.ncolours
+
+
+def tile_node_table(ntiles, node_ptr, nodes):
+    """The tiles' node lists as a fixed-arity table (rows padded with -1)."""
+    counts = np.diff(node_ptr[:ntiles + 1]).astype(np.int64)
+    width = int(counts.max(initial=0))
+    table = np.full((max(ntiles, 1), max(width, 1)), -1, dtype=np.int32)
+    rows = np.repeat(np.arange(ntiles, dtype=np.int64), counts)
+    cols = np.arange(int(counts.sum()), dtype=np.int64) - np.repeat(node_ptr[:ntiles].astype(np.int64), counts)
+    table[rows, cols] = nodes[:int(counts.sum())]
+    return table
+
+
+def tile_colours(ntiles, node_ptrs, nodes, sizes):
+    """Greedy colouring of tiles over their node lists (one list per map):
+    the same first-free-colour bitmask colouring as the elements'
+    (``op2hip_colour_greedy``), with each tile's nodes as its map row.
+    Returns (host colours int32[ntiles], number of colours)."""
+    L = dev.lib()
+    tables = [tile_node_table(ntiles, p, n) for p, n in zip(node_ptrs, nodes)]
+    nm = len(tables)
+    ptrs = (ctypes.c_void_p * max(nm, 1))(*[t.ctypes.data for t in tables])
+    ar = np.array([t.shape[1] for t in tables] or [0], dtype=np.int32)
+    tid = np.arange(max(nm, 1), dtype=np.int32)
+    tsz = np.array(sizes or [0], dtype=np.int32)
+    out = np.empty(max(ntiles, 1), dtype=np.int32)
+    nc = ctypes.c_int32()
+    dev.check(L.op2hip_colour_greedy(int(ntiles), nm, ptrs, ar.ctypes.data, tid.ctypes.data,
+                                     tsz.ctypes.data, nm, out.ctypes.data, ctypes.byref(nc)),
+              "colour_greedy (tiles)")
+    return out[:ntiles], int(nc.value)
+
+
+def colour_tiles(ntiles, node_ptrs, nodes, sizes):
+    """Tiles sorted by their greedy colour (``tile_colours``), on the device."""
+    colours, nc = tile_colours(ntiles, node_ptrs, nodes, sizes)
+    order = np.argsort(colours, kind="stable").astype(np.int32)
+    counts = np.bincount(colours, minlength=nc)
+    offsets = np.zeros(nc + 1, dtype=np.int32)
+    np.cumsum(counts, out=offsets[1:])
+    return TileColouring(dev.to_device(order), offsets, nc)
 
 
 def tile_reduce_plan(L, ntiles, node_ptr, nodes, nexcl, nnodes):
@@ -375,7 +446,9 @@ def tile_plan(maps, start, end, tile_size, node_cap=0):
                     [dev.to_device(x) for x in lcs], [int(v) for v in mx[:nm]],
                     [dev.to_device(r[0]) for r in red], [r[1] for r in red],
                     [dev.to_device(r[2]) for r in red], [dev.to_device(r[3]) for r in red],
-                    [dev.to_device(r[4]) for r in red], [r[5] for r in red])
+                    [dev.to_device(r[4]) for r in red], [r[5] for r in red],
+                    host_node_ptr=[x[:nt + 1] for x in nps],
+                    host_nodes=[nds[k][:int(nps[k][nt])] for k in range(nm)])
     owner._cache[ck] = plan
     return plan
 

@@ -213,7 +213,7 @@ class HandWrittenKernel:
         subset = isinstance(parloop.iterset, Subset)
         if strategy.startswith("rowgather") and (not has_mat or subset):
             return "atomic"
-        if strategy == "staged" and (subset or "staged" not in self.spec):
+        if strategy in ("staged", "staged_coloured") and (subset or "staged" not in self.spec):
             return "atomic"
         if strategy == "split" and "split" not in self.spec:
             return "atomic"
@@ -373,11 +373,20 @@ class HandWrittenKernel:
                                      [(m, id(m.toset)) for m in self._inc_maps(parloop)], key)
             plan.exec_fields(ex)
             keep.append(plan)
-        if strategy == "staged":
+        if strategy in ("staged", "staged_coloured"):
             tp = self._tile_plan(args, start, end, int(cfg.get("tile") or _TILE), int(cfg.get("lds") or _LDS))
             if tp is None:
                 strategy = "atomic"
                 ex.strategy = dev.STRATEGIES[strategy]
+            elif strategy == "staged_coloured":
+                # tiles coloured over the nodes of the maps they increment:
+                # one launch per colour, every node written without atomics
+                inc = [m for m, (_, nout) in enumerate(self.spec["staged_vals"]) if nout > 0]
+                sizes = [args[self.spec["staged"][m]].maps[0].toset.total_size for m in inc]
+                col = tp.colouring(inc, sizes)
+                tp.exec_fields(ex, None)
+                col.exec_fields(ex)
+                keep += [tp, col]
             else:
                 nouts = [nout for _, nout in self.spec["staged_vals"]]
                 partials = cfg.get("partials", configuration["staged_partials"])

@@ -27,7 +27,7 @@ def _jitter(m, rng, amp=0.1):
     return np.ascontiguousarray(X)
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic"])
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic", "staged_coloured"])
 def test_adjoint_loop_matches_oracle(device, strategy):
     use_strategy(strategy)
     m = unit_cube_tet_p2(6, rng=workload_rng(4))

@@ -23,7 +23,8 @@ def mesh256():
     return unit_square_tri(256, rng=workload_rng(1))
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic", "auto"])
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic", "staged_coloured",
+                                      "auto"])
 @pytest.mark.parametrize("split", ["all", "nocore", "half"])
 def test_p1_residual_and_mass(device, mesh256, strategy, split):
     n = mesh256.ncells
@@ -40,7 +41,7 @@ def test_p1_residual_and_mass(device, mesh256, strategy, split):
     assert abs(M.data_ro.sum() - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "staged"])
+@pytest.mark.parametrize("strategy", ["atomic", "staged", "staged_coloured"])
 def test_p1_inc_accumulates(device, mesh256, strategy):
     """INC adds into existing values (two loops = twice the residual)."""
     configuration["strategy"] = strategy

@@ -19,7 +19,7 @@ def tet8():
     return unit_cube_tet_p2(8, rng=workload_rng(3))
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic"])
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic", "staged_coloured"])
 @pytest.mark.parametrize("split", ["all", "half"])
 def test_p2_residual_f64(device, tet8, strategy, split):
     use_strategy(strategy)
@@ -31,7 +31,7 @@ def test_p2_residual_f64(device, tet8, strategy, split):
     assert relerr(R.data_ro, oracle_p2("p2_elasticity_residual", tet8)) < 1e-12
 
 
-@pytest.mark.parametrize("strategy", ["atomic", "staged", "staged_atomic"])
+@pytest.mark.parametrize("strategy", ["atomic", "staged", "staged_atomic", "staged_coloured"])
 def test_p2_residual_f32(device, tet8, strategy):
     use_strategy(strategy)
     p = TetProblem(tet8, dtype=np.float32)

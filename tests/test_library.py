@@ -408,3 +408,41 @@ def test_integration_p2_stub_argtypes_match_the_abi():
     want = [getattr(t, "_type_", None) for t in dev._SIGNATURES["op2hip_wrap_p2_elasticity_fused"][1]]
     got = [kinds[n]._type_ for n in names]
     assert got == want
+
+
+@pytest.mark.parametrize("maps_used,tile", [((0,), 256), ((0, 1), 128), ((0,), 37)])
+def test_tile_colouring_is_conflict_free_and_matches_the_oracle(maps_used, tile):
+    """Partition colouring of the staged tiles (``plan.rst:46-51, 78-80``):
+    tiles of one colour share no node of the coloured maps, and the colours
+    equal the oracle's greedy colouring of the tiles' padded node lists (the
+    whole tile's nodes as one row).  A planted mistake -- colouring over the
+    tiles' exclusive nodes only, dropping the shared ones -- must show a
+    conflict."""
+    from pyop2_amd import plans
+    m = unit_cube_tet_p2(8, rng=workload_rng(3))
+    maps = [np.ascontiguousarray(m.cell_node, np.int32), np.ascontiguousarray(m.cell_vertex, np.int32)]
+    sizes = [m.nnodes, m.nverts]
+    nt, tp, te, nps, nds, nxs, lcs, mx = _tiles(maps, sizes, 0, m.ncells, tile)
+    sel = list(maps_used)
+    colours, nc = plans.tile_colours(nt, [nps[k] for k in sel], [nds[k] for k in sel], [sizes[k] for k in sel])
+    tables = [plans.tile_node_table(nt, nps[k], nds[k]) for k in sel]
+    want, nwant = osp.greedy_colour(nt, tables, list(range(len(sel))), [sizes[k] for k in sel])
+    assert nc == nwant and np.array_equal(colours, want)
+    assert colours.min() == 0 and colours.max() == nc - 1
+
+    def conflicts(col, k):
+        seen = {}
+        for t in range(nt):
+            for v in nds[k][nps[k][t]:nps[k][t + 1]]:
+                key = (int(col[t]), int(v))
+                if key in seen and seen[key] != t:
+                    return True
+                seen[key] = t
+        return False
+    for k in sel:
+        assert not conflicts(colours, k)
+    k = sel[0]
+    excl = [nds[k][nps[k][t]:nps[k][t] + nxs[k][t]] for t in range(nt)]
+    eptr = np.concatenate([[0], np.cumsum([len(x) for x in excl])]).astype(np.int32)
+    wrong, _ = plans.tile_colours(nt, [eptr], [np.concatenate(excl).astype(np.int32)], [sizes[k]])
+    assert conflicts(wrong, k)
