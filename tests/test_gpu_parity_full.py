@@ -167,6 +167,51 @@ def test_p2_fused_32cube_split_overlap(device, tet32, tet32_oracle, monkeypatch,
     assert relerr(R.data_ro, 2 * Rref) < 1e-12
 
 
+@pytest.mark.parametrize("overlap", [None, "after"])
+def test_p2_fused_split_captured_in_a_hip_graph(device, monkeypatch, overlap):
+    """The split P2 loop (and its side-stream overlap) captured in a HIP graph
+    after one eager pass: every replay adds one more pass, checked against the
+    oracle after each replay, with eager work and synchronisation between
+    replays (device arrays read directly, past the Dats' host copies)."""
+    import torch
+    from pyop2_amd import autotune
+    m = unit_cube_tet_p2(8, rng=workload_rng(3))
+    Rref = oracle_p2("p2_elasticity_residual", m)
+    Aref = oracle_p2("p2_elasticity_matrix", m)
+    configuration["strategy"] = "auto"
+    cfg = {"strategy": "split", "overlap": overlap,
+           "parts": [{"strategy": "rowgather_slots"},
+                     {"strategy": "staged", "tile": 256, "lds": 65536, "block": 256}]}
+    monkeypatch.setattr(autotune, "choice", lambda ex, pl: cfg)
+    p = TetProblem(m)
+    A = op2.Mat(p.sparsity(), np.float64)
+    R = op2.Dat(p.nodes ** 3, name="R")
+    pl = op2.ParLoop(elements.p2_elasticity_fused(), p.cells, A(op2.INC, (p.c2n, p.c2n)), R(op2.INC, p.c2n),
+                     p.X(op2.READ, p.c2v), p.u(op2.READ, p.c2n))
+    pl()
+    torch.cuda.synchronize()
+    capture = torch.cuda.Stream()
+    capture.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(capture):
+        with torch.cuda.graph(g, stream=capture):
+            pl()
+    torch.cuda.current_stream().wait_stream(capture)
+    torch.cuda.synchronize()
+    rows, cols, _ = A._host_csr()
+    assert np.array_equal(rows, Aref.rowptr) and np.array_equal(cols, Aref.colidx)
+    for k in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        scratch = torch.ones(1024, dtype=torch.float64, device=R._device_array().device).sum()
+        torch.cuda.synchronize()
+        assert float(scratch) == 1024.0
+        got_r = R._device_array().cpu().numpy()
+        _, _, got_a = A._host_csr()
+        assert relerr(got_r, (k + 2) * Rref) < 1e-12, k
+        assert relerr(got_a, (k + 2) * Aref.values) < 1e-12, k
+
+
 def test_p2_fused_32cube_f32(device, tet32):
     Rref = oracle_p2("p2_elasticity_residual", tet32, coords=tet32.coords.astype(np.float32).astype(np.float64),
                      u=tet32.u.astype(np.float32).astype(np.float64))
