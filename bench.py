@@ -369,7 +369,11 @@ def c3_headline(args):
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": tr_meas[0] if tr_meas else None,
             "traffic_source": tr_meas[1] if tr_meas else None, "traffic_key": tkey,
-            "kernel": f"p2_elasticity_fused ({strategy}): " + ", ".join(f"{k} {v:.1f} us" for k, v in parts.items()),
+            "kernel": (f"p2_elasticity_fused ({strategy}"
+                       + (f", residual overlapped {launch_config['overlap']} the assembly"
+                          if isinstance(launch_config, dict) and launch_config.get("overlap") else "")
+                       + "): " + ", ".join(f"{k} {v:.1f} us" for k, v in parts.items())
+                       + (" between each part's own event pair" if parts else "")),
             "kernel_us": kms * 1e3, "alg_bytes_per_launch": alg}
     detail = dict(cells=m3.ncells, kernel_us=kms * 1e3, kernel_us_bracketed=kms_bracketed * 1e3,
                   kernel_parts_us=parts, launch_config=launch_config,
