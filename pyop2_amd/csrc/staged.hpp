@@ -174,19 +174,6 @@ staged_reduce_kernel(int32_t nshared, const int32_t *__restrict__ nodes,
   for (int a = 0; a < kOut; ++a) o[a] = T(double(o[a]) + sum[a]);
 }
 
-// Elements that read their inputs from the tile's LDS and add their
-// increments into it themselves (Elem::kLdsElement) provide
-//   void element_lds(int32_t e, const T *in0, const uint16_t (&l0)[A0],
-//                    const T *in1, const uint16_t (&l1)[A1], double *acc0, double *acc1) const;
-template <class Elem, class = void>
-struct LdsElementOf {
-  static constexpr bool value = false;
-};
-template <class Elem>
-struct LdsElementOf<Elem, std::void_t<decltype(Elem::kLdsElement)>> {
-  static constexpr bool value = Elem::kLdsElement;
-};
-
 // Tile b of a staged launch (the workgroup's share, or one work item of a
 // batched launch).  The local node ids of a thread's first element are
 // loaded before the staging barrier.
@@ -225,10 +212,6 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
 
   for (int32_t t = threadIdx.x; t < ne; t += blockDim.x) {
     if (t != (int32_t)threadIdx.x) load_locals(p0 + t);
-    if constexpr (LdsElementOf<Elem>::value) {
-      elem.element_lds(elem_id, in0, l0, in1, l1, acc0, acc1);
-      continue;
-    }
     T x0[A0][I0], x1[A1][I1], y0[A0][O0], y1[A1][O1];
 #pragma unroll
     for (int i = 0; i < A0; ++i)
