@@ -156,15 +156,6 @@ __device__ __forceinline__ void test_gradients(const T (&s)[3][3], const Geom<T>
     }
 }
 
-template <typename T, int Q>
-__device__ __forceinline__ void residual_q(const T (&uu)[10][3], const Geom<T> &g, T lam, T mu,
-                                           T (&re)[10][3]) {
-  T gu[3][3], s[3][3];
-  field_grad<T, Q>(uu, g, gu);
-  stress(gu, lam, mu, s);
-  test_gradients<T, Q>(s, g, re);
-}
-
 
 // Vertex-stress form of the element residual.  grad u is linear on the
 // element, so the stress is linear too: s(L) = sum_n L_n s_n with s_n the
@@ -245,9 +236,7 @@ __device__ __forceinline__ void residual_vertex_form(const Geom<T> &g, T lam, T 
     sym_stress(G, T(4) * lam, T(4) * mu, sv[n]);
 #pragma unroll
     for (int c = 0; c < 6; ++c) sv[n][c] -= sP[c];
-    // memory clobber: each vertex's reads stay in its own iteration (not
-    // hoisted and merged across iterations, which keeps them all live)
-    }
+  }
   // pass 3: the ten increments, S = sum_n s_n
   T S[6];
 #pragma unroll
@@ -279,7 +268,9 @@ __device__ __forceinline__ void residual_vertex_form(const Geom<T> &g, T lam, T 
 }
 
 // The element residual in the vertex-stress form (about 1000 flops; the
-// point-by-point form residual_q gives the same values to rounding).
+// point-by-point quadrature form -- field_grad, stress and test_gradients at
+// each of the four points, as the adjoint evaluates it -- gives the same
+// values to rounding).
 template <typename T>
 __device__ __forceinline__ void element_residual(const T (&uu)[10][3], const Geom<T> &g, T lam,
                                                  T mu, T (&re)[10][3]) {
