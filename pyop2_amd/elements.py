@@ -25,6 +25,7 @@ w = 1/24 on the reference tetrahedron) integrates the degree-2 integrands of
 the P2 stiffness exactly.
 """
 import hashlib
+import re
 
 __all__ = ["p1_lumped_mass", "p1_laplace_residual", "p1_laplace_matrix",
            "p2_elasticity_residual", "p2_elasticity_matrix", "p2_elasticity_fused",
@@ -343,9 +344,29 @@ def kernel_source(family, lam=1.0, mu=0.5):
         raise ValueError(f"unknown element kernel family {family!r}") from None
 
 
+# C tokens: string and character literals, identifiers, numbers, operators
+_TOKEN = re.compile(r'"(?:\\.|[^"\\])*"|\'(?:\\.|[^\'\\])*\'|[A-Za-z_]\w*'
+                    r'|(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?[fFlLuU]*'
+                    r'|->|\+\+|--|<<=|>>=|<<|>>|[-+*/%&|^!<>=]=|&&|\|\||[-+*/%&|^!~<>=?:;,.(){}\[\]#]')
+
+
+def normalised_source(code, name):
+    """The kernel's C tokens, comments and layout dropped and its function
+    name replaced by a placeholder: two sources that differ only in
+    formatting, comments or the kernel's name give the same text."""
+    code = re.sub(r"/\*.*?\*/", " ", code, flags=re.S)
+    code = re.sub(r"//[^\n]*", " ", code)
+    toks = _TOKEN.findall(code)
+    if "".join(toks) != re.sub(r"\s+", "", code):
+        # a character no token covers: keep the text itself (nothing dropped)
+        return name + "\0" + code
+    return " ".join("@kernel" if t == name else t for t in toks)
+
+
 def code_digest(code, name):
-    """Stable digest of a kernel's (code, name), the key of the HIP registry."""
-    return hashlib.sha256((name + "\0" + code).encode()).hexdigest()
+    """Digest of a kernel's normalised source (``normalised_source``), the
+    key of the HIP registry."""
+    return hashlib.sha256(normalised_source(code, name).encode()).hexdigest()
 
 
 def _make(family, **params):

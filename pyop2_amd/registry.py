@@ -74,6 +74,20 @@ FAMILIES = {
         staged=[0, 1], staged_vals=[(6, 3), (3, 3)]),
 }
 
+
+def _register_known_sources():
+    """Register the element kernels' sources at their default parameters
+    (the Lame parameters 1 and 0.5 of the P2 kernels), so a loop whose
+    kernel text is one of them -- up to formatting, comments and its name --
+    takes the hand-written entry point however the kernel was built."""
+    from pyop2_amd.elements import kernel_source
+    for family in FAMILIES:
+        params = {} if family.startswith("p1_") else {"lam": 1.0, "mu": 0.5}
+        register_element_kernel(kernel_source(family, **params), family, family, params)
+
+
+_register_known_sources()
+
 # "staged": arguments whose maps are the staged maps 0 and 1 of the entry
 # point's OP2HIP_STAGED launch (tiles of _TILE elements); "staged_vals":
 # per staged map, the values staged (Mat dtype) and accumulated (double) per

@@ -96,3 +96,20 @@ def test_p1_csr_full_size_properties(device):
     rowsum = torch.zeros(m.nverts, dtype=vals.dtype, device=vals.device).index_add_(0, rows, vals)
     diag_scale = vals.abs().max().item()
     assert rowsum.abs().max().item() < 1e-12 * diag_scale
+
+
+def test_user_written_copy_runs_the_hand_written_loop(device, mesh256):
+    """A reformatted and renamed copy of the P1 residual kernel, built with
+    op2.Kernel directly, runs the hand-written entry point and gives the
+    oracle's residual."""
+    import re
+    src = elements.kernel_source("p1_laplace_residual").replace("p1_laplace_residual", "laplace_res")
+    src = "/* user copy */\n" + re.sub(r"[ \t]+", "  ", src.replace(";", " ;\n"))
+    use_strategy("auto")
+    p = TriProblem(mesh256)
+    R = op2.Dat(p.verts, name="R")
+    pl = op2.ParLoop(op2.Kernel(src, "laplace_res"), p.cells, R(op2.INC, p.c2v), p.X(op2.READ, p.c2v),
+                     p.u(op2.READ, p.c2v))
+    pl()
+    assert pl.global_kernel.is_hand_written
+    assert relerr(R.data_ro, oracle_p1("p1_laplace_residual", mesh256)) < TOL
