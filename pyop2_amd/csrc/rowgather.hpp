@@ -785,14 +785,27 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
   }
 }
 
+// Whether the slot-table kernel of an element hands each XCD a contiguous
+// range of batches (Elem::kXcdContiguous, default true) or deals
+// consecutive batches to the XCDs in turn.
+template <class Elem, class = void>
+struct XcdContiguousOf {
+  static constexpr bool value = true;
+};
+template <class Elem>
+struct XcdContiguousOf<Elem, std::void_t<decltype(Elem::kXcdContiguous)>> {
+  static constexpr bool value = Elem::kXcdContiguous;
+};
+
 template <class Elem, int kPre>
 __global__ void __launch_bounds__(256)
 rowgather_slots_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
                        const BatchDesc *__restrict__ descs, int32_t nbatches,
                        const SlotEntry<Elem::kCar> *__restrict__ table,
                        typename Elem::T *__restrict__ rowdat) {
-  slots_batch<Elem, kPre>(descs[xcd_contiguous(blockIdx.x, nbatches)], start, end, elem, A, table,
-                          rowdat);
+  const int32_t b = XcdContiguousOf<Elem>::value ? xcd_contiguous(blockIdx.x, nbatches)
+                                                 : (int32_t)blockIdx.x;
+  slots_batch<Elem, kPre>(descs[b], start, end, elem, A, table, rowdat);
 }
 
 // Deepest old-value prefetch of an element (Elem::kMaxPrefetch when given):
