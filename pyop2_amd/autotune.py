@@ -18,9 +18,10 @@ is the conflict-resolution strategy plus its launch parameters:
   size;
 * row gathers: the batch budget (``batch`` = most row incidences, most
   blocks of a batch, which sets its LDS) and the workgroup size;
-* split loops: each part's configuration, tuned as a loop of its own, then
-  whether the parts that write no matrix overlap the assembly on a side
-  stream (``overlap``).
+* split loops: each part's configuration, tuned as a loop of its own (the
+  best of each staged strategy kept for the parts that write no matrix),
+  then which of those runs beside the assembly and whether it overlaps the
+  assembly on a side stream (``overlap``).
 
 Tuning runs in two stages: every applicable strategy with its default
 parameters, then the parameter candidates of the fastest strategy.  The key
@@ -232,11 +233,16 @@ def _time(executor, parloop, cfg, outputs, repeats):
 
 
 def _tune_split(executor, parloop, outputs, repeats, report):
-    """Each part of a split loop tuned as a loop of its own (strategy fixed
-    to the part's kind, parameters tuned), on the split loop's scratch
-    outputs."""
+    """Each part of a split loop tuned as a loop of its own (the assembly
+    part under the slot-table row gather; the other parts under each staged
+    strategy, partials and coloured tiles), on the split loop's scratch
+    outputs.  The best configuration of every strategy of a part is kept:
+    a part that is slower alone can be the faster one beside the assembly
+    on a side stream, so the overlap modes are timed over those
+    alternatives."""
+    import itertools
     from pyop2_amd.registry import _SubLoop
-    parts = []
+    alternatives = []
     for sub, idx in executor._split_parts():
         part = _SubLoop(parloop.iterset, [parloop.arguments[k] for k in idx])
         sub_out = {}
@@ -245,27 +251,37 @@ def _tune_split(executor, parloop, outputs, repeats, report):
                 sub_out[n] = outputs[k]
         if "mat" in outputs:
             sub_out["mat"] = outputs["mat"]
-        strat = "rowgather_slots" if sub.spec["args"][0][0] == "mat" else "staged"
-        strat = sub.applicable(part, strat)
-        best, best_t = {"strategy": strat}, None
-        for params in param_candidates(sub, strat):
-            cfg = dict(params, strategy=strat)
-            t = _time(sub, part, cfg, sub_out, repeats)
+        wanted = (["rowgather_slots"] if sub.spec["args"][0][0] == "mat"
+                  else ["staged", "staged_coloured"])
+        strats = []
+        for w in wanted:
+            strat = sub.applicable(part, w)
+            if strat not in strats:
+                strats.append(strat)
+        alts = []
+        for strat in strats:
+            best, best_t = {"strategy": strat}, None
+            for params in param_candidates(sub, strat):
+                cfg = dict(params, strategy=strat)
+                t = _time(sub, part, cfg, sub_out, repeats)
+                if report is not None:
+                    report[f"{sub.family}:{json.dumps(cfg, sort_keys=True)}"] = t
+                if t is not None and (best_t is None or t < best_t):
+                    best, best_t = cfg, t
+            alts.append(best)
+        alternatives.append(alts)
+    best, best_t = None, None
+    for parts in itertools.product(*alternatives):
+        for overlap in SPLIT_OVERLAP:
+            cfg = {"strategy": "split", "parts": list(parts)}
+            if overlap:
+                cfg["overlap"] = overlap
+            t = _time(executor, parloop, cfg, outputs, repeats)
             if report is not None:
-                report[f"{sub.family}:{json.dumps(cfg, sort_keys=True)}"] = t
+                names = ",".join(p["strategy"] for p in parts)
+                report[f"split:{names}:overlap={overlap}"] = t
             if t is not None and (best_t is None or t < best_t):
                 best, best_t = cfg, t
-        parts.append(best)
-    best, best_t = None, None
-    for overlap in SPLIT_OVERLAP:
-        cfg = {"strategy": "split", "parts": parts}
-        if overlap:
-            cfg["overlap"] = overlap
-        t = _time(executor, parloop, cfg, outputs, repeats)
-        if report is not None:
-            report[f"split:overlap={overlap}"] = t
-        if t is not None and (best_t is None or t < best_t):
-            best, best_t = cfg, t
     return best
 
 

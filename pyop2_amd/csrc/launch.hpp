@@ -54,4 +54,15 @@ __device__ __forceinline__ int32_t xcd_contiguous(int32_t bid, int32_t nb) {
   return x * q + min(x, rem) + idx;
 }
 
+// Workgroup id -> batch id dealing chunks of kChunk consecutive batches to
+// the XCDs in turn: the chip works through one window of 8 * kChunk batches
+// at a time, each XCD on kChunk neighbours of it.  Workgroups of the last,
+// partial window keep their own id.
+template <int32_t kChunk>
+__device__ __forceinline__ int32_t xcd_dealt(int32_t bid, int32_t nb) {
+  constexpr int32_t kXcd = 8, kWindow = kXcd * kChunk;
+  const int32_t w = bid / kWindow, r = bid % kWindow;
+  return (w + 1) * kWindow <= nb ? w * kWindow + (r % kXcd) * kChunk + r / kXcd : bid;
+}
+
 }  // namespace op2hip

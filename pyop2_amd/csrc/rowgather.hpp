@@ -785,16 +785,17 @@ __device__ __forceinline__ void slots_batch(const BatchDesc &d, int32_t start, i
   }
 }
 
-// Whether the slot-table kernel of an element hands each XCD a contiguous
-// range of batches (Elem::kXcdContiguous, default true) or deals
-// consecutive batches to the XCDs in turn.
+// How the slot-table kernel of an element maps workgroups to batches
+// (Elem::kXcdChunk): 0 (the default) hands each XCD a contiguous range of
+// batches, kChunk > 0 deals chunks of kChunk consecutive batches to the
+// XCDs in turn.
 template <class Elem, class = void>
-struct XcdContiguousOf {
-  static constexpr bool value = true;
+struct XcdChunkOf {
+  static constexpr int32_t value = 0;
 };
 template <class Elem>
-struct XcdContiguousOf<Elem, std::void_t<decltype(Elem::kXcdContiguous)>> {
-  static constexpr bool value = Elem::kXcdContiguous;
+struct XcdChunkOf<Elem, std::void_t<decltype(Elem::kXcdChunk)>> {
+  static constexpr int32_t value = Elem::kXcdChunk;
 };
 
 template <class Elem, int kPre>
@@ -803,8 +804,12 @@ rowgather_slots_kernel(int32_t start, int32_t end, Elem elem, op2hip_mat A,
                        const BatchDesc *__restrict__ descs, int32_t nbatches,
                        const SlotEntry<Elem::kCar> *__restrict__ table,
                        typename Elem::T *__restrict__ rowdat) {
-  const int32_t b = XcdContiguousOf<Elem>::value ? xcd_contiguous(blockIdx.x, nbatches)
-                                                 : (int32_t)blockIdx.x;
+  constexpr int32_t kChunk = XcdChunkOf<Elem>::value;
+  int32_t b;
+  if constexpr (kChunk > 0)
+    b = xcd_dealt<kChunk>(blockIdx.x, nbatches);
+  else
+    b = xcd_contiguous(blockIdx.x, nbatches);
   slots_batch<Elem, kPre>(descs[b], start, end, elem, A, table, rowdat);
 }
 

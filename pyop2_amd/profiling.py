@@ -10,7 +10,10 @@ Usage::
             [--copy SUB --copy-bytes N]
         HBM bytes per dispatch of the kernels whose name contains SUB, from a
         FETCH_SIZE run and a WRITE_SIZE run (both in KiB per dispatch), each
-        scaled by bytes / counter of a plain copy kernel of known size
+        scaled by bytes / counter of a plain copy kernel of known size;
+        with --per LOOPSUB, bytes per dispatch of the kernel LOOPSUB instead
+        (the run's total over SUB's dispatches / LOOPSUB's dispatch count:
+        per loop launch, for a kernel dispatched once per colour)
 
 The ``traffic`` report ends with one line per kernel,
 ``traffic_bytes_per_launch <kernel-substring> <bytes>``, that the benchmark
@@ -82,8 +85,9 @@ def _pick(table, sub):
     return hits
 
 
-def traffic(fetch_dirs, write_dirs, kernel_sub, copy_sub=None, copy_bytes=None):
-    """Measured HBM bytes per dispatch: (report lines, {kernel: bytes})."""
+def traffic(fetch_dirs, write_dirs, kernel_sub, copy_sub=None, copy_bytes=None, per_sub=None):
+    """Measured HBM bytes per dispatch (per dispatch of the kernel matching
+    ``per_sub`` when given): (report lines, {kernel: bytes})."""
     fetch = counter_means(fetch_dirs)
     write = counter_means(write_dirs)
     ffac = wfac = 1.0
@@ -96,15 +100,23 @@ def traffic(fetch_dirs, write_dirs, kernel_sub, copy_sub=None, copy_bytes=None):
         lines.append(f"calibration kernel: {ck[:100]}")
         lines.append(f"  moves {copy_bytes} B each way; FETCH_SIZE reads {fk:.0f} B (factor {ffac:.3f}), "
                      f"WRITE_SIZE reads {wk:.0f} B (factor {wfac:.3f})")
+    launches = None
+    if per_sub:
+        pk = _pick(fetch, per_sub)[0]
+        launches = fetch[pk]["FETCH_SIZE"][1]
+        lines.append(f"per launch: {launches} dispatches of {pk[:100]}")
     res = {}
     for k in _pick(fetch, kernel_sub):
-        f = fetch[k]["FETCH_SIZE"][0] * 1024.0 * ffac
+        n = fetch[k]["FETCH_SIZE"][1]
+        scale = n / launches if launches else 1.0
+        f = fetch[k]["FETCH_SIZE"][0] * 1024.0 * ffac * scale
         wks = [w for w in write if w == k]
-        w = write[wks[0]]["WRITE_SIZE"][0] * 1024.0 * wfac if wks else float("nan")
+        w = write[wks[0]]["WRITE_SIZE"][0] * 1024.0 * wfac * scale if wks else float("nan")
         res[k] = f + w
         lines.append(f"kernel: {k[:140]}")
-        lines.append(f"  dispatches {fetch[k]['FETCH_SIZE'][1]}; fetched {f:.4g} B, written {w:.4g} B, "
-                     f"total {f + w:.4g} B per dispatch (calibrated)")
+        unit = "per launch" if launches else "per dispatch"
+        lines.append(f"  dispatches {n}; fetched {f:.4g} B, written {w:.4g} B, "
+                     f"total {f + w:.4g} B {unit} (calibrated)")
     return lines, res
 
 
@@ -130,6 +142,7 @@ def main(argv=None):
     t.add_argument("--key", help="substring written on the traffic line (default: --kernel)")
     t.add_argument("--copy")
     t.add_argument("--copy-bytes", type=float)
+    t.add_argument("--per", help="report bytes per dispatch of the kernel containing this")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         st = kernel_stats(a.dirs)
@@ -148,7 +161,7 @@ def main(argv=None):
             for ctr, (v, n) in sorted(ctrs.items()):
                 print(f"    {ctr:28s} {v:16.6g}   ({n} dispatches)")
     else:
-        lines, res = traffic(a.fetch, a.write, a.kernel, a.copy, a.copy_bytes)
+        lines, res = traffic(a.fetch, a.write, a.kernel, a.copy, a.copy_bytes, a.per)
         print("\n".join(lines))
         for k, v in res.items():
             print(f"traffic_bytes_per_launch {a.key or a.kernel} {v:.0f}")

@@ -141,11 +141,14 @@ SPLIT_PARTS = {
     "defaults": None,
     "tiles384_atomics": [{"strategy": "rowgather_slots", "batch": [384, 480], "block": 192},
                          {"strategy": "staged", "tile": 384, "lds": 40960, "block": 192, "partials": False}],
+    "tiles384_coloured": [{"strategy": "rowgather_slots", "batch": [384, 480], "block": 192},
+                          {"strategy": "staged_coloured", "tile": 384, "lds": 40960, "block": 192}],
 }
 
 
 @pytest.mark.parametrize("overlap,parts", [("before", "defaults"), ("after", "defaults"),
-                                           ("after", "tiles384_atomics")])
+                                           ("after", "tiles384_atomics"),
+                                           ("after", "tiles384_coloured")])
 def test_p2_fused_32cube_split_overlap(device, tet32, tet32_oracle, monkeypatch, overlap, parts):
     """The split loop with its residual overlapped on a side stream: two
     passes in a row (the second adds onto the first) equal twice the
@@ -167,8 +170,9 @@ def test_p2_fused_32cube_split_overlap(device, tet32, tet32_oracle, monkeypatch,
     assert relerr(R.data_ro, 2 * Rref) < 1e-12
 
 
-@pytest.mark.parametrize("overlap", [None, "after"])
-def test_p2_fused_split_captured_in_a_hip_graph(device, monkeypatch, overlap):
+@pytest.mark.parametrize("overlap,residual", [(None, "staged"), ("after", "staged"),
+                                              ("after", "staged_coloured")])
+def test_p2_fused_split_captured_in_a_hip_graph(device, monkeypatch, overlap, residual):
     """The split P2 loop (and its side-stream overlap) captured in a HIP graph
     after one eager pass: every replay adds one more pass, checked against the
     oracle after each replay, with eager work and synchronisation between
@@ -181,7 +185,7 @@ def test_p2_fused_split_captured_in_a_hip_graph(device, monkeypatch, overlap):
     configuration["strategy"] = "auto"
     cfg = {"strategy": "split", "overlap": overlap,
            "parts": [{"strategy": "rowgather_slots"},
-                     {"strategy": "staged", "tile": 256, "lds": 65536, "block": 256}]}
+                     {"strategy": residual, "tile": 256, "lds": 65536, "block": 256}]}
     monkeypatch.setattr(autotune, "choice", lambda ex, pl: cfg)
     p = TetProblem(m)
     A = op2.Mat(p.sparsity(), np.float64)
