@@ -708,13 +708,19 @@ def run(args):
     if not args.primary_only:
         out.update(secondary(args, m3))
         out["c5_batched"] = batched(args)
-    copy_gbs, copy_ms = copy_bandwidth()
-    out["stream_copy"] = dict(bytes_each_way=1 << 29, ms=copy_ms, gbs=copy_gbs)
+    # 512 MiB each way: the calibration kernel of the FETCH_SIZE / WRITE_SIZE
+    # profiles; 4 GiB each way: the streaming rate with the 256 MB last-level
+    # cache out of play, which the roofline's copy fraction uses
+    cal_gbs, cal_ms = copy_bandwidth()
+    copy_gbs, copy_ms = copy_bandwidth(1 << 32, reps=5)
+    out["stream_copy"] = dict(bytes_each_way=1 << 32, ms=copy_ms, gbs=copy_gbs,
+                              calibration=dict(bytes_each_way=1 << 29, ms=cal_ms, gbs=cal_gbs))
     # every launch configuration the autotuner holds (key: family, precision,
     # iteration set sizes, map content digests)
     from pyop2_amd import autotune
     out["autotune"] = autotune.decisions()
-    log(f"streaming y = -x over 512 MiB: {copy_ms * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)")
+    log(f"streaming y = -x over 512 MiB: {cal_ms * 1e3:.1f} us = {cal_gbs:.0f} GB/s; over 4 GiB: "
+        f"{copy_ms * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)")
     head["roofline"]["frac_of_measured_copy"] = head["roofline"]["achieved"] / copy_gbs
     n3 = head["workload"]
     d = head["detail"]
