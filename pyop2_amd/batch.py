@@ -28,7 +28,12 @@ __all__ = ["BatchedLaplace", "BatchMesh"]
 
 BATCH_TILE = 0x40000000
 _ROW_BUDGET = (1024, 2048)   # (max row incidences, max nnz) of a row batch
-_TILE = 256
+# elements per residual tile: the batched tiles add the nodes they share with
+# device atomics, so larger tiles (fewer shared nodes, more work per item)
+# pay until their LDS limits the workgroups a CU holds.  512 meshes of
+# 16^2..128^2 squares: 346 / 291 / 273 / 257 / 262 / 278 us with tiles of
+# 256 / 512 / 768 / 1024 / 1280 / 1536 elements.
+_TILE = 1024
 
 
 class BatchMesh(ctypes.Structure):
