@@ -142,19 +142,27 @@ def measured_traffic(key):
     return None
 
 
-def copy_bandwidth(nbytes=1 << 29, reps=10):
-    """Streaming kernel y = -x over ``nbytes`` each way (HIP events): the
-    measured copy bandwidth, and the calibration kernel (torch's neg kernel,
-    known bytes) of the FETCH_SIZE / WRITE_SIZE profiles."""
+def copy_bandwidth(nbytes=1 << 29, reps=10, calibration=True):
+    """Streaming kernel over ``nbytes`` each way (HIP events): the measured
+    copy bandwidth.  With ``calibration`` it is y = -x (torch's neg kernel),
+    the calibration kernel of known bytes of the FETCH_SIZE / WRITE_SIZE
+    profiles, so only those launches may use it; otherwise y = 2 x (torch's
+    multiply kernel, the same streaming)."""
     from pyop2_amd import device as dev
     torch = dev.torch()
     x = torch.ones(nbytes // 8, dtype=torch.float64, device=dev.device())
     y = torch.empty_like(x)
-    torch.neg(x, out=y)
+
+    def step():
+        if calibration:
+            torch.neg(x, out=y)
+        else:
+            torch.mul(x, 2.0, out=y)
+    step()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     for a, b in ev:
         a.record()
-        torch.neg(x, out=y)
+        step()
         b.record()
     torch.cuda.synchronize()
     ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
@@ -712,14 +720,14 @@ def run(args):
     # profiles; 4 GiB each way: the streaming rate with the 256 MB last-level
     # cache out of play, which the roofline's copy fraction uses
     cal_gbs, cal_ms = copy_bandwidth()
-    copy_gbs, copy_ms = copy_bandwidth(1 << 32, reps=5)
+    copy_gbs, copy_ms = copy_bandwidth(1 << 32, reps=5, calibration=False)
     out["stream_copy"] = dict(bytes_each_way=1 << 32, ms=copy_ms, gbs=copy_gbs,
                               calibration=dict(bytes_each_way=1 << 29, ms=cal_ms, gbs=cal_gbs))
     # every launch configuration the autotuner holds (key: family, precision,
     # iteration set sizes, map content digests)
     from pyop2_amd import autotune
     out["autotune"] = autotune.decisions()
-    log(f"streaming y = -x over 512 MiB: {cal_ms * 1e3:.1f} us = {cal_gbs:.0f} GB/s; over 4 GiB: "
+    log(f"streaming y = -x over 512 MiB: {cal_ms * 1e3:.1f} us = {cal_gbs:.0f} GB/s; y = 2 x over 4 GiB: "
         f"{copy_ms * 1e3:.1f} us = {copy_gbs:.0f} GB/s (read + write)")
     head["roofline"]["frac_of_measured_copy"] = head["roofline"]["achieved"] / copy_gbs
     n3 = head["workload"]
