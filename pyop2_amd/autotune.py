@@ -168,7 +168,13 @@ def choice(executor, parloop):
     """The loop's tuned configuration (tuned now on a first encounter)."""
     if not configuration["autotune"]:
         return default(executor, parloop)
-    key = _key(executor, parloop)
+    # the key is fixed by the loop object (set sizes, map contents): built
+    # once per loop and family
+    held = getattr(parloop, "__dict__", None)
+    keys = held.setdefault("_tune_keys", {}) if held is not None else {}
+    key = keys.get((executor.family, executor.variant))
+    if key is None:
+        key = keys[(executor.family, executor.variant)] = _key(executor, parloop)
     with _lock:
         _load()
         hit = _memo.get(key)

@@ -24,6 +24,7 @@ do).  The 4-point Gauss rule (a = 0.5854101966249685, b = 0.1381966011250105,
 w = 1/24 on the reference tetrahedron) integrates the degree-2 integrands of
 the P2 stiffness exactly.
 """
+import functools
 import hashlib
 import re
 
@@ -363,9 +364,11 @@ def normalised_source(code, name):
     return " ".join("@kernel" if t == name else t for t in toks)
 
 
+@functools.lru_cache(maxsize=1024)
 def code_digest(code, name):
     """Digest of a kernel's normalised source (``normalised_source``), the
-    key of the HIP registry."""
+    key of the HIP registry (kept per source text: a program that builds the
+    same kernel for every loop call normalises it once)."""
     return hashlib.sha256(normalised_source(code, name).encode()).hexdigest()
 
 

@@ -101,9 +101,17 @@ class LocalKernel:
     @property
     def arguments(self):
         accesses, dtypes = self.accesses, self.dtypes
+        # built once per signature (read several times by every loop call);
+        # only tuples are trusted not to change under the cache
+        hit = self.__dict__.get("_arguments")
+        if hit is not None and hit[0] is accesses and hit[1] is dtypes:
+            return hit[2]
         if accesses is None or dtypes is None or len(accesses) != len(dtypes):
             raise RuntimeError(f"kernel {self.name} has no complete argument signature yet")
-        return tuple(map(LocalKernelArg, accesses, dtypes))
+        out = tuple(map(LocalKernelArg, accesses, dtypes))
+        if isinstance(accesses, tuple) and isinstance(dtypes, tuple):
+            self._arguments = (accesses, dtypes, out)
+        return out
 
     @property
     def num_flops(self):

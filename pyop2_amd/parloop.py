@@ -231,6 +231,23 @@ class Parloop:
 
     def replace_data(self, index, new_argument):
         self.arguments[index].data = new_argument
+        self._static = None
+
+    def _static_parts(self):
+        """What every call reads but only the bound arguments fix: the written
+        arguments, the indirect Dats each exchange half visits (first use
+        order, each once), and whether any of them has a halo at all."""
+        st = self.__dict__.get("_static")
+        if st is None:
+            written = [(garg, arg, larg.access) for larg, garg, arg in self._triples()
+                       if larg.access != Access.READ and arg.kind != "passthrough"]
+            exch = []
+            for larg, garg, arg in self._triples():
+                if isinstance(garg, (DatKernelArg, MixedDatKernelArg)) and garg.is_indirect:
+                    exch.append((larg.access, arg.data))
+            halo = any(data._has_halo() for _, data in exch)
+            st = self._static = (written, exch, halo)
+        return st
 
     # -- running --------------------------------------------------------------------
     def _compute(self, part):
@@ -254,8 +271,7 @@ class Parloop:
         self._exchange("local_to_global_end", _EXCHANGED_AFTER)
 
     def _written(self):
-        return [(garg, arg, larg.access) for larg, garg, arg in self._triples()
-                if larg.access != Access.READ and arg.kind != "passthrough"]
+        return self._static_parts()[0]
 
     def _prepare_outputs(self):
         """New versions for written data (a temporary's Global is the one
@@ -272,14 +288,16 @@ class Parloop:
 
     def _exchange(self, step, modes):
         """Calls ``step`` (a halo exchange half) once per distinct indirect Dat
-        argument accessed with one of ``modes``."""
+        argument accessed with one of ``modes`` (nothing to do when none of
+        them has a halo, as on one rank)."""
+        _, exch, halo = self._static_parts()
+        if not halo:
+            return
         done = []
-        for larg, garg, arg in self._triples():
-            if not isinstance(garg, (DatKernelArg, MixedDatKernelArg)) or not garg.is_indirect:
-                continue
-            if larg.access in modes and not any(arg.data is d for d in done):
-                done.append(arg.data)
-                getattr(arg.data, step)(larg.access)
+        for access, data in exch:
+            if access in modes and not any(data is d for d in done):
+                done.append(data)
+                getattr(data, step)(access)
 
     def _mark_outputs(self):
         for garg, arg, access in self._written():

@@ -303,6 +303,10 @@ class AbstractDat(DataCarrier, DeviceStorage):
         halo = self.dataset.halo
         return None if (halo is None or self._halo_frozen) else halo
 
+    def _has_halo(self):
+        """Whether the Dat's set has a halo object (exchanges can do work)."""
+        return self.dataset.halo is not None
+
     def _reset_halo_entries(self, access_mode):
         from pyop2_amd.datatypes import dtype_limits
         lowest, highest = dtype_limits(self.dtype)

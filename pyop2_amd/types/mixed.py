@@ -283,6 +283,9 @@ class MixedDat:
     def norm(self):
         return self.inner(self) ** 0.5
 
+    def _has_halo(self):
+        return any(d._has_halo() for d in self._dats)
+
     def global_to_local_begin(self, access_mode):
         for d in self._dats:
             d.global_to_local_begin(access_mode)
