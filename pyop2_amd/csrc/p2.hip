@@ -728,6 +728,11 @@ struct ResidualStaged {
 template <typename TT>
 struct AdjointStaged {
   using T = TT;
+  // reads u and Rbar from the tile's LDS records at each quadrature point:
+  // 210 registers (two waves per SIMD) against 256 + 40 accumulation
+  // registers (one wave) for the same element taking its 60 inputs as
+  // arrays; 230 against 257-262 us at 64^3
+  static constexpr bool kLdsElement = true;
   static constexpr int kMaps = 2;
   static constexpr int kAr0 = 10, kIn0 = 6, kOut0 = 3;
   static constexpr int kAr1 = 4, kIn1 = 3, kOut1 = 3;
@@ -752,8 +757,9 @@ struct AdjointStaged {
   }
   template <int M>
   __host__ __device__ __forceinline__ T *out() const { return M == 0 ? ubar : Xbar; }
-  __device__ __forceinline__ void element(int32_t, const T (&ue)[10][6], const T (&xv)[4][3],
-                                          T (&ub)[10][3], T (&xb)[4][3]) const;
+  __device__ __forceinline__ void element_lds(int32_t, const T *in0, const uint16_t (&l0)[10],
+                                              const T *in1, const uint16_t (&l1)[4],
+                                              T (&ub)[10][3], T (&xb)[4][3]) const;
 };
 
 // Adjoint of the residual loop for one element (see header of the entry point).
@@ -817,19 +823,109 @@ __device__ __forceinline__ void element_adjoint(const T (&uu)[10][3], const T (&
   }
 }
 
-template <typename TT>
-__device__ __forceinline__ void AdjointStaged<TT>::element(int32_t, const TT (&ue)[10][6],
-                                                           const TT (&xv)[4][3], TT (&ub)[10][3],
-                                                           TT (&xb)[4][3]) const {
-  TT uu[10][3], ee[10][3];
+
+// Barycentric-form gradient at point Q of a field held in LDS: component c
+// of node j at in[l[j] * stride + off + c].
+template <typename T, int Q>
+__device__ __forceinline__ void field_grad_lds(const T *in, const uint16_t (&l)[10], int stride,
+                                               int off, const Geom<T> &g, T (&grad)[3][3]) {
+  T w[4][3];
 #pragma unroll
-  for (int j = 0; j < 10; ++j)
+  for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      uu[j][a] = ue[j][a];
-      ee[j][a] = ue[j][3 + a];
+    for (int b = 0; b < 3; ++b) w[m][b] = T(0);
+#pragma unroll
+  for (int j = 0; j < 10; ++j) {
+    const T *f = in + (size_t)l[j] * stride + off;
+    const T f0 = f[0], f1 = f[1], f2 = f[2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (dcoef(Q, j, m) == 0.0) continue;
+      const T c = T(dcoef(Q, j, m));
+      w[m][0] += c * f0;
+      w[m][1] += c * f1;
+      w[m][2] += c * f2;
     }
-  element_adjoint(uu, ee, geometry_of(xv), lam, mu, ub, xb);
+  }
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T acc = T(0);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc += w[m][b] * g.gl[m][k];
+      grad[b][k] = acc;
+    }
+}
+
+// The shape term of the adjoint at point Q, P += phi I - gu^T se - ge^T su
+// (u and Rbar read from the staged records, six values a node).  The
+// scheduling barrier keeps the next point's reads after this point's work,
+// so the element never holds all its inputs at once.
+template <typename T, int Q>
+__device__ __forceinline__ void shape_q_lds(const T *in0, const uint16_t (&l0)[10],
+                                            const Geom<T> &g, T lam, T mu, T (&P)[3][3]) {
+  T gu[3][3], ge[3][3], su[3][3], se[3][3];
+  field_grad_lds<T, Q>(in0, l0, 6, 0, g, gu);
+  field_grad_lds<T, Q>(in0, l0, 6, 3, g, ge);
+  stress(gu, lam, mu, su);
+  stress(ge, lam, mu, se);
+  T phi = T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) phi += su[a][k] * ge[a][k];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T acc = (l == k) ? phi : T(0);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc -= gu[b][l] * se[b][k] + ge[b][l] * su[b][k];
+      P[l][k] += acc;
+    }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Staged adjoint element: xb from the shape term at the four points, ub =
+// K_e Rbar as the residual of Rbar (the residual is linear in u and K_e
+// symmetric), both reading the staged records on demand.
+template <typename TT>
+__device__ __forceinline__ void AdjointStaged<TT>::element_lds(int32_t, const TT *in0,
+                                                               const uint16_t (&l0)[10],
+                                                               const TT *in1,
+                                                               const uint16_t (&l1)[4],
+                                                               TT (&ub)[10][3],
+                                                               TT (&xb)[4][3]) const {
+  TT xv[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xv[v][a] = in1[(size_t)l1[v] * 3 + a];
+  const Geom<TT> g = geometry_of(xv);
+  TT P[3][3] = {};
+  shape_q_lds<TT, 0>(in0, l0, g, lam, mu, P);
+  shape_q_lds<TT, 1>(in0, l0, g, lam, mu, P);
+  shape_q_lds<TT, 2>(in0, l0, g, lam, mu, P);
+  shape_q_lds<TT, 3>(in0, l0, g, lam, mu, P);
+  // dS/dJ = wdet * P K^T ; X[m+1] gets column m, X[0] minus the row sums
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    TT rowsum = TT(0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const TT v = g.wdet * (P[l][0] * g.K[m][0] + P[l][1] * g.K[m][1] + P[l][2] * g.K[m][2]);
+      xb[m + 1][l] = v;
+      rowsum += v;
+    }
+    xb[0][l] = -rowsum;
+  }
+  residual_vertex_form(
+      g, lam, mu, [&](int j, int c) { return in0[(size_t)l0[j] * 6 + 3 + c]; },
+      [&](int i, const TT (&r)[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ub[i][c] = r[c];
+      });
 }
 
 template <typename T, bool kAtomic>

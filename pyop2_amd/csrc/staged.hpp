@@ -21,6 +21,8 @@
 //   template <int M> T *out() const;                             // INC target, kOutM per node
 //   void element(int32_t e, const T (&x0)[A0][I0], const T (&x1)[A1][I1],
 //                T (&y0)[A0][O0], T (&y1)[A1][O1]) const;     // A/I/O >= 1
+//   or, with kLdsElement = true, element_lds(e, in0, l0, in1, l1, y0, y1)
+//   reading the staged records from LDS itself (see LdsElementOf).
 #pragma once
 
 #include "launch.hpp"
@@ -174,6 +176,20 @@ staged_reduce_kernel(int32_t nshared, const int32_t *__restrict__ nodes,
   for (int a = 0; a < kOut; ++a) o[a] = T(double(o[a]) + sum[a]);
 }
 
+// Whether an element reads its staged values from LDS itself
+// (Elem::kLdsElement, with element_lds(e, in0, l0, in1, l1, y0, y1): in0 and
+// in1 the tile's staged records in LDS, l0 and l1 the element's local node
+// ids).  An element whose inputs each feed several terms keeps fewer values
+// live by reading them where they are used than by taking them as arrays.
+template <class Elem, class = void>
+struct LdsElementOf {
+  static constexpr bool value = false;
+};
+template <class Elem>
+struct LdsElementOf<Elem, std::void_t<decltype(Elem::kLdsElement)>> {
+  static constexpr bool value = Elem::kLdsElement;
+};
+
 // Tile b of a staged launch (the workgroup's share, or one work item of a
 // batched launch).  The local node ids of a thread's first element are
 // loaded before the staging barrier.
@@ -227,17 +243,22 @@ __device__ __forceinline__ void staged_tile(const Elem &elem, const TilePlan &P,
     const int32_t pos = pos_of(t);
     if (pos >= ne) continue;
     if (t != (int32_t)threadIdx.x) load_locals(p0 + pos);
-    T x0[A0][I0], x1[A1][I1], y0[A0][O0], y1[A1][O1];
+    T y0[A0][O0], y1[A1][O1];
+    if constexpr (LdsElementOf<Elem>::value) {
+      elem.element_lds(elem_id, in0, l0, in1, l1, y0, y1);
+    } else {
+      T x0[A0][I0], x1[A1][I1];
 #pragma unroll
-    for (int i = 0; i < A0; ++i)
+      for (int i = 0; i < A0; ++i)
 #pragma unroll
-      for (int a = 0; a < I0; ++a) x0[i][a] = Elem::kIn0 > 0 ? in0[(size_t)l0[i] * I0 + a] : T(0);
+        for (int a = 0; a < I0; ++a) x0[i][a] = Elem::kIn0 > 0 ? in0[(size_t)l0[i] * I0 + a] : T(0);
 #pragma unroll
-    for (int i = 0; i < A1; ++i)
+      for (int i = 0; i < A1; ++i)
 #pragma unroll
-      for (int a = 0; a < I1; ++a)
-        x1[i][a] = (Elem::kMaps > 1 && Elem::kIn1 > 0) ? in1[(size_t)l1[i] * I1 + a] : T(0);
-    elem.element(elem_id, x0, x1, y0, y1);
+        for (int a = 0; a < I1; ++a)
+          x1[i][a] = (Elem::kMaps > 1 && Elem::kIn1 > 0) ? in1[(size_t)l1[i] * I1 + a] : T(0);
+      elem.element(elem_id, x0, x1, y0, y1);
+    }
     if constexpr (Elem::kOut0 > 0) {
 #pragma unroll
       for (int i = 0; i < A0; ++i)
