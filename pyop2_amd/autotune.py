@@ -56,6 +56,7 @@ STAGED_PARAMS = [
     {"tile": 384, "lds": 40960, "block": 192},
     {"tile": 384, "lds": 40960, "block": 128},
     {"tile": 384, "lds": 65536, "block": 128},
+    {"tile": 384, "lds": 65536, "block": 192},
     {"tile": 512, "lds": 65536, "block": 128},
     {"tile": 512, "lds": 54272, "block": 256},
     {"tile": 512, "lds": 65536, "block": 256},
