@@ -100,3 +100,25 @@ def oracle_p2(family, mesh, lam=1.0, mu=0.5, u=None, coords=None):
         assert err == 0
         return A
     raise ValueError(family)
+
+
+def repeated_edge_nodes(mesh, fraction=0.05, seed=11):
+    """``mesh`` with the last edge node of a ``fraction`` of its cells replaced
+    by the cell's previous edge node, so those map rows name one node twice
+    (the geometry, read through the vertex map, is unchanged).  A cell is
+    only changed while the node it drops stays referenced by another cell,
+    so every node keeps at least one incidence."""
+    import dataclasses
+    cn = mesh.cell_node.copy()
+    refs = np.bincount(cn.ravel(), minlength=mesh.nnodes)
+    rng = np.random.default_rng(seed)
+    changed = 0
+    for e in rng.permutation(mesh.ncells)[: int(fraction * mesh.ncells)]:
+        if refs[cn[e, 9]] > 1:
+            refs[cn[e, 9]] -= 1
+            refs[cn[e, 8]] += 1
+            cn[e, 9] = cn[e, 8]
+            changed += 1
+    assert changed > 0 and refs.min() > 0
+    return dataclasses.replace(mesh, cell_node=cn)
+

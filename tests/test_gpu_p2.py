@@ -6,7 +6,7 @@ reference Mat is FP64-only, so FP32 has no reference counterpart).
 import numpy as np
 import pytest
 
-from helpers import TetProblem, oracle_p2, relerr, use_strategy
+from helpers import TetProblem, oracle_p2, relerr, repeated_edge_nodes, use_strategy
 from pyop2_amd import elements, op2
 from pyop2_amd.configuration import configuration
 from pyop2_amd.meshes import unit_cube_tet_p2, workload_rng
@@ -85,3 +85,40 @@ def test_p2_bsr_assembly_f32(device, tet8, strategy):
     ref = oracle_p2("p2_elasticity_matrix", tet8, coords=tet8.coords.astype(np.float32).astype(np.float64))
     _, _, v = A._host_csr()
     assert relerr(v, ref.values) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def tet8_repeated(tet8):
+    m = repeated_edge_nodes(tet8)
+    return m, oracle_p2("p2_elasticity_matrix", m), oracle_p2("p2_elasticity_residual", m)
+
+
+@pytest.mark.parametrize("strategy", ["split", "rowgather_slots", "rowgather_rows", "rowgather_cells",
+                                      "rowgather_staged", "rowgather_thread", "atomic"])
+def test_p2_fused_with_repeated_map_entries(device, tet8_repeated, strategy):
+    """Map rows naming a node twice: the reference's wrapper adds both local
+    rows and columns into that node's row and entry, so the Mat and the
+    residual hold the sum of both contributions."""
+    m, Aref, Rref = tet8_repeated
+    configuration["strategy"] = strategy
+    p = TetProblem(m)
+    A = op2.Mat(p.sparsity(), np.float64)
+    R = op2.Dat(p.nodes ** 3, name="R")
+    op2.par_loop(elements.p2_elasticity_fused(), p.cells, A(op2.INC, (p.c2n, p.c2n)), R(op2.INC, p.c2n),
+                 p.X(op2.READ, p.c2v), p.u(op2.READ, p.c2n))
+    A.assemble()
+    rp, ci, v = A._host_csr()
+    assert np.array_equal(rp, Aref.rowptr) and np.array_equal(ci, Aref.colidx)
+    assert relerr(v, Aref.values) < 1e-12
+    assert relerr(R.data_ro, Rref) < 1e-12
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic", "staged_coloured"])
+def test_p2_residual_with_repeated_map_entries(device, tet8_repeated, strategy):
+    m, _, Rref = tet8_repeated
+    use_strategy(strategy)
+    p = TetProblem(m)
+    R = op2.Dat(p.nodes ** 3, name="R")
+    op2.par_loop(elements.p2_elasticity_residual(), p.cells, R(op2.INC, p.c2n), p.X(op2.READ, p.c2v),
+                 p.u(op2.READ, p.c2n))
+    assert relerr(R.data_ro, Rref) < 1e-12

@@ -8,6 +8,7 @@ import re
 import numpy as np
 import pytest
 
+from helpers import repeated_edge_nodes
 from oracle import sparsity as osp
 from pyop2_amd import device as dev
 from pyop2_amd.meshes import unit_square_tri, unit_cube_tet_p2, workload_rng
@@ -106,14 +107,20 @@ def _tiles(maps, sizes, start, end, tile, cap=0):
     return nt, tp[:nt + 1], te[:n], nps, nds, nxs, lcs, mx
 
 
-@pytest.mark.parametrize("start,end,tile,cap", [(0, 3072, 256, 0), (100, 3000, 128, 0), (0, 37, 64, 0),
-                                               (0, 3072, 256, 300), (5, 2000, 256, 40)])
-def test_tiles_partition_elements_and_renumber_nodes(start, end, tile, cap):
+@pytest.mark.parametrize("start,end,tile,cap,repeated", [(0, 3072, 256, 0, False), (100, 3000, 128, 0, False),
+                                                        (0, 37, 64, 0, False), (0, 3072, 256, 300, False),
+                                                        (5, 2000, 256, 40, False), (0, 3072, 256, 0, True),
+                                                        (100, 3000, 128, 0, True)])
+def test_tiles_partition_elements_and_renumber_nodes(start, end, tile, cap, repeated):
     """Every element of [start, end) in exactly one tile of at most ``tile``;
     local ids give back the map entries; a tile's first nexcl nodes occur in
     no other tile, the others in at least two; node lists strictly increase
-    within each part."""
+    within each part.  Also with map rows that name a node twice (both local
+    ids then give the one tile node)."""
     m = unit_cube_tet_p2(8, rng=workload_rng(3))
+    if repeated:
+        m = repeated_edge_nodes(m)
+        assert (m.cell_node[:, 8] == m.cell_node[:, 9]).any()
     maps = [np.ascontiguousarray(m.cell_node, np.int32), np.ascontiguousarray(m.cell_vertex, np.int32)]
     nt, tp, te, nps, nds, nxs, lcs, mx = _tiles(maps, [m.nnodes, m.nverts], start, end, tile, cap)
     if cap:
