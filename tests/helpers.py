@@ -122,3 +122,19 @@ def repeated_edge_nodes(mesh, fraction=0.05, seed=11):
     assert changed > 0 and refs.min() > 0
     return dataclasses.replace(mesh, cell_node=cn)
 
+
+
+def unreferenced_nodes(mesh, nholes=37, seed=5):
+    """``mesh`` with ``nholes`` extra P2 nodes that no cell references,
+    spread through the node numbering: their Mat rows hold only the
+    diagonal block and no tile stages them."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    n = mesh.nnodes + nholes
+    holes = np.sort(rng.choice(n, nholes, replace=False))
+    keep = np.setdiff1d(np.arange(n), holes)        # new id of old node i
+    u = np.empty((n, 3))
+    u[keep] = mesh.u
+    u[holes] = rng.standard_normal((nholes, 3))
+    return dataclasses.replace(mesh, cell_node=keep[mesh.cell_node].astype(np.int32), u=u,
+                               nedges=mesh.nedges + nholes), holes

@@ -6,7 +6,7 @@ reference Mat is FP64-only, so FP32 has no reference counterpart).
 import numpy as np
 import pytest
 
-from helpers import TetProblem, oracle_p2, relerr, repeated_edge_nodes, use_strategy
+from helpers import TetProblem, oracle_p2, relerr, repeated_edge_nodes, unreferenced_nodes, use_strategy
 from pyop2_amd import elements, op2
 from pyop2_amd.configuration import configuration
 from pyop2_amd.meshes import unit_cube_tet_p2, workload_rng
@@ -122,3 +122,43 @@ def test_p2_residual_with_repeated_map_entries(device, tet8_repeated, strategy):
     op2.par_loop(elements.p2_elasticity_residual(), p.cells, R(op2.INC, p.c2n), p.X(op2.READ, p.c2v),
                  p.u(op2.READ, p.c2n))
     assert relerr(R.data_ro, Rref) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def tet8_holes(tet8):
+    m, holes = unreferenced_nodes(tet8)
+    return m, holes, oracle_p2("p2_elasticity_matrix", m), oracle_p2("p2_elasticity_residual", m)
+
+
+@pytest.mark.parametrize("strategy", ["split", "rowgather_slots", "rowgather_rows", "rowgather_cells",
+                                      "rowgather_staged", "rowgather_thread", "atomic"])
+def test_p2_fused_with_unreferenced_nodes(device, tet8_holes, strategy):
+    """Nodes no cell references: their Mat rows hold only the diagonal block
+    (zero), and their residual entries keep the values they had before the
+    loop (the loop increments a residual that starts at one)."""
+    m, holes, Aref, Rref = tet8_holes
+    configuration["strategy"] = strategy
+    p = TetProblem(m)
+    A = op2.Mat(p.sparsity(), np.float64)
+    R = op2.Dat(p.nodes ** 3, np.ones((m.nnodes, 3)), name="R")
+    op2.par_loop(elements.p2_elasticity_fused(), p.cells, A(op2.INC, (p.c2n, p.c2n)), R(op2.INC, p.c2n),
+                 p.X(op2.READ, p.c2v), p.u(op2.READ, p.c2n))
+    A.assemble()
+    rp, ci, v = A._host_csr()
+    assert np.array_equal(rp, Aref.rowptr) and np.array_equal(ci, Aref.colidx)
+    assert relerr(v, Aref.values) < 1e-12
+    assert not np.asarray(v)[rp[holes]].any()
+    assert relerr(R.data_ro, 1.0 + Rref) < 1e-12
+    assert np.array_equal(R.data_ro[holes], np.ones((len(holes), 3)))
+
+
+@pytest.mark.parametrize("strategy", ["atomic", "coloured", "staged", "staged_atomic", "staged_coloured"])
+def test_p2_residual_with_unreferenced_nodes(device, tet8_holes, strategy):
+    m, holes, _, Rref = tet8_holes
+    use_strategy(strategy)
+    p = TetProblem(m)
+    R = op2.Dat(p.nodes ** 3, np.ones((m.nnodes, 3)), name="R")
+    op2.par_loop(elements.p2_elasticity_residual(), p.cells, R(op2.INC, p.c2n), p.X(op2.READ, p.c2v),
+                 p.u(op2.READ, p.c2n))
+    assert relerr(R.data_ro, 1.0 + Rref) < 1e-12
+    assert np.array_equal(R.data_ro[holes], np.ones((len(holes), 3)))
