@@ -583,7 +583,7 @@ struct SplitRound {
     }
   }
 
-  __device__ __forceinline__ void eval(const Elem &elem, double *acc, int32_t *status) const {
+  __device__ __forceinline__ void eval(const Elem &elem, double *acc) const {
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
       if (!ok[u]) continue;
@@ -598,11 +598,11 @@ struct SplitRound {
       double *row = acc + (size_t)se[u].row_off * kBs;
       const SlotEntry<Elem::kCar> &s = se[u];
       elem.row_at(xv, i, [&](int j, int32_t, const T *blk) {
+        // slot 255 is a negative column id, which adds nothing (PETSc drops
+        // negative indices): a column outside the pattern was refused when
+        // the table was built
         const int sl = s.slot[j];
-        if (sl == 255) {
-          atomicAdd(status, 1);
-          return;
-        }
+        if (sl == 255) return;
 #pragma unroll
         for (int k = 0; k < kBs; ++k) atomic_add(&row[sl * kBs + k], double(blk[k]));
       });
@@ -663,7 +663,7 @@ __device__ __forceinline__ void slots_batch_split(const BatchDesc &d, int32_t st
   for (int32_t base = threadIdx.x; base < d.ni; base += R::kR * bd) {
     if (base != (int32_t)threadIdx.x)
       round.load(elem, base, d.ni, A.inc_ent + d.i0, table + d.i0, start, end);
-    round.eval(elem, acc, A.status);
+    round.eval(elem, acc);
   }
   __syncthreads();
 

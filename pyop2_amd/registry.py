@@ -146,13 +146,34 @@ def _variant(dtypes):
     return None
 
 
+def _refuse_generated(family, variant, why):
+    """The generated loop hands the kernel buffers of the data's type, and the
+    registered kernel sources take double arguments: their float32 variants
+    exist only as hand-written loops."""
+    from pyop2_amd.exceptions import DataTypeError
+    raise DataTypeError(f"{family}: {variant or 'mixed-precision'} data runs only in the hand-written "
+                        f"loops, which this loop does not fit ({why}); the kernel's C source takes "
+                        "double arguments")
+
+
 def lookup(gk):
     """Executor for ``gk`` if a hand-written entry point implements it."""
     lk = gk.local_kernel
     hit = _REGISTERED.get(code_digest(lk.code, lk.name))
-    if hit is None or getattr(gk, "_extruded", False):
+    if hit is None:
         return None
     family, params = hit
+    ex = _match(gk, lk, family, params)
+    if ex is None:
+        variant = _variant(lk.dtypes)
+        if variant != "f64":
+            _refuse_generated(family, variant, "extruded set or other arguments than the entry point's")
+    return ex
+
+
+def _match(gk, lk, family, params):
+    if getattr(gk, "_extruded", False):
+        return None
     spec = FAMILIES[family]
     if len(gk.arguments) != len(spec["args"]):
         return None
@@ -278,6 +299,9 @@ class HandWrittenKernel:
             raise RuntimeError("hand-written loops are launched by a Parloop")
         if not self._check_maps(parloop):
             # a loop the entry point does not take: the generated path
+            if self.variant != "f64":
+                _refuse_generated(self.family, self.variant,
+                                  "a map with negative entries, or indices beyond 32 bits")
             from pyop2_amd.codegen import GeneratedKernel
             return GeneratedKernel(self.gk)(start, end, ptrs, parloop)
         self.launch(start, end, parloop, self._choose(parloop))

@@ -138,3 +138,19 @@ def unreferenced_nodes(mesh, nholes=37, seed=5):
     u[holes] = rng.standard_normal((nholes, 3))
     return dataclasses.replace(mesh, cell_node=keep[mesh.cell_node].astype(np.int32), u=u,
                                nedges=mesh.nedges + nholes), holes
+
+
+def negative_entries(mesh, fraction=0.05, seed=13):
+    """``mesh`` with the last edge node of a ``fraction`` of its cells
+    replaced by -1 (no target: PETSc drops negative row and column ids), a
+    cell only changed while the node it drops stays referenced."""
+    import dataclasses
+    cn = mesh.cell_node.copy()
+    refs = np.bincount(cn.ravel(), minlength=mesh.nnodes)
+    rng = np.random.default_rng(seed)
+    for e in rng.permutation(mesh.ncells)[: int(fraction * mesh.ncells)]:
+        if refs[cn[e, 9]] > 1:
+            refs[cn[e, 9]] -= 1
+            cn[e, 9] = -1
+    assert (cn < 0).any() and refs.min() > 0
+    return dataclasses.replace(mesh, cell_node=cn)

@@ -6,7 +6,8 @@ reference Mat is FP64-only, so FP32 has no reference counterpart).
 import numpy as np
 import pytest
 
-from helpers import TetProblem, oracle_p2, relerr, repeated_edge_nodes, unreferenced_nodes, use_strategy
+from helpers import (TetProblem, negative_entries, oracle_p2, relerr, repeated_edge_nodes, unreferenced_nodes,
+                     use_strategy)
 from pyop2_amd import elements, op2
 from pyop2_amd.configuration import configuration
 from pyop2_amd.meshes import unit_cube_tet_p2, workload_rng
@@ -162,3 +163,34 @@ def test_p2_residual_with_unreferenced_nodes(device, tet8_holes, strategy):
                  p.u(op2.READ, p.c2n))
     assert relerr(R.data_ro, 1.0 + Rref) < 1e-12
     assert np.array_equal(R.data_ro[holes], np.ones((len(holes), 3)))
+
+
+def test_p2_assembly_with_negative_map_entries(device, tet8):
+    """A node map with -1 entries: those rows and columns add nothing, as
+    PETSc drops negative ids.  The hand-written loops assume every entry has
+    a target, so the loop takes the generated one."""
+    m = negative_entries(tet8)
+    configuration["strategy"] = "auto"
+    p = TetProblem(m)
+    A = op2.Mat(p.sparsity(), np.float64)
+    op2.par_loop(elements.p2_elasticity_matrix(), p.cells, A(op2.INC, (p.c2n, p.c2n)), p.X(op2.READ, p.c2v))
+    A.assemble()
+    ref = oracle_p2("p2_elasticity_matrix", m)
+    rp, ci, v = A._host_csr()
+    assert np.array_equal(rp, ref.rowptr) and np.array_equal(ci, ref.colidx)
+    assert relerr(v, ref.values) < 1e-12
+
+
+def test_p2_float32_loop_that_needs_the_generated_path_is_refused(device, tet8):
+    """The generated loop hands the kernel buffers of the data's type and the
+    kernel's C source takes doubles, so a float32 loop the hand-written
+    entry points do not take (here: negative map entries) raises instead of
+    computing on mistyped buffers."""
+    from pyop2_amd.exceptions import DataTypeError
+    m = negative_entries(tet8)
+    configuration["strategy"] = "auto"
+    p = TetProblem(m, dtype=np.float32)
+    A = op2.Mat(p.sparsity(), np.float32)
+    with pytest.raises(DataTypeError, match="float32|f32"):
+        op2.par_loop(elements.p2_elasticity_matrix(), p.cells, A(op2.INC, (p.c2n, p.c2n)),
+                     p.X(op2.READ, p.c2v))

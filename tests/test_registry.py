@@ -60,3 +60,31 @@ def test_user_written_copy_takes_the_hand_written_entry_point():
     assert isinstance(ex, HandWrittenKernel) and ex.family == P1_NAME
     wrong = reformatted(src, P1_NAME, "laplace_res").replace("+=", "-=", 1)
     assert lookup(_p1_loop(wrong, "laplace_res").global_kernel) is None
+
+
+def _p1_typed_loop(dtypes):
+    n = 6
+    cells = op2.Set(2 * n * n, "cells")
+    verts = op2.Set((n + 1) ** 2, "verts")
+    rng = np.random.default_rng(0)
+    c2v = op2.Map(cells, verts, 3, rng.integers(0, (n + 1) ** 2, (2 * n * n, 3)), "c2v")
+    X = op2.Dat(verts ** 2, rng.random(((n + 1) ** 2, 2)).astype(dtypes[1]), dtypes[1], name="X")
+    u = op2.Dat(verts, rng.random((n + 1) ** 2).astype(dtypes[2]), dtypes[2], name="u")
+    R = op2.Dat(verts, dtype=dtypes[0], name="R")
+    return op2.ParLoop(op2.Kernel(elements.kernel_source(P1_NAME), P1_NAME), cells, R(op2.INC, c2v),
+                       X(op2.READ, c2v), u(op2.READ, c2v))
+
+
+def test_registered_kernel_on_data_its_source_does_not_type_is_refused():
+    """The registered sources take double arguments and the generated loop
+    hands a kernel buffers of the data's type, so a registered kernel on
+    float32 or mixed data that no hand-written entry point takes (the P1
+    families have float64 entry points only) raises instead of running on
+    mistyped buffers."""
+    import pytest
+    from pyop2_amd.exceptions import DataTypeError
+    ok = lookup(_p1_typed_loop((np.float64,) * 3).global_kernel)
+    assert isinstance(ok, HandWrittenKernel) and ok.variant == "f64"
+    for dtypes in ((np.float32,) * 3, (np.float64, np.float32, np.float64)):
+        with pytest.raises(DataTypeError, match="double arguments"):
+            lookup(_p1_typed_loop(dtypes).global_kernel)
